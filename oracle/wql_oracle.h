@@ -1,0 +1,100 @@
+/*
+ * wql_oracle.h -- CPU restatement of the WaveQLab3D time-stepping hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * it.  The product (waveqlab3d-gh_amd/) never links or calls it.
+ *
+ * Plain C99, double precision throughout, written from the reference Fortran
+ * (file:line citations in wql_oracle.c).  It reproduces the reference's
+ * 2-MPI-rank behaviour: each block owns its own copy of the interface state
+ * and evolves it with its own face frame (domain.f90:600-609,
+ * RHS_Interior.f90:1750-1772).
+ *
+ * Layouts (identical to the reference's column-major Fortran arrays on the
+ * owned range):  volume  a[f][s][r][q]   (q fastest)
+ *                q-face  a[f][s][r],  r-face a[f][s][q],  s-face a[f][r][q]
+ */
+#ifndef WQL_ORACLE_H
+#define WQL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORA_LOCKED = 0, ORA_SLIP_WEAKENING = 1, ORA_RATE_STATE = 2 };
+
+/* fault law and prestress description (same meaning as wql_fault_params) */
+typedef struct {
+    int law;            /* ORA_LOCKED / ORA_SLIP_WEAKENING / ORA_RATE_STATE */
+    int stress_mode;    /* 0: constant tractions T0 + boxcar patches; 1: tensor s0 on the face frame */
+    double T0[3];       /* (n, m, l) background tractions, stress_mode 0 */
+    double s0[6];       /* sxx syy szz sxy sxz syz, stress_mode 1 */
+    int npatch;         /* boxcar patches added to T0_l: (yc, zc, wy, wz, amp) */
+    double patch[4][5];
+    int nuc;            /* 0 none, 1 cosine-tapered disk (yc, zc, r1, width, amp), 2 smooth bump (yc, zc, R, amp, T) */
+    double nucp[5];
+    double fs, fd, Dc;  /* linear slip weakening */
+    double bar_wy, bar_wz, bar_big; /* fs += bar_big*(1 - boxcar(y,bar_wy)*boxcar(z,bar_wz)) */
+    double rs_a, rs_da, rs_w, rs_wr, rs_b, rs_L, rs_V0, rs_f0, rs_Vin;
+} ora_fault;
+
+typedef struct {
+    int nq, nr, ns;
+    int cartesian;      /* 1: reference 'cartesian' interior kernel (JU..:16357-16445) */
+    int bc[6];          /* Lx Rx Ly Ry Lz Rz boundary codes (0 interface, 1 absorbing, 2 free, 3 clamped) */
+    int pml[6];         /* PML slab flags per side */
+    int npml;
+    double *X;          /* [3][NP] coordinates (input) */
+    double *metric;     /* [10][NP] q_x r_x s_x q_y r_y s_y q_z r_z s_z J (computed) */
+    double *M;          /* [3][NP] lambda mu rho */
+    double *nml[6];     /* per side: [9][face pts] = n(3), m(3), l(3) (computed) */
+    double *F, *DF;     /* [9][NP] */
+    double *Q[6], *DQ[6]; /* PML slab arrays [9][slab pts] (slab = owned range restricted to the layer) */
+    double *P;          /* [2][NP] plastic strain / strain rate (may be NULL) */
+} ora_block;
+
+typedef struct {        /* one block's copy of the interface state, q-face arrays */
+    double *S, *DS;     /* [4][nr*ns] */
+    double *W, *DW;     /* [nr*ns] */
+    double *trup;       /* [nr*ns] */
+    double *Svel;       /* [3][nr*ns] */
+    double *Uhat, *Vhat;/* [6][nr*ns] stage-1 output snapshot (hat + prestress) */
+} ora_iface;
+
+typedef struct {
+    ora_block b[2];
+    ora_iface I[2];     /* I[0]: block 1's copy, I[1]: block 2's copy */
+    ora_fault fault;
+    int plastic;        /* 0 elastic, 1 plastic_flow2 ('scec'), 2 plastic_flow (default model) */
+    double pl_c, pl_nu, pl_Tv;          /* plastic_flow2 constants */
+    double pl_mu, pl_beta, pl_eta;      /* plastic_flow parameters */
+    int s0_kind;        /* initial stress tensor for plasticity: 0 constant s0 (fault.s0), 1 TPV26 depth profile */
+    double t, dt;
+} ora_domain;
+
+/* metrics + Jacobian from X (grid.f90:396-469) and face frames (unit_normals.f90:8-282) */
+int ora_metrics(ora_block *b);
+/* DF += L(F): interior + boundary band + PML (RHS_Interior.f90:8-272, 1782-2201) */
+void ora_set_rates(ora_block *b);
+/* outer-boundary SAT terms (RHS_Interior.f90:1516-1703) */
+void ora_boundary(ora_block *b);
+/* fault SAT + friction for block ib (1 or 2) (RHS_Interior.f90:1706-1779) */
+int ora_interface(ora_domain *d, int ib, double t, int stage);
+/* nsteps of LSRK(5,4) (time_step.f90:68-150) */
+int ora_step(ora_domain *d, int nsteps);
+/* initial state variable and velocities for rate and state (Interface_Condition.f90:2722-2850) */
+void ora_init_state(ora_domain *d);
+/* one SBP first derivative row (for tests): D applied to u at index i of n points */
+double ora_d1(const double *u, int n, int i, double h);
+/* LSRK coefficients A[5], B[5], C[6] (time_step.f90:53-62) */
+void ora_rk_coeffs(double *A, double *B, double *C);
+/* pointwise friction solve (tests): slip-weakening / rate-and-state hat variables */
+int ora_hat(const ora_fault *f, const double u[6], const double v[6], const double mat1[3],
+            const double mat2[3], const double x[3], const double nrm[9], double slip,
+            double psi, double t, double uhat[6], double vhat[6], double *Vel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,171 @@
+"""Write the golden fixtures of tests/golden/*.npz from the reference build.
+
+Runs only in the development container: it needs the reference Fortran built
+by `make -C oracle ref` (objects and binaries in oracle/_ref/, never committed)
+and MPICH's mpiexec.  Every case runs with 2 MPI ranks (one block per rank),
+the configuration under which the reference accumulates fault slip once per
+stage (domain.f90:600-609).
+
+Each case is described by an input file in the reference's namelist format
+(written below, all block parameters explicit because the reference leaves
+unset `btp` fields uninitialised, domain.f90:99) and one golden_driver mode:
+  rhs  -> DF after the volume operator and after the SAT terms, for a
+          deterministic pattern state (oracle/golden_driver.f90:fill_pattern)
+  traj -> the state after n LSRK steps from the reference initial condition
+The .npz fixtures hold float64 arrays in C order (field, s, r, q) for volume
+arrays and (field, s, r) for fault arrays.
+"""
+import os, subprocess, sys, shutil
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, 'oracle', '_ref', 'golden')
+MPIEXEC = os.environ.get('MPIEXEC', '/opt/conda/bin/mpiexec')
+
+BASE_BLOCKS = dict(
+    aqrs=[(-3.0, 6.0, -1.5), (0.0, 6.0, -1.5)],
+    bqrs=[(0.0, 9.0, 1.5), (3.0, 9.0, 1.5)],
+)
+
+
+def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
+             plastic_model='default', pml_l=((False,) * 3,) * 2, pml_r=((False,) * 3,) * 2,
+             npml=0, lc=(1.0, 0.0), rc=(0.0, 1.0), profile='analytical_test_problem',
+             rho_s_p=(2.67, 3.464, 6.0), t_final=1.0):
+    T = lambda b: 'T' if b else 'F'
+    s = ["&problem_list",
+         f" name = 'data/{name}',", f" problem = '{problem}',", f" response = '{response}',",
+         f" plastic_model = '{plastic_model}',", " nblocks = 2,", f" t_final = {t_final}d0,",
+         " CFL = 0.5d0,", f" coupling = '{coupling}',", f" type_of_mesh = '{mesh}',",
+         " w_stride = 1,", " w_fault = F /", "", "&block_list"]
+    for b in range(2):
+        a, c = BASE_BLOCKS['aqrs'][b], BASE_BLOCKS['bqrs'][b]
+        i = b + 1
+        s += [f"btp({i})%nqrs = {n[0]}, {n[1]}, {n[2]},",
+              f"btp({i})%aqrs = {a[0]}d0, {a[1]}d0, {a[2]}d0,",
+              f"btp({i})%bqrs = {c[0]}d0, {c[1]}d0, {c[2]}d0,",
+              f"btp({i})%rho_s_p = {rho_s_p[0]}d0, {rho_s_p[1]}d0, {rho_s_p[2]}d0,",
+              "btp(%d)%%mu_beta_eta = 0d0, 0d0, 0d0," % i,
+              f"btp({i})%lqrs = {lq[b][0]}, {lq[b][1]}, {lq[b][2]},",
+              f"btp({i})%rqrs = {rq[b][0]}, {rq[b][1]}, {rq[b][2]},",
+              f"btp({i})%lc = {lc[b]}d0,", f"btp({i})%rc = {rc[b]}d0,",
+              f"btp({i})%profile_type = '{profile}',", f"btp({i})%profile_path = 'No-path',",
+              f"btp({i})%topography_type = 'none',", f"btp({i})%topography_path = 'No-path',",
+              "btp(%d)%%material_path = 'none', 'none', 'none'," % i,
+              f"btp({i})%pml_lqrs = {', '.join(T(x) for x in pml_l[b])},",
+              f"btp({i})%pml_rqrs = {', '.join(T(x) for x in pml_r[b])},",
+              f"btp({i})%npml = {npml},", f"btp({i})%faultsize = {n[1]}, {n[2]}" + (',' if b == 0 else ' /')]
+    s += ["", "&moment_list", " use_moment_tensor = F /", "", "&mms_list", " use_mms = F /", "",
+          "&output_list", " output_seismograms = F,", " output_fault_topo = F /", ""]
+    return "\n".join(s) + "\n"
+
+
+LQ_STD = [(1, 2, 1), (0, 2, 1)]
+RQ_STD = [(0, 1, 3), (1, 1, 1)]
+N_STD = (15, 14, 13)
+
+CASES = {
+    # name: (namelist kwargs, [(mode, args), ...])
+    'cart_sw': (dict(problem='TPV5', coupling='slip-weakening_friction', mesh='cartesian',
+                     n=N_STD, lq=LQ_STD, rq=RQ_STD),
+                [('rhs', ['0.3']), ('traj', ['12'])]),
+    'curv_sw': (dict(problem='TPV28', coupling='slip-weakening_friction', mesh='curvilinear',
+                     n=N_STD, lq=LQ_STD, rq=RQ_STD),
+                [('rhs', ['0.3']), ('traj', ['12'])]),
+    'curv_rs': (dict(problem='TPV101', coupling='rate-and-state_friction', mesh='curvilinear',
+                     n=N_STD, lq=LQ_STD, rq=RQ_STD),
+                [('rhs', ['0.55']), ('traj', ['12'])]),
+    'pml_locked': (dict(problem='TPV5', coupling='locked', mesh='cartesian', n=(16, 15, 14),
+                        lq=[(1, 1, 1), (0, 1, 1)], rq=[(0, 1, 1), (1, 2, 1)],
+                        pml_l=((True, True, False), (False, False, True)),
+                        pml_r=((False, False, True), (True, False, False)), npml=7),
+                   [('rhs', ['0.1']), ('traj', ['10', '-1.2', '7.4', '0.2', '0.5', '1.0'])]),
+    'plastic_sw': (dict(problem='TPV28', coupling='slip-weakening_friction', mesh='cartesian',
+                        n=N_STD, lq=LQ_STD, rq=RQ_STD, response='plastic', plastic_model='scec'),
+                   [('traj', ['10'])]),
+    'locked_pulse': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=N_STD,
+                          lq=LQ_STD, rq=RQ_STD),
+                     [('traj', ['10', '-0.8', '7.2', '0.3', '0.4', '1.0'])]),
+}
+
+
+def read_block(prefix, b, name, shape):
+    return np.fromfile(f'{prefix}_b{b}_{name}.bin', '<f8').reshape(shape, order='F')
+
+
+def run_case(name, kw, runs, work):
+    os.makedirs(work, exist_ok=True)
+    inp = os.path.join(work, f'{name}.in')
+    with open(inp, 'w') as f:
+        f.write(namelist(name, **kw))
+    for d in ('data', 'seismogram/block1/data', 'seismogram/block2/data'):
+        os.makedirs(os.path.join(work, d), exist_ok=True)
+    nq, nr, ns = kw['n']
+    out = {'n': np.array(kw['n'])}
+    for mode, args in runs:
+        prefix = os.path.join(work, f'{name}_{mode}')
+        cmd = [MPIEXEC, '-n', '2', REF, f'{name}.in', mode, prefix] + args
+        r = subprocess.run(cmd, cwd=work, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            print(r.stdout[-3000:], r.stderr[-3000:])
+            raise SystemExit(f'{name} {mode} failed')
+        vol = lambda b, nm, nf: read_block(prefix, b, nm, (nq, nr, ns, nf)).transpose(3, 2, 1, 0)
+        face = lambda b, nm, nf: read_block(prefix, b, nm, (nr, ns, nf)).transpose(2, 1, 0)
+        for b in (1, 2):
+            if mode == 'rhs':
+                out[f'rhs_t'] = np.array(float(args[0]))
+                out[f'b{b}_DFvol'] = vol(b, 'DFvol', 9)
+                out[f'b{b}_DFfull'] = vol(b, 'DFfull', 9)
+                out[f'b{b}_DS'] = face(b, 'DS', 4)
+                out[f'b{b}_DW'] = face(b, 'DW', 1)
+                for side in range(1, 7):
+                    for tag in ('DQvol', 'DQfull'):
+                        fn = f'{prefix}_b{b}_{tag}{side}.bin'
+                        if os.path.exists(fn):
+                            out[f'b{b}_{tag}{side}'] = np.fromfile(fn, '<f8')
+            else:
+                out['traj_args'] = np.array([float(a) for a in args])
+                out[f'b{b}_F'] = vol(b, 'F', 9)
+                out[f'b{b}_S'] = face(b, 'S', 4)
+                out[f'b{b}_W'] = face(b, 'W', 1)
+                out[f'b{b}_W0'] = face(b, 'W0', 1)
+                out[f'b{b}_trup'] = face(b, 'trup', 1)
+                out[f'b{b}_Svel'] = face(b, 'Svel', 3)
+                for side in range(1, 7):
+                    fn = f'{prefix}_b{b}_Q{side}.bin'
+                    if os.path.exists(fn):
+                        out[f'b{b}_Q{side}'] = np.fromfile(fn, '<f8')
+                fn = f'{prefix}_b{b}_P.bin'
+                if os.path.exists(fn):
+                    out[f'b{b}_P'] = vol(b, 'P', 2)
+                meta = dict(l.split() for l in open(f'{prefix}_meta.txt'))
+                out['dt'] = np.array(float(meta['dt']))
+                out['t_end'] = np.array(float(meta['t']))
+            if name == 'curv_sw':
+                out[f'b{b}_X'] = vol(b, 'X', 3)
+                if True:
+                    out[f'b{b}_metric'] = np.concatenate(
+                        [vol(b, 'metricx', 3), vol(b, 'metricy', 3), vol(b, 'metricz', 3),
+                         vol(b, 'J', 1)])
+                    nml = []
+                    for side in range(1, 7):
+                        a = np.fromfile(f'{prefix}_b{b}_nml{side}.bin', '<f8')
+                        nml.append(a)
+                    out[f'b{b}_nml'] = np.concatenate(nml)
+    path = os.path.join(HERE, f'{name}.npz')
+    np.savez_compressed(path, **out)
+    with open(os.path.join(HERE, 'inputs', f'{name}.in'), 'w') as f:
+        f.write(namelist(name, **kw))
+    print(f'wrote {path} ({os.path.getsize(path) / 1e3:.0f} kB)')
+
+
+if __name__ == '__main__':
+    if not os.path.exists(REF):
+        sys.exit('build the reference first: make -C oracle ref')
+    os.makedirs(os.path.join(HERE, 'inputs'), exist_ok=True)
+    work = os.path.join(ROOT, '_work', 'golden_runs')
+    names = sys.argv[1:] or list(CASES)
+    for nm in names:
+        kw, runs = CASES[nm]
+        run_case(nm, kw, runs, os.path.join(work, nm))

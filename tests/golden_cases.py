@@ -1,0 +1,82 @@
+"""Helpers shared by the parity tests: the golden cases of tests/golden/."""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, 'tests', 'golden')
+sys.path.insert(0, os.path.join(ROOT, 'waveqlab3d-gh_amd'))
+
+from wql import problem as P  # noqa: E402
+
+RHS_CASES = ['cart_sw', 'curv_sw', 'curv_rs', 'pml_locked']
+TRAJ_CASES = ['cart_sw', 'curv_sw', 'curv_rs', 'pml_locked', 'plastic_sw', 'locked_pulse']
+
+
+def load(case):
+    return np.load(os.path.join(GOLDEN, f'{case}.npz'))
+
+
+def problem(case):
+    return P.from_namelist(os.path.join(GOLDEN, 'inputs', f'{case}.in'))
+
+
+def pattern(shape_qrs, f1, blk, scale, i0=1):
+    """oracle/golden_driver.f90:pat on a (nq, nr, ns) grid -> [ns][nr][nq]."""
+    nq, nr, ns = shape_qrs
+    k, j, i = np.meshgrid(np.arange(1, ns + 1), np.arange(1, nr + 1), np.arange(i0, i0 + nq),
+                          indexing='ij')
+    idx = i.astype(np.int64) + 1000 * j + 1000000 * k + 1000000000 * np.int64(f1)
+    h = (idx * 7919 + 104729 * blk) % 1000003
+    return scale * (2.0 * (h / 1000003.0) - 1.0)
+
+
+def pattern_state(prob):
+    """The state golden_driver's fill_pattern writes, per block:
+    F [9][ns][nr][nq], PML Q per side [9][slab], S [4][ns][nr], W [1][ns][nr]."""
+    out = []
+    for ib, b in enumerate(prob.blocks):
+        nq, nr, ns = b.n
+        F = np.stack([pattern(b.n, f + 1, ib + 1, 10.0 if f >= 3 else 1.0) for f in range(9)])
+        Q = [None] * 6
+        for s in range(6):
+            if not b.pml[s]:
+                continue
+            lo, hi = [1, 1, 1], [nq, nr, ns]
+            d = s // 2
+            if s % 2 == 0:
+                hi[d] = b.npml
+            else:
+                lo[d] = b.n[d] - b.npml + 1
+            kk, jj, ii = np.meshgrid(np.arange(lo[2], hi[2] + 1), np.arange(lo[1], hi[1] + 1),
+                                     np.arange(lo[0], hi[0] + 1), indexing='ij')
+            q = []
+            for f in range(9):
+                idx = ii.astype(np.int64) + 1000 * jj + 1000000 * kk + 1000000000 * np.int64(f + 1 + 10 * (s + 1))
+                h = (idx * 7919 + 104729 * (ib + 3)) % 1000003
+                q.append((0.1 * (2.0 * (h / 1000003.0) - 1.0)).ravel())
+            Q[s] = np.stack(q)
+        kk, jj = np.meshgrid(np.arange(1, ns + 1), np.arange(1, nr + 1), indexing='ij')
+        S = []
+        for f in range(4):
+            idx = 1000 * jj.astype(np.int64) + 1000000 * kk + 1000000000 * np.int64(f + 1)
+            h = (idx * 7919 + 104729 * 7) % 1000003
+            S.append(0.25 * ((2.0 * (h / 1000003.0) - 1.0) + 1.0))
+        idx = 1000 * jj.astype(np.int64) + 1000000 * kk
+        h = (idx * 7919 + 104729 * 8) % 1000003
+        W = (0.6 + 0.05 * (2.0 * (h / 1000003.0) - 1.0))[None]
+        out.append(dict(F=F, Q=Q, S=np.stack(S), W=W))
+    return out
+
+
+def pulse(prob, X, args):
+    """Gaussian velocity pulse added by golden_driver's add_pulse."""
+    x0, y0, z0, sig, amp = args
+    r2 = (X[0] - x0) ** 2 + (X[1] - y0) ** 2 + (X[2] - z0) ** 2
+    return amp * np.exp(-r2 / (2.0 * sig ** 2))
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)

@@ -1,0 +1,150 @@
+"""Block geometry: node coordinates X[3][ns][nr][nq] of one block.
+
+Host-side initialisation (runs once per run, not on the time-stepping path).
+The coordinates are the input of the metric computation done on the device.
+
+* ``cartesian`` restates cartesian_grid_3d (grid.f90:213-240).
+* ``curvilinear`` restates the transfinite-interpolation generator
+  Curve_Grid_3D3 (grid.f90:471-1907) for the cases the hot path needs: the
+  'analytical_test_problem' fault profile (grid.f90:775-831), plus
+  caller-supplied fault-surface and free-surface perturbation fields (the
+  seeded topography / rough-fault inputs of the benchmark workloads, which the
+  reference reads from files through the same TFI; grid.f90:1349-1430).
+  Face TFI: Gen_Surf_Mesh (grid.f90:2418-2455); volume TFI: Gen3_Curve_Mesh
+  (grid.f90:2290-2416).
+"""
+import numpy as np
+
+PI = 3.141592653589793
+
+
+def cartesian(n, a, b):
+    """X for a uniform block with n=(nq,nr,ns) nodes spanning [a, b]."""
+    nq, nr, ns = n
+    X = np.empty((3, ns, nr, nq))
+    for d, m in enumerate((nq, nr, ns)):
+        h = (b[d] - a[d]) / float(m - 1)
+        c = a[d] + np.arange(m, dtype=np.float64) * h
+        shape = [1, 1, 1]
+        shape[2 - d] = m
+        X[d] = np.broadcast_to(c.reshape(shape), (ns, nr, nq))
+    return X
+
+
+def _surf(mx, my, XL, XR, XB, XT):
+    """Gen_Surf_Mesh: 2-D TFI; result indexed [i, j] with i < mx, j < my."""
+    hq = 1.0 / float(mx - 1)
+    hr = 1.0 / float(my - 1)
+    q = (np.arange(mx, dtype=np.float64) * hq)[:, None]
+    r = (np.arange(my, dtype=np.float64) * hr)[None, :]
+    XL = np.asarray(XL)[None, :]
+    XR = np.asarray(XR)[None, :]
+    XBi = np.asarray(XB)[:, None]
+    XTi = np.asarray(XT)[:, None]
+    return ((1.0 - q) * XL + q * XR + (1.0 - r) * XBi + r * XTi
+            - (1.0 - q) * (1.0 - r) * XL[:, :1] - q * (1.0 - r) * XR[:, :1]
+            - r * (1.0 - q) * XT[0] - (r * q) * XT[mx - 1])
+
+
+def _tfi3(n, Xleft, Xright, Xtop, Xbottom, Xfront, Xback):
+    """Gen3_Curve_Mesh for one coordinate component -> array [ns][nr][nq].
+
+    Face arrays: left/right [nr][ns] (j,k), top/bottom [nq][ns] (i,k),
+    front/back [nq][nr] (i,j)."""
+    nq, nr, ns = n
+    hq, hr, hs = 1.0 / float(nq - 1), 1.0 / float(nr - 1), 1.0 / float(ns - 1)
+    q = (np.arange(nq, dtype=np.float64) * hq)[None, None, :]
+    r = (np.arange(nr, dtype=np.float64) * hr)[None, :, None]
+    s = (np.arange(ns, dtype=np.float64) * hs)[:, None, None]
+    L = Xleft.T[:, :, None]       # [k][j][1]
+    R = Xright.T[:, :, None]
+    T = Xtop.T[:, None, :]        # [k][1][i]
+    B = Xbottom.T[:, None, :]
+    Fr = Xfront.T[None, :, :]     # [1][j][i]
+    Bk = Xback.T[None, :, :]
+    U = (1.0 - q) * L + q * R
+    V = (1.0 - r) * T + r * B
+    W = (1.0 - s) * Fr + s * Bk
+    UW = ((1.0 - q) * ((1.0 - r) * Xleft[0, :][:, None, None] + r * Xleft[nr - 1, :][:, None, None])
+          + q * ((1.0 - r) * Xright[0, :][:, None, None] + r * Xright[nr - 1, :][:, None, None]))
+    UV = ((1.0 - r) * ((1.0 - s) * Xtop[:, 0][None, None, :] + s * Xtop[:, ns - 1][None, None, :])
+          + r * ((1.0 - s) * Xbottom[:, 0][None, None, :] + s * Xbottom[:, ns - 1][None, None, :]))
+    VW = ((1.0 - s) * ((1.0 - q) * Xfront[0, :][None, :, None] + q * Xfront[nq - 1, :][None, :, None])
+          + s * ((1.0 - q) * Xback[0, :][None, :, None] + q * Xback[nq - 1, :][None, :, None]))
+    UVW1 = (1.0 - q) * ((1.0 - r) * ((1.0 - s) * Xtop[0, 0] + s * Xtop[0, ns - 1])
+                        + r * ((1.0 - s) * Xbottom[0, 0] + s * Xbottom[0, ns - 1]))
+    UVW2 = q * ((1.0 - r) * ((1.0 - s) * Xtop[nq - 1, 0] + s * Xtop[nq - 1, ns - 1])
+                + r * ((1.0 - s) * Xbottom[nq - 1, 0] + s * Xbottom[nq - 1, ns - 1]))
+    return U + V + W - UV - UW - VW + UVW1 + UVW2
+
+
+def curvilinear(n, a, b, lc=0.0, rc=0.0, profile='analytical_test_problem',
+                fault_left=None, fault_right=None, top=None):
+    """Curvilinear block with n=(nq,nr,ns) nodes over [a, b].
+
+    profile 'analytical_test_problem' adds the Gaussian-plus-sinusoid bump of
+    grid.f90:812-827 to the q-faces, weighted by rc (left face) and lc (right
+    face).  profile 'flat' keeps the q-faces planar; then ``fault_left`` /
+    ``fault_right`` ([nr][ns] arrays of x offsets) perturb the q-faces and
+    ``top`` ([nq][ns] array of y offsets) perturbs the r = 1 face.
+    """
+    mx, my, mz = n
+    ax, ay, az = a
+    bx, by, bz = b
+    hy = (by - ay) / float(my - 1)
+    hz = (bz - az) / float(mz - 1)
+    jy = ay + np.arange(my, dtype=np.float64) * hy
+    kz = az + np.arange(mz, dtype=np.float64) * hz
+    # x-faces (grid.f90:779-811)
+    Yleft = _surf(my, mz, np.full(mz, ay), np.full(mz, by), jy, jy)
+    Zleft = _surf(my, mz, kz, kz, np.full(my, az), np.full(my, bz))
+    Yright, Zright = Yleft.copy(), Zleft.copy()
+    if profile == 'analytical_test_problem':
+        Xleft = (ax + 0.1 * rc * np.exp(-((1.0 * Yleft - 0.5 * (ay + by)) ** 2
+                                          + 1.0 * (Zleft - 0.5 * (az + bz)) ** 2) / 0.025)
+                 + 0.1 * rc * np.sin(2.0 / (by - ay) * PI * Yleft) * np.sin(2.0 / (bz - az) * PI * Zleft))
+        Xright = (bx + 0.1 * lc * np.exp(-(1.0 * (Yright - 0.5 * (ay + by)) ** 2
+                                           + 1.0 * (Zright - 0.5 * (az + bz)) ** 2) / 0.025)
+                  + 0.1 * lc * np.sin(2.0 / (by - ay) * PI * Yright) * np.sin(2.0 / (bz - az) * PI * Zright))
+    else:
+        Xleft = np.full((my, mz), float(ax))
+        Xright = np.full((my, mz), float(bx))
+        if fault_left is not None:
+            Xleft = Xleft + fault_left
+        if fault_right is not None:
+            Xright = Xright + fault_right
+
+    def edge(lo, hi, m):
+        h = (hi - lo) / float(m - 1)
+        return lo + np.arange(m, dtype=np.float64) * h
+
+    # z-faces (grid.f90:930-1041)
+    faces = {}
+    for tag, k in (('front', 0), ('back', mz - 1)):
+        XL, XR = Xleft[:, k], Xright[:, k]
+        YL, YR = Yleft[:, k], Yright[:, k]
+        XB = edge(Xleft[0, k], Xright[0, k], mx)
+        XT = edge(Xleft[my - 1, k], Xright[my - 1, k], mx)
+        YB = edge(Yleft[0, k], Yright[0, k], mx)
+        YT = edge(Yleft[my - 1, k], Yright[my - 1, k], mx)
+        faces['X' + tag] = _surf(mx, my, XL, XR, XB, XT)
+        faces['Y' + tag] = _surf(mx, my, YL, YR, YB, YT)
+        faces['Z' + tag] = np.full((mx, my), az if k == 0 else bz)
+    # y-faces (grid.f90:1044-1158)
+    for tag, j in (('top', 0), ('bottom', my - 1)):
+        ZT = faces['Zback'][:, j]
+        ZB = faces['Zfront'][:, j]
+        XB = edge(Xleft[j, 0], Xright[j, 0], mx)
+        XT = edge(Xleft[j, mz - 1], Xright[j, mz - 1], mx)
+        XL, XR = Xleft[j, :], Xright[j, :]
+        ZL, ZR = Zleft[j, :], Zright[j, :]
+        faces['X' + tag] = _surf(mx, mz, XL, XR, XB, XT)
+        faces['Z' + tag] = _surf(mx, mz, ZL, ZR, ZB, ZT)
+        faces['Y' + tag] = np.full((mx, mz), ay if j == 0 else by)
+    if top is not None:
+        faces['Ytop'] = faces['Ytop'] + top
+    X = np.empty((3, mz, my, mx))
+    X[0] = _tfi3(n, Xleft, Xright, faces['Xtop'], faces['Xbottom'], faces['Xfront'], faces['Xback'])
+    X[1] = _tfi3(n, Yleft, Yright, faces['Ytop'], faces['Ybottom'], faces['Yfront'], faces['Yback'])
+    X[2] = _tfi3(n, Zleft, Zright, faces['Ztop'], faces['Zbottom'], faces['Zfront'], faces['Zback'])
+    return X

@@ -1,0 +1,209 @@
+"""Problem description: the reference's input-file API restated as data.
+
+A ``Problem`` holds what the reference's &problem_list / &block_list
+namelists carry for the hot path (domain.f90:105-136, datatypes.f90:255-266)
+plus the fault law parameters that the reference keys by problem name
+(Interface_Condition.f90:1268-2435).  ``from_namelist`` reads a reference
+input file; ``fault_params`` turns a problem key into explicit parameters.
+"""
+import math
+import re
+from dataclasses import dataclass, field
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import geometry
+
+LOCKED, SLIP_WEAKENING, RATE_STATE = 0, 1, 2
+
+
+@dataclass
+class Block:
+    n: Tuple[int, int, int]
+    a: Tuple[float, float, float]
+    b: Tuple[float, float, float]
+    rho_s_p: Tuple[float, float, float] = (2.67, 3.464, 6.0)   # rho, cs, cp
+    bc: Tuple[int, ...] = (1, 0, 2, 1, 1, 1)                   # Lx Rx Ly Ry Lz Rz
+    pml: Tuple[bool, ...] = (False,) * 6
+    npml: int = 0
+    lc: float = 0.0
+    rc: float = 0.0
+    mesh: str = 'cartesian'                                    # 'cartesian' | 'curvilinear'
+    profile: str = 'analytical_test_problem'
+    fault_offset: Optional[np.ndarray] = None                  # [nr][ns] x-offsets of the fault face
+    top_offset: Optional[np.ndarray] = None                    # [nq][ns] y-offsets of the r=1 face
+    material: Optional[np.ndarray] = None                      # [3][ns][nr][nq] lambda, mu, rho (heterogeneous)
+
+    def coords(self):
+        if self.mesh == 'cartesian':
+            return geometry.cartesian(self.n, self.a, self.b)
+        fl = fr = None
+        if self.fault_offset is not None:
+            if self.bc[1] == 0:
+                fr = self.fault_offset
+            else:
+                fl = self.fault_offset
+        return geometry.curvilinear(self.n, self.a, self.b, self.lc, self.rc,
+                                    self.profile, fault_left=fl, fault_right=fr, top=self.top_offset)
+
+    def lame(self):
+        """M = (lambda, mu, rho) on every node (material.f90:181-188)."""
+        if self.material is not None:
+            return np.ascontiguousarray(self.material, dtype=np.float64)
+        rho, cs, cp = self.rho_s_p
+        nq, nr, ns = self.n
+        M = np.empty((3, ns, nr, nq))
+        M[0] = rho * (cp ** 2 - 2.0 * cs ** 2)
+        M[1] = rho * cs ** 2
+        M[2] = rho
+        return M
+
+    def spacing(self):
+        return tuple((self.b[d] - self.a[d]) / float(self.n[d] - 1) for d in range(3))
+
+
+@dataclass
+class Fault:
+    law: int = LOCKED
+    stress_mode: int = 0
+    T0: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    s0: Tuple[float, ...] = (0.0,) * 6
+    patches: List[Tuple[float, float, float, float, float]] = field(default_factory=list)
+    nuc: int = 0
+    nucp: Tuple[float, ...] = (0.0,) * 5
+    fs: float = 0.0
+    fd: float = 0.0
+    Dc: float = 1.0
+    bar_wy: float = 1e30
+    bar_wz: float = 1e30
+    bar_big: float = 0.0
+    rs_a: float = 0.0
+    rs_da: float = 0.0
+    rs_w: float = 0.0
+    rs_wr: float = 1.0
+    rs_b: float = 0.0
+    rs_L: float = 1.0
+    rs_V0: float = 1.0
+    rs_f0: float = 0.0
+    rs_Vin: float = 0.0
+
+
+def fault_params(problem: str, coupling: str) -> Fault:
+    """Explicit parameters for the reference's problem keys."""
+    if coupling == 'locked':
+        return Fault(law=LOCKED)
+    if coupling == 'slip-weakening_friction':
+        if problem == 'TPV5':   # Interface_Condition.f90:1283-1299, 1822-1830
+            return Fault(law=SLIP_WEAKENING, stress_mode=0, T0=(-120.0, 0.0, 70.0),
+                         patches=[(7.5, -7.5, 1.5, 1.5, 8.0), (7.5, 0.0, 1.5, 1.5, 11.6),
+                                  (7.5, 7.5, 1.5, 1.5, -8.0)],
+                         fs=0.677, fd=0.525, Dc=0.4, bar_wy=15.0, bar_wz=15.0, bar_big=1.0e10)
+        if problem == 'TPV28':  # 1459-1474, 2049-2079, initial_stress_condition.f90:78-84
+            return Fault(law=SLIP_WEAKENING, stress_mode=1, s0=(-60.0, 0.0, 60.0, 0.0, 29.38, 0.0),
+                         nuc=1, nucp=(7.5, 0.0, 1.4, 0.6, 11.6),
+                         fs=0.677, fd=0.373, Dc=0.4, bar_wy=15.0, bar_wz=18.0, bar_big=1.0e10)
+    if coupling == 'rate-and-state_friction' and problem in ('TPV101', 'TPV102'):
+        # 386-400, 433-436, 2324-2385, 2756-2780, initial_stress_condition.f90:114-120
+        return Fault(law=RATE_STATE, stress_mode=1, s0=(-120.0, 0.0, -120.0, 0.0, 75.0, 0.0),
+                     nuc=2, nucp=(7.5, 0.0, 3.0, 25.0, 1.0),
+                     rs_a=0.008, rs_da=0.008, rs_w=15.0, rs_wr=3.0, rs_b=0.012, rs_L=0.02,
+                     rs_V0=1.0e-6, rs_f0=0.6, rs_Vin=1.0e-12)
+    raise ValueError(f'unsupported problem/coupling: {problem}/{coupling}')
+
+
+@dataclass
+class Problem:
+    blocks: List[Block]
+    fault: Fault
+    cfl: float = 0.5
+    t_final: float = 0.0
+    plastic: int = 0           # 0 elastic, 1 Drucker-Prager return map ('scec'), 2 default model
+    pl_c: float = 1.36
+    pl_nu: float = 0.1934
+    pl_Tv: float = 0.03
+    pl_mu: float = 0.0
+    pl_beta: float = 0.0
+    pl_eta: float = 0.0
+    s0_kind: int = 0           # plastic background stress: 0 = fault.s0 everywhere, 1 = TPV26 depth profile
+    name: str = 'default'
+
+    def dt(self):
+        """Global time step (domain.f90:222-235, block.f90:437-452)."""
+        dts = []
+        for b in self.blocks:
+            h = min(b.spacing())
+            dts.append(self.cfl * h / math.sqrt(b.rho_s_p[1] ** 2 + b.rho_s_p[2] ** 2))
+        return min(dts)
+
+    def nt(self):
+        return int(math.floor(self.t_final / self.dt()))
+
+
+def _nml_value(tok):
+    tok = tok.strip()
+    if tok in ('T', '.true.', '.TRUE.'):
+        return True
+    if tok in ('F', '.false.', '.FALSE.'):
+        return False
+    if tok[:1] in "'\"":
+        return tok.strip("'\"")
+    t = tok.replace('d', 'e').replace('D', 'e')
+    try:
+        return int(t)
+    except ValueError:
+        return float(t)
+
+
+def read_namelist(text, group):
+    """Minimal reader for the reference's &group ... / namelists."""
+    text = re.sub(r'!.*', '', text)
+    m = re.search(r'&' + group + r'\b(.*?)/\s*$', text, re.S | re.M)
+    if not m:
+        return {}
+    body = m.group(1)
+    out = {}
+    for key, val in re.findall(r"([A-Za-z_][\w%()]*)\s*=\s*((?:'[^']*'|[^=])*?)(?=,?\s*[A-Za-z_][\w%()]*\s*=|\s*$)", body, re.S):
+        vals = [v for v in re.findall(r"'[^']*'|[^,\s]+", val) if v.strip(',')]
+        parsed = [_nml_value(v) for v in vals]
+        out[key.lower()] = parsed[0] if len(parsed) == 1 else parsed
+    return out
+
+
+def from_namelist(path):
+    """Build a Problem from a reference input file (&problem_list, &block_list)."""
+    text = open(path).read()
+    pl = read_namelist(text, 'problem_list')
+    bl = read_namelist(text, 'block_list')
+    problem = pl.get('problem', 'TPV5')
+    coupling = pl.get('coupling', 'locked')
+    mesh = pl.get('type_of_mesh', 'cartesian')
+    blocks = []
+    for i in (1, 2):
+        g = lambda k, d=None: bl.get(f'btp({i})%{k}', d)
+        lq, rq = g('lqrs'), g('rqrs')
+        pml_l = g('pml_lqrs', [False] * 3)
+        pml_r = g('pml_rqrs', [False] * 3)
+        blocks.append(Block(
+            n=tuple(g('nqrs')), a=tuple(float(v) for v in g('aqrs')), b=tuple(float(v) for v in g('bqrs')),
+            rho_s_p=tuple(float(v) for v in g('rho_s_p', (2.67, 3.464, 6.0))),
+            bc=(lq[0], rq[0], lq[1], rq[1], lq[2], rq[2]),
+            pml=(pml_l[0], pml_r[0], pml_l[1], pml_r[1], pml_l[2], pml_r[2]),
+            npml=int(g('npml', 0) or 0), lc=float(g('lc', 0.0) or 0.0), rc=float(g('rc', 0.0) or 0.0),
+            mesh=mesh, profile=g('profile_type', 'analytical_test_problem')))
+    response = pl.get('response', 'elastic')
+    model = pl.get('plastic_model', 'default')
+    plastic = 0
+    if response == 'plastic':
+        plastic = 1 if model == 'scec' else 2
+    prob = Problem(blocks=blocks, fault=fault_params(problem, coupling),
+                   cfl=float(pl.get('cfl', 0.5)), t_final=float(pl.get('t_final', 0.0)),
+                   plastic=plastic, name=pl.get('name', 'default'))
+    if plastic:
+        f = fault_params(problem, 'slip-weakening_friction') if problem == 'TPV28' else prob.fault
+        prob.s0_kind = 1 if problem in ('TPV26', 'TPV27') else 0
+        if problem == 'TPV30':
+            prob.pl_c, prob.pl_nu, prob.pl_Tv = 1.18, 0.1680, 0.05
+        if prob.fault.stress_mode == 0 and problem == 'TPV28':
+            prob.fault.s0 = f.s0
+    return prob
