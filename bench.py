@@ -1,0 +1,160 @@
+"""Throughput of the LSRK time-stepping path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rupture]
+
+A step is one LSRK(5,4) time step (five RK stages) of the whole two-block
+problem.  The default workload is the curvilinear rupture problem
+(2 x 201x101x101 nodes, h = 100 m, slip-weakening fault, random topography),
+FP32 volume fields with the FP64 fault; the FP64 path is timed beside it.
+
+Metric: M grid-pt RK stages/s = nodes x 5 x steps / seconds / 1e6 (owned
+nodes, device-synchronised, no output).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, 'waveqlab3d-gh_amd'))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main_kernel_nodes(prob):
+    """nodes handled by the main volume kernel per launch (both blocks)"""
+    tot = 0
+    for b in prob.blocks:
+        nq, nr, ns = b.n
+        lo = lambda on: max(b.npml, 6) if on else 0
+        x0, x1 = lo(b.pml[0]), nq - lo(b.pml[1])
+        y0, y1 = lo(b.pml[2]), nr - lo(b.pml[3])
+        z0 = max(b.npml, 6) if b.pml[4] else 6
+        z1 = ns - (max(b.npml, 6) if b.pml[5] else 6)
+        tot += max(0, x1 - x0) * max(0, y1 - y0) * max(0, z1 - z0)
+    return tot
+
+
+def bytes_per_node(prob, precision):
+    """algorithmic HBM bytes per node and RK stage of the volume update:
+    F in 9 R, DF 9 R + 9 W, F out 9 W; + 10 (metrics + J) if curvilinear;
+    + 3 (lambda, mu, rho) if heterogeneous"""
+    vals = 36
+    if any(b.mesh != 'cartesian' for b in prob.blocks):
+        vals += 10
+    if any(b.material is not None for b in prob.blocks):
+        vals += 3
+    return vals * (4 if precision == 'fp32' else 8)
+
+
+def run(W, precision, steps, warmup, tile, zchunk, graph):
+    import torch
+    from wql.solver import Simulation
+    prob = W['problem']
+    sim = Simulation(prob, precision=precision, tile=tile, zchunk=zchunk, coords=W['coords'], graph=graph)
+    if W.get('F0') is not None:
+        for ib in range(2):
+            sim.set_fields(ib, W['F0'][ib])
+    if W.get('init_state'):
+        sim.init_state()
+    sim.step(warmup)
+    sim.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    # per-kernel timing of the main volume kernel (HIP events on the library stream)
+    sim.profile(True)
+    sim.step(2)
+    sim.sync()
+    kms = sim.volume_kernel_ms()
+    sim.profile(False)
+    bytes_dev = sim.device_bytes()
+    sim.close()
+    return sec, kms, bytes_dev
+
+
+def cpu_baseline(W):
+    """the oracle (C, 1 thread) on one LSRK step of a sub-volume of the same workload"""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle as O
+    from wql import workloads
+    name = W['problem'].name
+    sub = workloads.WORKLOADS.get(name, workloads.rupture)(scale=0.5)
+    prob = sub['problem']
+    od = O.OracleDomain(prob, coords=sub['coords'])
+    nsteps = 2
+    t0 = time.perf_counter()
+    od.step(nsteps)
+    sec = (time.perf_counter() - t0) / nsteps
+    nodes = workloads.points(prob)
+    return dict(value=nodes * 5 / sec / 1e6, unit='M grid-pt RK stages/s', cores=1, kind='port',
+                sample=f'oracle C restatement, 2 LSRK steps of the {name} workload at half the node count per axis '
+                       f'({"x".join(str(v) for v in prob.blocks[0].n)} per block, {nodes} nodes), {sec:.1f} s')
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=30)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', default='rupture')
+    ap.add_argument('--scale', type=float, default=1.0)
+    ap.add_argument('--tile', type=int, default=0)
+    ap.add_argument('--zchunk', type=int, default=32)
+    ap.add_argument('--graph', type=int, default=0)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-fp64', action='store_true')
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f'bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported '
+              '(the two-block problem would shard one block per GPU, see DESIGN.md)', file=sys.stderr)
+        sys.exit(2)
+    from wql import workloads
+    W = workloads.WORKLOADS[a.workload](scale=a.scale)
+    prob = W['problem']
+    nodes = workloads.points(prob)
+    res = {}
+    for prec in (['fp32'] if a.no_fp64 else ['fp64', 'fp32']):
+        sec, kms, dev = run(W, prec, a.steps, a.warmup, a.tile, a.zchunk, a.graph)
+        res[prec] = dict(sec=sec, kms=kms, dev=dev, mpts=nodes * 5 * a.steps / sec / 1e6)
+    r32 = res['fp32']
+    bpn = bytes_per_node(prob, 'fp32')
+    kn = main_kernel_nodes(prob)
+    achieved = kn * bpn / (r32['kms'] * 1e-3) / 1e9 if r32['kms'] > 0 else None
+    out = {
+        'metric': 'M grid-pt RK stages/s FP32 vs FP64 @<=1e-5 slip err; p50/p99 scenario latency',
+        'value': round(r32['mpts'], 2),
+        'unit': 'M grid-pt RK stages/s',
+        'n_gpus': 1,
+        'steps': a.steps,
+        'warmup': a.warmup,
+        'ms_per_step': round(r32['sec'] / a.steps * 1e3, 4),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32 volume / f64 fault',
+        'data': 'synthetic (seeded geometry, topography and friction)',
+        'config': {'workload': f'{a.workload}: 2 blocks x {"x".join(str(v) for v in prob.blocks[0].n)}',
+                   'nodes': nodes, 'tile_variant': a.tile, 'zchunk': a.zchunk},
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1) if achieved else None,
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(achieved / HBM_PEAK_GBS, 4) if achieved else None, 'traffic': None,
+                     'kernel': 'k_volume_main', 'kernel_ms': round(r32['kms'], 5),
+                     'algorithmic_bytes_per_launch': kn * bpn},
+    }
+    if 'fp64' in res:
+        out['fp64'] = {'value': round(res['fp64']['mpts'], 2),
+                       'ms_per_step': round(res['fp64']['sec'] / a.steps * 1e3, 4),
+                       'kernel_ms': round(res['fp64']['kms'], 5),
+                       'speedup_fp32_over_fp64': round(r32['mpts'] / res['fp64']['mpts'], 3)}
+    if not a.no_cpu:
+        out['cpu_baseline'] = cpu_baseline(W)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

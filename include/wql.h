@@ -1,0 +1,151 @@
+/*
+ * wql.h -- C ABI of the MI355X WaveQLab3D time-stepping library (libwql.so).
+ *
+ * The library replaces the reference's per-step loop
+ *     call time_step_RK(D, D%dt, RK, n)          (main.f90:85, time_step.f90:68-150)
+ * for a two-block domain joined by one frictional fault (domain.f90:243-244).
+ * The whole block state lives in HBM between calls; the host hands over plain
+ * arrays in the reference's column-major layout (q fastest, then r, s, field:
+ * datatypes.f90, mpi3dcomm.f90:736) restricted to the owned nodes.
+ *
+ * Every entry returns 0 on success or a negative WQL_E* code; the message of
+ * the last failure on the calling thread is wql_last_error().  Calls on one
+ * context are not reentrant.  No host pointer is retained after a call.
+ */
+#ifndef WQL_H
+#define WQL_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WQL_ABI_VERSION 1
+
+enum {
+    WQL_OK = 0,
+    WQL_E_ARG = -1,        /* invalid descriptor or argument */
+    WQL_E_NOMEM = -2,      /* device allocation failed / over the HBM budget */
+    WQL_E_HIP = -3,        /* HIP runtime error */
+    WQL_E_NONFINITE = -4,  /* NaN/Inf detected in the fault state */
+    WQL_E_SOLVER = -5,     /* friction solve failed (negative slip rate) */
+    WQL_E_STATE = -6       /* call out of order */
+};
+
+enum { WQL_FP64 = 0, WQL_FP32 = 1 };              /* volume precision */
+enum { WQL_LOCKED = 0, WQL_SLIP_WEAKENING = 1, WQL_RATE_STATE = 2 };
+
+/* One block: replaces the fields of block_type the hot path reads
+ * (datatypes.f90:118-135; block.f90:27-269 fills them). */
+typedef struct {
+    int nq, nr, ns;        /* owned nodes per direction (each >= 13) */
+    int bc[6];             /* Lx Rx Ly Ry Lz Rz: 0 fault, 1 absorbing, 2 free surface, 3 clamped
+                              (BoundaryConditions.f90:15-48) */
+    int pml[6];            /* PML slab per side (block.f90:160-178) */
+    int npml;              /* PML thickness in nodes (pml.f90:11) */
+    int cartesian;         /* 1: uniform Cartesian block (type_of_mesh='cartesian') */
+    const double *X;       /* [3][ns][nr][nq] node coordinates (grid.f90:13-395) */
+    const double *M;       /* [3][ns][nr][nq] lambda, mu, rho; NULL -> homogeneous rho_s_p */
+    double rho_s_p[3];     /* rho, cs, cp (material.f90:181-188) */
+} wql_block_desc;
+
+/* Fault law and prestress (Interface_Condition.f90:1268-2476 keyed tables,
+ * made explicit).  Same meaning as the oracle's ora_fault. */
+typedef struct {
+    int law;               /* WQL_LOCKED / WQL_SLIP_WEAKENING / WQL_RATE_STATE */
+    int stress_mode;       /* 0: tractions T0 + boxcar patches; 1: tensor s0 on the face frame */
+    double T0[3];
+    double s0[6];
+    int npatch;
+    double patch[4][5];    /* (yc, zc, wy, wz, amplitude) added to T0_l */
+    int nuc;               /* 0 none, 1 cosine disk (yc,zc,r1,width,amp), 2 smooth bump (yc,zc,R,amp,T) */
+    double nucp[5];
+    double fs, fd, Dc;
+    double bar_wy, bar_wz, bar_big;
+    double rs_a, rs_da, rs_w, rs_wr, rs_b, rs_L, rs_V0, rs_f0, rs_Vin;
+} wql_fault_params;
+
+typedef struct {
+    int precision;         /* WQL_FP64, or WQL_FP32: volume fields/metrics FP32, fault state FP64 */
+    double dt;             /* time step (domain.f90:222-235) */
+    double t0;             /* initial time */
+    int plastic;           /* 0 elastic, 1 Drucker-Prager viscoplastic ('scec', plastic.f90:153-217),
+                              2 default model (plastic.f90:221-290) */
+    double pl_c, pl_nu, pl_Tv;
+    double pl_mu, pl_beta, pl_eta;
+    int s0_kind;           /* plastic background stress: 0 constant fault.s0, 1 TPV26 depth profile */
+    wql_fault_params fault;
+} wql_problem;
+
+typedef struct wql_ctx wql_ctx;
+
+/* replaces init_domain's device-relevant part (domain.f90:31-564): uploads
+ * geometry/material, computes metrics and face frames on the device, zeroes
+ * the fields.  blocks[0] is the block on the minus side of the fault. */
+int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx **out);
+int wql_destroy(wql_ctx *ctx);
+const char *wql_last_error(void);
+int wql_abi_version(void);
+
+/* device memory the context holds, bytes */
+size_t wql_device_bytes(const wql_ctx *ctx);
+/* bytes a context for this problem would need (admission control) */
+size_t wql_estimate_bytes(const wql_problem *prob, const wql_block_desc blocks[2]);
+
+/* Init_Vel_State (Interface_Condition.f90:2722-2850): state variable and
+ * background velocity for rate-and-state faults. */
+int wql_init_state(wql_ctx *ctx);
+
+/* field upload/download, F[9][ns][nr][nq] in the reference field order
+ * vx vy vz sxx syy szz sxy sxz syz (RHS_Interior.f90:241-251) */
+int wql_set_fields(wql_ctx *ctx, int block, const double *F);
+int wql_get_fields(wql_ctx *ctx, int block, double *F);
+int wql_get_rates(wql_ctx *ctx, int block, double *DF);
+/* PML auxiliary fields of one slab [9][slab nodes] (pml.f90:11-168) */
+int wql_set_pml(wql_ctx *ctx, int block, int side, const double *Q);
+int wql_get_pml(wql_ctx *ctx, int block, int side, double *Q);
+int wql_get_pml_rates(wql_ctx *ctx, int block, int side, double *DQ);
+/* metrics [10][ns][nr][nq] = q_x r_x s_x q_y r_y s_y q_z r_z s_z J (grid.f90:396-469) */
+int wql_get_metrics(wql_ctx *ctx, int block, double *metric);
+/* face frame of one side [9][face nodes] = n, m, l (unit_normals.f90:8-168) */
+int wql_get_frame(wql_ctx *ctx, int block, int side, double *nml);
+
+/* fault state of one block's copy (iface.f90:13-48): S[4], W[1], trup[1], Svel[3],
+ * Uhat[6], Vhat[6] on [nr*ns] (q-face layout [f][s][r]); any pointer may be NULL */
+int wql_set_fault(wql_ctx *ctx, int block, const double *S, const double *W);
+int wql_get_fault(wql_ctx *ctx, int block, double *S, double *W, double *trup, double *Svel,
+                  double *Uhat, double *Vhat);
+int wql_get_fault_rates(wql_ctx *ctx, int block, double *DS, double *DW);
+
+/* One right-hand-side evaluation at time t with zero incoming rates:
+ * what = 0: volume operator only (set_rates, domain.f90:908-924)
+ * what = 1: plus boundary and fault SAT terms (enforce_bound_iface_conditions,
+ *           domain.f90:586-611) at RK stage `stage` */
+int wql_eval_rates(wql_ctx *ctx, double t, int what, int stage);
+
+/* nsteps LSRK(5,4) steps (time_step.f90:68-150); enqueued on the context's
+ * stream, returns without waiting. */
+int wql_step(wql_ctx *ctx, int nsteps);
+/* wait for the context's stream; reports NaN/Inf found in the fault state */
+int wql_sync(wql_ctx *ctx);
+double wql_time(const wql_ctx *ctx);
+
+/* launch shape of the volume kernel (autotuner hook): tile variant index and
+ * z-chunk length; returns WQL_E_ARG for unknown variants */
+int wql_set_tile(wql_ctx *ctx, int variant, int zchunk);
+int wql_tile_variants(void);
+
+/* capture one LSRK step into a HIP graph and replay it (1) or launch kernels
+ * directly (0) */
+int wql_use_graph(wql_ctx *ctx, int on);
+
+/* average device time (ms) of the main volume kernel over the last wql_step
+ * call when profiling is on (HIP events on the context's stream) */
+int wql_profile(wql_ctx *ctx, int on);
+double wql_volume_kernel_ms(wql_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -87,7 +87,7 @@ def fault_struct(f):
 class OracleDomain:
     """Owns numpy arrays for both blocks and both interface copies."""
 
-    def __init__(self, prob):
+    def __init__(self, prob, coords=None):
         L = lib()
         self.prob = prob
         self.d = OraDomain()
@@ -96,7 +96,8 @@ class OracleDomain:
         for ib, bs in enumerate(prob.blocks):
             nq, nr, ns = bs.n
             NP = nq * nr * ns
-            arr = dict(X=np.ascontiguousarray(bs.coords()), metric=np.zeros((10, ns, nr, nq)),
+            X = bs.coords() if coords is None else coords[ib]
+            arr = dict(X=np.ascontiguousarray(X, dtype=np.float64), metric=np.zeros((10, ns, nr, nq)),
                        M=bs.lame(), F=np.zeros((9, ns, nr, nq)), DF=np.zeros((9, ns, nr, nq)),
                        P=np.zeros((2, ns, nr, nq)))
             fp = [nr * ns, nr * ns, nq * ns, nq * ns, nq * nr, nq * nr]

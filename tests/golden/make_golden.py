@@ -76,6 +76,9 @@ CASES = {
     'curv_rs': (dict(problem='TPV101', coupling='rate-and-state_friction', mesh='curvilinear',
                      n=N_STD, lq=LQ_STD, rq=RQ_STD),
                 [('rhs', ['0.55']), ('traj', ['12'])]),
+    'curv_rs_long': (dict(problem='TPV101', coupling='rate-and-state_friction', mesh='curvilinear',
+                          n=N_STD, lq=LQ_STD, rq=RQ_STD, t_final=5.0),
+                     [('traj', ['60'])]),
     'pml_locked': (dict(problem='TPV5', coupling='locked', mesh='cartesian', n=(16, 15, 14),
                         lq=[(1, 1, 1), (0, 1, 1)], rq=[(0, 1, 1), (1, 2, 1)],
                         pml_l=((True, True, False), (False, False, True)),

@@ -11,7 +11,16 @@ sys.path.insert(0, os.path.join(ROOT, 'waveqlab3d-gh_amd'))
 from wql import problem as P  # noqa: E402
 
 RHS_CASES = ['cart_sw', 'curv_sw', 'curv_rs', 'pml_locked']
-TRAJ_CASES = ['cart_sw', 'curv_sw', 'curv_rs', 'pml_locked', 'plastic_sw', 'locked_pulse']
+TRAJ_CASES = ['cart_sw', 'curv_sw', 'curv_rs', 'curv_rs_long', 'pml_locked', 'plastic_sw', 'locked_pulse']
+
+# Relative-L2 bar for FP64 trajectories.  'curv_rs' stops before rupture:
+# slip rates there are ~1e-12 m/s, equal to the absolute step tolerance of
+# the reference's Regula Falsi (Interface_Condition.f90:1188), so its
+# iteration count -- and the slip rate to within 1e-12 m/s -- is decided by
+# last-bit rounding; fields near the fault are ~1e-11.  That window is held to
+# 1e-7; the same law during rupture ('curv_rs_long', V ~ 3 m/s) is held to
+# the FP64 bar.
+TRAJ_TOL = {'curv_rs': 1e-7}
 
 
 def load(case):

@@ -1,0 +1,70 @@
+"""The oracle against the reference's own outputs (CPU).
+
+Golden vectors were written by the reference Fortran build (oracle/_ref,
+tests/golden/make_golden.py) with 2 MPI ranks.  The oracle must reproduce
+them to FP64 rounding."""
+import numpy as np
+import pytest
+
+import golden_cases as G
+import gpu_helpers as H
+import oracle as O
+from wql import geometry
+
+
+def test_geometry_matches_reference_grid():
+    z = G.load('curv_sw')
+    prob = G.problem('curv_sw')
+    for ib in range(2):
+        X = prob.blocks[ib].coords()
+        assert np.abs(X - z[f'b{ib + 1}_X']).max() <= 1e-14
+
+
+def test_metrics_match_reference():
+    z = G.load('curv_sw')
+    od = O.OracleDomain(G.problem('curv_sw'))
+    for ib in range(2):
+        assert G.rel_l2(od.blocks[ib]['metric'], z[f'b{ib + 1}_metric']) <= 1e-14
+
+
+@pytest.mark.parametrize('case', G.RHS_CASES)
+def test_rhs(case):
+    z = G.load(case)
+    prob = G.problem(case)
+    od, vol, dqv = H.oracle_rhs(prob, float(z['rhs_t']))
+    for ib in range(2):
+        assert G.rel_l2(vol[ib], z[f'b{ib + 1}_DFvol']) <= 1e-15
+        assert G.rel_l2(od.blocks[ib]['DF'], z[f'b{ib + 1}_DFfull']) <= 1e-15
+        ia = od.blocks[ib]['iface']
+        assert np.abs(ia['DS'] - z[f'b{ib + 1}_DS']).max() <= 1e-13
+        assert np.abs(ia['DW'] - z[f'b{ib + 1}_DW']).max() <= 1e-13
+        for s in range(6):
+            if f'b{ib + 1}_DQvol{s + 1}' in z:
+                assert G.rel_l2(dqv[ib][s].ravel(), z[f'b{ib + 1}_DQvol{s + 1}']) <= 1e-15
+                assert G.rel_l2(od.blocks[ib]['DQ'][s].ravel(), z[f'b{ib + 1}_DQfull{s + 1}']) <= 1e-15
+
+
+@pytest.mark.parametrize('case', G.TRAJ_CASES)
+def test_trajectory(case):
+    z = G.load(case)
+    prob = G.problem(case)
+    od = O.OracleDomain(prob)
+    args = z['traj_args']
+    F0 = H.pulse_init(prob, args)
+    for ib in range(2):
+        od.blocks[ib]['F'][...] = F0[ib]
+    od.init_state()
+    assert od.d.dt == float(z['dt'])
+    od.step(int(args[0]))
+    assert od.t == pytest.approx(float(z['t_end']), rel=1e-15)
+    for ib in range(2):
+        a = od.blocks[ib]
+        assert G.rel_l2(a['F'], z[f'b{ib + 1}_F']) <= 1e-12
+        for key in ('S', 'W', 'Svel', 'trup'):
+            ref = z[f'b{ib + 1}_{key}']
+            assert np.abs(a['iface'][key] - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+        if f'b{ib + 1}_P' in z:
+            assert G.rel_l2(a['P'], z[f'b{ib + 1}_P']) <= 1e-12
+        for s in range(6):
+            if f'b{ib + 1}_Q{s + 1}' in z:
+                assert G.rel_l2(a['Q'][s].ravel(), z[f'b{ib + 1}_Q{s + 1}']) <= 1e-12

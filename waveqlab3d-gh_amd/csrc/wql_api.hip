@@ -1,0 +1,856 @@
+// wql_api.hip -- C ABI of libwql (declared in include/wql.h): context
+// lifetime, uploads/downloads, and the per-stage launch sequence.
+#include "wql.h"
+#include "wql_kernels.hip"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace wql;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(WQL_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// LSRK(5,4) of Carpenter & Kennedy (time_step.f90:53-62)
+const double RK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                        -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double RK_B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                        1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                        2277821191437.0 / 14882151754819.0};
+const double RK_C[6] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                        2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0, 1.0};
+
+const double TAU0 = 1.0 / (0.315949074074074);  // block.f90:191
+
+// closure table rows (JU_xJU_yJU_z6.f90:77-298) as (1-based column, coefficient)
+struct RowSrc { int n; int col[8]; double c[8]; };
+const RowSrc LEFT_SRC[6] = {
+    {6, {6, 5, 4, 3, 2, 1}, {0.036577936277544, 0.104488069284042, -0.450398306578272, -0.141512858744873, 2.033378678700676, -1.582533518939116}},
+    {5, {6, 5, 4, 3, 1}, {-0.014903449191300, -0.069112065532624, 0.258816087376832, 0.287258622978251, -0.462059195631158}},
+    {5, {6, 5, 4, 2, 1}, {-0.018129342917256, -0.022902190275815, 0.606235523609147, -0.636451095137907, 0.071247104721830}},
+    {6, {7, 6, 5, 3, 2, 1}, {0.013435342414630, -0.051642265516119, 0.520262285050482, -0.306681191361148, -0.290087484386815, 0.114713313798970}},
+    {7, {8, 7, 6, 4, 3, 2, 1}, {0.018281071473911, -0.164529643265203, 0.769199413962647, -0.707905442575989, 0.015764336127392, 0.105400944933782, -0.036210680656541}},
+    {8, {9, 8, 7, 5, 4, 3, 2, 1}, {0.016437980868017, -0.147941827812150, 0.739709139060752, -0.691649024426814, 0.063183694641876, 0.011220896474665, 0.020437334208704, -0.011398193015050}},
+};
+// right rows n-5..n: columns as offsets from n
+const RowSrc RIGHT_SRC[6] = {
+    {8, {-8, -7, -6, -4, -3, -2, -1, 0}, {-0.016437980868017, 0.147941827812150, -0.739709139060752, 0.691649024426814, -0.063183694641876, -0.011220896474665, -0.020437334208704, 0.011398193015050}},
+    {7, {-7, -6, -5, -3, -2, -1, 0}, {-0.018281071473911, 0.164529643265203, -0.769199413962647, 0.707905442575989, -0.015764336127392, -0.105400944933782, 0.036210680656541}},
+    {6, {-6, -5, -4, -2, -1, 0}, {-0.013435342414630, 0.051642265516119, -0.520262285050482, 0.306681191361148, 0.290087484386815, -0.114713313798970}},
+    {5, {-5, -4, -3, -1, 0}, {0.018129342917256, 0.022902190275815, -0.606235523609147, 0.636451095137907, -0.071247104721830}},
+    {5, {-5, -4, -3, -2, 0}, {0.014903449191300, 0.069112065532624, -0.258816087376832, -0.287258622978251, 0.462059195631158}},
+    {6, {-5, -4, -3, -2, -1, 0}, {-0.036577936277544, -0.104488069284042, 0.450398306578272, 0.141512858744873, -2.033378678700676, 1.582533518939116}},
+};
+
+SbpTable make_table()
+{
+    SbpTable T;
+    memset(&T, 0, sizeof T);
+    for (int r = 0; r < 6; r++) {
+        T.left[r].n = LEFT_SRC[r].n;
+        for (int t = 0; t < LEFT_SRC[r].n; t++) {
+            T.left[r].off[t] = (LEFT_SRC[r].col[t] - 1) - r;
+            T.left[r].c[t] = LEFT_SRC[r].c[t];
+        }
+        // row index n-6+r (0-based), column n-1+off  ->  relative off + 5 - r
+        T.right[r].n = RIGHT_SRC[r].n;
+        for (int t = 0; t < RIGHT_SRC[r].n; t++) {
+            T.right[r].off[t] = RIGHT_SRC[r].col[t] + 5 - r;
+            T.right[r].c[t] = RIGHT_SRC[r].c[t];
+        }
+    }
+    return T;
+}
+
+struct DevBlock {
+    int nq, nr, ns;
+    long NP;
+    int cart;
+    int bc[6], pml[6], npml;
+    double lam, mu, rho;
+    double gc[10];
+    void *F[2] = {nullptr, nullptr};
+    void *DF = nullptr;
+    void *G = nullptr;      // [10][NP] real (curvilinear)
+    void *M = nullptr;      // [3][NP] real (heterogeneous)
+    void *Q[6] = {}, *DQ[6] = {};
+    long SP[6] = {};
+    double *frame[6] = {}, *fgm[6] = {}, *fmat[6] = {};
+    double *fx = nullptr;
+    double *P = nullptr, *Y = nullptr;
+};
+
+}  // namespace
+
+struct wql_ctx {
+    int prec;
+    hipStream_t stream = nullptr;
+    DevBlock b[2];
+    Iface I[2];
+    FaultParams fault;
+    wql_problem prob;
+    double t, dt;
+    int cur = 0;
+    int variant = 0, zchunk = 32;
+    int use_graph = 0;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int *d_flag = nullptr;
+    double *d_t = nullptr;
+    size_t bytes = 0;
+    std::vector<void *> allocs;
+    int profile = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double vol_ms = 0.0;
+    long vol_launches = 0;
+};
+
+namespace {
+
+int dalloc(wql_ctx *c, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    c->allocs.push_back(*p);
+    c->bytes += bytes;
+    return 0;
+}
+
+template <typename real>
+Blk<real> blk_view(const wql_ctx *c, int ib, int cur)
+{
+    const DevBlock &d = c->b[ib];
+    Blk<real> B;
+    memset(&B, 0, sizeof B);
+    B.nq = d.nq; B.nr = d.nr; B.ns = d.ns; B.NP = d.NP;
+    B.Fin = (const real *)d.F[cur];
+    B.Fout = (real *)d.F[1 - cur];
+    B.DF = (real *)d.DF;
+    B.G = (const real *)d.G;
+    B.M = (const real *)d.M;
+    for (int k = 0; k < 10; k++) B.gc[k] = d.gc[k];
+    B.lam = d.lam; B.mu = d.mu; B.rho = d.rho;
+    B.cart = d.cart;
+    for (int s = 0; s < 6; s++) {
+        B.bc[s] = d.bc[s]; B.pml[s] = d.pml[s];
+        B.Q[s] = (real *)d.Q[s]; B.DQ[s] = (real *)d.DQ[s];
+        B.frame[s] = d.frame[s]; B.fgm[s] = d.fgm[s]; B.fmat[s] = d.fmat[s];
+    }
+    B.npml = d.npml;
+    B.fx = d.fx;
+    B.P = d.P;
+    B.Y = d.Y;
+    return B;
+}
+
+// node ranges of the main kernel (all nodes not in a PML slab and not within
+// six nodes of an s boundary)
+void main_range(const DevBlock &d, int r[6])
+{
+    int pz0 = d.pml[4] ? std::max(d.npml, 6) : 6, pz1 = d.pml[5] ? std::max(d.npml, 6) : 6;
+    int px0 = d.pml[0] ? std::max(d.npml, 6) : 0, px1 = d.pml[1] ? std::max(d.npml, 6) : 0;
+    int py0 = d.pml[2] ? std::max(d.npml, 6) : 0, py1 = d.pml[3] ? std::max(d.npml, 6) : 0;
+    r[0] = std::min(px0, d.nq); r[1] = std::max(d.nq - px1, r[0]);
+    r[2] = std::min(py0, d.nr); r[3] = std::max(d.nr - py1, r[2]);
+    r[4] = std::min(pz0, d.ns); r[5] = std::max(d.ns - pz1, r[4]);
+}
+
+const int VARIANT_TX[3] = {32, 64, 16};
+const int VARIANT_TY[3] = {8, 8, 16};
+
+TileList tile_list(const wql_ctx *c)
+{
+    TileList L;
+    memset(&L, 0, sizeof L);
+    int TX = VARIANT_TX[c->variant], TY = VARIANT_TY[c->variant];
+    long first = 0;
+    for (int ib = 0; ib < 2; ib++) {
+        int r[6];
+        main_range(c->b[ib], r);
+        if (r[1] <= r[0] || r[3] <= r[2] || r[5] <= r[4]) continue;
+        TileGeom &g = L.g[L.n];
+        g.blk = ib;
+        g.xa0 = r[0]; g.xa1 = r[1]; g.ya0 = r[2]; g.ya1 = r[3]; g.za0 = r[4]; g.za1 = r[5];
+        g.ntx = (r[1] - r[0] + TX - 1) / TX;
+        g.nty = (r[3] - r[2] + TY - 1) / TY;
+        g.zc = std::max(1, c->zchunk);
+        g.nzc = (r[5] - r[4] + g.zc - 1) / g.zc;
+        g.first = first;
+        first += (long)g.ntx * g.nty * g.nzc;
+        L.n++;
+    }
+    L.total = first;
+    return L;
+}
+
+BoxList box_list(const wql_ctx *c)
+{
+    BoxList L;
+    memset(&L, 0, sizeof L);
+    long start = 0;
+    auto add = [&](int ib, int x0, int x1, int y0, int y1, int z0, int z1) {
+        if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
+        Box &b = L.box[L.nbox++];
+        b.blk = ib; b.x0 = x0; b.x1 = x1; b.y0 = y0; b.y1 = y1; b.z0 = z0; b.z1 = z1;
+        b.start = start;
+        start += (long)(x1 - x0) * (y1 - y0) * (z1 - z0);
+    };
+    for (int ib = 0; ib < 2; ib++) {
+        const DevBlock &d = c->b[ib];
+        int r[6];
+        main_range(d, r);
+        int z0 = std::min(r[4], d.ns), z1 = std::max(r[5], z0);
+        add(ib, 0, d.nq, 0, d.nr, 0, z0);          // s-low planes
+        add(ib, 0, d.nq, 0, d.nr, z1, d.ns);       // s-high planes
+        add(ib, 0, r[0], 0, d.nr, z0, z1);         // q-low slab
+        add(ib, r[1], d.nq, 0, d.nr, z0, z1);      // q-high slab
+        add(ib, r[0], r[1], 0, r[2], z0, z1);      // r-low slab
+        add(ib, r[0], r[1], r[3], d.nr, z0, z1);   // r-high slab
+    }
+    L.total = start;
+    return L;
+}
+
+template <typename real, int TX, int TY>
+void launch_main_v(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0, real A1, real bdt)
+{
+    dim3 blk(TX, TY);
+    dim3 grd((unsigned)(((TL.total + 7) / 8) * 8));
+    bool curv = !c->b[0].cart || !c->b[1].cart;
+    bool het = c->b[0].M || c->b[1].M;
+    if (curv && het) k_volume_main<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+    else if (curv) k_volume_main<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+    else if (het) k_volume_main<real, TX, TY, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+    else k_volume_main<real, TX, TY, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+}
+
+template <typename real>
+void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int stage, int rates_only,
+                  int do_plastic)
+{
+    Pair<real> P;
+    P.b[0] = blk_view<real>(c, 0, cur);
+    P.b[1] = blk_view<real>(c, 1, cur);
+    // mixed grids: a Cartesian block inside a curvilinear pair still has no G
+    TileList TL = tile_list(c);
+    if (TL.total > 0) {
+        if (c->profile && !c->use_graph) hipEventRecord(c->ev0, c->stream);
+        switch (c->variant) {
+        case 1: launch_main_v<real, 64, 8>(c, P, TL, (real)A, (real)A, (real)bdt); break;
+        case 2: launch_main_v<real, 16, 16>(c, P, TL, (real)A, (real)A, (real)bdt); break;
+        default: launch_main_v<real, 32, 8>(c, P, TL, (real)A, (real)A, (real)bdt); break;
+        }
+        if (c->profile && !c->use_graph) {
+            hipEventRecord(c->ev1, c->stream);
+            hipEventSynchronize(c->ev1);
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, c->ev0, c->ev1);
+            c->vol_ms += ms;
+            c->vol_launches++;
+        }
+    }
+    BoxList BL = box_list(c);
+    if (BL.total > 0) {
+        unsigned g = (unsigned)((BL.total + 255) / 256);
+        k_volume_generic<real><<<g, 256, 0, c->stream>>>(P, BL, (real)A, (real)A, (real)bdt);
+    }
+    BoundaryArgs a;
+    memset(&a, 0, sizeof a);
+    a.fault = c->fault;
+    a.I[0] = c->I[0];
+    a.I[1] = c->I[1];
+    for (int ib = 0; ib < 2; ib++) {
+        const DevBlock &d = c->b[ib];
+        a.nodes[ib] = 2L * d.nr * d.ns + 2L * (d.nq - 2) * d.ns + 2L * (d.nq - 2) * (d.nr - 2);
+    }
+    a.t = t_stage;
+    a.A[0] = a.A[1] = A;
+    a.bdt = bdt;
+    a.tau0 = TAU0;
+    a.stage = stage;
+    a.rates_only = rates_only;
+    a.flag = c->d_flag;
+    unsigned gb = (unsigned)((a.nodes[0] + a.nodes[1] + 127) / 128);
+    k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
+    if (do_plastic && c->prob.plastic) {
+        PlasticArgs pa;
+        memset(&pa, 0, sizeof pa);
+        pa.model = c->prob.plastic;
+        pa.c = c->prob.pl_c; pa.nu = c->prob.pl_nu; pa.Tv = c->prob.pl_Tv;
+        pa.mu = c->prob.pl_mu; pa.beta = c->prob.pl_beta; pa.eta = c->prob.pl_eta;
+        pa.dt = c->dt;
+        pa.s0_kind = c->prob.s0_kind;
+        for (int k = 0; k < 6; k++) pa.s0[k] = c->prob.fault.s0[k];
+        long tot = 0;
+        for (int ib = 0; ib < 2; ib++) {
+            const DevBlock &d = c->b[ib];
+            int np = d.npml;
+            pa.x0[ib] = d.pml[0] ? np : 0; pa.x1[ib] = d.pml[1] ? d.nq - np : d.nq;
+            pa.y0[ib] = d.pml[2] ? np : 0; pa.y1[ib] = d.pml[3] ? d.nr - np : d.nr;
+            pa.z0[ib] = d.pml[4] ? np : 0; pa.z1[ib] = d.pml[5] ? d.ns - np : d.ns;
+            pa.n[ib] = (long)std::max(0, pa.x1[ib] - pa.x0[ib]) * std::max(0, pa.y1[ib] - pa.y0[ib]) *
+                       std::max(0, pa.z1[ib] - pa.z0[ib]);
+            tot += pa.n[ib];
+        }
+        // plastic acts on the stage output
+        Pair<real> Q = P;
+        if (tot > 0) k_plastic<real><<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(Q, pa);
+    }
+}
+
+template <typename real>
+int enqueue_step(wql_ctx *c, int cur, double t0)
+{
+    for (int s = 0; s < 5; s++) {
+        launch_stage<real>(c, cur, RK_A[s], RK_B[s] * c->dt, t0 + RK_C[s] * c->dt, s + 1, 0, s == 4);
+        cur = 1 - cur;
+    }
+    return 0;
+}
+
+int check_launch()
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(WQL_E_HIP, "kernel launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+template <typename real>
+int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
+{
+    static bool table_done = false;
+    if (!table_done) {
+        SbpTable T = make_table();
+        HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_sbp), &T, sizeof T));
+        table_done = true;
+    }
+    int *d_err = nullptr;
+    if (int rc = dalloc(c, (void **)&d_err, sizeof(int))) return rc;
+    HIPCHK(hipMemset(d_err, 0, sizeof(int)));
+    for (int ib = 0; ib < 2; ib++) {
+        const wql_block_desc &bd = blocks[ib];
+        DevBlock &d = c->b[ib];
+        d.nq = bd.nq; d.nr = bd.nr; d.ns = bd.ns;
+        d.NP = (long)bd.nq * bd.nr * bd.ns;
+        d.cart = bd.cartesian ? 1 : 0;
+        for (int s = 0; s < 6; s++) { d.bc[s] = bd.bc[s]; d.pml[s] = bd.pml[s] ? 1 : 0; }
+        d.npml = bd.npml;
+        const long NP = d.NP;
+        double *X = nullptr, *M64 = nullptr, *G64 = nullptr;
+        HIPCHK(hipMalloc(&X, 3 * NP * sizeof(double)));
+        HIPCHK(hipMalloc(&M64, 3 * NP * sizeof(double)));
+        HIPCHK(hipMalloc(&G64, 10 * NP * sizeof(double)));
+        HIPCHK(hipMemcpy(X, bd.X, 3 * NP * sizeof(double), hipMemcpyHostToDevice));
+        double rho = bd.rho_s_p[0], cs = bd.rho_s_p[1], cp = bd.rho_s_p[2];
+        d.lam = rho * (cp * cp - 2.0 * (cs * cs));
+        d.mu = rho * (cs * cs);
+        d.rho = rho;
+        unsigned g = (unsigned)((NP + 255) / 256);
+        if (bd.M) {
+            HIPCHK(hipMemcpy(M64, bd.M, 3 * NP * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            k_fill<double><<<g, 256, 0, c->stream>>>(NP, M64, d.lam);
+            k_fill<double><<<g, 256, 0, c->stream>>>(NP, M64 + NP, d.mu);
+            k_fill<double><<<g, 256, 0, c->stream>>>(NP, M64 + 2 * NP, d.rho);
+        }
+        k_metrics<<<g, 256, 0, c->stream>>>(d.nq, d.nr, d.ns, X, G64);
+        if (int rc = check_launch()) return rc;
+        for (int side = 0; side < 6; side++) {
+            int dir = side / 2;
+            long FP = dir == 0 ? (long)d.nr * d.ns : dir == 1 ? (long)d.nq * d.ns : (long)d.nq * d.nr;
+            if (int rc = dalloc(c, (void **)&d.frame[side], 9 * FP * sizeof(double))) return rc;
+            if (int rc = dalloc(c, (void **)&d.fgm[side], 3 * FP * sizeof(double))) return rc;
+            if (int rc = dalloc(c, (void **)&d.fmat[side], 3 * FP * sizeof(double))) return rc;
+            bool fault_side = (ib == 0 && side == 1) || (ib == 1 && side == 0);
+            if (fault_side)
+                if (int rc = dalloc(c, (void **)&d.fx, 3 * FP * sizeof(double))) return rc;
+            k_frames<<<(unsigned)((FP + 127) / 128), 128, 0, c->stream>>>(d.nq, d.nr, d.ns, side, G64, M64, X,
+                                                                          d.frame[side], d.fgm[side], d.fmat[side],
+                                                                          fault_side ? d.fx : nullptr, d_err);
+        }
+        if (c->prob.plastic && c->prob.s0_kind == 1) {
+            if (int rc = dalloc(c, (void **)&d.Y, NP * sizeof(double))) return rc;
+            HIPCHK(hipMemcpyAsync(d.Y, X + NP, NP * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        }
+        if (c->prob.plastic) {
+            if (int rc = dalloc(c, (void **)&d.P, 2 * NP * sizeof(double))) return rc;
+            k_fill<double><<<(unsigned)((2 * NP + 255) / 256), 256, 0, c->stream>>>(2 * NP, d.P, 0.0);
+        }
+        if (!d.cart) {
+            if (int rc = dalloc(c, &d.G, 10 * NP * sizeof(real))) return rc;
+            k_convert<real><<<(unsigned)((10 * NP + 255) / 256), 256, 0, c->stream>>>(10 * NP, G64, (real *)d.G);
+        } else {
+            // metrics of a uniform block: the value at the block centre
+            long pc = (long)(d.nq / 2) + (long)d.nq * ((d.nr / 2) + (long)d.nr * (d.ns / 2));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int k = 0; k < 10; k++)
+                HIPCHK(hipMemcpy(&d.gc[k], G64 + k * NP + pc, sizeof(double), hipMemcpyDeviceToHost));
+        }
+        if (bd.M) {
+            if (int rc = dalloc(c, &d.M, 3 * NP * sizeof(real))) return rc;
+            k_convert<real><<<(unsigned)((3 * NP + 255) / 256), 256, 0, c->stream>>>(3 * NP, M64, (real *)d.M);
+        }
+        for (int k = 0; k < 2; k++) {
+            if (int rc = dalloc(c, &d.F[k], 9 * NP * sizeof(real))) return rc;
+            k_fill<real><<<(unsigned)((9 * NP + 255) / 256), 256, 0, c->stream>>>(9 * NP, (real *)d.F[k], (real)0);
+        }
+        if (int rc = dalloc(c, &d.DF, 9 * NP * sizeof(real))) return rc;
+        k_fill<real><<<(unsigned)((9 * NP + 255) / 256), 256, 0, c->stream>>>(9 * NP, (real *)d.DF, (real)0);
+        for (int s = 0; s < 6; s++) {
+            if (!d.pml[s]) continue;
+            int dir = s / 2;
+            long n3[3] = {d.nq, d.nr, d.ns};
+            n3[dir] = d.npml;
+            d.SP[s] = n3[0] * n3[1] * n3[2];
+            if (int rc = dalloc(c, &d.Q[s], 9 * d.SP[s] * sizeof(real))) return rc;
+            if (int rc = dalloc(c, &d.DQ[s], 9 * d.SP[s] * sizeof(real))) return rc;
+            unsigned gq = (unsigned)((9 * d.SP[s] + 255) / 256);
+            k_fill<real><<<gq, 256, 0, c->stream>>>(9 * d.SP[s], (real *)d.Q[s], (real)0);
+            k_fill<real><<<gq, 256, 0, c->stream>>>(9 * d.SP[s], (real *)d.DQ[s], (real)0);
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (int rc = check_launch()) return rc;
+        hipFree(X);
+        hipFree(M64);
+        hipFree(G64);
+    }
+    int herr = 0;
+    HIPCHK(hipMemcpy(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) return fail(WQL_E_ARG, "face frame: metric gradient is not a unit vector after normalisation");
+    // interface state per block copy
+    for (int ib = 0; ib < 2; ib++) {
+        long FP = (long)c->b[ib].nr * c->b[ib].ns;
+        Iface &I = c->I[ib];
+        double **arrs[8] = {&I.S, &I.DS, &I.W, &I.DW, &I.trup, &I.Svel, &I.Uhat, &I.Vhat};
+        int nf[8] = {4, 4, 1, 1, 1, 3, 6, 6};
+        for (int k = 0; k < 8; k++) {
+            if (int rc = dalloc(c, (void **)arrs[k], nf[k] * FP * sizeof(double))) return rc;
+            double v = (k == 4) ? 1.0e9 : 0.0;  // trup = 1e9 (iface.f90:41)
+            k_fill<double><<<(unsigned)((nf[k] * FP + 255) / 256), 256, 0, c->stream>>>(nf[k] * FP, *arrs[k], v);
+        }
+    }
+    if (int rc = dalloc(c, (void **)&c->d_flag, sizeof(int))) return rc;
+    HIPCHK(hipMemset(c->d_flag, 0, sizeof(int)));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return check_launch();
+}
+
+long face_nodes(const DevBlock &d, int side)
+{
+    int dir = side / 2;
+    return dir == 0 ? (long)d.nr * d.ns : dir == 1 ? (long)d.nq * d.ns : (long)d.nq * d.nr;
+}
+
+int copy_field(wql_ctx *c, const void *dev, double *host, long n)
+{
+    if (c->prec == WQL_FP64) {
+        HIPCHK(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> tmp(n);
+        HIPCHK(hipMemcpyAsync(tmp.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (long k = 0; k < n; k++) host[k] = tmp[k];
+    }
+    return 0;
+}
+
+int put_field(wql_ctx *c, void *dev, const double *host, long n)
+{
+    if (c->prec == WQL_FP64) {
+        HIPCHK(hipMemcpyAsync(dev, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> tmp(host, host + n);
+        HIPCHK(hipMemcpyAsync(dev, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int valid_block(const wql_block_desc &b)
+{
+    if (b.nq < 13 || b.nr < 13 || b.ns < 13) return fail(WQL_E_ARG, "block needs at least 13 nodes per direction");
+    if (!b.X) return fail(WQL_E_ARG, "block coordinates X are required");
+    for (int s = 0; s < 6; s++)
+        if (b.bc[s] < 0 || b.bc[s] > 3) return fail(WQL_E_ARG, "boundary code %d out of range", b.bc[s]);
+    for (int s = 0; s < 6; s++)
+        if (b.pml[s] && (b.npml < 1 || b.npml > (s < 2 ? b.nq : s < 4 ? b.nr : b.ns)))
+            return fail(WQL_E_ARG, "PML thickness %d does not fit the block", b.npml);
+    if (!b.M && (b.rho_s_p[0] <= 0 || b.rho_s_p[1] <= 0 || b.rho_s_p[2] <= 0))
+        return fail(WQL_E_ARG, "rho_s_p must be positive");
+    return 0;
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char *wql_last_error(void) { return g_err.c_str(); }
+int wql_abi_version(void) { return WQL_ABI_VERSION; }
+int wql_tile_variants(void) { return 3; }
+
+size_t wql_estimate_bytes(const wql_problem *prob, const wql_block_desc blocks[2])
+{
+    size_t r = prob && prob->precision == WQL_FP32 ? 4 : 8, tot = 0;
+    for (int ib = 0; ib < 2; ib++) {
+        const wql_block_desc &b = blocks[ib];
+        size_t NP = (size_t)b.nq * b.nr * b.ns;
+        size_t v = 27 * NP * r;                                // F x2, DF
+        if (!b.cartesian) v += 10 * NP * r;                    // metrics + J
+        if (b.M) v += 3 * NP * r;
+        if (prob && prob->plastic) v += 2 * NP * 8;
+        size_t init = 16 * NP * 8;                             // X, M64, G64 during wql_create
+        for (int s = 0; s < 6; s++)
+            if (b.pml[s]) {
+                size_t n3[3] = {(size_t)b.nq, (size_t)b.nr, (size_t)b.ns};
+                n3[s / 2] = (size_t)b.npml;
+                v += 18 * n3[0] * n3[1] * n3[2] * r;
+            }
+        size_t faces = 2 * ((size_t)b.nr * b.ns + (size_t)b.nq * b.ns + (size_t)b.nq * b.nr) * 18 * 8;
+        tot += std::max(v, v / 2 + init) + faces + 25 * (size_t)b.nr * b.ns * 8;
+    }
+    return tot;
+}
+
+int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx **out)
+{
+    if (!prob || !blocks || !out) return fail(WQL_E_ARG, "null argument");
+    *out = nullptr;
+    if (prob->precision != WQL_FP64 && prob->precision != WQL_FP32) return fail(WQL_E_ARG, "bad precision");
+    if (!(prob->dt > 0)) return fail(WQL_E_ARG, "dt must be positive");
+    for (int ib = 0; ib < 2; ib++)
+        if (int rc = valid_block(blocks[ib])) return rc;
+    if (blocks[0].nr != blocks[1].nr || blocks[0].ns != blocks[1].ns)
+        return fail(WQL_E_ARG, "blocks must share the fault face size (nr, ns)");
+    if (blocks[0].bc[1] != 0 || blocks[1].bc[0] != 0)
+        return fail(WQL_E_ARG, "the fault joins block 0 side Rx and block 1 side Lx (bc code 0)");
+    if (prob->fault.law < 0 || prob->fault.law > 2 || prob->fault.npatch < 0 || prob->fault.npatch > 4)
+        return fail(WQL_E_ARG, "bad fault description");
+    wql_ctx *c = new wql_ctx();
+    c->prec = prob->precision;
+    c->prob = *prob;
+    c->dt = prob->dt;
+    c->t = prob->t0;
+    static_assert(sizeof(FaultParams) == sizeof(wql_fault_params), "fault layout");
+    memcpy(&c->fault, &prob->fault, sizeof(FaultParams));
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(WQL_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    int rc = c->prec == WQL_FP64 ? create_typed<double>(c, blocks) : create_typed<float>(c, blocks);
+    if (rc) {
+        std::string msg = g_err;
+        wql_destroy(c);
+        g_err = msg;
+        return rc;
+    }
+    if (hipMalloc(&c->d_t, sizeof(double)) != hipSuccess) {
+        wql_destroy(c);
+        return fail(WQL_E_NOMEM, "hipMalloc(time)");
+    }
+    c->allocs.push_back(c->d_t);
+    hipEventCreate(&c->ev0);
+    hipEventCreate(&c->ev1);
+    *out = c;
+    return 0;
+}
+
+int wql_destroy(wql_ctx *c)
+{
+    if (!c) return 0;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 2; k++)
+        if (c->gexec[k]) hipGraphExecDestroy(c->gexec[k]);
+    for (void *p : c->allocs) hipFree(p);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+size_t wql_device_bytes(const wql_ctx *c) { return c ? c->bytes : 0; }
+double wql_time(const wql_ctx *c) { return c ? c->t : 0.0; }
+
+int wql_init_state(wql_ctx *c)
+{
+    if (!c) return fail(WQL_E_ARG, "null context");
+    if (c->fault.law != WQL_RATE_STATE) return 0;
+    // background slip velocity of Init_Vel_State (Interface_Condition.f90:2758-2761)
+    double vz[2] = {std::copysign(0.5 * c->fault.rs_Vin, -1.0), std::copysign(0.5 * c->fault.rs_Vin, 1.0)};
+    for (int ib = 0; ib < 2; ib++) {
+        DevBlock &d = c->b[ib];
+        unsigned g = (unsigned)((d.NP + 255) / 256);
+        if (c->prec == WQL_FP64)
+            k_fill<double><<<g, 256, 0, c->stream>>>(d.NP, (double *)d.F[c->cur] + 2 * d.NP, vz[ib]);
+        else
+            k_fill<float><<<g, 256, 0, c->stream>>>(d.NP, (float *)d.F[c->cur] + 2 * d.NP, (float)vz[ib]);
+    }
+    long n = (long)c->b[0].nr * c->b[0].ns + (long)c->b[1].nr * c->b[1].ns;
+    unsigned g = (unsigned)((n + 127) / 128);
+    if (c->prec == WQL_FP64) {
+        Pair<double> P;
+        P.b[0] = blk_view<double>(c, 0, c->cur);
+        P.b[1] = blk_view<double>(c, 1, c->cur);
+        k_init_state<double><<<g, 128, 0, c->stream>>>(P, c->fault, c->I[0], c->I[1]);
+    } else {
+        Pair<float> P;
+        P.b[0] = blk_view<float>(c, 0, c->cur);
+        P.b[1] = blk_view<float>(c, 1, c->cur);
+        k_init_state<float><<<g, 128, 0, c->stream>>>(P, c->fault, c->I[0], c->I[1]);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return check_launch();
+}
+
+int wql_set_fields(wql_ctx *c, int ib, const double *F)
+{
+    if (!c || ib < 0 || ib > 1 || !F) return fail(WQL_E_ARG, "bad argument");
+    return put_field(c, c->b[ib].F[c->cur], F, 9 * c->b[ib].NP);
+}
+
+int wql_get_fields(wql_ctx *c, int ib, double *F)
+{
+    if (!c || ib < 0 || ib > 1 || !F) return fail(WQL_E_ARG, "bad argument");
+    return copy_field(c, c->b[ib].F[c->cur], F, 9 * c->b[ib].NP);
+}
+
+int wql_get_rates(wql_ctx *c, int ib, double *DF)
+{
+    if (!c || ib < 0 || ib > 1 || !DF) return fail(WQL_E_ARG, "bad argument");
+    return copy_field(c, c->b[ib].DF, DF, 9 * c->b[ib].NP);
+}
+
+int wql_set_pml(wql_ctx *c, int ib, int side, const double *Q)
+{
+    if (!c || ib < 0 || ib > 1 || side < 0 || side > 5 || !Q || !c->b[ib].pml[side])
+        return fail(WQL_E_ARG, "bad argument or no PML on that side");
+    return put_field(c, c->b[ib].Q[side], Q, 9 * c->b[ib].SP[side]);
+}
+
+int wql_get_pml(wql_ctx *c, int ib, int side, double *Q)
+{
+    if (!c || ib < 0 || ib > 1 || side < 0 || side > 5 || !Q || !c->b[ib].pml[side])
+        return fail(WQL_E_ARG, "bad argument or no PML on that side");
+    return copy_field(c, c->b[ib].Q[side], Q, 9 * c->b[ib].SP[side]);
+}
+
+int wql_get_pml_rates(wql_ctx *c, int ib, int side, double *DQ)
+{
+    if (!c || ib < 0 || ib > 1 || side < 0 || side > 5 || !DQ || !c->b[ib].pml[side])
+        return fail(WQL_E_ARG, "bad argument or no PML on that side");
+    return copy_field(c, c->b[ib].DQ[side], DQ, 9 * c->b[ib].SP[side]);
+}
+
+int wql_get_metrics(wql_ctx *c, int ib, double *metric)
+{
+    if (!c || ib < 0 || ib > 1 || !metric) return fail(WQL_E_ARG, "bad argument");
+    const DevBlock &d = c->b[ib];
+    if (d.cart) {
+        for (int k = 0; k < 10; k++)
+            for (long p = 0; p < d.NP; p++) metric[k * d.NP + p] = d.gc[k];
+        return 0;
+    }
+    return copy_field(c, d.G, metric, 10 * d.NP);
+}
+
+int wql_get_frame(wql_ctx *c, int ib, int side, double *nml)
+{
+    if (!c || ib < 0 || ib > 1 || side < 0 || side > 5 || !nml) return fail(WQL_E_ARG, "bad argument");
+    long FP = face_nodes(c->b[ib], side);
+    HIPCHK(hipMemcpy(nml, c->b[ib].frame[side], 9 * FP * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int wql_set_fault(wql_ctx *c, int ib, const double *S, const double *W)
+{
+    if (!c || ib < 0 || ib > 1) return fail(WQL_E_ARG, "bad argument");
+    long FP = (long)c->b[ib].nr * c->b[ib].ns;
+    if (S) HIPCHK(hipMemcpy(c->I[ib].S, S, 4 * FP * sizeof(double), hipMemcpyHostToDevice));
+    if (W) HIPCHK(hipMemcpy(c->I[ib].W, W, FP * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int wql_get_fault(wql_ctx *c, int ib, double *S, double *W, double *trup, double *Svel, double *Uhat, double *Vhat)
+{
+    if (!c || ib < 0 || ib > 1) return fail(WQL_E_ARG, "bad argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    long FP = (long)c->b[ib].nr * c->b[ib].ns;
+    const Iface &I = c->I[ib];
+    if (S) HIPCHK(hipMemcpy(S, I.S, 4 * FP * sizeof(double), hipMemcpyDeviceToHost));
+    if (W) HIPCHK(hipMemcpy(W, I.W, FP * sizeof(double), hipMemcpyDeviceToHost));
+    if (trup) HIPCHK(hipMemcpy(trup, I.trup, FP * sizeof(double), hipMemcpyDeviceToHost));
+    if (Svel) HIPCHK(hipMemcpy(Svel, I.Svel, 3 * FP * sizeof(double), hipMemcpyDeviceToHost));
+    if (Uhat) HIPCHK(hipMemcpy(Uhat, I.Uhat, 6 * FP * sizeof(double), hipMemcpyDeviceToHost));
+    if (Vhat) HIPCHK(hipMemcpy(Vhat, I.Vhat, 6 * FP * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int wql_get_fault_rates(wql_ctx *c, int ib, double *DS, double *DW)
+{
+    if (!c || ib < 0 || ib > 1) return fail(WQL_E_ARG, "bad argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    long FP = (long)c->b[ib].nr * c->b[ib].ns;
+    if (DS) HIPCHK(hipMemcpy(DS, c->I[ib].DS, 4 * FP * sizeof(double), hipMemcpyDeviceToHost));
+    if (DW) HIPCHK(hipMemcpy(DW, c->I[ib].DW, FP * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
+{
+    if (!c || what < 0 || what > 1) return fail(WQL_E_ARG, "bad argument");
+    for (int ib = 0; ib < 2; ib++) {
+        DevBlock &d = c->b[ib];
+        unsigned g = (unsigned)((9 * d.NP + 255) / 256);
+        if (c->prec == WQL_FP64) k_fill<double><<<g, 256, 0, c->stream>>>(9 * d.NP, (double *)d.DF, 0.0);
+        else k_fill<float><<<g, 256, 0, c->stream>>>(9 * d.NP, (float *)d.DF, 0.0f);
+        for (int s = 0; s < 6; s++)
+            if (d.pml[s]) {
+                unsigned gq = (unsigned)((9 * d.SP[s] + 255) / 256);
+                if (c->prec == WQL_FP64) k_fill<double><<<gq, 256, 0, c->stream>>>(9 * d.SP[s], (double *)d.DQ[s], 0.0);
+                else k_fill<float><<<gq, 256, 0, c->stream>>>(9 * d.SP[s], (float *)d.DQ[s], 0.0f);
+            }
+        long FP = (long)d.nr * d.ns;
+        k_fill<double><<<(unsigned)((4 * FP + 255) / 256), 256, 0, c->stream>>>(4 * FP, c->I[ib].DS, 0.0);
+        k_fill<double><<<(unsigned)((FP + 255) / 256), 256, 0, c->stream>>>(FP, c->I[ib].DW, 0.0);
+    }
+    // volume kernels with A = 0 and a zero update step; boundary in rates-only mode
+    if (what == 0) {
+        // run the volume kernels only: boundary kernel skipped by using a copy of the
+        // launch sequence without it
+        Pair<double> Pd;
+        Pair<float> Pf;
+        if (c->prec == WQL_FP64) {
+            Pd.b[0] = blk_view<double>(c, 0, c->cur);
+            Pd.b[1] = blk_view<double>(c, 1, c->cur);
+        } else {
+            Pf.b[0] = blk_view<float>(c, 0, c->cur);
+            Pf.b[1] = blk_view<float>(c, 1, c->cur);
+        }
+        TileList TL = tile_list(c);
+        BoxList BL = box_list(c);
+        if (c->prec == WQL_FP64) {
+            if (TL.total > 0) {
+                switch (c->variant) {
+                case 1: launch_main_v<double, 64, 8>(c, Pd, TL, 0.0, 0.0, 0.0); break;
+                case 2: launch_main_v<double, 16, 16>(c, Pd, TL, 0.0, 0.0, 0.0); break;
+                default: launch_main_v<double, 32, 8>(c, Pd, TL, 0.0, 0.0, 0.0); break;
+                }
+            }
+            if (BL.total > 0)
+                k_volume_generic<double><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(Pd, BL, 0.0, 0.0, 0.0);
+        } else {
+            if (TL.total > 0) {
+                switch (c->variant) {
+                case 1: launch_main_v<float, 64, 8>(c, Pf, TL, 0.f, 0.f, 0.f); break;
+                case 2: launch_main_v<float, 16, 16>(c, Pf, TL, 0.f, 0.f, 0.f); break;
+                default: launch_main_v<float, 32, 8>(c, Pf, TL, 0.f, 0.f, 0.f); break;
+                }
+            }
+            if (BL.total > 0)
+                k_volume_generic<float><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(Pf, BL, 0.f, 0.f, 0.f);
+        }
+    } else {
+        if (c->prec == WQL_FP64) launch_stage<double>(c, c->cur, 0.0, 0.0, t, stage, 1, 0);
+        else launch_stage<float>(c, c->cur, 0.0, 0.0, t, stage, 1, 0);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return check_launch();
+}
+
+int wql_step(wql_ctx *c, int nsteps)
+{
+    if (!c || nsteps < 0) return fail(WQL_E_ARG, "bad argument");
+    for (int n = 0; n < nsteps; n++) {
+        if (c->use_graph) {
+            // step graphs depend on the ping-pong parity and on t (baked in);
+            // rebuild when t changed since capture
+            hipGraph_t graph;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            if (c->prec == WQL_FP64) enqueue_step<double>(c, c->cur, c->t);
+            else enqueue_step<float>(c, c->cur, c->t);
+            HIPCHK(hipStreamEndCapture(c->stream, &graph));
+            hipGraphExec_t ex;
+            HIPCHK(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
+            HIPCHK(hipGraphLaunch(ex, c->stream));
+            hipGraphExecDestroy(ex);
+            hipGraphDestroy(graph);
+        } else {
+            if (c->prec == WQL_FP64) enqueue_step<double>(c, c->cur, c->t);
+            else enqueue_step<float>(c, c->cur, c->t);
+        }
+        c->cur = 1 - c->cur;  // five stages: odd number of buffer swaps
+        c->t = c->t + RK_C[5] * c->dt;
+        if (int rc = check_launch()) return rc;
+    }
+    return 0;
+}
+
+int wql_sync(wql_ctx *c)
+{
+    if (!c) return fail(WQL_E_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int flag = 0;
+    HIPCHK(hipMemcpy(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag == 3) return fail(WQL_E_SOLVER, "friction solve produced a negative slip rate");
+    if (flag == 4) return fail(WQL_E_NONFINITE, "non-finite value in the fault state");
+    if (flag) return fail(WQL_E_SOLVER, "fault error code %d", flag);
+    return 0;
+}
+
+int wql_set_tile(wql_ctx *c, int variant, int zchunk)
+{
+    if (!c || variant < 0 || variant > 2 || zchunk < 1) return fail(WQL_E_ARG, "bad tile variant");
+    c->variant = variant;
+    c->zchunk = zchunk;
+    return 0;
+}
+
+int wql_use_graph(wql_ctx *c, int on)
+{
+    if (!c) return fail(WQL_E_ARG, "null context");
+    c->use_graph = on ? 1 : 0;
+    return 0;
+}
+
+int wql_profile(wql_ctx *c, int on)
+{
+    if (!c) return fail(WQL_E_ARG, "null context");
+    c->profile = on ? 1 : 0;
+    c->vol_ms = 0.0;
+    c->vol_launches = 0;
+    return 0;
+}
+
+double wql_volume_kernel_ms(wql_ctx *c)
+{
+    if (!c || c->vol_launches == 0) return 0.0;
+    return c->vol_ms / (double)c->vol_launches;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// wql_device.h -- device-side data structures and small helpers shared by the
+// kernels of libwql (gfx950).  Host code fills these structs; kernels take
+// them by value.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wql {
+
+constexpr int NF = 9;  // vx vy vz sxx syy szz sxy sxz syz (RHS_Interior.f90:241-251)
+
+// 6th-order SBP first-derivative closure (JU_xJU_yJU_z6.f90:77-298,
+// metrics.f90:16-160): boundary rows 0..5 and their mirror rows n-6..n-1, as
+// tap offsets relative to the row and coefficients, in the order the
+// reference sums them.  Filled once per process by wql_create.
+struct SbpRow {
+    int n;
+    int off[8];
+    double c[8];
+};
+struct SbpTable {
+    SbpRow left[6];
+    SbpRow right[6];
+};
+
+// Fault description, identical in meaning to wql_fault_params.
+struct FaultParams {
+    int law;
+    int stress_mode;
+    double T0[3];
+    double s0[6];
+    int npatch;
+    double patch[4][5];
+    int nuc;
+    double nucp[5];
+    double fs, fd, Dc;
+    double bar_wy, bar_wz, bar_big;
+    double rs_a, rs_da, rs_w, rs_wr, rs_b, rs_L, rs_V0, rs_f0, rs_Vin;
+};
+
+// One block as the kernels see it.
+template <typename real>
+struct Blk {
+    int nq, nr, ns;
+    long NP;                 // nq*nr*ns
+    const real *Fin;         // [9][NP] stage input
+    real *Fout;              // [9][NP] stage output
+    real *DF;                // [9][NP] low-storage RK rates
+    const real *G;           // [10][NP] metrics + J (curvilinear); nullptr for Cartesian
+    const real *M;           // [3][NP] lambda mu rho (heterogeneous); nullptr if homogeneous
+    double gc[10];           // constant metrics + J (Cartesian)
+    double lam, mu, rho;     // homogeneous material
+    int cart;                // 1: constant metrics
+    int bc[6];
+    int pml[6];
+    int npml;
+    real *Q[6], *DQ[6];      // PML slabs [9][slab nodes]
+    // face data (FP64): per side, nodes in face layout (q: j + nr*k, r: i + nq*k, s: i + nq*j)
+    const double *frame[6];  // [9][face] n m l
+    const double *fgm[6];    // [3][face] metric components of the face axis (a_x, a_y, a_z)
+    const double *fmat[6];   // [3][face] lambda mu rho
+    const double *fx;        // [3][nr*ns] coordinates of the fault face (q-face carrying bc==0)
+    double *P;               // [2][NP] plastic strain rate / accumulated plastic strain (FP64)
+    const double *Y;         // [NP] y-coordinate (plastic depth-dependent background stress)
+    double *Xall;            // [3][NP] coordinates (FP64, only when needed by plasticity)
+};
+
+// One block's copy of the interface state (iface.f90:13-48), FP64.
+struct Iface {
+    double *S, *DS;          // [4][nr*ns]
+    double *W, *DW;          // [nr*ns]
+    double *trup;            // [nr*ns]
+    double *Svel;            // [3][nr*ns]
+    double *Uhat, *Vhat;     // [6][nr*ns]
+};
+
+__device__ __forceinline__ long pidx(int nq, int nr, int i, int j, int k)
+{
+    return (long)i + (long)nq * ((long)j + (long)nr * (long)k);
+}
+
+}  // namespace wql

@@ -1,0 +1,1163 @@
+// wql_kernels.hip -- gfx950 kernels of the WaveQLab3D time-stepping path.
+//
+// Per RK stage (time_step.f90:109-147) the host enqueues:
+//   k_volume_main    interior + x/y boundary rows, LDS-tiled x/y stencils with
+//                    a z-marching register queue; fused DF = A*DF + L(F),
+//                    F_out = F + B*dt*DF                     (HBM-bound)
+//   k_volume_generic the z boundary planes and PML slabs, per node in the
+//                    reference's boundary-band form (RHS_Interior.f90:221-268)
+//                    with the PML auxiliary update (1782-2201)
+//   k_boundary       every outer-face node of both blocks: characteristic SAT
+//                    terms (1516-1703) and the fault SAT + friction solve
+//                    (1706-1779, CouplingForcing.f90, Interface_Condition.f90),
+//                    then the final F_out / PML Q / slip / state update
+//   k_plastic        last stage only: viscoplastic return map (plastic.f90)
+#include "wql_device.h"
+
+#include <math.h>
+
+namespace wql {
+
+__constant__ SbpTable c_sbp;
+
+// ----------------------------------------------------------------------------
+// SBP stencil taps for row i of an n-node line (absolute indices)
+__device__ __forceinline__ int sbp_taps(int i, int n, int *idx, double *c)
+{
+    if (i >= 6 && i <= n - 7) {
+        idx[0] = i - 3; c[0] = -1.0 / 60.0;
+        idx[1] = i - 2; c[1] = 0.15;
+        idx[2] = i - 1; c[2] = -0.75;
+        idx[3] = i + 1; c[3] = 0.75;
+        idx[4] = i + 2; c[4] = -0.15;
+        idx[5] = i + 3; c[5] = 1.0 / 60.0;
+        return 6;
+    }
+    const SbpRow &R = (i < 6) ? c_sbp.left[i] : c_sbp.right[i - (n - 6)];
+    for (int t = 0; t < R.n; t++) { idx[t] = i + R.off[t]; c[t] = R.c[t]; }
+    return R.n;
+}
+
+// ----------------------------------------------------------------------------
+// init: metrics and Jacobian in FP64 from node coordinates (grid.f90:396-469)
+__global__ void k_metrics(int nq, int nr, int ns, const double *__restrict__ X, double *__restrict__ G)
+{
+    long NP = (long)nq * nr * ns;
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= NP) return;
+    int i = (int)(p % nq), j = (int)((p / nq) % nr), k = (int)(p / ((long)nq * nr));
+    double D[3][3];  // D[axis][component] = dX_comp / d axis
+    for (int a = 0; a < 3; a++) {
+        int n = a == 0 ? nq : a == 1 ? nr : ns;
+        int pos = a == 0 ? i : a == 1 ? j : k;
+        int idx[8]; double c[8];
+        int m = sbp_taps(pos, n, idx, c);
+        double h = 1.0 / (double)(n - 1);
+        for (int f = 0; f < 3; f++) {
+            double s = 0.0;
+            for (int t = 0; t < m; t++) {
+                int ii = i, jj = j, kk = k;
+                if (a == 0) ii = idx[t]; else if (a == 1) jj = idx[t]; else kk = idx[t];
+                double v = c[t] * X[f * NP + pidx(nq, nr, ii, jj, kk)];
+                s = (t == 0) ? v : s + v;
+            }
+            D[a][f] = (1.0 / h) * s;
+        }
+    }
+    const double *Xq = D[0], *Xr = D[1], *Xs = D[2];
+    double J = Xq[0] * (Xr[1] * Xs[2] - Xr[2] * Xs[1]) - Xq[1] * (Xr[0] * Xs[2] - Xr[2] * Xs[0]) +
+               Xq[2] * (Xr[0] * Xs[1] - Xr[1] * Xs[0]);
+    double iJ = 1.0 / J;
+    G[0 * NP + p] = iJ * (Xr[1] * Xs[2] - Xr[2] * Xs[1]);
+    G[1 * NP + p] = iJ * (Xq[2] * Xs[1] - Xq[1] * Xs[2]);
+    G[2 * NP + p] = iJ * (Xq[1] * Xr[2] - Xq[2] * Xr[1]);
+    G[3 * NP + p] = iJ * (Xr[2] * Xs[0] - Xr[0] * Xs[2]);
+    G[4 * NP + p] = iJ * (Xq[0] * Xs[2] - Xq[2] * Xs[0]);
+    G[5 * NP + p] = iJ * (Xq[2] * Xr[0] - Xq[0] * Xr[2]);
+    G[6 * NP + p] = iJ * (Xr[0] * Xs[1] - Xr[1] * Xs[0]);
+    G[7 * NP + p] = iJ * (Xq[1] * Xs[0] - Xq[0] * Xs[1]);
+    G[8 * NP + p] = iJ * (Xq[0] * Xr[1] - Xq[1] * Xr[0]);
+    G[9 * NP + p] = J;
+}
+
+// init: face frames (unit_normals.f90:8-168, gen_orth_vectors 232-266,
+// Gram_Schmidt 269-282) and face copies of metric / material / coordinates
+__global__ void k_frames(int nq, int nr, int ns, int side, const double *__restrict__ G,
+                         const double *__restrict__ M, const double *__restrict__ X,
+                         double *__restrict__ frame, double *__restrict__ fgm, double *__restrict__ fmat,
+                         double *__restrict__ fx, int *__restrict__ err)
+{
+    int dir = side / 2;
+    int na = dir == 0 ? nr : nq;
+    int nb = dir == 2 ? nr : ns;
+    long FP = (long)na * nb;
+    long fp = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (fp >= FP) return;
+    int jj = (int)(fp % na), kk = (int)(fp / na);
+    int i, j, k;
+    if (dir == 0) { i = (side == 0) ? 0 : nq - 1; j = jj; k = kk; }
+    else if (dir == 1) { j = (side == 2) ? 0 : nr - 1; i = jj; k = kk; }
+    else { k = (side == 4) ? 0 : ns - 1; i = jj; j = kk; }
+    long NP = (long)nq * nr * ns;
+    long p = pidx(nq, nr, i, j, k);
+    double ax = G[(0 + dir) * NP + p], ay = G[(3 + dir) * NP + p], az = G[(6 + dir) * NP + p];
+    double norm = sqrt(ax * ax + ay * ay + az * az);
+    double n[3] = {ax / norm, ay / norm, az / norm};
+    double m[3] = {0.0, 1.0, 0.0};
+    const double tol = 1.0e-12;
+    if (fabs(sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) - 1.0) >= tol) err[0] = 1;
+    double d1 = sqrt((n[0] - m[0]) * (n[0] - m[0]) + (n[1] - m[1]) * (n[1] - m[1]) + (n[2] - m[2]) * (n[2] - m[2]));
+    double d2 = sqrt((n[0] + m[0]) * (n[0] + m[0]) + (n[1] + m[1]) * (n[1] + m[1]) + (n[2] + m[2]) * (n[2] + m[2]));
+    if (!(d1 >= tol && d2 >= tol)) { m[0] = 0.0; m[1] = 0.0; m[2] = 1.0; }
+    double a = n[0] * m[0] + n[1] * m[1] + n[2] * m[2];
+    for (int t = 0; t < 3; t++) m[t] = m[t] - a * n[t];
+    double s = 1.0 / sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    for (int t = 0; t < 3; t++) m[t] = s * m[t];
+    double l[3] = {n[1] * m[2] - n[2] * m[1], n[2] * m[0] - n[0] * m[2], n[0] * m[1] - n[1] * m[0]};
+    for (int t = 0; t < 3; t++) {
+        frame[(0 + t) * FP + fp] = n[t];
+        frame[(3 + t) * FP + fp] = m[t];
+        frame[(6 + t) * FP + fp] = l[t];
+        fmat[t * FP + fp] = M[t * NP + p];
+        if (fx) fx[t * FP + fp] = X[t * NP + p];
+    }
+    fgm[0 * FP + fp] = ax;
+    fgm[1 * FP + fp] = ay;
+    fgm[2 * FP + fp] = az;
+}
+
+// storage conversion FP64 -> real
+template <typename real>
+__global__ void k_convert(long n, const double *__restrict__ src, real *__restrict__ dst)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = (real)src[p];
+}
+
+// ----------------------------------------------------------------------------
+// volume: generic per-node path (reference boundary-band form)
+
+struct Box {
+    int blk;
+    int x0, x1, y0, y1, z0, z1;
+    long start;  // prefix of node counts
+};
+struct BoxList {
+    int nbox;
+    Box box[16];
+    long total;
+};
+
+template <typename real>
+struct Pair {
+    Blk<real> b[2];
+};
+
+template <typename real>
+__device__ __forceinline__ double gmet(const Blk<real> &B, int c, long p)
+{
+    return B.cart ? B.gc[c] : (double)B.G[(long)c * B.NP + p];
+}
+
+template <typename real>
+__device__ void volume_node_generic(const Blk<real> &B, int i, int j, int k, real A, real bdt)
+{
+    const long NP = B.NP;
+    const long p = pidx(B.nq, B.nr, i, j, k);
+    real U[3][NF];  // Ux, Uy, Uz (JU_xJU_yJU_z6.f90:12-812, summed as in 808)
+    const int nn[3] = {B.nq, B.nr, B.ns};
+    const int pos[3] = {i, j, k};
+    for (int a = 0; a < 3; a++) {
+        int idx[8]; double c[8];
+        int m = sbp_taps(pos[a], nn[a], idx, c);
+        real sv[3] = {0, 0, 0};
+        real ss[3][6];
+        for (int fam = 0; fam < 3; fam++)
+            for (int f = 0; f < 6; f++) ss[fam][f] = 0;
+        for (int t = 0; t < m; t++) {
+            int ii = i, jj = j, kk = k;
+            if (a == 0) ii = idx[t]; else if (a == 1) jj = idx[t]; else kk = idx[t];
+            long q = pidx(B.nq, B.nr, ii, jj, kk);
+            real ct = (real)c[t];
+            real J = (real)gmet(B, 9, q);
+            real mt[3] = {(real)gmet(B, 0 + a, q), (real)gmet(B, 3 + a, q), (real)gmet(B, 6 + a, q)};
+            for (int f = 0; f < 3; f++) {
+                real v = ct * B.Fin[f * NP + q];
+                sv[f] = (t == 0) ? v : sv[f] + v;
+            }
+            for (int f = 0; f < 6; f++) {
+                real cf = ct * B.Fin[(3 + f) * NP + q] * J;
+                for (int fam = 0; fam < 3; fam++) {
+                    real v = cf * mt[fam];
+                    ss[fam][f] = (t == 0) ? v : ss[fam][f] + v;
+                }
+            }
+        }
+        real ih = (real)(1.0 / (1.0 / (double)(nn[a] - 1)));
+        for (int fam = 0; fam < 3; fam++) {
+            real mc = (real)gmet(B, 3 * fam + a, p);
+            for (int f = 0; f < 3; f++) {
+                real v = ih * (mc * sv[f]);
+                U[fam][f] = (a == 0) ? v : U[fam][f] + v;
+            }
+            for (int f = 0; f < 6; f++) {
+                real v = ih * ss[fam][f];
+                U[fam][3 + f] = (a == 0) ? v : U[fam][3 + f] + v;
+            }
+        }
+    }
+    const real *Ux = U[0], *Uy = U[1], *Uz = U[2];
+    real lam, mu, rho;
+    if (B.M) { lam = B.M[p]; mu = B.M[NP + p]; rho = B.M[2 * NP + p]; }
+    else { lam = (real)B.lam; mu = (real)B.mu; rho = (real)B.rho; }
+    real Jc = (real)gmet(B, 9, p);
+    real rhoJ_inv = (real)1 / (rho * Jc);
+    real l2m = lam + (real)2 * mu;
+    real DU[NF];
+    DU[0] = (Ux[3] + Uy[6] + Uz[7]) * rhoJ_inv;
+    DU[1] = (Ux[6] + Uy[4] + Uz[8]) * rhoJ_inv;
+    DU[2] = (Ux[7] + Uy[8] + Uz[5]) * rhoJ_inv;
+    DU[3] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
+    DU[4] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
+    DU[5] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
+    DU[6] = mu * (Uy[0] + Ux[1]);
+    DU[7] = mu * (Uz[0] + Ux[2]);
+    DU[8] = mu * (Uz[1] + Uy[2]);
+    // PML (RHS_Interior.f90:1782-2201), order z-low, z-high, y-low, y-high, x-low, x-high
+    const int order[6] = {4, 5, 2, 3, 0, 1};
+    const double d0 = 6.0 / (2.0 * 1.0) * log(1.0 / 1.0e-3);
+    const real cfs = (real)1;
+    for (int o = 0; o < 6; o++) {
+        int side = order[o];
+        if (!B.pml[side]) continue;
+        int dir = side / 2, np = B.npml;
+        int lo[3] = {0, 0, 0}, hi[3] = {B.nq - 1, B.nr - 1, B.ns - 1};
+        int n = nn[dir];
+        if (side % 2 == 0) hi[dir] = np - 1; else lo[dir] = n - np;
+        if (pos[0] < lo[0] || pos[0] > hi[0] || pos[1] < lo[1] || pos[1] > hi[1] || pos[2] < lo[2] || pos[2] > hi[2])
+            continue;
+        long n0 = hi[0] - lo[0] + 1, n1 = hi[1] - lo[1] + 1, n2 = hi[2] - lo[2] + 1;
+        long s = (long)(pos[0] - lo[0]) + n0 * ((long)(pos[1] - lo[1]) + n1 * (long)(pos[2] - lo[2]));
+        long SP = n0 * n1 * n2;
+        int p1 = pos[dir] + 1;
+        double dd;
+        if (side == 4) dd = d0 * fabs((double)(p1 - np) / (double)np);
+        else if (side % 2 == 0) dd = d0 * fabs((double)(p1 - (1 + np)) / (double)np);
+        else dd = d0 * fabs((double)(p1 - (n - np)) / (double)np);
+        real d = (real)dd;
+        real *Q = B.Q[side], *DQ = B.DQ[side];
+        real q[NF];
+        for (int f = 0; f < NF; f++) q[f] = Q[f * SP + s];
+        DU[0] = DU[0] - d * Jc * q[0] * rhoJ_inv;
+        DU[1] = DU[1] - d * Jc * q[1] * rhoJ_inv;
+        DU[2] = DU[2] - d * Jc * q[2] * rhoJ_inv;
+        DU[3] = DU[3] - d * (l2m * q[3] + lam * q[4] + lam * q[5]);
+        DU[4] = DU[4] - d * (lam * q[3] + l2m * q[4] + lam * q[5]);
+        DU[5] = DU[5] - d * (lam * q[3] + lam * q[4] + l2m * q[5]);
+        DU[6] = DU[6] - mu * d * q[6];
+        DU[7] = DU[7] - mu * d * q[7];
+        DU[8] = DU[8] - mu * d * q[8];
+        real src[NF];
+        if (dir == 2) {
+            src[0] = Uz[7] / Jc; src[1] = Uz[8] / Jc; src[2] = Uz[5] / Jc;
+            src[3] = 0; src[4] = 0; src[5] = Uz[2]; src[6] = 0; src[7] = Uz[0]; src[8] = Uz[1];
+        } else if (dir == 1) {
+            src[0] = Uy[6] / Jc; src[1] = Uy[4] / Jc; src[2] = Uy[8] / Jc;
+            src[3] = 0; src[4] = Uy[1]; src[5] = 0; src[6] = Uy[0]; src[7] = 0; src[8] = Uy[2];
+        } else {
+            src[0] = Ux[3] / Jc; src[1] = Ux[6] / Jc; src[2] = Ux[7] / Jc;
+            src[3] = Ux[0]; src[4] = 0; src[5] = 0; src[6] = Ux[1]; src[7] = Ux[2]; src[8] = 0;
+        }
+        for (int f = 0; f < NF; f++) {
+            real dq = A * DQ[f * SP + s];
+            dq = dq + src[f] - (d + cfs) * q[f];
+            DQ[f * SP + s] = dq;
+            Q[f * SP + s] = q[f] + bdt * dq;
+        }
+    }
+    for (int f = 0; f < NF; f++) {
+        real df = A * B.DF[f * NP + p];
+        df = df + DU[f];
+        B.DF[f * NP + p] = df;
+        B.Fout[f * NP + p] = B.Fin[f * NP + p] + bdt * df;
+    }
+}
+
+template <typename real>
+__global__ void __launch_bounds__(256) k_volume_generic(Pair<real> P, BoxList L, real A0, real A1, real bdt)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.total) return;
+    int bi = 0;
+    for (int b = 1; b < L.nbox; b++)
+        if (t >= L.box[b].start) bi = b;
+    const Box &bx = L.box[bi];
+    long r = t - bx.start;
+    int w = bx.x1 - bx.x0, h = bx.y1 - bx.y0;
+    int i = bx.x0 + (int)(r % w);
+    int j = bx.y0 + (int)((r / w) % h);
+    int k = bx.z0 + (int)(r / ((long)w * h));
+    const Blk<real> &B = P.b[bx.blk];
+    volume_node_generic<real>(B, i, j, k, bx.blk == 0 ? A0 : A1, bdt);
+}
+
+// ----------------------------------------------------------------------------
+// volume: main tiled kernel
+//
+// A thread block owns a TX x TY tile of (q, r) columns and marches a chunk of
+// s planes.  Velocities and the q-/r-direction stress fluxes
+//     f_a,i = J (a_x s_xi + a_y s_yi + a_z s_zi),  a = q, r
+// of the current plane sit in LDS with a 3-node halo; the s-direction data
+// (velocities and s-fluxes of planes z-3..z+3) ride in a register queue.
+// Boundary closure rows in q and r read the same LDS tile: the tiling keeps
+// all six closure rows and their taps inside one tile (first tile starts at
+// the domain edge, last tile is right-aligned, TX, TY >= 6).
+
+struct TileGeom {
+    int blk;
+    int xa0, xa1, ya0, ya1, za0, za1;  // node range of this kernel in the block
+    int ntx, nty, nzc, zc;             // tiles in x, y; z chunks and chunk length
+    long first;                        // first logical tile id of this block
+};
+struct TileList {
+    int n;
+    TileGeom g[2];
+    long total;
+};
+
+template <int T>
+__device__ __forceinline__ void tile_range(int t, int nt, int a0, int a1, int nfull, int &start, int &c0, int &c1)
+{
+    // start: first node held by the tile; [c0,c1): nodes computed by it
+    int cut = (a1 == nfull) ? nfull - 6 : a1;
+    if (t < nt - 1) {
+        start = a0 + t * T;
+        c0 = start;
+        c1 = min(start + T, cut);
+        if (c1 < c0) c1 = c0;
+    } else {
+        start = max(a0, a1 - T);
+        c0 = min(a0 + t * T, cut);
+        if (c0 < a0) c0 = a0;
+        c1 = a1;
+    }
+}
+
+template <typename real>
+__device__ __forceinline__ real d1_row(const real *row, int lx, int gx, int n, real ih)
+{
+    // row: LDS row, lx: local index of the node in row, gx: global index
+    if (gx >= 6 && gx <= n - 7) {
+        return ih * ((real)0.75 * (row[lx + 1] - row[lx - 1]) + (real)(-0.15) * (row[lx + 2] - row[lx - 2]) +
+                     (real)(1.0 / 60.0) * (row[lx + 3] - row[lx - 3]));
+    }
+    const SbpRow &R = (gx < 6) ? c_sbp.left[gx] : c_sbp.right[gx - (n - 6)];
+    real s = 0;
+    for (int t = 0; t < R.n; t++) s = s + (real)R.c[t] * row[lx + R.off[t]];
+    return ih * s;
+}
+
+template <typename real>
+__device__ __forceinline__ real d1_col(const real *col, int stride, int ly, int gy, int n, real ih)
+{
+    if (gy >= 6 && gy <= n - 7) {
+        return ih * ((real)0.75 * (col[(ly + 1) * stride] - col[(ly - 1) * stride]) +
+                     (real)(-0.15) * (col[(ly + 2) * stride] - col[(ly - 2) * stride]) +
+                     (real)(1.0 / 60.0) * (col[(ly + 3) * stride] - col[(ly - 3) * stride]));
+    }
+    const SbpRow &R = (gy < 6) ? c_sbp.left[gy] : c_sbp.right[gy - (n - 6)];
+    real s = 0;
+    for (int t = 0; t < R.n; t++) s = s + (real)R.c[t] * col[(ly + R.off[t]) * stride];
+    return ih * s;
+}
+
+template <typename real, bool CURV>
+__device__ __forceinline__ void load_node(const Blk<real> &B, long p, real v[3], real sg[6])
+{
+    const long NP = B.NP;
+#pragma unroll
+    for (int f = 0; f < 3; f++) v[f] = B.Fin[f * NP + p];
+#pragma unroll
+    for (int f = 0; f < 6; f++) sg[f] = B.Fin[(3 + f) * NP + p];
+}
+
+// flux along axis a at node p: J (a_x s_x. + a_y s_y. + a_z s_z.)
+template <typename real, bool CURV>
+__device__ __forceinline__ void flux(const Blk<real> &B, long p, int a, const real sg[6], real fl[3])
+{
+    real J, ax, ay, az;
+    if (CURV) {
+        const long NP = B.NP;
+        J = B.G[9 * NP + p];
+        ax = B.G[(0 + a) * NP + p];
+        ay = B.G[(3 + a) * NP + p];
+        az = B.G[(6 + a) * NP + p];
+    } else {
+        J = (real)B.gc[9]; ax = (real)B.gc[0 + a]; ay = (real)B.gc[3 + a]; az = (real)B.gc[6 + a];
+    }
+    real Jx = J * ax, Jy = J * ay, Jz = J * az;
+    // s_x. = (sxx, sxy, sxz), s_y. = (sxy, syy, syz), s_z. = (sxz, syz, szz)
+    fl[0] = Jx * sg[0] + Jy * sg[3] + Jz * sg[4];
+    fl[1] = Jx * sg[3] + Jy * sg[1] + Jz * sg[5];
+    fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
+}
+
+template <typename real, int TX, int TY, bool CURV, bool HET>
+__global__ void __launch_bounds__(TX * TY) k_volume_main(Pair<real> P, TileList TL, real A0, real A1, real bdt)
+{
+    constexpr int WX = TX + 6, WY = TY + 6;
+    __shared__ real sv[3][WY][WX];   // velocities on the cross-shaped halo tile
+    __shared__ real sfq[3][TY][WX];  // q-fluxes on x-extended rows
+    __shared__ real sfr[3][WY][TX];  // r-fluxes on y-extended columns
+
+    // XCD-aware ordering: hardware deals consecutive workgroups round-robin to
+    // the 8 XCDs; give each XCD a contiguous range of logical tiles so that
+    // neighbouring tiles (which share halo rows) share an L2.
+    const long nwg = (long)gridDim.x;
+    const long per = (nwg + 7) / 8;
+    const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
+    if (logical >= TL.total) return;
+    const int gi = (TL.n > 1 && logical >= TL.g[1].first) ? 1 : 0;
+    const TileGeom &T = TL.g[gi];
+    const Blk<real> &B = P.b[T.blk];
+    const real A = T.blk == 0 ? A0 : A1;
+    long lid = logical - T.first;
+    const int tx_i = (int)(lid % T.ntx);
+    const int ty_i = (int)((lid / T.ntx) % T.nty);
+    const int zc_i = (int)(lid / ((long)T.ntx * T.nty));
+
+    int xs, xc0, xc1, ys, yc0, yc1;
+    tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
+    tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
+    const int z0 = T.za0 + zc_i * T.zc;
+    const int z1 = min(z0 + T.zc, T.za1);
+
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int gx = xs + lx, gy = ys + ly;
+    const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
+    const bool inb = (gx < B.nq && gy < B.nr);
+    const long NP = B.NP;
+    const long plane = (long)B.nq * B.nr;
+    const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
+
+    // s-direction register queue: planes z-3 .. z+3 of this column
+    real qv[7][3], qf[7][3];
+#pragma unroll
+    for (int a = 0; a < 7; a++)
+#pragma unroll
+        for (int f = 0; f < 3; f++) { qv[a][f] = 0; qf[a][f] = 0; }
+
+    auto push = [&](int z) {
+#pragma unroll
+        for (int a = 0; a < 6; a++)
+#pragma unroll
+            for (int f = 0; f < 3; f++) { qv[a][f] = qv[a + 1][f]; qf[a][f] = qf[a + 1][f]; }
+        if (inb) {
+            long p = (long)gx + (long)B.nq * gy + plane * z;
+            real v[3], sg[6], fl[3];
+            load_node<real, CURV>(B, p, v, sg);
+            flux<real, CURV>(B, p, 2, sg, fl);
+#pragma unroll
+            for (int f = 0; f < 3; f++) { qv[6][f] = v[f]; qf[6][f] = fl[f]; }
+        }
+    };
+
+    for (int z = z0 - 3; z < z0 + 3; z++) push(z);
+
+    const int nthr = TX * TY;
+    const int tid = ly * TX + lx;
+    for (int z = z0; z < z1; z++) {
+        push(z + 3);
+        __syncthreads();
+        // fill LDS for plane z: x-extended rows (center + x halo)
+        for (int c = tid; c < WX * TY; c += nthr) {
+            int cx = c % WX, cy = c / WX;
+            int ggx = xs - 3 + cx, ggy = ys + cy;
+            if (ggx < 0 || ggx >= B.nq || ggy >= B.nr) continue;
+            long p = (long)ggx + (long)B.nq * ggy + plane * z;
+            real v[3], sg[6], fl[3];
+            load_node<real, CURV>(B, p, v, sg);
+            flux<real, CURV>(B, p, 0, sg, fl);
+#pragma unroll
+            for (int f = 0; f < 3; f++) { sv[f][cy + 3][cx] = v[f]; sfq[f][cy][cx] = fl[f]; }
+            if (cx >= 3 && cx < TX + 3) {
+                flux<real, CURV>(B, p, 1, sg, fl);
+#pragma unroll
+                for (int f = 0; f < 3; f++) sfr[f][cy + 3][cx - 3] = fl[f];
+            }
+        }
+        // y halo rows
+        for (int c = tid; c < TX * 6; c += nthr) {
+            int cx = c % TX, hy = c / TX;
+            int cy = hy < 3 ? hy : TY + hy;  // 0,1,2 and TY+3..TY+5
+            int ggx = xs + cx, ggy = ys - 3 + cy;
+            if (ggx >= B.nq || ggy < 0 || ggy >= B.nr) continue;
+            long p = (long)ggx + (long)B.nq * ggy + plane * z;
+            real v[3], sg[6], fl[3];
+            load_node<real, CURV>(B, p, v, sg);
+            flux<real, CURV>(B, p, 1, sg, fl);
+#pragma unroll
+            for (int f = 0; f < 3; f++) { sv[f][cy][cx + 3] = v[f]; sfr[f][cy][cx] = fl[f]; }
+        }
+        __syncthreads();
+        if (!own) continue;
+        const long p = (long)gx + (long)B.nq * gy + plane * z;
+        // derivatives along q and r from LDS, along s from the queue
+        real dvq[3], dvr[3], dvs[3], div[3];
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            dvq[f] = d1_row<real>(&sv[f][ly + 3][0], lx + 3, gx, B.nq, ihq);
+            dvr[f] = d1_col<real>(&sv[f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
+            dvs[f] = ihs * ((real)0.75 * (qv[4][f] - qv[2][f]) + (real)(-0.15) * (qv[5][f] - qv[1][f]) +
+                            (real)(1.0 / 60.0) * (qv[6][f] - qv[0][f]));
+            real a = d1_row<real>(&sfq[f][ly][0], lx + 3, gx, B.nq, ihq);
+            real b = d1_col<real>(&sfr[f][0][lx], TX, ly + 3, gy, B.nr, ihr);
+            real c = ihs * ((real)0.75 * (qf[4][f] - qf[2][f]) + (real)(-0.15) * (qf[5][f] - qf[1][f]) +
+                            (real)(1.0 / 60.0) * (qf[6][f] - qf[0][f]));
+            div[f] = a + b + c;
+        }
+        real g[10];
+        if (CURV) {
+#pragma unroll
+            for (int c = 0; c < 10; c++) g[c] = B.G[c * NP + p];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 10; c++) g[c] = (real)B.gc[c];
+        }
+        real lam, mu, rho;
+        if (HET) { lam = B.M[p]; mu = B.M[NP + p]; rho = B.M[2 * NP + p]; }
+        else { lam = (real)B.lam; mu = (real)B.mu; rho = (real)B.rho; }
+        // physical velocity gradients (chain rule with the centre metrics)
+        real Ux[3], Uy[3], Uz[3];
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            Ux[f] = g[0] * dvq[f] + g[1] * dvr[f] + g[2] * dvs[f];
+            Uy[f] = g[3] * dvq[f] + g[4] * dvr[f] + g[5] * dvs[f];
+            Uz[f] = g[6] * dvq[f] + g[7] * dvr[f] + g[8] * dvs[f];
+        }
+        real rhoJ_inv = (real)1 / (rho * g[9]);
+        real l2m = lam + (real)2 * mu;
+        real DU[NF];
+        DU[0] = div[0] * rhoJ_inv;
+        DU[1] = div[1] * rhoJ_inv;
+        DU[2] = div[2] * rhoJ_inv;
+        DU[3] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
+        DU[4] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
+        DU[5] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
+        DU[6] = mu * (Uy[0] + Ux[1]);
+        DU[7] = mu * (Uz[0] + Ux[2]);
+        DU[8] = mu * (Uz[1] + Uy[2]);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            real df = A * B.DF[f * NP + p] + DU[f];
+            B.DF[f * NP + p] = df;
+            real fin = (f < 3) ? sv[f][ly + 3][lx + 3] : B.Fin[f * NP + p];
+            B.Fout[f * NP + p] = fin + bdt * df;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// boundary: outer-face SAT terms and the fault interface
+//
+// FP64 throughout and without FMA contraction, so that the friction solve
+// rounds as the reference's scalar code does (the Regula Falsi iteration
+// count depends on last-bit differences near its 1e-12 step tolerance).
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ double boxcar(double x, double w)  // Interface_Condition.f90:2438-2456
+{
+    const double tol = 100.0 * 2.220446049250313e-16;
+    if (-w + tol < x && x < w - tol) return 1.0;
+    if (fabs(-w - x) <= tol || fabs(x - w) <= tol) return 0.5;
+    return 0.0;
+}
+
+__device__ __forceinline__ double smooth_boxcar(double x, double w, double wr)  // 2459-2476
+{
+    if (-w <= x && x <= w) return 1.0;
+    if (fabs(x) > w && fabs(x) < w + wr) return 0.5 * (1.0 + tanh(wr / (fabs(x) - w - wr) + wr / (fabs(x) - w)));
+    return 0.0;
+}
+
+// prestress (Interface_Condition.f90:1802-2435)
+__device__ void prestress(const FaultParams &f, const double x[3], double t, const double l[3], const double m[3],
+                          const double n[3], double &Tn, double &Tm, double &Tl)
+{
+    if (f.stress_mode == 0) {
+        Tn = f.T0[0]; Tm = f.T0[1]; Tl = f.T0[2];
+        for (int k = 0; k < f.npatch; k++)
+            Tl = Tl + f.patch[k][4] * boxcar(x[1] - f.patch[k][0], f.patch[k][2]) * boxcar(x[2] - f.patch[k][1], f.patch[k][3]);
+    } else {
+        const double *s = f.s0;
+        double Tx = n[0] * s[0] + n[1] * s[3] + n[2] * s[4];
+        double Ty = n[0] * s[3] + n[1] * s[1] + n[2] * s[5];
+        double Tz = n[0] * s[4] + n[1] * s[5] + n[2] * s[2];
+        Tn = n[0] * Tx + n[1] * Ty + n[2] * Tz;
+        Tm = m[0] * Tx + m[1] * Ty + m[2] * Tz;
+        Tl = l[0] * Tx + l[1] * Ty + l[2] * Tz;
+    }
+    if (f.nuc == 1) {
+        double dy = x[1] - f.nucp[0], dz = x[2] - f.nucp[1];
+        double r = sqrt(dy * dy + dz * dz);
+        double tn = 0.0;
+        if (r <= f.nucp[2]) tn = f.nucp[4];
+        else if (r >= f.nucp[2] && r <= f.nucp[2] + f.nucp[3])
+            tn = 0.5 * f.nucp[4] * (1.0 + cos(3.141592653589793 * (r - f.nucp[2]) / f.nucp[3]));
+        Tl = Tl + tn;
+    } else if (f.nuc == 2) {
+        double dy = x[1] - f.nucp[0], dz = x[2] - f.nucp[1];
+        double r = sqrt(dy * dy + dz * dz);
+        double R = f.nucp[2], T = f.nucp[4];
+        double Fv = 0.0, Gv = 0.0;
+        if (r < R) Fv = exp(r * r / (r * r - R * R));
+        if (t > 0.0 && t < T) Gv = exp((t - T) * (t - T) / (t * (t - 2.0 * T)));
+        if (t >= T) Gv = 1.0;
+        Tl = Tl + f.nucp[3] * Fv * Gv;
+    }
+}
+
+__device__ __forceinline__ double rs_f(double V, double Phi, double eta, double sn, double psi, double V0, double a)
+{  // Interface_Condition.f90:1262
+    return V + a * sn / eta * asinh(0.5 * V / V0 * exp(psi / a)) - Phi / eta;
+}
+
+// Regula Falsi of Interface_Condition.f90:1170-1239, kept step for step
+__device__ double regula_falsi(double V, double Phi, double eta, double sn, double psi, double V0, double a)
+{
+    double err = 1.0, tol = 1.0e-12;
+    double Vl = 0.0, Vr = Phi / eta;
+    int k = 1;
+    const int maxit = 5000;
+    double fv = rs_f(V, Phi, eta, sn, psi, V0, a);
+    double fl = rs_f(Vl, Phi, eta, sn, psi, V0, a);
+    double fr = rs_f(Vr, Phi, eta, sn, psi, V0, a);
+    if (fabs(fv) <= tol) return V;
+    if (fabs(fl) <= tol) return Vl;
+    if (fabs(fr) <= tol) return Vr;
+    while (k <= maxit && err >= tol) {
+        double V1 = V;
+        if (fv * fl > tol) {
+            Vl = V;
+            V = Vr - (Vr - Vl) * fr / (fr - fl);
+            fl = fv;
+        } else if (fv * fr > tol) {
+            Vr = V;
+            V = Vr - (Vr - Vl) * fr / (fr - fl);
+            fr = fv;
+        }
+        fv = rs_f(V, Phi, eta, sn, psi, V0, a);
+        err = fabs(V - V1);
+        k = k + 1;
+    }
+    return V;
+}
+
+// hat variables of one fault node (Interface_RHS 62-452; Welded 516-637;
+// Slip_Weakening 745-916; rate_state 918-1067); returns 0 or an error code
+__device__ int hat_vars(const FaultParams &f, const double u[6], const double v[6], const double m1[3],
+                        const double m2[3], const double x[3], const double nn[3], const double mm[3],
+                        const double ll[3], double slip, double psi, double t, double uo[6], double vo[6],
+                        double &Vel)
+{
+    double c1p = sqrt((2.0 * m1[1] + m1[0]) / m1[2]), c1s = sqrt(m1[1] / m1[2]);
+    double c2p = sqrt((2.0 * m2[1] + m2[0]) / m2[2]), c2s = sqrt(m2[1] / m2[2]);
+    double Z1s = m1[2] * c1s, Z2s = m2[2] * c2s, Z1p = m1[2] * c1p, Z2p = m2[2] * c2p;
+    double p_n = Z1p * u[0] - u[3], p_m = Z1s * u[1] - u[4], p_l = Z1s * u[2] - u[5];
+    double q_n = Z2p * v[0] + v[3], q_m = Z2s * v[1] + v[4], q_l = Z2s * v[2] + v[5];
+    Vel = 0.0;
+    if (f.law == 0) {
+        double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
+        double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
+        double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
+        double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
+        double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
+        uo[0] = (q_n - phi_n) / Z2p; uo[3] = phi_n;
+        vo[0] = (p_n + phi_n) / Z1p; vo[3] = phi_n;
+        uo[2] = (q_l - phi_l) / Z2s; uo[5] = phi_l;
+        vo[2] = (p_l + phi_l) / Z1s; vo[5] = phi_l;
+        uo[1] = (q_m - phi_m) / Z2s; uo[4] = phi_m;
+        vo[1] = (p_m + phi_m) / Z1s; vo[4] = phi_m;
+        return 0;
+    }
+    if (Z1s <= 1.0e-9 || Z2s <= 1.0e-9) return 2;
+    double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
+    double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
+    double T0n, T0m, T0l;
+    prestress(f, x, t, ll, mm, nn, T0n, T0m, T0l);
+    double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
+    double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
+    double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
+    double u_n_hat = (q_n - phi_n) / Z2p, Tu_n_hat = phi_n;
+    double v_n_hat = (p_n + phi_n) / Z1p, Tv_n_hat = phi_n;
+    double sigma_n = fmax(0.0, -(T0n + phi_n));
+    double vv_m, vv_l, Tu_m, Tu_l, Tv_m, Tv_l;
+    if (f.law == 1) {
+        double tau_lock = sqrt((T0l + phi_l) * (T0l + phi_l) + (T0m + phi_m) * (T0m + phi_m));
+        double fs = f.fs + f.bar_big * (1.0 - boxcar(x[1], f.bar_wy) * boxcar(x[2], f.bar_wz));
+        double fr = fs - (fs - f.fd) * fmin(slip, f.Dc) / f.Dc;
+        double tau_str = sigma_n * fr;
+        if (tau_lock > tau_str) {
+            double phi_1 = T0m + phi_m, phi_2 = T0l + phi_l;
+            double Phi = sqrt(phi_1 * phi_1 + phi_2 * phi_2);
+            double V = (Phi - tau_str) / eta_s;
+            if (V < 0.0) return 3;
+            Vel = V;
+            vv_m = phi_1 / (eta_s + tau_str / V);
+            vv_l = phi_2 / (eta_s + tau_str / V);
+            double tau_m = phi_1 - eta_s * vv_m, tau_l = phi_2 - eta_s * vv_l;
+            Tu_m = tau_m - T0m; Tu_l = tau_l - T0l;
+            Tv_m = tau_m - T0m; Tv_l = tau_l - T0l;
+        } else {
+            Tu_m = phi_m; Tu_l = phi_l; Tv_m = phi_m; Tv_l = phi_l;
+            vv_m = 0.0; vv_l = 0.0; Vel = 0.0;
+        }
+    } else {
+        double a = f.rs_a + f.rs_da * (1.0 - smooth_boxcar(x[1], f.rs_w, f.rs_wr) * smooth_boxcar(x[2], f.rs_w, f.rs_wr));
+        double V0 = f.rs_V0;
+        vv_m = v[1] - u[1];
+        vv_l = v[2] - u[2];
+        double phi_1 = T0m + phi_m, phi_2 = T0l + phi_l;
+        double Phi = sqrt(phi_1 * phi_1 + phi_2 * phi_2);
+        double V = sqrt(vv_m * vv_m + vv_l * vv_l);
+        if (V > Phi) V = 0.5 * Phi / eta_s;
+        V = regula_falsi(V, Phi, eta_s, sigma_n, psi, V0, a);
+        if (V < 0.0) return 3;
+        Vel = V;
+        double fv = sigma_n * a * asinh(0.5 * V / V0 * exp(psi / a));
+        vv_m = phi_1 / (eta_s + fv / V);
+        vv_l = phi_2 / (eta_s + fv / V);
+        double tau_m = phi_1 - eta_s * vv_m, tau_l = phi_2 - eta_s * vv_l;
+        Tu_m = tau_m - T0m; Tu_l = tau_l - T0l;
+        Tv_m = tau_m - T0m; Tv_l = tau_l - T0l;
+    }
+    uo[0] = u_n_hat; vo[0] = v_n_hat;
+    uo[1] = (q_m - Tv_m) / Z2s - vv_m;
+    uo[2] = (q_l - Tv_l) / Z2s - vv_l;
+    vo[1] = (Tu_m + p_m) / Z1s + vv_m;
+    vo[2] = (Tu_l + p_l) / Z1s + vv_l;
+    uo[3] = Tu_n_hat; vo[3] = Tv_n_hat;
+    uo[4] = Tu_m; uo[5] = Tu_l;
+    vo[4] = Tv_m; vo[5] = Tv_l;
+    return 0;
+}
+
+// characteristic boundary SAT of one node (BoundaryConditions.f90:777-918)
+// followed by `structure` (RHS_Interior.f90:2203-2223)
+__device__ void bc_sat(const double u[9], double rho, double mu, double lambda, const double l[3],
+                       const double m[3], const double n[3], double norm, double r, int plus,
+                       double ubc[9], double Ux[9])
+{
+    double c_p = sqrt((2.0 * mu + lambda) / rho), c_s = sqrt(mu / rho);
+    double Zp = rho * c_p, Zs = rho * c_s;
+    double T0 = n[0] * u[3] + n[1] * u[6] + n[2] * u[7];
+    double T1 = n[0] * u[6] + n[1] * u[4] + n[2] * u[8];
+    double T2 = n[0] * u[7] + n[1] * u[8] + n[2] * u[5];
+    double vl = l[0] * u[0] + l[1] * u[1] + l[2] * u[2];
+    double vm = m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
+    double vn = n[0] * u[0] + n[1] * u[1] + n[2] * u[2];
+    double Tl = l[0] * T0 + l[1] * T1 + l[2] * T2;
+    double Tm = m[0] * T0 + m[1] * T1 + m[2] * T2;
+    double Tn = n[0] * T0 + n[1] * T1 + n[2] * T2;
+    double s = plus ? 1.0 : -1.0;
+    double pl = 0.5 * ((1.0 - r) * Zs * vl + s * ((1.0 + r) * Tl));
+    double pm = 0.5 * ((1.0 - r) * Zs * vm + s * ((1.0 + r) * Tm));
+    double pn = 0.5 * ((1.0 - r) * Zp * vn + s * ((1.0 + r) * Tn));
+    double ql = 0.5 * ((1.0 - r) * vl + s * ((1.0 + r) / Zs * Tl));
+    double qm = 0.5 * ((1.0 - r) * vm + s * ((1.0 + r) / Zs * Tm));
+    double qn = 0.5 * ((1.0 - r) * vn + s * ((1.0 + r) / Zp * Tn));
+    double Fx = l[0] * pl + m[0] * pm + n[0] * pn;
+    double Fy = l[1] * pl + m[1] * pm + n[1] * pn;
+    double Fz = l[2] * pl + m[2] * pm + n[2] * pn;
+    double F_x = l[0] * ql + m[0] * qm + n[0] * qn;
+    double F_y = l[1] * ql + m[1] * qm + n[1] * qn;
+    double F_z = l[2] * ql + m[2] * qm + n[2] * qn;
+    ubc[0] = norm * Fx;
+    ubc[1] = norm * Fy;
+    ubc[2] = norm * Fz;
+    double sn = s * norm;
+    ubc[3] = sn * n[0] * F_x;
+    ubc[4] = sn * n[1] * F_y;
+    ubc[5] = sn * n[2] * F_z;
+    ubc[6] = sn * (n[1] * F_x + n[0] * F_y);
+    ubc[7] = sn * (n[2] * F_x + n[0] * F_z);
+    ubc[8] = sn * (n[2] * F_y + n[1] * F_z);
+    Ux[0] = 1.0 / rho * ubc[0];
+    Ux[1] = 1.0 / rho * ubc[1];
+    Ux[2] = 1.0 / rho * ubc[2];
+    Ux[3] = (2.0 * mu + lambda) * ubc[3] + lambda * (ubc[4] + ubc[5]);
+    Ux[4] = (2.0 * mu + lambda) * ubc[4] + lambda * (ubc[3] + ubc[5]);
+    Ux[5] = (2.0 * mu + lambda) * ubc[5] + lambda * (ubc[3] + ubc[4]);
+    Ux[6] = mu * ubc[6];
+    Ux[7] = mu * ubc[7];
+    Ux[8] = mu * ubc[8];
+}
+
+// fault SAT for block `me` (0: minus side, face q = nq-1; 1: plus side,
+// face q = 0) at fault node (j, k): CouplingForcing.f90:9-467 with
+// Interface_RHS; adds -tau0/h_q * bc to sat[9]
+template <typename real>
+__device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams &fp, int me, int j, int k,
+                          double t, int stage, double A, double bdt, double sat[9], double tau0)
+{
+    const Blk<real> &B1 = P.b[0], &B2 = P.b[1];
+    const Blk<real> &B = P.b[me];
+    const int side = me == 0 ? 1 : 0;
+    const int nr = B.nr;
+    const long FP = (long)nr * B.ns;
+    const long fpi = (long)j + (long)nr * k;
+    const long pu = pidx(B1.nq, B1.nr, B1.nq - 1, j, k), pv = pidx(B2.nq, B2.nr, 0, j, k);
+    double u[9], v[9];
+    for (int f = 0; f < 9; f++) { u[f] = (double)B1.Fin[f * B1.NP + pu]; v[f] = (double)B2.Fin[f * B2.NP + pv]; }
+    const double *fr = B.frame[side];
+    double nn[3], mm[3], ll[3];
+    for (int c = 0; c < 3; c++) { nn[c] = fr[c * FP + fpi]; mm[c] = fr[(3 + c) * FP + fpi]; ll[c] = fr[(6 + c) * FP + fpi]; }
+    // tractions and rotation into (n, m, l) (CouplingForcing.f90:105-191, 470-524)
+    double ur[6], vr[6];
+    ur[0] = u[0]; ur[1] = u[1]; ur[2] = u[2];
+    ur[3] = nn[0] * u[3] + nn[1] * u[6] + nn[2] * u[7];
+    ur[4] = nn[0] * u[6] + nn[1] * u[4] + nn[2] * u[8];
+    ur[5] = nn[0] * u[7] + nn[1] * u[8] + nn[2] * u[5];
+    vr[0] = v[0]; vr[1] = v[1]; vr[2] = v[2];
+    vr[3] = nn[0] * v[3] + nn[1] * v[6] + nn[2] * v[7];
+    vr[4] = nn[0] * v[6] + nn[1] * v[4] + nn[2] * v[8];
+    vr[5] = nn[0] * v[7] + nn[1] * v[8] + nn[2] * v[5];
+    double uL[6], vL[6];
+    for (int g = 0; g < 2; g++) {
+        const double *s = g == 0 ? ur : vr;
+        double *d = g == 0 ? uL : vL;
+        d[0] = nn[0] * s[0] + nn[1] * s[1] + nn[2] * s[2];
+        d[1] = mm[0] * s[0] + mm[1] * s[1] + mm[2] * s[2];
+        d[2] = ll[0] * s[0] + ll[1] * s[1] + ll[2] * s[2];
+        d[3] = nn[0] * s[3] + nn[1] * s[4] + nn[2] * s[5];
+        d[4] = mm[0] * s[3] + mm[1] * s[4] + mm[2] * s[5];
+        d[5] = ll[0] * s[3] + ll[1] * s[4] + ll[2] * s[5];
+    }
+    const double *fm1 = B1.fmat[1], *fm2 = B2.fmat[0];
+    double mat1[3] = {fm1[fpi], fm1[FP + fpi], fm1[2 * FP + fpi]};
+    double mat2[3] = {fm2[fpi], fm2[FP + fpi], fm2[2 * FP + fpi]};
+    double x[3] = {B.fx[fpi], B.fx[FP + fpi], B.fx[2 * FP + fpi]};
+    double slip = I.S[3 * FP + fpi], psi = I.W[fpi];
+    double uh[6], vh[6], Vel;
+    int rc = hat_vars(fp, uL, vL, mat1, mat2, x, nn, mm, ll, slip, psi, t, uh, vh, Vel);
+    if (rc) return rc;
+    // fault state rates (Interface_Condition.f90:431-436, 470-476) and RK update
+    if (fp.law == 2) {
+        double dw = A * I.DW[fpi];
+        dw = dw + fp.rs_b * fp.rs_V0 / fp.rs_L * exp(-(psi - fp.rs_f0) / fp.rs_b) - Vel * fp.rs_b / fp.rs_L;
+        I.DW[fpi] = dw;
+        I.W[fpi] = psi + bdt * dw;
+    } else {
+        double dw = A * I.DW[fpi];
+        I.DW[fpi] = dw;
+        I.W[fpi] = psi + bdt * dw;
+    }
+    double slip_rate = sqrt((vh[1] - uh[1]) * (vh[1] - uh[1]) + (vh[2] - uh[2]) * (vh[2] - uh[2]));
+    for (int c = 0; c < 4; c++) {
+        double ds = A * I.DS[c * FP + fpi];
+        ds = ds + (c < 3 ? vh[c] - uh[c] : slip_rate);
+        I.DS[c * FP + fpi] = ds;
+        I.S[c * FP + fpi] = I.S[c * FP + fpi] + bdt * ds;
+    }
+    if (!isfinite(slip_rate) || !isfinite(uh[4]) || !isfinite(uh[5])) return 4;
+    if (stage == 1) {  // 480-509, CouplingForcing.f90:213-228, 584-605
+        double T0n = 0.0, T0m = 0.0, T0l = 0.0;
+        if (fp.law != 0) prestress(fp, x, t, ll, mm, nn, T0n, T0m, T0l);
+        for (int c = 0; c < 3; c++) {
+            double sv = vh[c] - uh[c];
+            I.Svel[c * FP + fpi] = sv;
+            I.Uhat[c * FP + fpi] = sv;
+            I.Vhat[c * FP + fpi] = sv;
+        }
+        I.Uhat[3 * FP + fpi] = uh[3] + T0n; I.Vhat[3 * FP + fpi] = vh[3] + T0n;
+        I.Uhat[4 * FP + fpi] = uh[4] + T0m; I.Vhat[4 * FP + fpi] = vh[4] + T0m;
+        I.Uhat[5 * FP + fpi] = uh[5] + T0l; I.Vhat[5 * FP + fpi] = vh[5] + T0l;
+        double s1 = I.Svel[FP + fpi], s2 = I.Svel[2 * FP + fpi];
+        if (sqrt(s1 * s1 + s2 * s2) > 1e-3) I.trup[fpi] = fmin(I.trup[fpi], t);
+    }
+    // rotate back (527-581) and SAT forcing (242-463)
+    double uX[6], vX[6], uH[6], vH[6];
+    const double *srcs[4] = {uL, vL, uh, vh};
+    double *dsts[4] = {uX, vX, uH, vH};
+    for (int g = 0; g < 4; g++)
+        for (int c = 0; c < 3; c++) {
+            dsts[g][c] = nn[c] * srcs[g][0] + mm[c] * srcs[g][1] + ll[c] * srcs[g][2];
+            dsts[g][3 + c] = nn[c] * srcs[g][3] + mm[c] * srcs[g][4] + ll[c] * srcs[g][5];
+        }
+    const double *gm = B.fgm[side];
+    double q_x = gm[fpi], q_y = gm[FP + fpi], q_z = gm[2 * FP + fpi];
+    double norm = sqrt(q_x * q_x + q_y * q_y + q_z * q_z);
+    const double *mat = me == 0 ? mat1 : mat2;
+    double lam = mat[0], mu = mat[1], rho = mat[2];
+    double Zp = rho * sqrt((2.0 * mu + lam) / rho), Zs = rho * sqrt(mu / rho);
+    double n_1 = nn[0], n_2 = nn[1], n_3 = nn[2], m_1 = mm[0], m_2 = mm[1], m_3 = mm[2];
+    double l_1 = ll[0], l_2 = ll[1], l_3 = ll[2];
+    double M11 = n_1 * n_1 * Zp + n_2 * n_2 * Zs + n_3 * n_3 * Zs;
+    double M12 = n_1 * m_1 * Zp + n_2 * m_2 * Zs + n_3 * m_3 * Zs;
+    double M13 = n_1 * l_1 * Zp + n_2 * l_2 * Zs + n_3 * l_3 * Zs;
+    double M22 = m_1 * m_1 * Zp + m_2 * m_2 * Zs + m_3 * m_3 * Zs;
+    double M23 = m_1 * l_1 * Zp + m_2 * l_2 * Zs + m_3 * l_3 * Zs;
+    double M31 = l_1 * n_1 * Zp + l_2 * n_2 * Zs + l_3 * n_3 * Zs;
+    double M32 = l_1 * m_1 * Zp + l_2 * m_2 * Zs + l_3 * m_3 * Zs;
+    double M33 = l_1 * l_1 * Zp + l_2 * l_2 * Zs + l_3 * l_3 * Zs;
+    double P11 = n_1 * n_1 * 1.0 / Zp + n_2 * n_2 * 1.0 / Zs + n_3 * n_3 * 1.0 / Zs;
+    double P12 = n_1 * m_1 * 1.0 / Zp + n_2 * m_2 * 1.0 / Zs + n_3 * m_3 * 1.0 / Zs;
+    double P13 = n_1 * l_1 * 1.0 / Zp + n_2 * l_2 * 1.0 / Zs + n_3 * l_3 * 1.0 / Zs;
+    double P22 = m_1 * m_1 * 1.0 / Zp + m_2 * m_2 * 1.0 / Zs + m_3 * m_3 * 1.0 / Zs;
+    double P23 = m_1 * l_1 * 1.0 / Zp + m_2 * l_2 * 1.0 / Zs + m_3 * l_3 * 1.0 / Zs;
+    double P31 = l_1 * n_1 * 1.0 / Zp + l_2 * n_2 * 1.0 / Zs + l_3 * n_3 * 1.0 / Zs;
+    double P32 = l_1 * m_1 * 1.0 / Zp + l_2 * m_2 * 1.0 / Zs + l_3 * m_3 * 1.0 / Zs;
+    double P33 = l_1 * l_1 * 1.0 / Zp + l_2 * l_2 * 1.0 / Zs + l_3 * l_3 * 1.0 / Zs;
+    double bc[9];
+    if (me == 0) {
+        double p1h = 0.5 * (M11 * uH[0] + M12 * uH[1] + M13 * uH[2] + uH[3]);
+        double p2h = 0.5 * (M12 * uH[0] + M22 * uH[1] + M23 * uH[2] + uH[4]);
+        double p3h = 0.5 * (M31 * uH[0] + M32 * uH[1] + M33 * uH[2] + uH[5]);
+        double p1 = 0.5 * (M11 * uX[0] + M12 * uX[1] + M13 * uX[2] + uX[3]);
+        double p2 = 0.5 * (M12 * uX[0] + M22 * uX[1] + M23 * uX[2] + uX[4]);
+        double p3 = 0.5 * (M31 * uX[0] + M32 * uX[1] + M33 * uX[2] + uX[5]);
+        double e1 = P11 * (p1 - p1h) + P12 * (p2 - p2h) + P13 * (p3 - p3h);
+        double e2 = P12 * (p1 - p1h) + P22 * (p2 - p2h) + P23 * (p3 - p3h);
+        double e3 = P31 * (p1 - p1h) + P32 * (p2 - p2h) + P33 * (p3 - p3h);
+        bc[0] = norm / rho * (p1 - p1h);
+        bc[1] = norm / rho * (p2 - p2h);
+        bc[2] = norm / rho * (p3 - p3h);
+        bc[3] = (2.0 * mu + lam) * q_x * e1 + lam * q_y * e2 + lam * q_z * e3;
+        bc[4] = (2.0 * mu + lam) * q_y * e2 + lam * q_z * e3 + lam * q_x * e1;
+        bc[5] = (2.0 * mu + lam) * q_z * e3 + lam * q_y * e2 + lam * q_x * e1;
+        bc[6] = mu * (q_y * e1 + q_x * e2);
+        bc[7] = mu * (q_z * e1 + q_x * e3);
+        bc[8] = mu * (q_z * e2 + q_y * e3);
+    } else {
+        double q1h = 0.5 * (M11 * vH[0] + M12 * vH[1] + M13 * vH[2] - vH[3]);
+        double q2h = 0.5 * (M12 * vH[0] + M22 * vH[1] + M23 * vH[2] - vH[4]);
+        double q3h = 0.5 * (M31 * vH[0] + M32 * vH[1] + M33 * vH[2] - vH[5]);
+        double q1 = 0.5 * (M11 * vX[0] + M12 * vX[1] + M13 * vX[2] - vX[3]);
+        double q2 = 0.5 * (M12 * vX[0] + M22 * vX[1] + M23 * vX[2] - vX[4]);
+        double q3 = 0.5 * (M31 * vX[0] + M32 * vX[1] + M33 * vX[2] - vX[5]);
+        double e1 = P11 * (q1 - q1h) + P12 * (q2 - q2h) + P13 * (q3 - q3h);
+        double e2 = P12 * (q1 - q1h) + P22 * (q2 - q2h) + P23 * (q3 - q3h);
+        double e3 = P31 * (q1 - q1h) + P32 * (q2 - q2h) + P33 * (q3 - q3h);
+        bc[0] = norm / rho * (q1 - q1h);
+        bc[1] = norm / rho * (q2 - q2h);
+        bc[2] = norm / rho * (q3 - q3h);
+        bc[3] = -(2.0 * mu + lam) * q_x * e1 - lam * q_y * e2 - lam * q_z * e3;
+        bc[4] = -(2.0 * mu + lam) * q_y * e2 - lam * q_z * e3 - lam * q_x * e1;
+        bc[5] = -(2.0 * mu + lam) * q_z * e3 - lam * q_y * e2 - lam * q_x * e1;
+        bc[6] = -mu * (q_y * e1 + q_x * e2);
+        bc[7] = -mu * (q_z * e1 + q_x * e3);
+        bc[8] = -mu * (q_z * e2 + q_y * e3);
+    }
+    double hq = 1.0 / (double)(B.nq - 1);
+    for (int f = 0; f < 9; f++) sat[f] = sat[f] - tau0 / hq * bc[f];
+    return 0;
+}
+
+struct BoundaryArgs {
+    FaultParams fault;
+    Iface I[2];
+    long nodes[2];        // boundary nodes per block
+    double t, A[2], bdt, tau0;
+    int stage;
+    int rates_only;       // 1: RHS evaluation without the RK update (no F_out / S / W write)
+    int *flag;
+};
+
+template <typename real>
+__global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int me = 0;
+    if (t >= a.nodes[0]) { me = 1; t -= a.nodes[0]; }
+    if (t >= a.nodes[me]) return;
+    const Blk<real> &B = P.b[me];
+    const int nq = B.nq, nr = B.nr, ns = B.ns;
+    // unique enumeration: q-faces, then r-faces without q edges, then s-faces
+    int i, j, k;
+    long nqf = 2L * nr * ns, nrf = 2L * (nq - 2) * ns;
+    if (t < nqf) {
+        i = (t < (long)nr * ns) ? 0 : nq - 1;
+        long r = t % ((long)nr * ns);
+        j = (int)(r % nr); k = (int)(r / nr);
+    } else if (t < nqf + nrf) {
+        long r = t - nqf;
+        j = (r < (long)(nq - 2) * ns) ? 0 : nr - 1;
+        r = r % ((long)(nq - 2) * ns);
+        i = 1 + (int)(r % (nq - 2)); k = (int)(r / (nq - 2));
+    } else {
+        long r = t - nqf - nrf;
+        k = (r < (long)(nq - 2) * (nr - 2)) ? 0 : ns - 1;
+        r = r % ((long)(nq - 2) * (nr - 2));
+        i = 1 + (int)(r % (nq - 2)); j = 1 + (int)(r / (nq - 2));
+    }
+    const long NP = B.NP;
+    const long p = pidx(nq, nr, i, j, k);
+    double u[9];
+    for (int f = 0; f < 9; f++) u[f] = (double)B.Fin[f * NP + p];
+    double sat[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int pos[3] = {i, j, k};
+    const int nn[3] = {nq, nr, ns};
+    const double A = a.A[me];
+    for (int side = 0; side < 6; side++) {
+        int dir = side / 2;
+        int on = (side % 2 == 0) ? (pos[dir] == 0) : (pos[dir] == nn[dir] - 1);
+        if (!on || B.bc[side] <= 0) continue;
+        long fpi;
+        long FP;
+        if (dir == 0) { fpi = j + (long)nr * k; FP = (long)nr * ns; }
+        else if (dir == 1) { fpi = i + (long)nq * k; FP = (long)nq * ns; }
+        else { fpi = i + (long)nq * j; FP = (long)nq * nr; }
+        const double *fr = B.frame[side], *gm = B.fgm[side], *fm = B.fmat[side];
+        double n[3], m[3], l[3];
+        for (int c = 0; c < 3; c++) { n[c] = fr[c * FP + fpi]; m[c] = fr[(3 + c) * FP + fpi]; l[c] = fr[(6 + c) * FP + fpi]; }
+        double ax = gm[fpi], ay = gm[FP + fpi], az = gm[2 * FP + fpi];
+        double norm = sqrt(ax * ax + ay * ay + az * az);
+        int code = B.bc[side];
+        double r = code == 1 ? 0.0 : code == 2 ? 1.0 : -1.0;
+        double ubc[9], Ux[9];
+        bc_sat(u, fm[2 * FP + fpi], fm[FP + fpi], fm[fpi], l, m, n, norm, r, side % 2, ubc, Ux);
+        double h = 1.0 / (double)(nn[dir] - 1);
+        for (int f = 0; f < 9; f++) sat[f] = sat[f] - a.tau0 / h * Ux[f];
+        if (B.pml[side]) {  // RHS_Interior.f90:1574-1700: SAT on the PML auxiliary rates
+            int np = B.npml;
+            int lo[3] = {0, 0, 0}, hi[3] = {nq - 1, nr - 1, ns - 1};
+            if (side % 2 == 0) hi[dir] = np - 1; else lo[dir] = nn[dir] - np;
+            long n0 = hi[0] - lo[0] + 1, n1 = hi[1] - lo[1] + 1, n2 = hi[2] - lo[2] + 1;
+            long s = (long)(i - lo[0]) + n0 * ((long)(j - lo[1]) + n1 * (long)(k - lo[2]));
+            long SP = n0 * n1 * n2;
+            for (int f = 0; f < 9; f++) {
+                real dsat = (real)(-(a.tau0 / h) * ubc[f]);
+                real dq = B.DQ[side][f * SP + s];
+                B.DQ[side][f * SP + s] = dq + dsat;
+                if (!a.rates_only) B.Q[side][f * SP + s] = B.Q[side][f * SP + s] + (real)a.bdt * dsat;
+            }
+        }
+    }
+    // fault face (domain.f90:600-609): block 0's face q = nq-1, block 1's face q = 0
+    int on_fault = (me == 0) ? (i == nq - 1 && B.bc[1] == 0) : (i == 0 && B.bc[0] == 0);
+    if (on_fault) {
+        int rc = fault_node<real>(P, a.I[me], a.fault, me, j, k, a.t, a.stage, A, a.rates_only ? 0.0 : a.bdt,
+                                  sat, a.tau0);
+        if (rc) a.flag[0] = rc;
+    }
+    for (int f = 0; f < 9; f++) {
+        real df = B.DF[f * NP + p] + (real)sat[f];
+        B.DF[f * NP + p] = df;
+        if (!a.rates_only) B.Fout[f * NP + p] = B.Fin[f * NP + p] + (real)a.bdt * df;
+    }
+}
+
+#pragma clang fp contract(fast)
+// ----------------------------------------------------------------------------
+// plasticity (plastic.f90:11-151, 153-217, 221-290, 293-352); initial stress
+// initial_stress_condition.f90:8-120
+struct PlasticArgs {
+    int model;          // 1 plastic_flow2, 2 plastic_flow
+    double c, nu, Tv, mu, beta, eta, dt;
+    int s0_kind;
+    double s0[6];
+    int x0[2], x1[2], y0[2], y1[2], z0[2], z1[2];
+    long n[2];
+};
+
+template <typename real>
+__global__ void k_plastic(Pair<real> P, PlasticArgs a)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int me = 0;
+    if (t >= a.n[0]) { me = 1; t -= a.n[0]; }
+    if (t >= a.n[me]) return;
+    const Blk<real> &B = P.b[me];
+    int w = a.x1[me] - a.x0[me], h = a.y1[me] - a.y0[me];
+    int i = a.x0[me] + (int)(t % w), j = a.y0[me] + (int)((t / w) % h), k = a.z0[me] + (int)(t / ((long)w * h));
+    const long NP = B.NP;
+    const long p = pidx(B.nq, B.nr, i, j, k);
+    double s0[6];
+    if (a.s0_kind == 1) {
+        double y = B.Y[p];
+        double Omega = 0.0;
+        if (y <= 15.0) Omega = 1.0;
+        else if (y >= 15.0 && y <= 20.0) Omega = (20.0 - y) / 5.0;
+        double b11 = 0.926793, b33 = 1.073206, b13 = -0.169029;
+        double Pf = 9.8 * y, s22 = -2.67 * 9.8 * y;
+        s0[0] = Omega * (b33 * (s22 + Pf) - Pf) + (1.0 - Omega) * s22 + Pf;
+        s0[1] = s22 + Pf;
+        s0[2] = Omega * (b11 * (s22 + Pf) - Pf) + (1.0 - Omega) * s22 + Pf;
+        s0[3] = 0.0;
+        s0[4] = Omega * (b13 * (s22 + Pf));
+        s0[5] = 0.0;
+    } else {
+        for (int c = 0; c < 6; c++) s0[c] = a.s0[c];
+    }
+    double sa[6];
+    for (int c = 0; c < 6; c++) sa[c] = (double)B.Fout[(3 + c) * NP + p] + s0[c];
+    double sigma = (sa[0] + sa[1] + sa[2]) / 3.0;
+    double J2 = ((sa[0] - sa[1]) * (sa[0] - sa[1]) + (sa[1] - sa[2]) * (sa[1] - sa[2]) +
+                 (sa[2] - sa[0]) * (sa[2] - sa[0])) / 6.0 + sa[3] * sa[3] + sa[4] * sa[4] + sa[5] * sa[5];
+    double tau = sqrt(J2);
+    const double delta[6] = {1, 1, 1, 0, 0, 0};
+    if (a.model == 1) {
+        double Ys = fmax(0.0, a.c * cos(atan(a.nu)) - sigma * sin(atan(a.nu)));
+        if (tau - Ys <= 0.0) return;
+        double r = exp(-a.dt / a.Tv) + (1.0 - exp(-a.dt / a.Tv)) * Ys / tau;
+        for (int c = 0; c < 6; c++) sa[c] = sigma * delta[c] + r * (sa[c] - sigma * delta[c]);
+        // plastic strain accumulator (FP64): equivalent plastic strain increment
+        // dgamma = (1 - r) tau / G, G = shear modulus
+        if (B.P) {
+            double G = B.M ? (double)B.M[NP + p] : B.mu;
+            double dg = (1.0 - r) * tau / G;
+            B.P[p] = dg / a.dt;
+            B.P[NP + p] = B.P[NP + p] + dg;
+        }
+    } else {
+        double Y = tau + a.mu * sigma;
+        if (Y <= 0.0) { if (B.P) B.P[p] = 0.0; return; }
+        double G = B.M ? (double)B.M[NP + p] : B.mu;
+        double lam_e = B.M ? (double)B.M[p] : B.lam;
+        double K = lam_e + 2.0 / 3.0 * G;
+        double sd[6];
+        for (int c = 0; c < 6; c++) sd[c] = sa[c] - sigma * delta[c];
+        double lambda = (tau + a.mu * sigma) / (a.eta + a.dt * (G + a.mu * a.beta * K));
+        if (B.P) { B.P[p] = lambda; B.P[NP + p] = B.P[NP + p] + a.dt * lambda; }
+        double taun = tau - a.dt * lambda * G;
+        sigma = sigma - a.dt * lambda * a.beta * K;
+        for (int c = 0; c < 6; c++) sa[c] = sigma * delta[c] + sd[c] * taun / (taun + a.dt * lambda * G);
+    }
+    for (int c = 0; c < 6; c++) B.Fout[(3 + c) * NP + p] = (real)(sa[c] - s0[c]);
+}
+
+// rate-and-state initial state (Interface_Condition.f90:2722-2806)
+template <typename real>
+__global__ void k_init_state(Pair<real> P, FaultParams f, Iface I0, Iface I1)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int me = 0;
+    long n0 = (long)P.b[0].nr * P.b[0].ns;
+    if (t >= n0) { me = 1; t -= n0; }
+    const Blk<real> &B = P.b[me];
+    long FP = (long)B.nr * B.ns;
+    if (t >= FP) return;
+    int side = me == 0 ? 1 : 0;
+    const double *fr = B.frame[side];
+    double n[3], m[3], l[3], x[3];
+    for (int c = 0; c < 3; c++) {
+        n[c] = fr[c * FP + t]; m[c] = fr[(3 + c) * FP + t]; l[c] = fr[(6 + c) * FP + t];
+        x[c] = B.fx[c * FP + t];
+    }
+    double T0n, T0m, T0l;
+    prestress(f, x, 0.0, l, m, n, T0n, T0m, T0l);
+    double a = f.rs_a + f.rs_da * (1.0 - smooth_boxcar(x[1], f.rs_w, f.rs_wr) * smooth_boxcar(x[2], f.rs_w, f.rs_wr));
+    double sn = -T0n;
+    double theta = f.rs_L / f.rs_V0 *
+                   exp(((a * log(2.0 * sinh(T0l / (a * sn)))) - f.rs_f0 - a * log(f.rs_Vin / f.rs_V0)) / f.rs_b);
+    (me == 0 ? I0 : I1).W[t] = f.rs_f0 + f.rs_b * log(f.rs_V0 / f.rs_L * theta);
+}
+
+template <typename real>
+__global__ void k_fill(long n, real *__restrict__ a, real v)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) a[p] = v;
+}
+
+}  // namespace wql

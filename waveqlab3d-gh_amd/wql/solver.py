@@ -1,0 +1,210 @@
+"""Python host mirror of the reference's domain/time-stepping API over libwql.
+
+``Simulation`` plays the role of the reference's `domain_type` plus
+`time_step_RK` (domain.f90, time_step.f90): it builds the two blocks from a
+``Problem``, keeps the whole state on the device and advances it.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import native
+from .native import check, dp
+
+
+def _p(a):
+    return a.ctypes.data_as(dp)
+
+
+def fault_struct(f):
+    s = native.FaultParams()
+    s.law, s.stress_mode = f.law, f.stress_mode
+    s.T0[:] = f.T0
+    s.s0[:] = f.s0
+    s.npatch = len(f.patches)
+    for k, pt in enumerate(f.patches):
+        s.patch[k][:] = pt
+    s.nuc = f.nuc
+    s.nucp[:] = f.nucp
+    for k in ('fs', 'fd', 'Dc', 'bar_wy', 'bar_wz', 'bar_big', 'rs_a', 'rs_da', 'rs_w', 'rs_wr',
+              'rs_b', 'rs_L', 'rs_V0', 'rs_f0', 'rs_Vin'):
+        setattr(s, k, getattr(f, k))
+    return s
+
+
+def problem_desc(prob, precision='fp64', dt=None):
+    d = native.ProblemDesc()
+    d.precision = native.WQL_FP32 if precision == 'fp32' else native.WQL_FP64
+    d.dt = prob.dt() if dt is None else dt
+    d.t0 = 0.0
+    d.plastic = prob.plastic
+    for k in ('pl_c', 'pl_nu', 'pl_Tv', 'pl_mu', 'pl_beta', 'pl_eta', 's0_kind'):
+        setattr(d, k, getattr(prob, k))
+    d.fault = fault_struct(prob.fault)
+    return d
+
+
+def block_descs(prob, coords=None):
+    arr = (native.BlockDesc * 2)()
+    keep = []
+    for ib, b in enumerate(prob.blocks):
+        X = np.ascontiguousarray(coords[ib] if coords is not None else b.coords(), dtype=np.float64)
+        keep.append(X)
+        d = arr[ib]
+        d.nq, d.nr, d.ns = b.n
+        d.bc[:] = b.bc
+        d.pml[:] = [int(v) for v in b.pml]
+        d.npml = b.npml
+        d.cartesian = 1 if b.mesh == 'cartesian' else 0
+        d.X = _p(X)
+        if b.material is not None:
+            M = np.ascontiguousarray(b.material, dtype=np.float64)
+            keep.append(M)
+            d.M = _p(M)
+        d.rho_s_p[:] = b.rho_s_p
+    return arr, keep
+
+
+def estimate_bytes(prob, precision='fp64'):
+    """Device bytes a Simulation of ``prob`` would hold (admission control)."""
+    pd = problem_desc(prob, precision)
+    arr = (native.BlockDesc * 2)()
+    for ib, b in enumerate(prob.blocks):
+        d = arr[ib]
+        d.nq, d.nr, d.ns = b.n
+        d.pml[:] = [int(v) for v in b.pml]
+        d.npml = b.npml
+        d.cartesian = 1 if b.mesh == 'cartesian' else 0
+    return native.lib().wql_estimate_bytes(C.byref(pd), arr)
+
+
+class Simulation:
+    def __init__(self, prob, precision='fp64', tile=None, zchunk=None, coords=None, graph=False, dt=None):
+        L = native.lib()
+        self.prob = prob
+        self.precision = precision
+        self.ctx = C.c_void_p()
+        pd = problem_desc(prob, precision, dt)
+        arr, keep = block_descs(prob, coords)
+        check(L.wql_create(C.byref(pd), arr, C.byref(self.ctx)))
+        self.dt = pd.dt
+        if tile is not None or zchunk is not None:
+            check(L.wql_set_tile(self.ctx, 0 if tile is None else tile, 32 if zchunk is None else zchunk))
+        if graph:
+            check(L.wql_use_graph(self.ctx, 1))
+
+    def close(self):
+        if self.ctx:
+            native.lib().wql_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state -----------------------------------------------------------
+    def _shape(self, ib, nf):
+        nq, nr, ns = self.prob.blocks[ib].n
+        return (nf, ns, nr, nq)
+
+    def set_fields(self, ib, F):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        assert F.shape == self._shape(ib, 9)
+        check(native.lib().wql_set_fields(self.ctx, ib, _p(F)))
+
+    def fields(self, ib):
+        F = np.empty(self._shape(ib, 9))
+        check(native.lib().wql_get_fields(self.ctx, ib, _p(F)))
+        return F
+
+    def rates(self, ib):
+        F = np.empty(self._shape(ib, 9))
+        check(native.lib().wql_get_rates(self.ctx, ib, _p(F)))
+        return F
+
+    def metrics(self, ib):
+        G = np.empty(self._shape(ib, 10))
+        check(native.lib().wql_get_metrics(self.ctx, ib, _p(G)))
+        return G
+
+    def frame(self, ib, side):
+        nq, nr, ns = self.prob.blocks[ib].n
+        fp = [nr * ns, nr * ns, nq * ns, nq * ns, nq * nr, nq * nr][side]
+        a = np.empty((9, fp))
+        check(native.lib().wql_get_frame(self.ctx, ib, side, _p(a)))
+        return a
+
+    def slab_shape(self, ib, side):
+        b = self.prob.blocks[ib]
+        n = list(b.n)
+        n[side // 2] = b.npml
+        return (9, n[0] * n[1] * n[2])
+
+    def set_pml(self, ib, side, Q):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        check(native.lib().wql_set_pml(self.ctx, ib, side, _p(Q)))
+
+    def pml(self, ib, side):
+        Q = np.empty(self.slab_shape(ib, side))
+        check(native.lib().wql_get_pml(self.ctx, ib, side, _p(Q)))
+        return Q
+
+    def pml_rates(self, ib, side):
+        Q = np.empty(self.slab_shape(ib, side))
+        check(native.lib().wql_get_pml_rates(self.ctx, ib, side, _p(Q)))
+        return Q
+
+    def set_fault(self, ib, S=None, W=None):
+        L = native.lib()
+        Sp = _p(np.ascontiguousarray(S, dtype=np.float64)) if S is not None else None
+        Wp = _p(np.ascontiguousarray(W, dtype=np.float64)) if W is not None else None
+        keep = (S, W)
+        check(L.wql_set_fault(self.ctx, ib, Sp, Wp))
+
+    def fault(self, ib):
+        nq, nr, ns = self.prob.blocks[ib].n
+        out = dict(S=np.empty((4, ns, nr)), W=np.empty((1, ns, nr)), trup=np.empty((1, ns, nr)),
+                   Svel=np.empty((3, ns, nr)), Uhat=np.empty((6, ns, nr)), Vhat=np.empty((6, ns, nr)))
+        check(native.lib().wql_get_fault(self.ctx, ib, *(_p(out[k]) for k in
+                                                         ('S', 'W', 'trup', 'Svel', 'Uhat', 'Vhat'))))
+        return out
+
+    def fault_rates(self, ib):
+        nq, nr, ns = self.prob.blocks[ib].n
+        DS, DW = np.empty((4, ns, nr)), np.empty((1, ns, nr))
+        check(native.lib().wql_get_fault_rates(self.ctx, ib, _p(DS), _p(DW)))
+        return DS, DW
+
+    # -- stepping ----------------------------------------------------------
+    def init_state(self):
+        check(native.lib().wql_init_state(self.ctx))
+
+    def eval_rates(self, t, what=1, stage=2):
+        check(native.lib().wql_eval_rates(self.ctx, t, what, stage))
+
+    def step(self, n=1):
+        check(native.lib().wql_step(self.ctx, n))
+
+    def sync(self):
+        check(native.lib().wql_sync(self.ctx))
+
+    @property
+    def t(self):
+        return native.lib().wql_time(self.ctx)
+
+    def device_bytes(self):
+        return native.lib().wql_device_bytes(self.ctx)
+
+    def profile(self, on=True):
+        check(native.lib().wql_profile(self.ctx, 1 if on else 0))
+
+    def volume_kernel_ms(self):
+        return native.lib().wql_volume_kernel_ms(self.ctx)
