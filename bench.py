@@ -73,8 +73,9 @@ def run(W, precision, steps, warmup, tile, zchunk, graph):
     kms = sim.volume_kernel_ms()
     sim.profile(False)
     bytes_dev = sim.device_bytes()
+    shape = sim.tile
     sim.close()
-    return sec, kms, bytes_dev
+    return sec, kms, bytes_dev, shape
 
 
 def cpu_baseline(W):
@@ -103,8 +104,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='rupture')
     ap.add_argument('--scale', type=float, default=1.0)
-    ap.add_argument('--tile', type=int, default=0)
-    ap.add_argument('--zchunk', type=int, default=32)
+    ap.add_argument('--tile', type=int, default=None, help='main-kernel launch variant (default: tuned)')
+    ap.add_argument('--zchunk', type=int, default=None)
     ap.add_argument('--graph', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
@@ -119,8 +120,8 @@ def main():
     nodes = workloads.points(prob)
     res = {}
     for prec in (['fp32'] if a.no_fp64 else ['fp64', 'fp32']):
-        sec, kms, dev = run(W, prec, a.steps, a.warmup, a.tile, a.zchunk, a.graph)
-        res[prec] = dict(sec=sec, kms=kms, dev=dev, mpts=nodes * 5 * a.steps / sec / 1e6)
+        sec, kms, dev, shape = run(W, prec, a.steps, a.warmup, a.tile, a.zchunk, a.graph)
+        res[prec] = dict(sec=sec, kms=kms, dev=dev, shape=shape, mpts=nodes * 5 * a.steps / sec / 1e6)
     r32 = res['fp32']
     bpn = bytes_per_node(prob, 'fp32')
     kn = main_kernel_nodes(prob)
@@ -139,7 +140,7 @@ def main():
         'dtype': 'f32 volume / f64 fault',
         'data': 'synthetic (seeded geometry, topography and friction)',
         'config': {'workload': f'{a.workload}: 2 blocks x {"x".join(str(v) for v in prob.blocks[0].n)}',
-                   'nodes': nodes, 'tile_variant': a.tile, 'zchunk': a.zchunk},
+                   'nodes': nodes, 'tile_variant': r32['shape'][0], 'zchunk': r32['shape'][1]},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1) if achieved else None,
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved / HBM_PEAK_GBS, 4) if achieved else None, 'traffic': None,

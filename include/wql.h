@@ -134,6 +134,7 @@ double wql_time(const wql_ctx *ctx);
 /* launch shape of the volume kernel (autotuner hook): tile variant index and
  * z-chunk length; returns WQL_E_ARG for unknown variants */
 int wql_set_tile(wql_ctx *ctx, int variant, int zchunk);
+int wql_get_tile(const wql_ctx *ctx, int *variant, int *zchunk);
 int wql_tile_variants(void);
 
 /* capture one LSRK step into a HIP graph and replay it (1) or launch kernels

@@ -26,7 +26,7 @@ def _say(*a):
 
 
 @pytest.mark.parametrize('case', G.RHS_CASES)
-@pytest.mark.parametrize('tile', [0, 1, 2])
+@pytest.mark.parametrize('tile', list(range(24)))
 def test_rhs_vs_golden(case, tile):
     _say(f'[gpu] rhs {case} tile={tile}')
     z = G.load(case)
@@ -120,7 +120,7 @@ def test_rhs_vs_oracle_larger(n, mesh):
     for b in prob.blocks:
         b.n = n
     od, vol, _ = H.oracle_rhs(prob, 0.3)
-    for tile in range(3):
+    for tile in range(24):
         with _sim(prob, tile=tile, zchunk=7) as sim:
             H.load_pattern_state(sim, prob)
             sim.eval_rates(0.3, what=0)
@@ -130,3 +130,60 @@ def test_rhs_vs_oracle_larger(n, mesh):
             sim.eval_rates(0.3, what=1, stage=2)
             for ib in range(2):
                 assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= TOL, (tile, ib)
+
+
+@pytest.mark.parametrize('case', ['curv_rs', 'pml_locked', 'plastic_sw'])
+def test_graph_replays_match_eager(case):
+    """a captured step graph (device-side clock) gives the eager launches'
+    results bit for bit, over several replays with host work in between"""
+    _say(f'[gpu] graph {case}')
+    z = G.load(case)
+    prob = G.problem(case)
+    args = z['traj_args']
+    F0 = H.pulse_init(prob, args)
+    sims = [_sim(prob), _sim(prob, graph=True)]
+    try:
+        for sim in sims:
+            for ib in range(2):
+                sim.set_fields(ib, F0[ib])
+            sim.init_state()
+        for chunk in (1, 2, 1, 3):
+            for sim in sims:
+                sim.step(chunk)
+                sim.sync()
+            assert sims[0].t == sims[1].t
+            for ib in range(2):
+                assert np.array_equal(sims[0].fields(ib), sims[1].fields(ib)), f'F block {ib + 1}'
+                fa, fb = sims[0].fault(ib), sims[1].fault(ib)
+                for key in ('S', 'W', 'Svel', 'trup'):
+                    assert np.array_equal(fa[key], fb[key]), key
+    finally:
+        for sim in sims:
+            sim.close()
+
+
+@pytest.mark.parametrize('case', ['curv_sw', 'curv_rs', 'pml_locked'])
+def test_fp32_volume_tracks_fp64(case):
+    """FP32 volume fields with the FP64 fault stay close to the FP64 path:
+    relative L2 of the fields and of the slip <= 1e-5 over the trajectory"""
+    _say(f'[gpu] fp32 {case}')
+    z = G.load(case)
+    prob = G.problem(case)
+    args = z['traj_args']
+    F0 = H.pulse_init(prob, args)
+    out = {}
+    for prec in ('fp64', 'fp32'):
+        with _sim(prob, precision=prec) as sim:
+            for ib in range(2):
+                sim.set_fields(ib, F0[ib])
+            sim.init_state()
+            sim.step(int(args[0]))
+            sim.sync()
+            out[prec] = [(sim.fields(ib).astype(np.float64), sim.fault(ib)) for ib in range(2)]
+    for ib in range(2):
+        F64, f64 = out['fp64'][ib]
+        F32, f32 = out['fp32'][ib]
+        assert G.rel_l2(F32, F64) <= 1e-5, f'F block {ib + 1}'
+        slip64 = f64['S']
+        if np.abs(slip64).max() > 0:
+            assert G.rel_l2(f32['S'], slip64) <= 1e-5, 'slip'

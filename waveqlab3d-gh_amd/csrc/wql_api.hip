@@ -114,7 +114,10 @@ struct wql_ctx {
     wql_problem prob;
     double t, dt;
     int cur = 0;
-    int variant = 0, zchunk = 32;
+    // launch shape of the main volume kernel; the default is the fastest one
+    // measured on the 2 x 201x101x101 curvilinear problem (pipelined march,
+    // 32 x 8 tiles, 4-plane s chunks, q tiles fastest)
+    int variant = 4, zchunk = 4, order = 0;
     int use_graph = 0;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int *d_flag = nullptr;
@@ -177,8 +180,11 @@ void main_range(const DevBlock &d, int r[6])
     r[4] = std::min(pz0, d.ns); r[5] = std::max(d.ns - pz1, r[4]);
 }
 
-const int VARIANT_TX[3] = {32, 64, 16};
-const int VARIANT_TY[3] = {8, 8, 16};
+// variants 0-3: k_volume_main (s-march); 4-7: software-pipelined march
+// k_volume_pipe; 8-11: plane-parallel k_volume_plane
+const int NVARIANT = 12;
+const int VARIANT_TX[NVARIANT] = {32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32};
+const int VARIANT_TY[NVARIANT] = {8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16};
 
 TileList tile_list(const wql_ctx *c)
 {
@@ -195,16 +201,52 @@ TileList tile_list(const wql_ctx *c)
         g.xa0 = r[0]; g.xa1 = r[1]; g.ya0 = r[2]; g.ya1 = r[3]; g.za0 = r[4]; g.za1 = r[5];
         g.ntx = (r[1] - r[0] + TX - 1) / TX;
         g.nty = (r[3] - r[2] + TY - 1) / TY;
-        g.zc = std::max(1, c->zchunk);
+        g.zc = c->variant >= 8 ? 1 : std::max(1, c->zchunk);
         g.nzc = (r[5] - r[4] + g.zc - 1) / g.zc;
         g.first = first;
         first += (long)g.ntx * g.nty * g.nzc;
         L.n++;
     }
     L.total = first;
+    L.order = c->order;
     return L;
 }
 
+// s-band planes handled by the plane-parallel kernel (no PML on that side)
+TileList band_list(const wql_ctx *c)
+{
+    TileList L;
+    memset(&L, 0, sizeof L);
+    const int TX = 32, TY = 8;
+    long first = 0;
+    for (int ib = 0; ib < 2; ib++) {
+        const DevBlock &d = c->b[ib];
+        int r[6];
+        main_range(d, r);
+        if (r[1] <= r[0] || r[3] <= r[2]) continue;
+        int lo0 = d.pml[4] ? r[4] : 0, hi1 = d.pml[5] ? r[5] : d.ns;
+        int seg[2][2] = {{lo0, r[4]}, {r[5], hi1}};
+        for (int k = 0; k < 2; k++) {
+            if (seg[k][1] <= seg[k][0]) continue;
+            TileGeom &g = L.g[L.n++];
+            g.blk = ib;
+            g.xa0 = r[0]; g.xa1 = r[1]; g.ya0 = r[2]; g.ya1 = r[3];
+            g.za0 = seg[k][0]; g.za1 = seg[k][1];
+            g.ntx = (r[1] - r[0] + TX - 1) / TX;
+            g.nty = (r[3] - r[2] + TY - 1) / TY;
+            g.zc = 1;
+            g.nzc = g.za1 - g.za0;
+            g.first = first;
+            first += (long)g.ntx * g.nty * g.nzc;
+        }
+    }
+    L.total = first;
+    L.order = 1;
+    return L;
+}
+
+// nodes of the per-node generic kernel: the PML slabs (including their
+// boundary band rows)
 BoxList box_list(const wql_ctx *c)
 {
     BoxList L;
@@ -221,9 +263,9 @@ BoxList box_list(const wql_ctx *c)
         const DevBlock &d = c->b[ib];
         int r[6];
         main_range(d, r);
-        int z0 = std::min(r[4], d.ns), z1 = std::max(r[5], z0);
-        add(ib, 0, d.nq, 0, d.nr, 0, z0);          // s-low planes
-        add(ib, 0, d.nq, 0, d.nr, z1, d.ns);       // s-high planes
+        int z0 = d.pml[4] ? r[4] : 0, z1 = d.pml[5] ? r[5] : d.ns;
+        add(ib, 0, d.nq, 0, d.nr, 0, z0);          // s-low PML slab
+        add(ib, 0, d.nq, 0, d.nr, z1, d.ns);       // s-high PML slab
         add(ib, 0, r[0], 0, d.nr, z0, z1);         // q-low slab
         add(ib, r[1], d.nq, 0, d.nr, z0, z1);      // q-high slab
         add(ib, r[0], r[1], 0, r[2], z0, z1);      // r-low slab
@@ -240,6 +282,20 @@ void launch_main_v(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0,
     dim3 grd((unsigned)(((TL.total + 7) / 8) * 8));
     bool curv = !c->b[0].cart || !c->b[1].cart;
     bool het = c->b[0].M || c->b[1].M;
+    if (c->variant >= 8) {
+        if (curv && het) k_volume_plane<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (curv) k_volume_plane<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (het) k_volume_plane<real, TX, TY, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else k_volume_plane<real, TX, TY, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        return;
+    }
+    if (c->variant >= 4) {
+        if (curv && het) k_volume_pipe<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (curv) k_volume_pipe<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (het) k_volume_pipe<real, TX, TY, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else k_volume_pipe<real, TX, TY, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        return;
+    }
     if (curv && het) k_volume_main<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
     else if (curv) k_volume_main<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
     else if (het) k_volume_main<real, TX, TY, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
@@ -247,8 +303,34 @@ void launch_main_v(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0,
 }
 
 template <typename real>
+void launch_main(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0, real A1, real bdt)
+{
+    switch (c->variant % 4) {
+    case 1: launch_main_v<real, 64, 8>(c, P, TL, A0, A1, bdt); break;
+    case 2: launch_main_v<real, 16, 16>(c, P, TL, A0, A1, bdt); break;
+    case 3: launch_main_v<real, 32, 16>(c, P, TL, A0, A1, bdt); break;
+    default: launch_main_v<real, 32, 8>(c, P, TL, A0, A1, bdt); break;
+    }
+}
+
+template <typename real>
+void launch_band(const wql_ctx *c, Pair<real> &P, real A, real bdt)
+{
+    TileList TB = band_list(c);
+    if (TB.total <= 0) return;
+    dim3 blk(32, 8);
+    dim3 grd((unsigned)(((TB.total + 7) / 8) * 8));
+    bool curv = !c->b[0].cart || !c->b[1].cart;
+    bool het = c->b[0].M || c->b[1].M;
+    if (curv && het) k_volume_plane<real, 32, 8, true, true><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+    else if (curv) k_volume_plane<real, 32, 8, true, false><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+    else if (het) k_volume_plane<real, 32, 8, false, true><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+    else k_volume_plane<real, 32, 8, false, false><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+}
+
+template <typename real>
 void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int stage, int rates_only,
-                  int do_plastic)
+                  int do_plastic, const double *tdev = nullptr, double tinc = 0.0)
 {
     Pair<real> P;
     P.b[0] = blk_view<real>(c, 0, cur);
@@ -257,11 +339,7 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
     TileList TL = tile_list(c);
     if (TL.total > 0) {
         if (c->profile && !c->use_graph) hipEventRecord(c->ev0, c->stream);
-        switch (c->variant) {
-        case 1: launch_main_v<real, 64, 8>(c, P, TL, (real)A, (real)A, (real)bdt); break;
-        case 2: launch_main_v<real, 16, 16>(c, P, TL, (real)A, (real)A, (real)bdt); break;
-        default: launch_main_v<real, 32, 8>(c, P, TL, (real)A, (real)A, (real)bdt); break;
-        }
+        launch_main<real>(c, P, TL, (real)A, (real)A, (real)bdt);
         if (c->profile && !c->use_graph) {
             hipEventRecord(c->ev1, c->stream);
             hipEventSynchronize(c->ev1);
@@ -271,6 +349,7 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
             c->vol_launches++;
         }
     }
+    launch_band<real>(c, P, (real)A, (real)bdt);
     BoxList BL = box_list(c);
     if (BL.total > 0) {
         unsigned g = (unsigned)((BL.total + 255) / 256);
@@ -286,6 +365,8 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
         a.nodes[ib] = 2L * d.nr * d.ns + 2L * (d.nq - 2) * d.ns + 2L * (d.nq - 2) * (d.nr - 2);
     }
     a.t = t_stage;
+    a.tdev = tdev;
+    a.tinc = tinc;
     a.A[0] = a.A[1] = A;
     a.bdt = bdt;
     a.tau0 = TAU0;
@@ -328,6 +409,28 @@ int enqueue_step(wql_ctx *c, int cur, double t0)
         cur = 1 - cur;
     }
     return 0;
+}
+
+// the step as a graph body: stage times read from c->d_t, which the last node
+// advances by one step
+template <typename real>
+int enqueue_step_graph(wql_ctx *c, int cur)
+{
+    for (int s = 0; s < 5; s++) {
+        launch_stage<real>(c, cur, RK_A[s], RK_B[s] * c->dt, 0.0, s + 1, 0, s == 4, c->d_t, RK_C[s] * c->dt);
+        cur = 1 - cur;
+    }
+    k_advance_time<<<1, 64, 0, c->stream>>>(c->d_t, RK_C[5] * c->dt);
+    return 0;
+}
+
+void drop_graphs(wql_ctx *c)
+{
+    for (int k = 0; k < 2; k++)
+        if (c->gexec[k]) {
+            hipGraphExecDestroy(c->gexec[k]);
+            c->gexec[k] = nullptr;
+        }
 }
 
 int check_launch()
@@ -511,7 +614,7 @@ extern "C" {
 
 const char *wql_last_error(void) { return g_err.c_str(); }
 int wql_abi_version(void) { return WQL_ABI_VERSION; }
-int wql_tile_variants(void) { return 3; }
+int wql_tile_variants(void) { return 2 * NVARIANT; }
 
 size_t wql_estimate_bytes(const wql_problem *prob, const wql_block_desc blocks[2])
 {
@@ -755,23 +858,13 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
         TileList TL = tile_list(c);
         BoxList BL = box_list(c);
         if (c->prec == WQL_FP64) {
-            if (TL.total > 0) {
-                switch (c->variant) {
-                case 1: launch_main_v<double, 64, 8>(c, Pd, TL, 0.0, 0.0, 0.0); break;
-                case 2: launch_main_v<double, 16, 16>(c, Pd, TL, 0.0, 0.0, 0.0); break;
-                default: launch_main_v<double, 32, 8>(c, Pd, TL, 0.0, 0.0, 0.0); break;
-                }
-            }
+            if (TL.total > 0) launch_main<double>(c, Pd, TL, 0.0, 0.0, 0.0);
+            launch_band<double>(c, Pd, 0.0, 0.0);
             if (BL.total > 0)
                 k_volume_generic<double><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(Pd, BL, 0.0, 0.0, 0.0);
         } else {
-            if (TL.total > 0) {
-                switch (c->variant) {
-                case 1: launch_main_v<float, 64, 8>(c, Pf, TL, 0.f, 0.f, 0.f); break;
-                case 2: launch_main_v<float, 16, 16>(c, Pf, TL, 0.f, 0.f, 0.f); break;
-                default: launch_main_v<float, 32, 8>(c, Pf, TL, 0.f, 0.f, 0.f); break;
-                }
-            }
+            if (TL.total > 0) launch_main<float>(c, Pf, TL, 0.f, 0.f, 0.f);
+            launch_band<float>(c, Pf, 0.f, 0.f);
             if (BL.total > 0)
                 k_volume_generic<float><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(Pf, BL, 0.f, 0.f, 0.f);
         }
@@ -786,24 +879,30 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
 int wql_step(wql_ctx *c, int nsteps)
 {
     if (!c || nsteps < 0) return fail(WQL_E_ARG, "bad argument");
-    for (int n = 0; n < nsteps; n++) {
-        if (c->use_graph) {
-            // step graphs depend on the ping-pong parity and on t (baked in);
-            // rebuild when t changed since capture
-            hipGraph_t graph;
+    const bool graph = c->use_graph && !c->profile;
+    if (graph && nsteps > 0) {
+        // the device clock starts from the host clock; one graph per
+        // ping-pong parity, captured on first use
+        k_set_time<<<1, 64, 0, c->stream>>>(c->d_t, c->t);
+        for (int k = 0; k < 2; k++) {
+            if (c->gexec[k]) continue;
+            hipGraph_t g;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            if (c->prec == WQL_FP64) enqueue_step<double>(c, c->cur, c->t);
-            else enqueue_step<float>(c, c->cur, c->t);
-            HIPCHK(hipStreamEndCapture(c->stream, &graph));
-            hipGraphExec_t ex;
-            HIPCHK(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
-            HIPCHK(hipGraphLaunch(ex, c->stream));
-            hipGraphExecDestroy(ex);
-            hipGraphDestroy(graph);
-        } else {
-            if (c->prec == WQL_FP64) enqueue_step<double>(c, c->cur, c->t);
-            else enqueue_step<float>(c, c->cur, c->t);
+            if (c->prec == WQL_FP64) enqueue_step_graph<double>(c, k);
+            else enqueue_step_graph<float>(c, k);
+            HIPCHK(hipStreamEndCapture(c->stream, &g));
+            hipError_t e = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                c->gexec[k] = nullptr;
+                return fail(WQL_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+            }
         }
+    }
+    for (int n = 0; n < nsteps; n++) {
+        if (graph) HIPCHK(hipGraphLaunch(c->gexec[c->cur], c->stream));
+        else if (c->prec == WQL_FP64) enqueue_step<double>(c, c->cur, c->t);
+        else enqueue_step<float>(c, c->cur, c->t);
         c->cur = 1 - c->cur;  // five stages: odd number of buffer swaps
         c->t = c->t + RK_C[5] * c->dt;
         if (int rc = check_launch()) return rc;
@@ -825,9 +924,20 @@ int wql_sync(wql_ctx *c)
 
 int wql_set_tile(wql_ctx *c, int variant, int zchunk)
 {
-    if (!c || variant < 0 || variant > 2 || zchunk < 1) return fail(WQL_E_ARG, "bad tile variant");
+    if (!c || variant < 0 || variant >= 2 * NVARIANT || zchunk < 1) return fail(WQL_E_ARG, "bad tile variant");
+    drop_graphs(c);
+    c->order = variant >= NVARIANT ? 1 : 0;
+    variant %= NVARIANT;
     c->variant = variant;
     c->zchunk = zchunk;
+    return 0;
+}
+
+int wql_get_tile(const wql_ctx *c, int *variant, int *zchunk)
+{
+    if (!c || !variant || !zchunk) return fail(WQL_E_ARG, "null argument");
+    *variant = c->variant + (c->order ? NVARIANT : 0);
+    *zchunk = c->zchunk;
     return 0;
 }
 
@@ -835,6 +945,7 @@ int wql_use_graph(wql_ctx *c, int on)
 {
     if (!c) return fail(WQL_E_ARG, "null context");
     c->use_graph = on ? 1 : 0;
+    if (!on) drop_graphs(c);
     return 0;
 }
 

@@ -321,9 +321,25 @@ struct TileGeom {
 };
 struct TileList {
     int n;
-    TileGeom g[2];
+    TileGeom g[4];
     long total;
+    int order;  // 0: q-tiles fastest, then r-tiles, then s; 1: s fastest
 };
+
+// logical tile id -> (q tile, r tile, s chunk or plane)
+__device__ __forceinline__ void tile_coords(const TileList &TL, const TileGeom &T, long lid, int nz, int &tx, int &ty,
+                                            int &tz)
+{
+    if (TL.order == 0) {
+        tx = (int)(lid % T.ntx);
+        ty = (int)((lid / T.ntx) % T.nty);
+        tz = (int)(lid / ((long)T.ntx * T.nty));
+    } else {
+        tz = (int)(lid % nz);
+        tx = (int)((lid / nz) % T.ntx);
+        ty = (int)(lid / ((long)nz * T.ntx));
+    }
+}
 
 template <int T>
 __device__ __forceinline__ void tile_range(int t, int nt, int a0, int a1, int nfull, int &start, int &c0, int &c1)
@@ -417,14 +433,15 @@ __global__ void __launch_bounds__(TX * TY) k_volume_main(Pair<real> P, TileList 
     const long per = (nwg + 7) / 8;
     const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
     if (logical >= TL.total) return;
-    const int gi = (TL.n > 1 && logical >= TL.g[1].first) ? 1 : 0;
+    int gi = 0;
+    for (int k = 1; k < TL.n; k++)
+        if (logical >= TL.g[k].first) gi = k;
     const TileGeom &T = TL.g[gi];
     const Blk<real> &B = P.b[T.blk];
     const real A = T.blk == 0 ? A0 : A1;
     long lid = logical - T.first;
-    const int tx_i = (int)(lid % T.ntx);
-    const int ty_i = (int)((lid / T.ntx) % T.nty);
-    const int zc_i = (int)(lid / ((long)T.ntx * T.nty));
+    int tx_i, ty_i, zc_i;
+    tile_coords(TL, T, lid, T.nzc, tx_i, ty_i, zc_i);
 
     int xs, xc0, xc1, ys, yc0, yc1;
     tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
@@ -554,6 +571,440 @@ __global__ void __launch_bounds__(TX * TY) k_volume_main(Pair<real> P, TileList 
             real fin = (f < 3) ? sv[f][ly + 3][lx + 3] : B.Fin[f * NP + p];
             B.Fout[f * NP + p] = fin + bdt * df;
         }
+    }
+}
+
+// Software-pipelined variant of the main kernel.  Each thread owns its
+// centre node and at most one halo node of the plane, so the loads of plane
+// z+1 (centre, halo, s-queue lead z+4, DF) are all issued before the
+// arithmetic of plane z and are in flight while it runs.  LDS is double
+// buffered: one barrier per plane.
+template <typename real, bool CURV>
+struct PlanePrefetch {
+    real v[3], s[6], g[10];   // centre node: velocity, stress, metrics + J
+    real hv[3], hs[6], hg[4]; // halo node: velocity, stress, J + axis metrics
+    real lv[3], ls[6], lg[4]; // s-queue lead (plane z+3): velocity, stress, J + s metrics
+    real df[9];               // centre rates
+};
+
+template <typename real, int TX, int TY, bool CURV, bool HET>
+__global__ void __launch_bounds__(TX * TY) k_volume_pipe(Pair<real> P, TileList TL, real A0, real A1, real bdt)
+{
+    constexpr int WX = TX + 6, WY = TY + 6;
+    __shared__ real sv[2][3][WY][WX];
+    __shared__ real sfq[2][3][TY][WX];
+    __shared__ real sfr[2][3][WY][TX];
+
+    const long nwg = (long)gridDim.x;
+    const long per = (nwg + 7) / 8;
+    const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
+    if (logical >= TL.total) return;
+    int gi = 0;
+    for (int k = 1; k < TL.n; k++)
+        if (logical >= TL.g[k].first) gi = k;
+    const TileGeom &T = TL.g[gi];
+    const Blk<real> &B = P.b[T.blk];
+    const real A = T.blk == 0 ? A0 : A1;
+    long lid = logical - T.first;
+    int tx_i, ty_i, zc_i;
+    tile_coords(TL, T, lid, T.nzc, tx_i, ty_i, zc_i);
+
+    int xs, xc0, xc1, ys, yc0, yc1;
+    tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
+    tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
+    const int z0 = T.za0 + zc_i * T.zc;
+    const int z1 = min(z0 + T.zc, T.za1);
+
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int gx = xs + lx, gy = ys + ly;
+    const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
+    const bool inb = (gx < B.nq && gy < B.nr);
+    const long NP = B.NP;
+    const long plane = (long)B.nq * B.nr;
+    const long pcol = (long)gx + (long)B.nq * gy;
+    const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
+
+    // this thread's halo node (x halo: 6*TY nodes, then y halo: 6*TX nodes)
+    const int tid = ly * TX + lx;
+    int hk = 0, hcx = 0, hcy = 0;
+    long hcol = 0;
+    if (tid < 6 * TY) {
+        int hx = tid % 6, hy = tid / 6;
+        hcx = hx < 3 ? hx : TX + hx;
+        hcy = hy;
+        int ggx = xs - 3 + hcx, ggy = ys + hy;
+        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcol = (long)ggx + (long)B.nq * ggy; }
+    } else if (tid < 6 * TY + 6 * TX) {
+        int t2 = tid - 6 * TY;
+        int hx = t2 % TX, hy = t2 / TX;
+        hcy = hy < 3 ? hy : TY + hy;
+        hcx = hx;
+        int ggx = xs + hx, ggy = ys - 3 + hcy;
+        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcol = (long)ggx + (long)B.nq * ggy; }
+    }
+    const int hax = hk == 2 ? 1 : 0;  // metric axis of the halo flux
+
+    auto gload = [&](long p, int c) -> real { return CURV ? B.G[(long)c * NP + p] : (real)B.gc[c]; };
+
+    auto prefetch = [&](PlanePrefetch<real, CURV> &R, int z) {
+        if (inb) {
+            long p = pcol + plane * z;
+#pragma unroll
+            for (int f = 0; f < 3; f++) R.v[f] = B.Fin[f * NP + p];
+#pragma unroll
+            for (int f = 0; f < 6; f++) R.s[f] = B.Fin[(3 + f) * NP + p];
+            if (CURV) {
+#pragma unroll
+                for (int c = 0; c < 10; c++) R.g[c] = B.G[(long)c * NP + p];
+            }
+            long pl = pcol + plane * (z + 3);
+#pragma unroll
+            for (int f = 0; f < 3; f++) R.lv[f] = B.Fin[f * NP + pl];
+#pragma unroll
+            for (int f = 0; f < 6; f++) R.ls[f] = B.Fin[(3 + f) * NP + pl];
+            if (CURV) {
+                R.lg[0] = B.G[9 * NP + pl];
+                R.lg[1] = B.G[2 * NP + pl];
+                R.lg[2] = B.G[5 * NP + pl];
+                R.lg[3] = B.G[8 * NP + pl];
+            }
+        }
+        if (own) {
+            long p = pcol + plane * z;
+#pragma unroll
+            for (int f = 0; f < 9; f++) R.df[f] = B.DF[f * NP + p];
+        }
+        if (hk) {
+            long p = hcol + plane * z;
+#pragma unroll
+            for (int f = 0; f < 3; f++) R.hv[f] = B.Fin[f * NP + p];
+#pragma unroll
+            for (int f = 0; f < 6; f++) R.hs[f] = B.Fin[(3 + f) * NP + p];
+            if (CURV) {
+                R.hg[0] = B.G[9 * NP + p];
+                R.hg[1] = B.G[(0 + hax) * NP + p];
+                R.hg[2] = B.G[(3 + hax) * NP + p];
+                R.hg[3] = B.G[(6 + hax) * NP + p];
+            }
+        }
+    };
+    auto fluxv = [&](const real sg[6], real J, real ax, real ay, real az, real fl[3]) {
+        real Jx = J * ax, Jy = J * ay, Jz = J * az;
+        fl[0] = Jx * sg[0] + Jy * sg[3] + Jz * sg[4];
+        fl[1] = Jx * sg[3] + Jy * sg[1] + Jz * sg[5];
+        fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
+    };
+
+    real qv[7][3], qf[7][3];
+#pragma unroll
+    for (int a = 0; a < 7; a++)
+#pragma unroll
+        for (int f = 0; f < 3; f++) { qv[a][f] = 0; qf[a][f] = 0; }
+    // warm the s-queue with planes z0-3 .. z0+2
+    if (inb) {
+#pragma unroll
+        for (int a = 1; a < 7; a++) {
+            long p = pcol + plane * (z0 - 4 + a);
+            real sg[6];
+#pragma unroll
+            for (int f = 0; f < 3; f++) qv[a][f] = B.Fin[f * NP + p];
+#pragma unroll
+            for (int f = 0; f < 6; f++) sg[f] = B.Fin[(3 + f) * NP + p];
+            fluxv(sg, gload(p, 9), gload(p, 2), gload(p, 5), gload(p, 8), qf[a]);
+        }
+    }
+
+    PlanePrefetch<real, CURV> cur, nxt;
+    prefetch(cur, z0);
+    for (int z = z0; z < z1; z++) {
+        const int b = z & 1;
+        // (1) publish plane z: centre and halo into LDS buffer b, lead into the queue
+        real cv[3], cs[6], cg[10], cdf[9];
+        if (inb) {
+            real J, g9[9];
+            if (CURV) {
+#pragma unroll
+                for (int c = 0; c < 10; c++) cg[c] = cur.g[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 10; c++) cg[c] = (real)B.gc[c];
+            }
+            J = cg[9];
+            (void)g9;
+            real fq[3], fr[3];
+            fluxv(cur.s, J, cg[0], cg[3], cg[6], fq);
+            fluxv(cur.s, J, cg[1], cg[4], cg[7], fr);
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                sv[b][f][ly + 3][lx + 3] = cur.v[f];
+                sfq[b][f][ly][lx + 3] = fq[f];
+                sfr[b][f][ly + 3][lx] = fr[f];
+                cv[f] = cur.v[f];
+            }
+#pragma unroll
+            for (int f = 0; f < 6; f++) cs[f] = cur.s[f];
+#pragma unroll
+            for (int a = 0; a < 6; a++)
+#pragma unroll
+                for (int f = 0; f < 3; f++) { qv[a][f] = qv[a + 1][f]; qf[a][f] = qf[a + 1][f]; }
+            real lJ, lx_, ly_, lz_;
+            if (CURV) { lJ = cur.lg[0]; lx_ = cur.lg[1]; ly_ = cur.lg[2]; lz_ = cur.lg[3]; }
+            else { lJ = (real)B.gc[9]; lx_ = (real)B.gc[2]; ly_ = (real)B.gc[5]; lz_ = (real)B.gc[8]; }
+#pragma unroll
+            for (int f = 0; f < 3; f++) qv[6][f] = cur.lv[f];
+            fluxv(cur.ls, lJ, lx_, ly_, lz_, qf[6]);
+        }
+        if (own) {
+#pragma unroll
+            for (int f = 0; f < 9; f++) cdf[f] = cur.df[f];
+        }
+        if (hk) {
+            real hJ, hx_, hy_, hz_;
+            if (CURV) { hJ = cur.hg[0]; hx_ = cur.hg[1]; hy_ = cur.hg[2]; hz_ = cur.hg[3]; }
+            else { hJ = (real)B.gc[9]; hx_ = (real)B.gc[0 + hax]; hy_ = (real)B.gc[3 + hax]; hz_ = (real)B.gc[6 + hax]; }
+            real fl[3];
+            fluxv(cur.hs, hJ, hx_, hy_, hz_, fl);
+            if (hk == 1) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) { sv[b][f][hcy + 3][hcx] = cur.hv[f]; sfq[b][f][hcy][hcx] = fl[f]; }
+            } else {
+#pragma unroll
+                for (int f = 0; f < 3; f++) { sv[b][f][hcy][hcx + 3] = cur.hv[f]; sfr[b][f][hcy][hcx] = fl[f]; }
+            }
+        }
+        __syncthreads();
+        // (2) loads of plane z+1 go out now and overlap the arithmetic below
+        if (z + 1 < z1) prefetch(nxt, z + 1);
+        // (3) plane z
+        if (own) {
+            const long p = pcol + plane * z;
+            real dvq[3], dvr[3], dvs[3], div[3];
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                dvq[f] = d1_row<real>(&sv[b][f][ly + 3][0], lx + 3, gx, B.nq, ihq);
+                dvr[f] = d1_col<real>(&sv[b][f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
+                dvs[f] = ihs * ((real)0.75 * (qv[4][f] - qv[2][f]) + (real)(-0.15) * (qv[5][f] - qv[1][f]) +
+                                (real)(1.0 / 60.0) * (qv[6][f] - qv[0][f]));
+                real a = d1_row<real>(&sfq[b][f][ly][0], lx + 3, gx, B.nq, ihq);
+                real bb = d1_col<real>(&sfr[b][f][0][lx], TX, ly + 3, gy, B.nr, ihr);
+                real c = ihs * ((real)0.75 * (qf[4][f] - qf[2][f]) + (real)(-0.15) * (qf[5][f] - qf[1][f]) +
+                                (real)(1.0 / 60.0) * (qf[6][f] - qf[0][f]));
+                div[f] = a + bb + c;
+            }
+            real lam, mu, rho;
+            if (HET) { lam = B.M[p]; mu = B.M[NP + p]; rho = B.M[2 * NP + p]; }
+            else { lam = (real)B.lam; mu = (real)B.mu; rho = (real)B.rho; }
+            real Ux[3], Uy[3], Uz[3];
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                Ux[f] = cg[0] * dvq[f] + cg[1] * dvr[f] + cg[2] * dvs[f];
+                Uy[f] = cg[3] * dvq[f] + cg[4] * dvr[f] + cg[5] * dvs[f];
+                Uz[f] = cg[6] * dvq[f] + cg[7] * dvr[f] + cg[8] * dvs[f];
+            }
+            real rhoJ_inv = (real)1 / (rho * cg[9]);
+            real l2m = lam + (real)2 * mu;
+            real DU[NF];
+            DU[0] = div[0] * rhoJ_inv;
+            DU[1] = div[1] * rhoJ_inv;
+            DU[2] = div[2] * rhoJ_inv;
+            DU[3] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
+            DU[4] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
+            DU[5] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
+            DU[6] = mu * (Uy[0] + Ux[1]);
+            DU[7] = mu * (Uz[0] + Ux[2]);
+            DU[8] = mu * (Uz[1] + Uy[2]);
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                real df = A * cdf[f] + DU[f];
+                B.DF[f * NP + p] = df;
+                real fin = f < 3 ? cv[f] : cs[f - 3];
+                B.Fout[f * NP + p] = fin + bdt * df;
+            }
+        }
+        cur = nxt;
+    }
+}
+
+// Plane-parallel variant: one thread per node and one workgroup per
+// (tile, s-plane).  The q/r stencils read an LDS tile (3-node halo, boundary
+// closure rows included), the s stencil reads the six s-neighbours straight
+// from memory; those loads hit L2 because the workgroups of consecutive
+// planes of a tile column are dispatched together to one XCD.  No loop-carried
+// state, so occupancy and memory-level parallelism are high.
+template <typename real, int TX, int TY, bool CURV, bool HET>
+__global__ void __launch_bounds__(TX * TY) k_volume_plane(Pair<real> P, TileList TL, real A0, real A1, real bdt)
+{
+    constexpr int WX = TX + 6, WY = TY + 6;
+    __shared__ real sv[3][WY][WX];
+    __shared__ real sfq[3][TY][WX];
+    __shared__ real sfr[3][WY][TX];
+
+    const long nwg = (long)gridDim.x;
+    const long per = (nwg + 7) / 8;
+    const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
+    if (logical >= TL.total) return;
+    int gi = 0;
+    for (int k = 1; k < TL.n; k++)
+        if (logical >= TL.g[k].first) gi = k;
+    const TileGeom &T = TL.g[gi];
+    const Blk<real> &B = P.b[T.blk];
+    const real A = T.blk == 0 ? A0 : A1;
+    long lid = logical - T.first;
+    const int nzp = T.za1 - T.za0;
+    int tx_i, ty_i, zl;
+    tile_coords(TL, T, lid, nzp, tx_i, ty_i, zl);
+    const int z = T.za0 + zl;
+
+    int xs, xc0, xc1, ys, yc0, yc1;
+    tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
+    tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
+
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int gx = xs + lx, gy = ys + ly;
+    const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
+    const bool inb = (gx < B.nq && gy < B.nr);
+    const long NP = B.NP;
+    const long plane = (long)B.nq * B.nr;
+    const long p = (long)gx + (long)B.nq * gy + plane * z;
+    const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
+
+    auto fluxv = [&](const real sg[6], real J, real ax, real ay, real az, real fl[3]) {
+        real Jx = J * ax, Jy = J * ay, Jz = J * az;
+        fl[0] = Jx * sg[0] + Jy * sg[3] + Jz * sg[4];
+        fl[1] = Jx * sg[3] + Jy * sg[1] + Jz * sg[5];
+        fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
+    };
+    auto gl = [&](long q, int c) -> real { return CURV ? B.G[(long)c * NP + q] : (real)B.gc[c]; };
+
+    // halo node of this thread (x halo 6*TY nodes, then y halo 6*TX nodes)
+    const int tid = ly * TX + lx;
+    int hk = 0, hcx = 0, hcy = 0;
+    long hp = 0;
+    if (tid < 6 * TY) {
+        int hx = tid % 6, hy = tid / 6;
+        hcx = hx < 3 ? hx : TX + hx;
+        hcy = hy;
+        int ggx = xs - 3 + hcx, ggy = ys + hy;
+        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hp = (long)ggx + (long)B.nq * ggy + plane * z; }
+    } else if (tid < 6 * TY + 6 * TX) {
+        int t2 = tid - 6 * TY;
+        int hx = t2 % TX, hy = t2 / TX;
+        hcy = hy < 3 ? hy : TY + hy;
+        hcx = hx;
+        int ggx = xs + hx, ggy = ys - 3 + hcy;
+        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hp = (long)ggx + (long)B.nq * ggy + plane * z; }
+    }
+
+    real cv[3], cs[6], cg[10];
+    if (inb) {
+#pragma unroll
+        for (int f = 0; f < 3; f++) cv[f] = B.Fin[f * NP + p];
+#pragma unroll
+        for (int f = 0; f < 6; f++) cs[f] = B.Fin[(3 + f) * NP + p];
+#pragma unroll
+        for (int c = 0; c < 10; c++) cg[c] = gl(p, c);
+        real fq[3], fr[3];
+        fluxv(cs, cg[9], cg[0], cg[3], cg[6], fq);
+        fluxv(cs, cg[9], cg[1], cg[4], cg[7], fr);
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            sv[f][ly + 3][lx + 3] = cv[f];
+            sfq[f][ly][lx + 3] = fq[f];
+            sfr[f][ly + 3][lx] = fr[f];
+        }
+    }
+    if (hk) {
+        const int ax = hk == 2 ? 1 : 0;
+        real hv[3], hs[6], fl[3];
+#pragma unroll
+        for (int f = 0; f < 3; f++) hv[f] = B.Fin[f * NP + hp];
+#pragma unroll
+        for (int f = 0; f < 6; f++) hs[f] = B.Fin[(3 + f) * NP + hp];
+        fluxv(hs, gl(hp, 9), gl(hp, 0 + ax), gl(hp, 3 + ax), gl(hp, 6 + ax), fl);
+        if (hk == 1) {
+#pragma unroll
+            for (int f = 0; f < 3; f++) { sv[f][hcy + 3][hcx] = hv[f]; sfq[f][hcy][hcx] = fl[f]; }
+        } else {
+#pragma unroll
+            for (int f = 0; f < 3; f++) { sv[f][hcy][hcx + 3] = hv[f]; sfr[f][hcy][hcx] = fl[f]; }
+        }
+    }
+    // s derivatives from the s-neighbours read straight from memory: central
+    // rows as +-k pairs, boundary closure rows (planes within six nodes of an
+    // s face) through the closure table
+    real dvs[3] = {0, 0, 0}, dfs[3] = {0, 0, 0};
+    if (own && (z < 6 || z > B.ns - 7)) {
+        int idx[8]; double cz[8];
+        int m = sbp_taps(z, B.ns, idx, cz);
+        for (int t = 0; t < m; t++) {
+            long pt = p + plane * (idx[t] - z);
+            real st[6], ft[3];
+#pragma unroll
+            for (int f = 0; f < 6; f++) st[f] = B.Fin[(3 + f) * NP + pt];
+            fluxv(st, gl(pt, 9), gl(pt, 2), gl(pt, 5), gl(pt, 8), ft);
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                dvs[f] += (real)cz[t] * B.Fin[f * NP + pt];
+                dfs[f] += (real)cz[t] * ft[f];
+            }
+        }
+    } else if (own) {
+        const real cc[3] = {(real)0.75, (real)(-0.15), (real)(1.0 / 60.0)};
+#pragma unroll
+        for (int k = 1; k <= 3; k++) {
+            long pp = p + plane * k, pm = p - plane * k;
+            real sp[6], sm[6], fp[3], fm[3];
+#pragma unroll
+            for (int f = 0; f < 6; f++) { sp[f] = B.Fin[(3 + f) * NP + pp]; sm[f] = B.Fin[(3 + f) * NP + pm]; }
+            fluxv(sp, gl(pp, 9), gl(pp, 2), gl(pp, 5), gl(pp, 8), fp);
+            fluxv(sm, gl(pm, 9), gl(pm, 2), gl(pm, 5), gl(pm, 8), fm);
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                dvs[f] += cc[k - 1] * (B.Fin[f * NP + pp] - B.Fin[f * NP + pm]);
+                dfs[f] += cc[k - 1] * (fp[f] - fm[f]);
+            }
+        }
+    }
+    __syncthreads();
+    if (!own) return;
+    real dvq[3], dvr[3], div[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        dvq[f] = d1_row<real>(&sv[f][ly + 3][0], lx + 3, gx, B.nq, ihq);
+        dvr[f] = d1_col<real>(&sv[f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
+        real a = d1_row<real>(&sfq[f][ly][0], lx + 3, gx, B.nq, ihq);
+        real b = d1_col<real>(&sfr[f][0][lx], TX, ly + 3, gy, B.nr, ihr);
+        div[f] = a + b + ihs * dfs[f];
+        dvs[f] = ihs * dvs[f];
+    }
+    real lam, mu, rho;
+    if (HET) { lam = B.M[p]; mu = B.M[NP + p]; rho = B.M[2 * NP + p]; }
+    else { lam = (real)B.lam; mu = (real)B.mu; rho = (real)B.rho; }
+    real Ux[3], Uy[3], Uz[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        Ux[f] = cg[0] * dvq[f] + cg[1] * dvr[f] + cg[2] * dvs[f];
+        Uy[f] = cg[3] * dvq[f] + cg[4] * dvr[f] + cg[5] * dvs[f];
+        Uz[f] = cg[6] * dvq[f] + cg[7] * dvr[f] + cg[8] * dvs[f];
+    }
+    real rhoJ_inv = (real)1 / (rho * cg[9]);
+    real l2m = lam + (real)2 * mu;
+    real DU[NF];
+    DU[0] = div[0] * rhoJ_inv;
+    DU[1] = div[1] * rhoJ_inv;
+    DU[2] = div[2] * rhoJ_inv;
+    DU[3] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
+    DU[4] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
+    DU[5] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
+    DU[6] = mu * (Uy[0] + Ux[1]);
+    DU[7] = mu * (Uz[0] + Ux[2]);
+    DU[8] = mu * (Uz[1] + Uy[2]);
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        real df = A * B.DF[f * NP + p] + DU[f];
+        B.DF[f * NP + p] = df;
+        real fin = f < 3 ? cv[f] : cs[f - 3];
+        B.Fout[f * NP + p] = fin + bdt * df;
     }
 }
 
@@ -958,6 +1409,8 @@ struct BoundaryArgs {
     Iface I[2];
     long nodes[2];        // boundary nodes per block
     double t, A[2], bdt, tau0;
+    const double *tdev;   // non-null: stage time = *tdev + tinc (step time kept on the device)
+    double tinc;
     int stage;
     int rates_only;       // 1: RHS evaluation without the RK update (no F_out / S / W write)
     int *flag;
@@ -1036,7 +1489,8 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
     // fault face (domain.f90:600-609): block 0's face q = nq-1, block 1's face q = 0
     int on_fault = (me == 0) ? (i == nq - 1 && B.bc[1] == 0) : (i == 0 && B.bc[0] == 0);
     if (on_fault) {
-        int rc = fault_node<real>(P, a.I[me], a.fault, me, j, k, a.t, a.stage, A, a.rates_only ? 0.0 : a.bdt,
+        const double ts = a.tdev ? *a.tdev + a.tinc : a.t;
+        int rc = fault_node<real>(P, a.I[me], a.fault, me, j, k, ts, a.stage, A, a.rates_only ? 0.0 : a.bdt,
                                   sat, a.tau0);
         if (rc) a.flag[0] = rc;
     }
@@ -1151,6 +1605,18 @@ __global__ void k_init_state(Pair<real> P, FaultParams f, Iface I0, Iface I1)
     double theta = f.rs_L / f.rs_V0 *
                    exp(((a * log(2.0 * sinh(T0l / (a * sn)))) - f.rs_f0 - a * log(f.rs_Vin / f.rs_V0)) / f.rs_b);
     (me == 0 ? I0 : I1).W[t] = f.rs_f0 + f.rs_b * log(f.rs_V0 / f.rs_L * theta);
+}
+
+// step time advance of a captured step graph: t = t + inc, the same FP64
+// operation as the host-side clock
+__global__ void k_advance_time(double *t, double inc)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) t[0] = t[0] + inc;
+}
+
+__global__ void k_set_time(double *t, double v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) t[0] = v;
 }
 
 template <typename real>

@@ -88,10 +88,22 @@ class Simulation:
         arr, keep = block_descs(prob, coords)
         check(L.wql_create(C.byref(pd), arr, C.byref(self.ctx)))
         self.dt = pd.dt
+        if tile is None and zchunk is None:
+            from . import autotune
+            best = autotune.lookup(prob, precision)
+            if best is not None:
+                tile, zchunk = best['variant'], best['zchunk']
         if tile is not None or zchunk is not None:
-            check(L.wql_set_tile(self.ctx, 0 if tile is None else tile, 32 if zchunk is None else zchunk))
+            check(L.wql_set_tile(self.ctx, 4 if tile is None else tile, 4 if zchunk is None else zchunk))
         if graph:
             check(L.wql_use_graph(self.ctx, 1))
+
+    @property
+    def tile(self):
+        """(variant, zchunk) of the main volume kernel's launch shape"""
+        v, z = C.c_int(), C.c_int()
+        check(native.lib().wql_get_tile(self.ctx, C.byref(v), C.byref(z)))
+        return v.value, z.value
 
     def close(self):
         if self.ctx:
