@@ -20,6 +20,37 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, 'waveqlab3d-gh_amd'))
 
 HBM_PEAK_GBS = 8000.0
+FAMILIES = ['k_volume_main', 'k_volume_pipe', 'k_volume_plane', 'k_volume_buf', 'k_volume_split']
+TILES = [(32, 8), (64, 8), (16, 16), (32, 16)]
+
+
+def kernel_name(prob, precision, variant):
+    """template name of the main volume kernel a launch variant selects"""
+    v = variant % 20
+    tx, ty = TILES[v % 4]
+    curv = 'true' if any(b.mesh != 'cartesian' for b in prob.blocks) else 'false'
+    het = 'true' if any(b.material is not None for b in prob.blocks) else 'false'
+    real = 'float' if precision == 'fp32' else 'double'
+    return f'{FAMILIES[v // 4]}<{real}, {tx}, {ty}, {curv}, {het}>'
+
+
+def measured_traffic(workload, shape, kname):
+    """HBM bytes per launch of the kernel from the committed FETCH_SIZE /
+    WRITE_SIZE counter summary (copy-calibrated), if it was taken at this
+    workload and launch shape"""
+    try:
+        with open(os.path.join(ROOT, 'profiles', 'traffic.json')) as f:
+            allj = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for entry in allj.values():
+        if entry.get('workload') != workload or entry.get('tile_variant') != shape[0] or \
+                entry.get('zchunk') != shape[1] or entry.get('tile_order', 0) != shape[2]:
+            continue
+        k = entry['kernels'].get(kname)
+        if k:
+            return k['read_bytes'] + k['write_bytes']
+    return None
 
 
 def main_kernel_nodes(prob):
@@ -48,11 +79,14 @@ def bytes_per_node(prob, precision):
     return vals * (4 if precision == 'fp32' else 8)
 
 
-def run(W, precision, steps, warmup, tile, zchunk, graph):
+def run(W, precision, steps, warmup, tile, zchunk, graph, order=None):
     import torch
+    from wql import native
     from wql.solver import Simulation
     prob = W['problem']
     sim = Simulation(prob, precision=precision, tile=tile, zchunk=zchunk, coords=W['coords'], graph=graph)
+    if order is not None:
+        native.check(native.lib().wql_set_tile_order(sim.ctx, *order))
     if W.get('F0') is not None:
         for ib in range(2):
             sim.set_fields(ib, W['F0'][ib])
@@ -84,7 +118,7 @@ def cpu_baseline(W):
     import oracle as O
     from wql import workloads
     name = W['problem'].name
-    sub = workloads.WORKLOADS.get(name, workloads.rupture)(scale=0.5)
+    sub = workloads.WORKLOADS.get(name, workloads.rupture)(scale=1.0)
     prob = sub['problem']
     od = O.OracleDomain(prob, coords=sub['coords'])
     nsteps = 2
@@ -93,8 +127,9 @@ def cpu_baseline(W):
     sec = (time.perf_counter() - t0) / nsteps
     nodes = workloads.points(prob)
     return dict(value=nodes * 5 / sec / 1e6, unit='M grid-pt RK stages/s', cores=1, kind='port',
-                sample=f'oracle C restatement, 2 LSRK steps of the {name} workload at half the node count per axis '
-                       f'({"x".join(str(v) for v in prob.blocks[0].n)} per block, {nodes} nodes), {sec:.1f} s')
+                sample=f'oracle C restatement (1 thread), {nsteps} LSRK steps of the full {name} workload '
+                       f'({"x".join(str(v) for v in prob.blocks[0].n)} per block, {nodes} nodes), '
+                       f'{sec * nsteps:.1f} s')
 
 
 def main():
@@ -106,6 +141,7 @@ def main():
     ap.add_argument('--scale', type=float, default=1.0)
     ap.add_argument('--tile', type=int, default=None, help='main-kernel launch variant (default: tuned)')
     ap.add_argument('--zchunk', type=int, default=None)
+    ap.add_argument('--order', default=None, help='tile dispatch order:group, e.g. 2:2')
     ap.add_argument('--graph', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
@@ -120,9 +156,13 @@ def main():
     nodes = workloads.points(prob)
     res = {}
     for prec in (['fp32'] if a.no_fp64 else ['fp64', 'fp32']):
-        sec, kms, dev, shape = run(W, prec, a.steps, a.warmup, a.tile, a.zchunk, a.graph)
+        order = tuple(int(v) for v in a.order.split(':')) if a.order else None
+        sec, kms, dev, shape = run(W, prec, a.steps, a.warmup, a.tile, a.zchunk, a.graph, order)
         res[prec] = dict(sec=sec, kms=kms, dev=dev, shape=shape, mpts=nodes * 5 * a.steps / sec / 1e6)
     r32 = res['fp32']
+    workload = f'{a.workload}: 2 blocks x {"x".join(str(v) for v in prob.blocks[0].n)}'
+    kname = kernel_name(prob, 'fp32', r32['shape'][0])
+    traffic = measured_traffic(workload, r32['shape'], kname)
     bpn = bytes_per_node(prob, 'fp32')
     kn = main_kernel_nodes(prob)
     achieved = kn * bpn / (r32['kms'] * 1e-3) / 1e9 if r32['kms'] > 0 else None
@@ -139,12 +179,13 @@ def main():
         'vs_baseline': None,
         'dtype': 'f32 volume / f64 fault',
         'data': 'synthetic (seeded geometry, topography and friction)',
-        'config': {'workload': f'{a.workload}: 2 blocks x {"x".join(str(v) for v in prob.blocks[0].n)}',
-                   'nodes': nodes, 'tile_variant': r32['shape'][0], 'zchunk': r32['shape'][1]},
+        'config': {'workload': workload,
+                   'nodes': nodes, 'tile_variant': r32['shape'][0], 'zchunk': r32['shape'][1],
+                   'tile_order': r32['shape'][2], 'tile_group': r32['shape'][3]},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1) if achieved else None,
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved / HBM_PEAK_GBS, 4) if achieved else None, 'traffic': None,
-                     'kernel': 'k_volume_main', 'kernel_ms': round(r32['kms'], 5),
+                     'frac': round(achieved / HBM_PEAK_GBS, 4) if achieved else None, 'traffic': traffic,
+                     'kernel': kname, 'kernel_ms': round(r32['kms'], 5),
                      'algorithmic_bytes_per_launch': kn * bpn},
     }
     if 'fp64' in res:

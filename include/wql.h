@@ -135,6 +135,10 @@ double wql_time(const wql_ctx *ctx);
  * z-chunk length; returns WQL_E_ARG for unknown variants */
 int wql_set_tile(wql_ctx *ctx, int variant, int zchunk);
 int wql_get_tile(const wql_ctx *ctx, int *variant, int *zchunk);
+/* dispatch order of the volume tiles: 0 q-tiles fastest, 1 s-chunks fastest,
+ * 2 per-XCD groups of `group` q-tile columns walked plane by plane */
+int wql_set_tile_order(wql_ctx *ctx, int order, int group);
+int wql_get_tile_order(const wql_ctx *ctx, int *order, int *group);
 int wql_tile_variants(void);
 
 /* capture one LSRK step into a HIP graph and replay it (1) or launch kernels

@@ -114,10 +114,10 @@ struct wql_ctx {
     wql_problem prob;
     double t, dt;
     int cur = 0;
-    // launch shape of the main volume kernel; the default is the fastest one
-    // measured on the 2 x 201x101x101 curvilinear problem (pipelined march,
-    // 32 x 8 tiles, 4-plane s chunks, q tiles fastest)
-    int variant = 4, zchunk = 4, order = 0;
+    // launch shape of the main volume kernel (set per precision in
+    // wql_create: the fastest measured on the 2 x 201x101x101 curvilinear
+    // problem)
+    int variant = 16, zchunk = 4, order = 0, group = 2;
     int use_graph = 0;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int *d_flag = nullptr;
@@ -180,11 +180,13 @@ void main_range(const DevBlock &d, int r[6])
     r[4] = std::min(pz0, d.ns); r[5] = std::max(d.ns - pz1, r[4]);
 }
 
-// variants 0-3: k_volume_main (s-march); 4-7: software-pipelined march
-// k_volume_pipe; 8-11: plane-parallel k_volume_plane
-const int NVARIANT = 12;
-const int VARIANT_TX[NVARIANT] = {32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32};
-const int VARIANT_TY[NVARIANT] = {8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16};
+// variants 0-3: k_volume_main (s-march); 4-7: software-pipelined march;
+// 8-11: plane-parallel; 12-15: buffer-addressed pipelined march; 16-19:
+// role-split pipelined march.  Within each family: 32x8, 64x8, 16x16, 32x16
+// tiles.
+const int NVARIANT = 20;
+const int VARIANT_TX[NVARIANT] = {32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32};
+const int VARIANT_TY[NVARIANT] = {8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16};
 
 TileList tile_list(const wql_ctx *c)
 {
@@ -201,7 +203,7 @@ TileList tile_list(const wql_ctx *c)
         g.xa0 = r[0]; g.xa1 = r[1]; g.ya0 = r[2]; g.ya1 = r[3]; g.za0 = r[4]; g.za1 = r[5];
         g.ntx = (r[1] - r[0] + TX - 1) / TX;
         g.nty = (r[3] - r[2] + TY - 1) / TY;
-        g.zc = c->variant >= 8 ? 1 : std::max(1, c->zchunk);
+        g.zc = (c->variant >= 8 && c->variant < 12) ? 1 : std::max(1, c->zchunk);
         g.nzc = (r[5] - r[4] + g.zc - 1) / g.zc;
         g.first = first;
         first += (long)g.ntx * g.nty * g.nzc;
@@ -209,6 +211,7 @@ TileList tile_list(const wql_ctx *c)
     }
     L.total = first;
     L.order = c->order;
+    L.gw = std::max(1, c->group);
     return L;
 }
 
@@ -282,6 +285,25 @@ void launch_main_v(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0,
     dim3 grd((unsigned)(((TL.total + 7) / 8) * 8));
     bool curv = !c->b[0].cart || !c->b[1].cart;
     bool het = c->b[0].M || c->b[1].M;
+    // buffer addressing needs 32-bit byte offsets: every array under 2 GiB
+    bool buf_ok = true;
+    for (int ib = 0; ib < 2; ib++)
+        if ((double)c->b[ib].NP * 10.0 * sizeof(real) >= 2147483647.0) buf_ok = false;
+    if (c->variant >= 16 && buf_ok) {
+        dim3 grd2((unsigned)(((2 * TL.total + 7) / 8) * 8));
+        if (curv && het) k_volume_split<real, TX, TY, true, true><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (curv) k_volume_split<real, TX, TY, true, false><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (het) k_volume_split<real, TX, TY, false, true><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else k_volume_split<real, TX, TY, false, false><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        return;
+    }
+    if (c->variant >= 12 && buf_ok) {
+        if (curv && het) k_volume_buf<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (curv) k_volume_buf<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (het) k_volume_buf<real, TX, TY, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else k_volume_buf<real, TX, TY, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        return;
+    }
     if (c->variant >= 8) {
         if (curv && het) k_volume_plane<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
         else if (curv) k_volume_plane<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
@@ -289,7 +311,7 @@ void launch_main_v(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0,
         else k_volume_plane<real, TX, TY, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
         return;
     }
-    if (c->variant >= 4) {
+    if (c->variant >= 4 && (c->variant < 8 || c->variant >= 12)) {
         if (curv && het) k_volume_pipe<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
         else if (curv) k_volume_pipe<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
         else if (het) k_volume_pipe<real, TX, TY, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
@@ -655,6 +677,10 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
         return fail(WQL_E_ARG, "bad fault description");
     wql_ctx *c = new wql_ctx();
     c->prec = prob->precision;
+    // role-split march: FP32 32x8 tiles with 4-plane s chunks, FP64 64x8
+    // tiles with 16-plane chunks
+    c->variant = prob->precision == WQL_FP64 ? 17 : 16;
+    c->zchunk = prob->precision == WQL_FP64 ? 16 : 4;
     c->prob = *prob;
     c->dt = prob->dt;
     c->t = prob->t0;
@@ -936,8 +962,25 @@ int wql_set_tile(wql_ctx *c, int variant, int zchunk)
 int wql_get_tile(const wql_ctx *c, int *variant, int *zchunk)
 {
     if (!c || !variant || !zchunk) return fail(WQL_E_ARG, "null argument");
-    *variant = c->variant + (c->order ? NVARIANT : 0);
+    *variant = c->variant + (c->order == 1 ? NVARIANT : 0);
     *zchunk = c->zchunk;
+    return 0;
+}
+
+int wql_set_tile_order(wql_ctx *c, int order, int group)
+{
+    if (!c || order < 0 || order > 2 || group < 1) return fail(WQL_E_ARG, "bad tile order");
+    drop_graphs(c);
+    c->order = order;
+    c->group = group;
+    return 0;
+}
+
+int wql_get_tile_order(const wql_ctx *c, int *order, int *group)
+{
+    if (!c || !order || !group) return fail(WQL_E_ARG, "null argument");
+    *order = c->order;
+    *group = c->group;
     return 0;
 }
 

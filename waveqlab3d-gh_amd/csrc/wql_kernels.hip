@@ -323,7 +323,14 @@ struct TileList {
     int n;
     TileGeom g[4];
     long total;
-    int order;  // 0: q-tiles fastest, then r-tiles, then s; 1: s fastest
+    // 0: q-tiles fastest, then r-tiles, then s chunks
+    // 1: s chunks fastest, then q-tiles, then r-tiles
+    // 2: groups of gw q-tile columns; inside a group s chunks outermost,
+    //    then r-tiles, then q-tiles.  Consecutive logical ids land on one
+    //    XCD, so each XCD walks its group plane by plane and the s-neighbour
+    //    planes and q/r halos of a tile are still in that XCD's L2.
+    int order;
+    int gw;
 };
 
 // logical tile id -> (q tile, r tile, s chunk or plane)
@@ -334,10 +341,29 @@ __device__ __forceinline__ void tile_coords(const TileList &TL, const TileGeom &
         tx = (int)(lid % T.ntx);
         ty = (int)((lid / T.ntx) % T.nty);
         tz = (int)(lid / ((long)T.ntx * T.nty));
-    } else {
+    } else if (TL.order == 1) {
         tz = (int)(lid % nz);
         tx = (int)((lid / nz) % T.ntx);
         ty = (int)(lid / ((long)nz * T.ntx));
+    } else {
+        const int gw = TL.gw;
+        const int ngx = (T.ntx + gw - 1) / gw;
+        const long full = (long)gw * T.nty * nz;
+        int g = (int)(lid / full);
+        int w = gw;
+        long rem;
+        if (g >= ngx - 1) {
+            g = ngx - 1;
+            w = T.ntx - g * gw;
+            rem = lid - (long)g * full;
+        } else {
+            rem = lid - (long)g * full;
+        }
+        const long pl = (long)w * T.nty;
+        tz = (int)(rem / pl);
+        const int r2 = (int)(rem - (long)tz * pl);
+        ty = r2 / w;
+        tx = g * gw + (r2 - ty * w);
     }
 }
 
@@ -822,6 +848,671 @@ __global__ void __launch_bounds__(TX * TY) k_volume_pipe(Pair<real> P, TileList 
             }
         }
         cur = nxt;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Buffer-addressed pipelined march.  Same algorithm and tiling as
+// k_volume_pipe; every global access is a raw buffer load/store whose
+// per-thread byte offset (one VGPR per node column) is shared by all fields
+// and whose field offset rides in an SGPR, so no 64-bit address arithmetic is
+// spent per load.  Used when every array of a block spans < 2 GiB.
+template <typename real>
+__device__ __forceinline__ real bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(real) == 4) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+    } else {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+    }
+}
+
+template <typename real>
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, real v)
+{
+    if constexpr (sizeof(real) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+    } else {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)voff, (int)soff, 0);
+    }
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void *p, unsigned bytes)
+{
+    // raw (stride 0) buffer over [p, p + bytes): out-of-range loads return 0
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename real, int TX, int TY, bool CURV, bool HET>
+__global__ void __launch_bounds__(TX * TY) k_volume_buf(Pair<real> P, TileList TL, real A0, real A1, real bdt)
+{
+    constexpr int WX = TX + 6, WY = TY + 6;
+    constexpr unsigned RS = sizeof(real);
+    __shared__ real sv[2][3][WY][WX];
+    __shared__ real sfq[2][3][TY][WX];
+    __shared__ real sfr[2][3][WY][TX];
+
+    const long nwg = (long)gridDim.x;
+    const long per = (nwg + 7) / 8;
+    const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
+    if (logical >= TL.total) return;
+    int gi = 0;
+    for (int k = 1; k < TL.n; k++)
+        if (logical >= TL.g[k].first) gi = k;
+    const TileGeom &T = TL.g[gi];
+    const Blk<real> &B = P.b[T.blk];
+    const real A = T.blk == 0 ? A0 : A1;
+    long lid = logical - T.first;
+    int tx_i, ty_i, zc_i;
+    tile_coords(TL, T, lid, T.nzc, tx_i, ty_i, zc_i);
+
+    int xs, xc0, xc1, ys, yc0, yc1;
+    tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
+    tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
+    const int z0 = T.za0 + zc_i * T.zc;
+    const int z1 = min(z0 + T.zc, T.za1);
+
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int gx = xs + lx, gy = ys + ly;
+    const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
+    const bool inb = (gx < B.nq && gy < B.nr);
+    const unsigned NPb = (unsigned)B.NP * RS;           // bytes per field
+    const unsigned planeb = (unsigned)(B.nq * B.nr) * RS;
+    const unsigned colb = (unsigned)(gx + B.nq * gy) * RS;
+    const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
+
+    const __amdgpu_buffer_rsrc_t rF = brsrc(B.Fin, 9u * NPb);
+    const __amdgpu_buffer_rsrc_t rD = brsrc(B.DF, 9u * NPb);
+    const __amdgpu_buffer_rsrc_t rO = brsrc(B.Fout, 9u * NPb);
+    const __amdgpu_buffer_rsrc_t rG = brsrc(CURV ? (const void *)B.G : (const void *)B.Fin, CURV ? 10u * NPb : 0u);
+    const __amdgpu_buffer_rsrc_t rM = brsrc(HET ? (const void *)B.M : (const void *)B.Fin, HET ? 3u * NPb : 0u);
+
+    // this thread's halo node (x halo: 6*TY nodes, then y halo: 6*TX nodes)
+    const int tid = ly * TX + lx;
+    int hk = 0, hcx = 0, hcy = 0;
+    unsigned hcolb = 0;
+    if (tid < 6 * TY) {
+        int hx = tid % 6, hy = tid / 6;
+        hcx = hx < 3 ? hx : TX + hx;
+        hcy = hy;
+        int ggx = xs - 3 + hcx, ggy = ys + hy;
+        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcolb = (unsigned)(ggx + B.nq * ggy) * RS; }
+    } else if (tid < 6 * TY + 6 * TX) {
+        int t2 = tid - 6 * TY;
+        int hx = t2 % TX, hy = t2 / TX;
+        hcy = hy < 3 ? hy : TY + hy;
+        hcx = hx;
+        int ggx = xs + hx, ggy = ys - 3 + hcy;
+        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcolb = (unsigned)(ggx + B.nq * ggy) * RS; }
+    }
+    const unsigned hax = hk == 2 ? 1u : 0u;  // metric axis of the halo flux
+
+    auto fluxv = [&](const real sg[6], real J, real ax, real ay, real az, real fl[3]) {
+        real Jx = J * ax, Jy = J * ay, Jz = J * az;
+        fl[0] = Jx * sg[0] + Jy * sg[3] + Jz * sg[4];
+        fl[1] = Jx * sg[3] + Jy * sg[1] + Jz * sg[5];
+        fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
+    };
+    auto gconst = [&](int c) -> real { return (real)B.gc[c]; };
+
+    struct PF {
+        real v[3], s[6], g[10];
+        real hv[3], hs[6], hg[4];
+        real lv[3], ls[6], lg[4];
+        real df[9];
+    };
+    auto prefetch = [&](PF &R, int z) {
+        if (inb) {
+            const unsigned o = colb + planeb * (unsigned)z;
+#pragma unroll
+            for (int f = 0; f < 3; f++) R.v[f] = bload<real>(rF, o, f * NPb);
+#pragma unroll
+            for (int f = 0; f < 6; f++) R.s[f] = bload<real>(rF, o, (3 + f) * NPb);
+            if (CURV) {
+#pragma unroll
+                for (int c = 0; c < 10; c++) R.g[c] = bload<real>(rG, o, c * NPb);
+            }
+            const unsigned ol = o + 3u * planeb;
+#pragma unroll
+            for (int f = 0; f < 3; f++) R.lv[f] = bload<real>(rF, ol, f * NPb);
+#pragma unroll
+            for (int f = 0; f < 6; f++) R.ls[f] = bload<real>(rF, ol, (3 + f) * NPb);
+            if (CURV) {
+                R.lg[0] = bload<real>(rG, ol, 9u * NPb);
+                R.lg[1] = bload<real>(rG, ol, 2u * NPb);
+                R.lg[2] = bload<real>(rG, ol, 5u * NPb);
+                R.lg[3] = bload<real>(rG, ol, 8u * NPb);
+            }
+        }
+        if (own) {
+            const unsigned o = colb + planeb * (unsigned)z;
+#pragma unroll
+            for (int f = 0; f < 9; f++) R.df[f] = bload<real>(rD, o, f * NPb);
+        }
+        if (hk) {
+            const unsigned o = hcolb + planeb * (unsigned)z;
+#pragma unroll
+            for (int f = 0; f < 3; f++) R.hv[f] = bload<real>(rF, o, f * NPb);
+#pragma unroll
+            for (int f = 0; f < 6; f++) R.hs[f] = bload<real>(rF, o, (3 + f) * NPb);
+            if (CURV) {
+                R.hg[0] = bload<real>(rG, o, 9u * NPb);
+                R.hg[1] = bload<real>(rG, o, (0u + hax) * NPb);
+                R.hg[2] = bload<real>(rG, o, (3u + hax) * NPb);
+                R.hg[3] = bload<real>(rG, o, (6u + hax) * NPb);
+            }
+        }
+    };
+
+    real qv[7][3], qf[7][3];
+#pragma unroll
+    for (int a = 0; a < 7; a++)
+#pragma unroll
+        for (int f = 0; f < 3; f++) { qv[a][f] = 0; qf[a][f] = 0; }
+    // warm the s-queue with planes z0-3 .. z0+2
+    if (inb) {
+#pragma unroll
+        for (int a = 1; a < 7; a++) {
+            const unsigned o = colb + planeb * (unsigned)(z0 - 4 + a);
+            real sg[6];
+#pragma unroll
+            for (int f = 0; f < 3; f++) qv[a][f] = bload<real>(rF, o, f * NPb);
+#pragma unroll
+            for (int f = 0; f < 6; f++) sg[f] = bload<real>(rF, o, (3 + f) * NPb);
+            if (CURV)
+                fluxv(sg, bload<real>(rG, o, 9u * NPb), bload<real>(rG, o, 2u * NPb), bload<real>(rG, o, 5u * NPb),
+                      bload<real>(rG, o, 8u * NPb), qf[a]);
+            else
+                fluxv(sg, gconst(9), gconst(2), gconst(5), gconst(8), qf[a]);
+        }
+    }
+
+    auto body = [&](PF &cur, PF &nxt, int z) {
+        const int b = z & 1;
+        // (1) publish plane z: centre and halo into LDS buffer b, lead into the queue
+        real cv[3], cs[6], cg[10], cdf[9];
+        if (inb) {
+            if (CURV) {
+#pragma unroll
+                for (int c = 0; c < 10; c++) cg[c] = cur.g[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 10; c++) cg[c] = gconst(c);
+            }
+            real fq[3], fr[3];
+            fluxv(cur.s, cg[9], cg[0], cg[3], cg[6], fq);
+            fluxv(cur.s, cg[9], cg[1], cg[4], cg[7], fr);
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                sv[b][f][ly + 3][lx + 3] = cur.v[f];
+                sfq[b][f][ly][lx + 3] = fq[f];
+                sfr[b][f][ly + 3][lx] = fr[f];
+                cv[f] = cur.v[f];
+            }
+#pragma unroll
+            for (int f = 0; f < 6; f++) cs[f] = cur.s[f];
+#pragma unroll
+            for (int a = 0; a < 6; a++)
+#pragma unroll
+                for (int f = 0; f < 3; f++) { qv[a][f] = qv[a + 1][f]; qf[a][f] = qf[a + 1][f]; }
+#pragma unroll
+            for (int f = 0; f < 3; f++) qv[6][f] = cur.lv[f];
+            if (CURV) fluxv(cur.ls, cur.lg[0], cur.lg[1], cur.lg[2], cur.lg[3], qf[6]);
+            else fluxv(cur.ls, gconst(9), gconst(2), gconst(5), gconst(8), qf[6]);
+        }
+        if (own) {
+#pragma unroll
+            for (int f = 0; f < 9; f++) cdf[f] = cur.df[f];
+        }
+        if (hk) {
+            real fl[3];
+            if (CURV) fluxv(cur.hs, cur.hg[0], cur.hg[1], cur.hg[2], cur.hg[3], fl);
+            else fluxv(cur.hs, gconst(9), gconst(0 + hax), gconst(3 + hax), gconst(6 + hax), fl);
+            if (hk == 1) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) { sv[b][f][hcy + 3][hcx] = cur.hv[f]; sfq[b][f][hcy][hcx] = fl[f]; }
+            } else {
+#pragma unroll
+                for (int f = 0; f < 3; f++) { sv[b][f][hcy][hcx + 3] = cur.hv[f]; sfr[b][f][hcy][hcx] = fl[f]; }
+            }
+        }
+        __syncthreads();
+        // (2) loads of plane z+1 go out now and overlap the arithmetic below
+        if (z + 1 < z1) prefetch(nxt, z + 1);
+        // (3) plane z
+        if (own) {
+            const unsigned o = colb + planeb * (unsigned)z;
+            real dvq[3], dvr[3], dvs[3], div[3];
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                dvq[f] = d1_row<real>(&sv[b][f][ly + 3][0], lx + 3, gx, B.nq, ihq);
+                dvr[f] = d1_col<real>(&sv[b][f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
+                dvs[f] = ihs * ((real)0.75 * (qv[4][f] - qv[2][f]) + (real)(-0.15) * (qv[5][f] - qv[1][f]) +
+                                (real)(1.0 / 60.0) * (qv[6][f] - qv[0][f]));
+                real a = d1_row<real>(&sfq[b][f][ly][0], lx + 3, gx, B.nq, ihq);
+                real bb = d1_col<real>(&sfr[b][f][0][lx], TX, ly + 3, gy, B.nr, ihr);
+                real c = ihs * ((real)0.75 * (qf[4][f] - qf[2][f]) + (real)(-0.15) * (qf[5][f] - qf[1][f]) +
+                                (real)(1.0 / 60.0) * (qf[6][f] - qf[0][f]));
+                div[f] = a + bb + c;
+            }
+            real lam, mu, rho;
+            if (HET) { lam = bload<real>(rM, o, 0u); mu = bload<real>(rM, o, NPb); rho = bload<real>(rM, o, 2u * NPb); }
+            else { lam = (real)B.lam; mu = (real)B.mu; rho = (real)B.rho; }
+            real Ux[3], Uy[3], Uz[3];
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                Ux[f] = cg[0] * dvq[f] + cg[1] * dvr[f] + cg[2] * dvs[f];
+                Uy[f] = cg[3] * dvq[f] + cg[4] * dvr[f] + cg[5] * dvs[f];
+                Uz[f] = cg[6] * dvq[f] + cg[7] * dvr[f] + cg[8] * dvs[f];
+            }
+            real rhoJ_inv = (real)1 / (rho * cg[9]);
+            real l2m = lam + (real)2 * mu;
+            real DU[NF];
+            DU[0] = div[0] * rhoJ_inv;
+            DU[1] = div[1] * rhoJ_inv;
+            DU[2] = div[2] * rhoJ_inv;
+            DU[3] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
+            DU[4] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
+            DU[5] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
+            DU[6] = mu * (Uy[0] + Ux[1]);
+            DU[7] = mu * (Uz[0] + Ux[2]);
+            DU[8] = mu * (Uz[1] + Uy[2]);
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                real df = A * cdf[f] + DU[f];
+                bstore<real>(rD, o, f * NPb, df);
+                real fin = f < 3 ? cv[f] : cs[f - 3];
+                bstore<real>(rO, o, f * NPb, fin + bdt * df);
+            }
+        }
+    };
+
+    PF pa, pb;
+    prefetch(pa, z0);
+    int z = z0;
+    for (; z + 1 < z1; z += 2) {
+        body(pa, pb, z);
+        body(pb, pa, z + 1);
+    }
+    if (z < z1) body(pa, pb, z);
+}
+
+// Closure table staged in LDS in the working precision.  Reads inside the
+// s-march sit behind its barriers, so the compiler keeps them in the loop
+// instead of holding per-lane copies of the table in registers.
+template <typename real>
+struct SbpLds {
+    real c[12][8];
+    int off[12][8];
+    int n[12];
+};
+
+template <typename real>
+__device__ __forceinline__ void sbp_to_lds(SbpLds<real> &S, int tid, int nthreads)
+{
+    for (int e = tid; e < 12 * 8; e += nthreads) {
+        const int r = e / 8, t = e % 8;
+        const SbpRow &R = r < 6 ? c_sbp.left[r] : c_sbp.right[r - 6];
+        S.c[r][t] = t < R.n ? (real)R.c[t] : (real)0;
+        S.off[r][t] = t < R.n ? R.off[t] : 0;
+        if (t == 0) S.n[r] = R.n;
+    }
+}
+
+template <typename real>
+__device__ __forceinline__ real d1_row_s(const SbpLds<real> &S, const real *row, int lx, int gx, int n, real ih)
+{
+    if (gx >= 6 && gx <= n - 7) {
+        return ih * ((real)0.75 * (row[lx + 1] - row[lx - 1]) + (real)(-0.15) * (row[lx + 2] - row[lx - 2]) +
+                     (real)(1.0 / 60.0) * (row[lx + 3] - row[lx - 3]));
+    }
+    const int r = gx < 6 ? gx : 6 + gx - (n - 6);
+    real s = 0;
+    for (int t = 0; t < S.n[r]; t++) s = s + S.c[r][t] * row[lx + S.off[r][t]];
+    return ih * s;
+}
+
+template <typename real>
+__device__ __forceinline__ real d1_col_s(const SbpLds<real> &S, const real *col, int stride, int ly, int gy, int n,
+                                         real ih)
+{
+    if (gy >= 6 && gy <= n - 7) {
+        return ih * ((real)0.75 * (col[(ly + 1) * stride] - col[(ly - 1) * stride]) +
+                     (real)(-0.15) * (col[(ly + 2) * stride] - col[(ly - 2) * stride]) +
+                     (real)(1.0 / 60.0) * (col[(ly + 3) * stride] - col[(ly - 3) * stride]));
+    }
+    const int r = gy < 6 ? gy : 6 + gy - (n - 6);
+    real s = 0;
+    for (int t = 0; t < S.n[r]; t++) s = s + S.c[r][t] * col[(ly + S.off[r][t]) * stride];
+    return ih * s;
+}
+
+// ----------------------------------------------------------------------------
+// Role-split pipelined march.  The velocity rates depend on the stresses only
+// and the stress rates on the velocities only, so each tile is processed by
+// two workgroups with consecutive ids (one XCD, dispatched together): role 0
+// marches the stress fluxes and writes DF/F_out of vx vy vz, role 1 marches
+// the velocities and writes DF/F_out of the six stresses.  Each carries half
+// of the fused kernel's state (one 7-plane s-queue of 3 values), which keeps
+// occupancy high; the metric and field lines both roles touch are fetched
+// from HBM once and hit L2 for the second role.
+template <typename real, int TX, int TY, bool CURV, bool HET>
+__global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList TL, real A0, real A1, real bdt)
+{
+    constexpr int WX = TX + 6, WY = TY + 6;
+    constexpr unsigned RS = sizeof(real);
+    // role 0: flux tiles sfq/sfr; role 1: velocity tile sv (same storage)
+    constexpr int NQ = 2 * 3 * TY * WX, NR = 2 * 3 * WY * TX, NV = 2 * 3 * WY * WX;
+    constexpr int NL = (NQ + NR) > NV ? (NQ + NR) : NV;
+    __shared__ real lds[NL];
+    real(*sfq)[3][TY][WX] = reinterpret_cast<real(*)[3][TY][WX]>(lds);
+    real(*sfr)[3][WY][TX] = reinterpret_cast<real(*)[3][WY][TX]>(lds + NQ);
+    real(*sv)[3][WY][WX] = reinterpret_cast<real(*)[3][WY][WX]>(lds);
+    __shared__ SbpLds<real> sbp;
+
+    const long nwg = (long)gridDim.x;
+    const long per = (nwg + 7) / 8;
+    const long logical2 = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
+    const int role = (int)(logical2 & 1);
+    const long logical = logical2 >> 1;
+    if (logical >= TL.total) return;
+    int gi = 0;
+    for (int k = 1; k < TL.n; k++)
+        if (logical >= TL.g[k].first) gi = k;
+    const TileGeom &T = TL.g[gi];
+    const Blk<real> &B = P.b[T.blk];
+    const real A = T.blk == 0 ? A0 : A1;
+    long lid = logical - T.first;
+    int tx_i, ty_i, zc_i;
+    tile_coords(TL, T, lid, T.nzc, tx_i, ty_i, zc_i);
+
+    int xs, xc0, xc1, ys, yc0, yc1;
+    tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
+    tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
+    const int z0 = T.za0 + zc_i * T.zc;
+    const int z1 = min(z0 + T.zc, T.za1);
+
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int gx = xs + lx, gy = ys + ly;
+    const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
+    const bool inb = (gx < B.nq && gy < B.nr);
+    const unsigned NPb = (unsigned)B.NP * RS;
+    const unsigned planeb = (unsigned)(B.nq * B.nr) * RS;
+    const unsigned colb = (unsigned)(gx + B.nq * gy) * RS;
+    const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
+
+    const __amdgpu_buffer_rsrc_t rF = brsrc(B.Fin, 9u * NPb);
+    const __amdgpu_buffer_rsrc_t rD = brsrc(B.DF, 9u * NPb);
+    const __amdgpu_buffer_rsrc_t rO = brsrc(B.Fout, 9u * NPb);
+    const __amdgpu_buffer_rsrc_t rG = brsrc(CURV ? (const void *)B.G : (const void *)B.Fin, CURV ? 10u * NPb : 0u);
+    const __amdgpu_buffer_rsrc_t rM = brsrc(HET ? (const void *)B.M : (const void *)B.Fin, HET ? 3u * NPb : 0u);
+    auto G = [&](unsigned o, unsigned c) -> real { return CURV ? bload<real>(rG, o, c * NPb) : (real)B.gc[c]; };
+
+    // halo node of this thread (x halo: 6*TY nodes, then y halo: 6*TX nodes)
+    const int tid = ly * TX + lx;
+    sbp_to_lds<real>(sbp, tid, TX * TY);  // published by the first barrier of the march
+    int hk = 0, hcx = 0, hcy = 0;
+    unsigned hcolb = 0;
+    if (tid < 6 * TY) {
+        int hx = tid % 6, hy = tid / 6;
+        hcx = hx < 3 ? hx : TX + hx;
+        hcy = hy;
+        int ggx = xs - 3 + hcx, ggy = ys + hy;
+        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcolb = (unsigned)(ggx + B.nq * ggy) * RS; }
+    } else if (tid < 6 * TY + 6 * TX) {
+        int t2 = tid - 6 * TY;
+        int hx = t2 % TX, hy = t2 / TX;
+        hcy = hy < 3 ? hy : TY + hy;
+        hcx = hx;
+        int ggx = xs + hx, ggy = ys - 3 + hcy;
+        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcolb = (unsigned)(ggx + B.nq * ggy) * RS; }
+    }
+    const unsigned hax = hk == 2 ? 1u : 0u;
+
+    auto fluxv = [&](const real sg[6], real J, real ax, real ay, real az, real fl[3]) {
+        real Jx = J * ax, Jy = J * ay, Jz = J * az;
+        fl[0] = Jx * sg[0] + Jy * sg[3] + Jz * sg[4];
+        fl[1] = Jx * sg[3] + Jy * sg[1] + Jz * sg[5];
+        fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
+    };
+    const real c1 = (real)0.75, c2 = (real)(-0.15), c3 = (real)(1.0 / 60.0);
+
+    if (role == 0) {
+        // ---------------- velocity rates: div of the stress fluxes / (rho J)
+        struct PF {
+            real s[6], g[7];   // centre stress; J, q_x q_y q_z, r_x r_y r_z
+            real v[3], df[3];  // centre velocity and rates (update)
+            real hs[6], hg[4]; // halo stress; J + axis metrics
+            real ls[6], lg[4]; // lead (z+3) stress; J + s metrics
+        };
+        auto prefetch = [&](PF &R, int z) {
+            if (inb) {
+                const unsigned o = colb + planeb * (unsigned)z;
+#pragma unroll
+                for (int f = 0; f < 6; f++) R.s[f] = bload<real>(rF, o, (3 + f) * NPb);
+                R.g[0] = G(o, 9); R.g[1] = G(o, 0); R.g[2] = G(o, 3); R.g[3] = G(o, 6);
+                R.g[4] = G(o, 1); R.g[5] = G(o, 4); R.g[6] = G(o, 7);
+                const unsigned ol = o + 3u * planeb;
+#pragma unroll
+                for (int f = 0; f < 6; f++) R.ls[f] = bload<real>(rF, ol, (3 + f) * NPb);
+                R.lg[0] = G(ol, 9); R.lg[1] = G(ol, 2); R.lg[2] = G(ol, 5); R.lg[3] = G(ol, 8);
+            }
+            if (own) {
+                const unsigned o = colb + planeb * (unsigned)z;
+#pragma unroll
+                for (int f = 0; f < 3; f++) { R.v[f] = bload<real>(rF, o, f * NPb); R.df[f] = bload<real>(rD, o, f * NPb); }
+            }
+            if (hk) {
+                const unsigned o = hcolb + planeb * (unsigned)z;
+#pragma unroll
+                for (int f = 0; f < 6; f++) R.hs[f] = bload<real>(rF, o, (3 + f) * NPb);
+                R.hg[0] = G(o, 9); R.hg[1] = G(o, 0 + hax); R.hg[2] = G(o, 3 + hax); R.hg[3] = G(o, 6 + hax);
+            }
+        };
+        real qf[7][3];
+#pragma unroll
+        for (int a = 0; a < 7; a++)
+#pragma unroll
+            for (int f = 0; f < 3; f++) qf[a][f] = 0;
+        if (inb) {
+            // warm the queue with planes z0-3 .. z0+2, one plane at a time
+#pragma unroll 1
+            for (int a = 1; a < 7; a++) {
+                const unsigned o = colb + planeb * (unsigned)(z0 - 4 + a);
+                real sg[6];
+#pragma unroll
+                for (int f = 0; f < 6; f++) sg[f] = bload<real>(rF, o, (3 + f) * NPb);
+#pragma unroll
+                for (int q = 0; q < 6; q++)
+#pragma unroll
+                    for (int f = 0; f < 3; f++) qf[q][f] = qf[q + 1][f];
+                fluxv(sg, G(o, 9), G(o, 2), G(o, 5), G(o, 8), qf[6]);
+            }
+        }
+        auto body = [&](PF &cur, PF &nxt, int z) {
+            const int b = z & 1;
+            real cv[3], cdf[3], J = 0;
+            if (inb) {
+                real fq[3], fr[3];
+                J = cur.g[0];
+                fluxv(cur.s, J, cur.g[1], cur.g[2], cur.g[3], fq);
+                fluxv(cur.s, J, cur.g[4], cur.g[5], cur.g[6], fr);
+#pragma unroll
+                for (int f = 0; f < 3; f++) { sfq[b][f][ly][lx + 3] = fq[f]; sfr[b][f][ly + 3][lx] = fr[f]; }
+#pragma unroll
+                for (int a = 0; a < 6; a++)
+#pragma unroll
+                    for (int f = 0; f < 3; f++) qf[a][f] = qf[a + 1][f];
+                fluxv(cur.ls, cur.lg[0], cur.lg[1], cur.lg[2], cur.lg[3], qf[6]);
+            }
+            if (own) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) { cv[f] = cur.v[f]; cdf[f] = cur.df[f]; }
+            }
+            if (hk) {
+                real fl[3];
+                fluxv(cur.hs, cur.hg[0], cur.hg[1], cur.hg[2], cur.hg[3], fl);
+                if (hk == 1) {
+#pragma unroll
+                    for (int f = 0; f < 3; f++) sfq[b][f][hcy][hcx] = fl[f];
+                } else {
+#pragma unroll
+                    for (int f = 0; f < 3; f++) sfr[b][f][hcy][hcx] = fl[f];
+                }
+            }
+            __syncthreads();
+            // unconditional: past the chunk's last plane this reads plane z1
+            // (and z1+3 for the lead), which lie inside the block because
+            // the march stops six planes short of the s faces; buffer loads
+            // return 0 for anything beyond the arrays
+            prefetch(nxt, z + 1);
+            if (own) {
+                const unsigned o = colb + planeb * (unsigned)z;
+                real rho = HET ? bload<real>(rM, o, 2u * NPb) : (real)B.rho;
+                real rhoJ_inv = (real)1 / (rho * J);
+#pragma unroll
+                for (int f = 0; f < 3; f++) {
+                    real a = d1_row_s<real>(sbp, &sfq[b][f][ly][0], lx + 3, gx, B.nq, ihq);
+                    real bb = d1_col_s<real>(sbp, &sfr[b][f][0][lx], TX, ly + 3, gy, B.nr, ihr);
+                    real c = ihs * (c1 * (qf[4][f] - qf[2][f]) + c2 * (qf[5][f] - qf[1][f]) + c3 * (qf[6][f] - qf[0][f]));
+                    real df = A * cdf[f] + (a + bb + c) * rhoJ_inv;
+                    bstore<real>(rD, o, f * NPb, df);
+                    bstore<real>(rO, o, f * NPb, cv[f] + bdt * df);
+                }
+            }
+        };
+        PF pa, pb;
+        prefetch(pa, z0);
+        int z = z0;
+        for (; z + 1 < z1; z += 2) {
+            body(pa, pb, z);
+            body(pb, pa, z + 1);
+        }
+        if (z < z1) body(pa, pb, z);
+    } else {
+        // ---------------- stress rates: Hooke's law on the velocity gradient
+        struct PF {
+            real v[3];          // centre velocity
+            real g[9];          // centre metrics (no J)
+            real s[6], df[6];   // centre stress and rates (update)
+            real hv[3], lv[3];  // halo and lead (z+3) velocity
+        };
+        auto prefetch = [&](PF &R, int z) {
+            if (inb) {
+                const unsigned o = colb + planeb * (unsigned)z;
+#pragma unroll
+                for (int f = 0; f < 3; f++) R.v[f] = bload<real>(rF, o, f * NPb);
+                const unsigned ol = o + 3u * planeb;
+#pragma unroll
+                for (int f = 0; f < 3; f++) R.lv[f] = bload<real>(rF, ol, f * NPb);
+            }
+            if (own) {
+                const unsigned o = colb + planeb * (unsigned)z;
+#pragma unroll
+                for (int c = 0; c < 9; c++) R.g[c] = G(o, c);
+#pragma unroll
+                for (int f = 0; f < 6; f++) {
+                    R.s[f] = bload<real>(rF, o, (3 + f) * NPb);
+                    R.df[f] = bload<real>(rD, o, (3 + f) * NPb);
+                }
+            }
+            if (hk) {
+                const unsigned o = hcolb + planeb * (unsigned)z;
+#pragma unroll
+                for (int f = 0; f < 3; f++) R.hv[f] = bload<real>(rF, o, f * NPb);
+            }
+        };
+        real qv[7][3];
+#pragma unroll
+        for (int a = 0; a < 7; a++)
+#pragma unroll
+            for (int f = 0; f < 3; f++) qv[a][f] = 0;
+        if (inb) {
+#pragma unroll 1
+            for (int a = 1; a < 7; a++) {
+                const unsigned o = colb + planeb * (unsigned)(z0 - 4 + a);
+#pragma unroll
+                for (int q = 0; q < 6; q++)
+#pragma unroll
+                    for (int f = 0; f < 3; f++) qv[q][f] = qv[q + 1][f];
+#pragma unroll
+                for (int f = 0; f < 3; f++) qv[6][f] = bload<real>(rF, o, f * NPb);
+            }
+        }
+        auto body = [&](PF &cur, PF &nxt, int z) {
+            const int b = z & 1;
+            real cg[9], cs[6], cdf[6];
+            if (inb) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) sv[b][f][ly + 3][lx + 3] = cur.v[f];
+#pragma unroll
+                for (int a = 0; a < 6; a++)
+#pragma unroll
+                    for (int f = 0; f < 3; f++) qv[a][f] = qv[a + 1][f];
+#pragma unroll
+                for (int f = 0; f < 3; f++) qv[6][f] = cur.lv[f];
+            }
+            if (own) {
+#pragma unroll
+                for (int c = 0; c < 9; c++) cg[c] = cur.g[c];
+#pragma unroll
+                for (int f = 0; f < 6; f++) { cs[f] = cur.s[f]; cdf[f] = cur.df[f]; }
+            }
+            if (hk) {
+                if (hk == 1) {
+#pragma unroll
+                    for (int f = 0; f < 3; f++) sv[b][f][hcy + 3][hcx] = cur.hv[f];
+                } else {
+#pragma unroll
+                    for (int f = 0; f < 3; f++) sv[b][f][hcy][hcx + 3] = cur.hv[f];
+                }
+            }
+            __syncthreads();
+            // unconditional: past the chunk's last plane this reads plane z1
+            // (and z1+3 for the lead), which lie inside the block because
+            // the march stops six planes short of the s faces; buffer loads
+            // return 0 for anything beyond the arrays
+            prefetch(nxt, z + 1);
+            if (own) {
+                const unsigned o = colb + planeb * (unsigned)z;
+                real Ux[3], Uy[3], Uz[3];
+#pragma unroll
+                for (int f = 0; f < 3; f++) {
+                    real dq = d1_row_s<real>(sbp, &sv[b][f][ly + 3][0], lx + 3, gx, B.nq, ihq);
+                    real dr = d1_col_s<real>(sbp, &sv[b][f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
+                    real ds = ihs * (c1 * (qv[4][f] - qv[2][f]) + c2 * (qv[5][f] - qv[1][f]) + c3 * (qv[6][f] - qv[0][f]));
+                    Ux[f] = cg[0] * dq + cg[1] * dr + cg[2] * ds;
+                    Uy[f] = cg[3] * dq + cg[4] * dr + cg[5] * ds;
+                    Uz[f] = cg[6] * dq + cg[7] * dr + cg[8] * ds;
+                }
+                real lam, mu;
+                if (HET) { lam = bload<real>(rM, o, 0u); mu = bload<real>(rM, o, NPb); }
+                else { lam = (real)B.lam; mu = (real)B.mu; }
+                real l2m = lam + (real)2 * mu;
+                real DU[6];
+                DU[0] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
+                DU[1] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
+                DU[2] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
+                DU[3] = mu * (Uy[0] + Ux[1]);
+                DU[4] = mu * (Uz[0] + Ux[2]);
+                DU[5] = mu * (Uz[1] + Uy[2]);
+#pragma unroll
+                for (int f = 0; f < 6; f++) {
+                    real df = A * cdf[f] + DU[f];
+                    bstore<real>(rD, o, (3 + f) * NPb, df);
+                    bstore<real>(rO, o, (3 + f) * NPb, cs[f] + bdt * df);
+                }
+            }
+        };
+        PF pa, pb;
+        prefetch(pa, z0);
+        int z = z0;
+        for (; z + 1 < z1; z += 2) {
+            body(pa, pb, z);
+            body(pb, pa, z + 1);
+        }
+        if (z < z1) body(pa, pb, z);
     }
 }
 

@@ -34,28 +34,35 @@ def lookup(prob, precision):
     return load_cache().get(key(prob, precision))
 
 
-def tune(prob, precision='fp32', coords=None, steps=2, variants=None, zchunks=ZCHUNKS, log=None):
-    """returns (variant, zchunk, ms) of the fastest main-kernel launch shape"""
-    variants = range(native.lib().wql_tile_variants()) if variants is None else variants
+def tune(prob, precision='fp32', coords=None, steps=2, variants=None, zchunks=ZCHUNKS, orders=((0, 1),),
+         log=None):
+    """times every (variant, zchunk, (order, group)) launch shape of the main
+    volume kernel; returns ((variant, zchunk, order, group, ms), results) of
+    the fastest and caches it"""
+    L = native.lib()
+    variants = range(L.wql_tile_variants() // 2) if variants is None else variants
     best = None
     results = []
     with Simulation(prob, precision=precision, coords=coords) as sim:
         for v in variants:
             for zc in zchunks:
-                native.check(native.lib().wql_set_tile(sim.ctx, v, zc))
-                sim.step(1)
-                sim.profile(True)
-                sim.step(steps)
-                sim.sync()
-                ms = sim.volume_kernel_ms()
-                sim.profile(False)
-                results.append((v, zc, ms))
-                if log:
-                    log(f'tile variant {v} zchunk {zc}: {ms:.4f} ms')
-                if best is None or ms < best[2]:
-                    best = (v, zc, ms)
+                for order, group in orders:
+                    native.check(L.wql_set_tile(sim.ctx, v, zc))
+                    native.check(L.wql_set_tile_order(sim.ctx, order, group))
+                    sim.step(1)
+                    sim.profile(True)
+                    sim.step(steps)
+                    sim.sync()
+                    ms = sim.volume_kernel_ms()
+                    sim.profile(False)
+                    results.append((v, zc, order, group, ms))
+                    if log:
+                        log(f'tile variant {v} zchunk {zc} order {order} group {group}: {ms:.4f} ms')
+                    if best is None or ms < best[4]:
+                        best = (v, zc, order, group, ms)
     cache = load_cache()
-    cache[key(prob, precision)] = {'variant': best[0], 'zchunk': best[1], 'ms': best[2]}
+    cache[key(prob, precision)] = {'variant': best[0], 'zchunk': best[1], 'order': best[2], 'group': best[3],
+                                   'ms': best[4]}
     try:
         with open(CACHE, 'w') as f:
             json.dump(cache, f, indent=1, sort_keys=True)

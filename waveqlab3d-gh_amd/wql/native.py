@@ -17,7 +17,7 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_estimate_bytes', 'wql_init_state', 'wql_set_fields', 'wql_get_fields', 'wql_get_rates',
            'wql_set_pml', 'wql_get_pml', 'wql_get_pml_rates', 'wql_get_metrics', 'wql_get_frame',
            'wql_set_fault', 'wql_get_fault', 'wql_get_fault_rates', 'wql_eval_rates', 'wql_step',
-           'wql_sync', 'wql_time', 'wql_set_tile', 'wql_get_tile', 'wql_tile_variants', 'wql_use_graph', 'wql_profile',
+           'wql_sync', 'wql_time', 'wql_set_tile', 'wql_get_tile', 'wql_set_tile_order', 'wql_get_tile_order', 'wql_tile_variants', 'wql_use_graph', 'wql_profile',
            'wql_volume_kernel_ms']
 
 
@@ -78,6 +78,8 @@ def lib():
             'wql_time': ([vp], C.c_double),
             'wql_set_tile': ([vp, C.c_int, C.c_int], C.c_int),
             'wql_get_tile': ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+            'wql_set_tile_order': ([vp, C.c_int, C.c_int], C.c_int),
+            'wql_get_tile_order': ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
             'wql_tile_variants': ([], C.c_int),
             'wql_use_graph': ([vp, C.c_int], C.c_int),
             'wql_profile': ([vp, C.c_int], C.c_int),

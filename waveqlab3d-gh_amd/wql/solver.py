@@ -88,22 +88,29 @@ class Simulation:
         arr, keep = block_descs(prob, coords)
         check(L.wql_create(C.byref(pd), arr, C.byref(self.ctx)))
         self.dt = pd.dt
+        order = None
         if tile is None and zchunk is None:
             from . import autotune
             best = autotune.lookup(prob, precision)
             if best is not None:
                 tile, zchunk = best['variant'], best['zchunk']
+                order = (best.get('order', 0), best.get('group', 2))
         if tile is not None or zchunk is not None:
-            check(L.wql_set_tile(self.ctx, 4 if tile is None else tile, 4 if zchunk is None else zchunk))
+            v0, z0 = C.c_int(), C.c_int()
+            check(L.wql_get_tile(self.ctx, C.byref(v0), C.byref(z0)))
+            check(L.wql_set_tile(self.ctx, v0.value if tile is None else tile, z0.value if zchunk is None else zchunk))
+        if order is not None:
+            check(L.wql_set_tile_order(self.ctx, *order))
         if graph:
             check(L.wql_use_graph(self.ctx, 1))
 
     @property
     def tile(self):
-        """(variant, zchunk) of the main volume kernel's launch shape"""
-        v, z = C.c_int(), C.c_int()
+        """(variant, zchunk, order, group) of the main volume kernel's launch shape"""
+        v, z, o, g = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         check(native.lib().wql_get_tile(self.ctx, C.byref(v), C.byref(z)))
-        return v.value, z.value
+        check(native.lib().wql_get_tile_order(self.ctx, C.byref(o), C.byref(g)))
+        return v.value, z.value, o.value, g.value
 
     def close(self):
         if self.ctx:
