@@ -121,7 +121,7 @@ def cpu_baseline(W):
     sub = workloads.WORKLOADS.get(name, workloads.rupture)(scale=1.0)
     prob = sub['problem']
     od = O.OracleDomain(prob, coords=sub['coords'])
-    nsteps = 2
+    nsteps = 3
     t0 = time.perf_counter()
     od.step(nsteps)
     sec = (time.perf_counter() - t0) / nsteps

@@ -87,7 +87,9 @@ SbpTable make_table()
 
 struct DevBlock {
     int nq, nr, ns;
-    long NP;
+    int px;                 // row pitch of the volume arrays
+    long NP;                // px*nr*ns (elements per field on the device)
+    long ND;                // nq*nr*ns (dense, as the host sees a field)
     int cart;
     int bc[6], pml[6], npml;
     double lam, mu, rho;
@@ -147,7 +149,7 @@ Blk<real> blk_view(const wql_ctx *c, int ib, int cur)
     const DevBlock &d = c->b[ib];
     Blk<real> B;
     memset(&B, 0, sizeof B);
-    B.nq = d.nq; B.nr = d.nr; B.ns = d.ns; B.NP = d.NP;
+    B.nq = d.nq; B.nr = d.nr; B.ns = d.ns; B.px = d.px; B.NP = d.NP;
     B.Fin = (const real *)d.F[cur];
     B.Fout = (real *)d.F[1 - cur];
     B.DF = (real *)d.DF;
@@ -215,8 +217,9 @@ TileList tile_list(const wql_ctx *c)
     return L;
 }
 
-// s-band planes handled by the plane-parallel kernel (no PML on that side)
-TileList band_list(const wql_ctx *c)
+// s-band planes: the six closure planes at each s face that has no PML, as
+// one 6-plane chunk per tile (per_plane = 0) or one plane per tile
+TileList band_list(const wql_ctx *c, int per_plane)
 {
     TileList L;
     memset(&L, 0, sizeof L);
@@ -227,8 +230,8 @@ TileList band_list(const wql_ctx *c)
         int r[6];
         main_range(d, r);
         if (r[1] <= r[0] || r[3] <= r[2]) continue;
-        int lo0 = d.pml[4] ? r[4] : 0, hi1 = d.pml[5] ? r[5] : d.ns;
-        int seg[2][2] = {{lo0, r[4]}, {r[5], hi1}};
+        // without PML main_range leaves exactly planes [0, 6) and [ns-6, ns)
+        int seg[2][2] = {{0, d.pml[4] ? 0 : r[4]}, {d.pml[5] ? d.ns : r[5], d.ns}};
         for (int k = 0; k < 2; k++) {
             if (seg[k][1] <= seg[k][0]) continue;
             TileGeom &g = L.g[L.n++];
@@ -237,14 +240,15 @@ TileList band_list(const wql_ctx *c)
             g.za0 = seg[k][0]; g.za1 = seg[k][1];
             g.ntx = (r[1] - r[0] + TX - 1) / TX;
             g.nty = (r[3] - r[2] + TY - 1) / TY;
-            g.zc = 1;
-            g.nzc = g.za1 - g.za0;
+            g.zc = per_plane ? 1 : g.za1 - g.za0;
+            g.nzc = per_plane ? g.za1 - g.za0 : 1;
             g.first = first;
             first += (long)g.ntx * g.nty * g.nzc;
         }
     }
     L.total = first;
-    L.order = 1;
+    L.order = 0;
+    L.gw = 1;
     return L;
 }
 
@@ -291,10 +295,10 @@ void launch_main_v(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0,
         if ((double)c->b[ib].NP * 10.0 * sizeof(real) >= 2147483647.0) buf_ok = false;
     if (c->variant >= 16 && buf_ok) {
         dim3 grd2((unsigned)(((2 * TL.total + 7) / 8) * 8));
-        if (curv && het) k_volume_split<real, TX, TY, true, true><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else if (curv) k_volume_split<real, TX, TY, true, false><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else if (het) k_volume_split<real, TX, TY, false, true><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else k_volume_split<real, TX, TY, false, false><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        if (curv && het) k_volume_split<real, TX, TY, true, true, 0><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (curv) k_volume_split<real, TX, TY, true, false, 0><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (het) k_volume_split<real, TX, TY, false, true, 0><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else k_volume_split<real, TX, TY, false, false, 0><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
         return;
     }
     if (c->variant >= 12 && buf_ok) {
@@ -338,16 +342,32 @@ void launch_main(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0, r
 template <typename real>
 void launch_band(const wql_ctx *c, Pair<real> &P, real A, real bdt)
 {
-    TileList TB = band_list(c);
-    if (TB.total <= 0) return;
-    dim3 blk(32, 8);
-    dim3 grd((unsigned)(((TB.total + 7) / 8) * 8));
+    // the six closure planes at each s face without PML: role-split march in
+    // closure mode, one 6-plane chunk per face and tile (32 x 8 tiles)
     bool curv = !c->b[0].cart || !c->b[1].cart;
     bool het = c->b[0].M || c->b[1].M;
-    if (curv && het) k_volume_plane<real, 32, 8, true, true><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
-    else if (curv) k_volume_plane<real, 32, 8, true, false><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
-    else if (het) k_volume_plane<real, 32, 8, false, true><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
-    else k_volume_plane<real, 32, 8, false, false><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+    dim3 blk(32, 8);
+    bool buf_ok = true;
+    for (int ib = 0; ib < 2; ib++)
+        if ((double)c->b[ib].NP * 10.0 * sizeof(real) >= 2147483647.0) buf_ok = false;
+    if (!buf_ok) {
+        // arrays beyond 32-bit buffer offsets: plane-parallel kernel per plane
+        TileList TP = band_list(c, 1);
+        if (TP.total <= 0) return;
+        dim3 grd((unsigned)(((TP.total + 7) / 8) * 8));
+        if (curv && het) k_volume_plane<real, 32, 8, true, true><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
+        else if (curv) k_volume_plane<real, 32, 8, true, false><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
+        else if (het) k_volume_plane<real, 32, 8, false, true><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
+        else k_volume_plane<real, 32, 8, false, false><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
+        return;
+    }
+    TileList TB = band_list(c, 0);
+    if (TB.total <= 0) return;
+    dim3 grd((unsigned)(((2 * TB.total + 7) / 8) * 8));
+    if (curv && het) k_volume_split<real, 32, 8, true, true, 1><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+    else if (curv) k_volume_split<real, 32, 8, true, false, 1><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+    else if (het) k_volume_split<real, 32, 8, false, true, 1><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+    else k_volume_split<real, 32, 8, false, false, 1><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
 }
 
 template <typename real>
@@ -469,6 +489,18 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
     if (!table_done) {
         SbpTable T = make_table();
         HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_sbp), &T, sizeof T));
+        double w64[12][9];
+        float w32[12][9];
+        memset(w64, 0, sizeof w64);
+        for (int r = 0; r < 6; r++) {
+            for (int t = 0; t < T.left[r].n; t++) w64[r][r + T.left[r].off[t]] += T.left[r].c[t];
+            // row n-6+r; plane (n-6+r)+off relative to n-9
+            for (int t = 0; t < T.right[r].n; t++) w64[6 + r][3 + r + T.right[r].off[t]] += T.right[r].c[t];
+        }
+        for (int r = 0; r < 12; r++)
+            for (int p = 0; p < 9; p++) w32[r][p] = (float)w64[r][p];
+        HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_sbw64), w64, sizeof w64));
+        HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_sbw32), w32, sizeof w32));
         table_done = true;
     }
     int *d_err = nullptr;
@@ -478,27 +510,32 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
         const wql_block_desc &bd = blocks[ib];
         DevBlock &d = c->b[ib];
         d.nq = bd.nq; d.nr = bd.nr; d.ns = bd.ns;
-        d.NP = (long)bd.nq * bd.nr * bd.ns;
+        // q rows padded to a multiple of 128 bytes: every row of a 32-wide
+        // tile then starts on a cache line
+        const int lane_elems = 128 / (int)sizeof(real);
+        d.px = (bd.nq + lane_elems - 1) / lane_elems * lane_elems;
+        d.NP = (long)d.px * bd.nr * bd.ns;
+        d.ND = (long)bd.nq * bd.nr * bd.ns;
         d.cart = bd.cartesian ? 1 : 0;
         for (int s = 0; s < 6; s++) { d.bc[s] = bd.bc[s]; d.pml[s] = bd.pml[s] ? 1 : 0; }
         d.npml = bd.npml;
-        const long NP = d.NP;
+        const long NP = d.NP, ND = d.ND;
         double *X = nullptr, *M64 = nullptr, *G64 = nullptr;
-        HIPCHK(hipMalloc(&X, 3 * NP * sizeof(double)));
-        HIPCHK(hipMalloc(&M64, 3 * NP * sizeof(double)));
-        HIPCHK(hipMalloc(&G64, 10 * NP * sizeof(double)));
-        HIPCHK(hipMemcpy(X, bd.X, 3 * NP * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&X, 3 * ND * sizeof(double)));
+        HIPCHK(hipMalloc(&M64, 3 * ND * sizeof(double)));
+        HIPCHK(hipMalloc(&G64, 10 * ND * sizeof(double)));
+        HIPCHK(hipMemcpy(X, bd.X, 3 * ND * sizeof(double), hipMemcpyHostToDevice));
         double rho = bd.rho_s_p[0], cs = bd.rho_s_p[1], cp = bd.rho_s_p[2];
         d.lam = rho * (cp * cp - 2.0 * (cs * cs));
         d.mu = rho * (cs * cs);
         d.rho = rho;
-        unsigned g = (unsigned)((NP + 255) / 256);
+        unsigned g = (unsigned)((ND + 255) / 256);
         if (bd.M) {
-            HIPCHK(hipMemcpy(M64, bd.M, 3 * NP * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(M64, bd.M, 3 * ND * sizeof(double), hipMemcpyHostToDevice));
         } else {
-            k_fill<double><<<g, 256, 0, c->stream>>>(NP, M64, d.lam);
-            k_fill<double><<<g, 256, 0, c->stream>>>(NP, M64 + NP, d.mu);
-            k_fill<double><<<g, 256, 0, c->stream>>>(NP, M64 + 2 * NP, d.rho);
+            k_fill<double><<<g, 256, 0, c->stream>>>(ND, M64, d.lam);
+            k_fill<double><<<g, 256, 0, c->stream>>>(ND, M64 + ND, d.mu);
+            k_fill<double><<<g, 256, 0, c->stream>>>(ND, M64 + 2 * ND, d.rho);
         }
         k_metrics<<<g, 256, 0, c->stream>>>(d.nq, d.nr, d.ns, X, G64);
         if (int rc = check_launch()) return rc;
@@ -517,7 +554,7 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
         }
         if (c->prob.plastic && c->prob.s0_kind == 1) {
             if (int rc = dalloc(c, (void **)&d.Y, NP * sizeof(double))) return rc;
-            HIPCHK(hipMemcpyAsync(d.Y, X + NP, NP * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            k_pad<double><<<(unsigned)((NP + 255) / 256), 256, 0, c->stream>>>(1, d.nq, d.nr, d.ns, d.px, X + ND, d.Y);
         }
         if (c->prob.plastic) {
             if (int rc = dalloc(c, (void **)&d.P, 2 * NP * sizeof(double))) return rc;
@@ -525,17 +562,17 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
         }
         if (!d.cart) {
             if (int rc = dalloc(c, &d.G, 10 * NP * sizeof(real))) return rc;
-            k_convert<real><<<(unsigned)((10 * NP + 255) / 256), 256, 0, c->stream>>>(10 * NP, G64, (real *)d.G);
+            k_pad<real><<<(unsigned)((10 * NP + 255) / 256), 256, 0, c->stream>>>(10, d.nq, d.nr, d.ns, d.px, G64, (real *)d.G);
         } else {
             // metrics of a uniform block: the value at the block centre
             long pc = (long)(d.nq / 2) + (long)d.nq * ((d.nr / 2) + (long)d.nr * (d.ns / 2));
             HIPCHK(hipStreamSynchronize(c->stream));
             for (int k = 0; k < 10; k++)
-                HIPCHK(hipMemcpy(&d.gc[k], G64 + k * NP + pc, sizeof(double), hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(&d.gc[k], G64 + k * ND + pc, sizeof(double), hipMemcpyDeviceToHost));
         }
         if (bd.M) {
             if (int rc = dalloc(c, &d.M, 3 * NP * sizeof(real))) return rc;
-            k_convert<real><<<(unsigned)((3 * NP + 255) / 256), 256, 0, c->stream>>>(3 * NP, M64, (real *)d.M);
+            k_pad<real><<<(unsigned)((3 * NP + 255) / 256), 256, 0, c->stream>>>(3, d.nq, d.nr, d.ns, d.px, M64, (real *)d.M);
         }
         for (int k = 0; k < 2; k++) {
             if (int rc = dalloc(c, &d.F[k], 9 * NP * sizeof(real))) return rc;
@@ -615,6 +652,52 @@ int put_field(wql_ctx *c, void *dev, const double *host, long n)
     return 0;
 }
 
+// volume arrays between the dense host layout [nf][ns][nr][nq] (FP64) and
+// the padded device layout [nf][ns][nr][px]: one contiguous copy through a
+// dense FP64 staging buffer and a (un)padding kernel on the context's stream
+template <typename real>
+int copy_vol_t(wql_ctx *c, const DevBlock &d, const void *dev, double *host, int nf)
+{
+    const long n = (long)nf * d.ND;
+    double *stage = nullptr;
+    if (hipMalloc(&stage, n * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(staging)");
+    k_unpad<real><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(nf, d.nq, d.nr, d.ns, d.px, (const real *)dev,
+                                                                       stage);
+    hipError_t e = hipMemcpyAsync(host, stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(stage);
+    if (e != hipSuccess) return fail(WQL_E_HIP, "volume download: %s", hipGetErrorString(e));
+    return check_launch();
+}
+
+template <typename real>
+int put_vol_t(wql_ctx *c, const DevBlock &d, void *dev, const double *host, int nf)
+{
+    const long n = (long)nf * d.ND;
+    double *stage = nullptr;
+    if (hipMalloc(&stage, n * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(staging)");
+    hipError_t e = hipMemcpyAsync(stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        const long np = (long)nf * d.nr * d.ns * d.px;
+        k_pad<real><<<(unsigned)((np + 255) / 256), 256, 0, c->stream>>>(nf, d.nq, d.nr, d.ns, d.px, stage,
+                                                                         (real *)dev);
+        e = hipStreamSynchronize(c->stream);
+    }
+    hipFree(stage);
+    if (e != hipSuccess) return fail(WQL_E_HIP, "volume upload: %s", hipGetErrorString(e));
+    return check_launch();
+}
+
+int copy_vol(wql_ctx *c, const DevBlock &d, const void *dev, double *host, int nf)
+{
+    return c->prec == WQL_FP64 ? copy_vol_t<double>(c, d, dev, host, nf) : copy_vol_t<float>(c, d, dev, host, nf);
+}
+
+int put_vol(wql_ctx *c, const DevBlock &d, void *dev, const double *host, int nf)
+{
+    return c->prec == WQL_FP64 ? put_vol_t<double>(c, d, dev, host, nf) : put_vol_t<float>(c, d, dev, host, nf);
+}
+
 int valid_block(const wql_block_desc &b)
 {
     if (b.nq < 13 || b.nr < 13 || b.ns < 13) return fail(WQL_E_ARG, "block needs at least 13 nodes per direction");
@@ -643,7 +726,8 @@ size_t wql_estimate_bytes(const wql_problem *prob, const wql_block_desc blocks[2
     size_t r = prob && prob->precision == WQL_FP32 ? 4 : 8, tot = 0;
     for (int ib = 0; ib < 2; ib++) {
         const wql_block_desc &b = blocks[ib];
-        size_t NP = (size_t)b.nq * b.nr * b.ns;
+        const size_t lane = 128 / r;
+        size_t NP = (b.nq + lane - 1) / lane * lane * (size_t)b.nr * b.ns;  // padded q rows
         size_t v = 27 * NP * r;                                // F x2, DF
         if (!b.cartesian) v += 10 * NP * r;                    // metrics + J
         if (b.M) v += 3 * NP * r;
@@ -760,19 +844,19 @@ int wql_init_state(wql_ctx *c)
 int wql_set_fields(wql_ctx *c, int ib, const double *F)
 {
     if (!c || ib < 0 || ib > 1 || !F) return fail(WQL_E_ARG, "bad argument");
-    return put_field(c, c->b[ib].F[c->cur], F, 9 * c->b[ib].NP);
+    return put_vol(c, c->b[ib], c->b[ib].F[c->cur], F, 9);
 }
 
 int wql_get_fields(wql_ctx *c, int ib, double *F)
 {
     if (!c || ib < 0 || ib > 1 || !F) return fail(WQL_E_ARG, "bad argument");
-    return copy_field(c, c->b[ib].F[c->cur], F, 9 * c->b[ib].NP);
+    return copy_vol(c, c->b[ib], c->b[ib].F[c->cur], F, 9);
 }
 
 int wql_get_rates(wql_ctx *c, int ib, double *DF)
 {
     if (!c || ib < 0 || ib > 1 || !DF) return fail(WQL_E_ARG, "bad argument");
-    return copy_field(c, c->b[ib].DF, DF, 9 * c->b[ib].NP);
+    return copy_vol(c, c->b[ib], c->b[ib].DF, DF, 9);
 }
 
 int wql_set_pml(wql_ctx *c, int ib, int side, const double *Q)
@@ -802,10 +886,10 @@ int wql_get_metrics(wql_ctx *c, int ib, double *metric)
     const DevBlock &d = c->b[ib];
     if (d.cart) {
         for (int k = 0; k < 10; k++)
-            for (long p = 0; p < d.NP; p++) metric[k * d.NP + p] = d.gc[k];
+            for (long p = 0; p < d.ND; p++) metric[k * d.ND + p] = d.gc[k];
         return 0;
     }
-    return copy_field(c, d.G, metric, 10 * d.NP);
+    return copy_vol(c, d, d.G, metric, 10);
 }
 
 int wql_get_frame(wql_ctx *c, int ib, int side, double *nml)

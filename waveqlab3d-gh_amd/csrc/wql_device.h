@@ -42,10 +42,12 @@ struct FaultParams {
 template <typename real>
 struct Blk {
     int nq, nr, ns;
-    long NP;                 // nq*nr*ns
+    int px;                  // row pitch of the volume arrays (>= nq, rows 128-byte aligned)
+    long NP;                 // elements per field: px*nr*ns
     const real *Fin;         // [9][NP] stage input
     real *Fout;              // [9][NP] stage output
     real *DF;                // [9][NP] low-storage RK rates
+    // volume arrays are [field][s][r][q] with q rows padded to px elements
     const real *G;           // [10][NP] metrics + J (curvilinear); nullptr for Cartesian
     const real *M;           // [3][NP] lambda mu rho (heterogeneous); nullptr if homogeneous
     double gc[10];           // constant metrics + J (Cartesian)
@@ -62,7 +64,6 @@ struct Blk {
     const double *fx;        // [3][nr*ns] coordinates of the fault face (q-face carrying bc==0)
     double *P;               // [2][NP] plastic strain rate / accumulated plastic strain (FP64)
     const double *Y;         // [NP] y-coordinate (plastic depth-dependent background stress)
-    double *Xall;            // [3][NP] coordinates (FP64, only when needed by plasticity)
 };
 
 // One block's copy of the interface state (iface.f90:13-48), FP64.

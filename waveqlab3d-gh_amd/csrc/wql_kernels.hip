@@ -19,6 +19,11 @@
 namespace wql {
 
 __constant__ SbpTable c_sbp;
+// The closure rows as dense weights over the 9 planes next to a face:
+// c_sbw*[r][p] for low rows r = 0..5 over planes 0..8, and for high rows
+// r = 6..11 (row n-12+r) over planes n-9..n-1.  Filled with c_sbp.
+__constant__ double c_sbw64[12][9];
+__constant__ float c_sbw32[12][9];
 
 // ----------------------------------------------------------------------------
 // SBP stencil taps for row i of an n-node line (absolute indices)
@@ -163,7 +168,7 @@ template <typename real>
 __device__ void volume_node_generic(const Blk<real> &B, int i, int j, int k, real A, real bdt)
 {
     const long NP = B.NP;
-    const long p = pidx(B.nq, B.nr, i, j, k);
+    const long p = pidx(B.px, B.nr, i, j, k);
     real U[3][NF];  // Ux, Uy, Uz (JU_xJU_yJU_z6.f90:12-812, summed as in 808)
     const int nn[3] = {B.nq, B.nr, B.ns};
     const int pos[3] = {i, j, k};
@@ -177,7 +182,7 @@ __device__ void volume_node_generic(const Blk<real> &B, int i, int j, int k, rea
         for (int t = 0; t < m; t++) {
             int ii = i, jj = j, kk = k;
             if (a == 0) ii = idx[t]; else if (a == 1) jj = idx[t]; else kk = idx[t];
-            long q = pidx(B.nq, B.nr, ii, jj, kk);
+            long q = pidx(B.px, B.nr, ii, jj, kk);
             real ct = (real)c[t];
             real J = (real)gmet(B, 9, q);
             real mt[3] = {(real)gmet(B, 0 + a, q), (real)gmet(B, 3 + a, q), (real)gmet(B, 6 + a, q)};
@@ -480,7 +485,7 @@ __global__ void __launch_bounds__(TX * TY) k_volume_main(Pair<real> P, TileList 
     const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
     const bool inb = (gx < B.nq && gy < B.nr);
     const long NP = B.NP;
-    const long plane = (long)B.nq * B.nr;
+    const long plane = (long)B.px * B.nr;
     const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
 
     // s-direction register queue: planes z-3 .. z+3 of this column
@@ -496,7 +501,7 @@ __global__ void __launch_bounds__(TX * TY) k_volume_main(Pair<real> P, TileList 
 #pragma unroll
             for (int f = 0; f < 3; f++) { qv[a][f] = qv[a + 1][f]; qf[a][f] = qf[a + 1][f]; }
         if (inb) {
-            long p = (long)gx + (long)B.nq * gy + plane * z;
+            long p = (long)gx + (long)B.px * gy + plane * z;
             real v[3], sg[6], fl[3];
             load_node<real, CURV>(B, p, v, sg);
             flux<real, CURV>(B, p, 2, sg, fl);
@@ -517,7 +522,7 @@ __global__ void __launch_bounds__(TX * TY) k_volume_main(Pair<real> P, TileList 
             int cx = c % WX, cy = c / WX;
             int ggx = xs - 3 + cx, ggy = ys + cy;
             if (ggx < 0 || ggx >= B.nq || ggy >= B.nr) continue;
-            long p = (long)ggx + (long)B.nq * ggy + plane * z;
+            long p = (long)ggx + (long)B.px * ggy + plane * z;
             real v[3], sg[6], fl[3];
             load_node<real, CURV>(B, p, v, sg);
             flux<real, CURV>(B, p, 0, sg, fl);
@@ -535,7 +540,7 @@ __global__ void __launch_bounds__(TX * TY) k_volume_main(Pair<real> P, TileList 
             int cy = hy < 3 ? hy : TY + hy;  // 0,1,2 and TY+3..TY+5
             int ggx = xs + cx, ggy = ys - 3 + cy;
             if (ggx >= B.nq || ggy < 0 || ggy >= B.nr) continue;
-            long p = (long)ggx + (long)B.nq * ggy + plane * z;
+            long p = (long)ggx + (long)B.px * ggy + plane * z;
             real v[3], sg[6], fl[3];
             load_node<real, CURV>(B, p, v, sg);
             flux<real, CURV>(B, p, 1, sg, fl);
@@ -544,7 +549,7 @@ __global__ void __launch_bounds__(TX * TY) k_volume_main(Pair<real> P, TileList 
         }
         __syncthreads();
         if (!own) continue;
-        const long p = (long)gx + (long)B.nq * gy + plane * z;
+        const long p = (long)gx + (long)B.px * gy + plane * z;
         // derivatives along q and r from LDS, along s from the queue
         real dvq[3], dvr[3], dvs[3], div[3];
 #pragma unroll
@@ -646,8 +651,8 @@ __global__ void __launch_bounds__(TX * TY) k_volume_pipe(Pair<real> P, TileList 
     const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
     const bool inb = (gx < B.nq && gy < B.nr);
     const long NP = B.NP;
-    const long plane = (long)B.nq * B.nr;
-    const long pcol = (long)gx + (long)B.nq * gy;
+    const long plane = (long)B.px * B.nr;
+    const long pcol = (long)gx + (long)B.px * gy;
     const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
 
     // this thread's halo node (x halo: 6*TY nodes, then y halo: 6*TX nodes)
@@ -659,14 +664,14 @@ __global__ void __launch_bounds__(TX * TY) k_volume_pipe(Pair<real> P, TileList 
         hcx = hx < 3 ? hx : TX + hx;
         hcy = hy;
         int ggx = xs - 3 + hcx, ggy = ys + hy;
-        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcol = (long)ggx + (long)B.nq * ggy; }
+        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcol = (long)ggx + (long)B.px * ggy; }
     } else if (tid < 6 * TY + 6 * TX) {
         int t2 = tid - 6 * TY;
         int hx = t2 % TX, hy = t2 / TX;
         hcy = hy < 3 ? hy : TY + hy;
         hcx = hx;
         int ggx = xs + hx, ggy = ys - 3 + hcy;
-        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcol = (long)ggx + (long)B.nq * ggy; }
+        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcol = (long)ggx + (long)B.px * ggy; }
     }
     const int hax = hk == 2 ? 1 : 0;  // metric axis of the halo flux
 
@@ -918,8 +923,8 @@ __global__ void __launch_bounds__(TX * TY) k_volume_buf(Pair<real> P, TileList T
     const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
     const bool inb = (gx < B.nq && gy < B.nr);
     const unsigned NPb = (unsigned)B.NP * RS;           // bytes per field
-    const unsigned planeb = (unsigned)(B.nq * B.nr) * RS;
-    const unsigned colb = (unsigned)(gx + B.nq * gy) * RS;
+    const unsigned planeb = (unsigned)(B.px * B.nr) * RS;
+    const unsigned colb = (unsigned)(gx + B.px * gy) * RS;
     const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
 
     const __amdgpu_buffer_rsrc_t rF = brsrc(B.Fin, 9u * NPb);
@@ -937,14 +942,14 @@ __global__ void __launch_bounds__(TX * TY) k_volume_buf(Pair<real> P, TileList T
         hcx = hx < 3 ? hx : TX + hx;
         hcy = hy;
         int ggx = xs - 3 + hcx, ggy = ys + hy;
-        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcolb = (unsigned)(ggx + B.nq * ggy) * RS; }
+        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcolb = (unsigned)(ggx + B.px * ggy) * RS; }
     } else if (tid < 6 * TY + 6 * TX) {
         int t2 = tid - 6 * TY;
         int hx = t2 % TX, hy = t2 / TX;
         hcy = hy < 3 ? hy : TY + hy;
         hcx = hx;
         int ggx = xs + hx, ggy = ys - 3 + hcy;
-        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcolb = (unsigned)(ggx + B.nq * ggy) * RS; }
+        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcolb = (unsigned)(ggx + B.px * ggy) * RS; }
     }
     const unsigned hax = hk == 2 ? 1u : 0u;  // metric axis of the halo flux
 
@@ -1197,7 +1202,7 @@ __device__ __forceinline__ real d1_col_s(const SbpLds<real> &S, const real *col,
 // of the fused kernel's state (one 7-plane s-queue of 3 values), which keeps
 // occupancy high; the metric and field lines both roles touch are fetched
 // from HBM once and hit L2 for the second role.
-template <typename real, int TX, int TY, bool CURV, bool HET>
+template <typename real, int TX, int TY, bool CURV, bool HET, int SB>
 __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList TL, real A0, real A1, real bdt)
 {
     constexpr int WX = TX + 6, WY = TY + 6;
@@ -1238,8 +1243,8 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
     const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
     const bool inb = (gx < B.nq && gy < B.nr);
     const unsigned NPb = (unsigned)B.NP * RS;
-    const unsigned planeb = (unsigned)(B.nq * B.nr) * RS;
-    const unsigned colb = (unsigned)(gx + B.nq * gy) * RS;
+    const unsigned planeb = (unsigned)(B.px * B.nr) * RS;
+    const unsigned colb = (unsigned)(gx + B.px * gy) * RS;
     const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
 
     const __amdgpu_buffer_rsrc_t rF = brsrc(B.Fin, 9u * NPb);
@@ -1259,14 +1264,14 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
         hcx = hx < 3 ? hx : TX + hx;
         hcy = hy;
         int ggx = xs - 3 + hcx, ggy = ys + hy;
-        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcolb = (unsigned)(ggx + B.nq * ggy) * RS; }
+        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcolb = (unsigned)(ggx + B.px * ggy) * RS; }
     } else if (tid < 6 * TY + 6 * TX) {
         int t2 = tid - 6 * TY;
         int hx = t2 % TX, hy = t2 / TX;
         hcy = hy < 3 ? hy : TY + hy;
         hcx = hx;
         int ggx = xs + hx, ggy = ys - 3 + hcy;
-        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcolb = (unsigned)(ggx + B.nq * ggy) * RS; }
+        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcolb = (unsigned)(ggx + B.px * ggy) * RS; }
     }
     const unsigned hax = hk == 2 ? 1u : 0u;
 
@@ -1277,6 +1282,16 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
         fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
     };
     const real c1 = (real)0.75, c2 = (real)(-0.15), c3 = (real)(1.0 / 60.0);
+    // SB = 0: interior chunk, s-queue of 7 planes; SB = 1: the six closure
+    // planes at an s face (low if the chunk starts at plane 0), with all 9
+    // planes they reach held in registers (planes 0..8, resp. ns-9..ns-1)
+    const bool slow = z0 == 0;
+    const int sbase = slow ? 0 : B.ns - 9;
+    auto wrow = [&](int z) -> int { return slow ? z : 6 + z - (B.ns - 6); };
+    auto sw = [&](int r, int p) -> real {
+        if constexpr (sizeof(real) == 4) return c_sbw32[r][p];
+        else return c_sbw64[r][p];
+    };
 
     if (role == 0) {
         // ---------------- velocity rates: div of the stress fluxes / (rho J)
@@ -1293,10 +1308,12 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
                 for (int f = 0; f < 6; f++) R.s[f] = bload<real>(rF, o, (3 + f) * NPb);
                 R.g[0] = G(o, 9); R.g[1] = G(o, 0); R.g[2] = G(o, 3); R.g[3] = G(o, 6);
                 R.g[4] = G(o, 1); R.g[5] = G(o, 4); R.g[6] = G(o, 7);
-                const unsigned ol = o + 3u * planeb;
+                if (SB == 0) {
+                    const unsigned ol = o + 3u * planeb;
 #pragma unroll
-                for (int f = 0; f < 6; f++) R.ls[f] = bload<real>(rF, ol, (3 + f) * NPb);
-                R.lg[0] = G(ol, 9); R.lg[1] = G(ol, 2); R.lg[2] = G(ol, 5); R.lg[3] = G(ol, 8);
+                    for (int f = 0; f < 6; f++) R.ls[f] = bload<real>(rF, ol, (3 + f) * NPb);
+                    R.lg[0] = G(ol, 9); R.lg[1] = G(ol, 2); R.lg[2] = G(ol, 5); R.lg[3] = G(ol, 8);
+                }
             }
             if (own) {
                 const unsigned o = colb + planeb * (unsigned)z;
@@ -1310,12 +1327,24 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
                 R.hg[0] = G(o, 9); R.hg[1] = G(o, 0 + hax); R.hg[2] = G(o, 3 + hax); R.hg[3] = G(o, 6 + hax);
             }
         };
-        real qf[7][3];
+        constexpr int NQF = SB == 0 ? 7 : 9;
+        real qf[NQF][3];
 #pragma unroll
-        for (int a = 0; a < 7; a++)
+        for (int a = 0; a < NQF; a++)
 #pragma unroll
             for (int f = 0; f < 3; f++) qf[a][f] = 0;
-        if (inb) {
+        if (SB != 0 && inb) {
+            // every plane the closure rows reach, loads all in flight together
+#pragma unroll
+            for (int a = 0; a < 9; a++) {
+                const unsigned o = colb + planeb * (unsigned)(sbase + a);
+                real sg[6];
+#pragma unroll
+                for (int f = 0; f < 6; f++) sg[f] = bload<real>(rF, o, (3 + f) * NPb);
+                fluxv(sg, G(o, 9), G(o, 2), G(o, 5), G(o, 8), qf[a]);
+            }
+        }
+        if (SB == 0 && inb) {
             // warm the queue with planes z0-3 .. z0+2, one plane at a time
 #pragma unroll 1
             for (int a = 1; a < 7; a++) {
@@ -1327,7 +1356,7 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
                 for (int q = 0; q < 6; q++)
 #pragma unroll
                     for (int f = 0; f < 3; f++) qf[q][f] = qf[q + 1][f];
-                fluxv(sg, G(o, 9), G(o, 2), G(o, 5), G(o, 8), qf[6]);
+                fluxv(sg, G(o, 9), G(o, 2), G(o, 5), G(o, 8), qf[NQF - 1]);
             }
         }
         auto body = [&](PF &cur, PF &nxt, int z) {
@@ -1340,11 +1369,13 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
                 fluxv(cur.s, J, cur.g[4], cur.g[5], cur.g[6], fr);
 #pragma unroll
                 for (int f = 0; f < 3; f++) { sfq[b][f][ly][lx + 3] = fq[f]; sfr[b][f][ly + 3][lx] = fr[f]; }
+                if (SB == 0) {
 #pragma unroll
-                for (int a = 0; a < 6; a++)
+                    for (int a = 0; a < NQF - 1; a++)
 #pragma unroll
-                    for (int f = 0; f < 3; f++) qf[a][f] = qf[a + 1][f];
-                fluxv(cur.ls, cur.lg[0], cur.lg[1], cur.lg[2], cur.lg[3], qf[6]);
+                        for (int f = 0; f < 3; f++) qf[a][f] = qf[a + 1][f];
+                    fluxv(cur.ls, cur.lg[0], cur.lg[1], cur.lg[2], cur.lg[3], qf[NQF - 1]);
+                }
             }
             if (own) {
 #pragma unroll
@@ -1375,7 +1406,16 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
                 for (int f = 0; f < 3; f++) {
                     real a = d1_row_s<real>(sbp, &sfq[b][f][ly][0], lx + 3, gx, B.nq, ihq);
                     real bb = d1_col_s<real>(sbp, &sfr[b][f][0][lx], TX, ly + 3, gy, B.nr, ihr);
-                    real c = ihs * (c1 * (qf[4][f] - qf[2][f]) + c2 * (qf[5][f] - qf[1][f]) + c3 * (qf[6][f] - qf[0][f]));
+                    real c;
+                    if constexpr (SB == 0) {
+                        c = ihs * (c1 * (qf[4][f] - qf[2][f]) + c2 * (qf[5][f] - qf[1][f]) + c3 * (qf[6][f] - qf[0][f]));
+                    } else {
+                        const int r = wrow(z);
+                        real acc = 0;
+#pragma unroll
+                        for (int q = 0; q < 9; q++) acc = acc + sw(r, q) * qf[q][f];
+                        c = ihs * acc;
+                    }
                     real df = A * cdf[f] + (a + bb + c) * rhoJ_inv;
                     bstore<real>(rD, o, f * NPb, df);
                     bstore<real>(rO, o, f * NPb, cv[f] + bdt * df);
@@ -1403,9 +1443,11 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
                 const unsigned o = colb + planeb * (unsigned)z;
 #pragma unroll
                 for (int f = 0; f < 3; f++) R.v[f] = bload<real>(rF, o, f * NPb);
-                const unsigned ol = o + 3u * planeb;
+                if (SB == 0) {
+                    const unsigned ol = o + 3u * planeb;
 #pragma unroll
-                for (int f = 0; f < 3; f++) R.lv[f] = bload<real>(rF, ol, f * NPb);
+                    for (int f = 0; f < 3; f++) R.lv[f] = bload<real>(rF, ol, f * NPb);
+                }
             }
             if (own) {
                 const unsigned o = colb + planeb * (unsigned)z;
@@ -1423,21 +1465,30 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
                 for (int f = 0; f < 3; f++) R.hv[f] = bload<real>(rF, o, f * NPb);
             }
         };
-        real qv[7][3];
+        constexpr int NQV = SB == 0 ? 7 : 9;
+        real qv[NQV][3];
 #pragma unroll
-        for (int a = 0; a < 7; a++)
+        for (int a = 0; a < NQV; a++)
 #pragma unroll
             for (int f = 0; f < 3; f++) qv[a][f] = 0;
-        if (inb) {
+        if (SB != 0 && inb) {
+#pragma unroll
+            for (int a = 0; a < 9; a++) {
+                const unsigned o = colb + planeb * (unsigned)(sbase + a);
+#pragma unroll
+                for (int f = 0; f < 3; f++) qv[a][f] = bload<real>(rF, o, f * NPb);
+            }
+        }
+        if (SB == 0 && inb) {
 #pragma unroll 1
             for (int a = 1; a < 7; a++) {
                 const unsigned o = colb + planeb * (unsigned)(z0 - 4 + a);
 #pragma unroll
-                for (int q = 0; q < 6; q++)
+                for (int q = 0; q < NQV - 1; q++)
 #pragma unroll
                     for (int f = 0; f < 3; f++) qv[q][f] = qv[q + 1][f];
 #pragma unroll
-                for (int f = 0; f < 3; f++) qv[6][f] = bload<real>(rF, o, f * NPb);
+                for (int f = 0; f < 3; f++) qv[NQV - 1][f] = bload<real>(rF, o, f * NPb);
             }
         }
         auto body = [&](PF &cur, PF &nxt, int z) {
@@ -1446,12 +1497,14 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
             if (inb) {
 #pragma unroll
                 for (int f = 0; f < 3; f++) sv[b][f][ly + 3][lx + 3] = cur.v[f];
+                if (SB == 0) {
 #pragma unroll
-                for (int a = 0; a < 6; a++)
+                    for (int a = 0; a < NQV - 1; a++)
 #pragma unroll
-                    for (int f = 0; f < 3; f++) qv[a][f] = qv[a + 1][f];
+                        for (int f = 0; f < 3; f++) qv[a][f] = qv[a + 1][f];
 #pragma unroll
-                for (int f = 0; f < 3; f++) qv[6][f] = cur.lv[f];
+                    for (int f = 0; f < 3; f++) qv[NQV - 1][f] = cur.lv[f];
+                }
             }
             if (own) {
 #pragma unroll
@@ -1481,7 +1534,16 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
                 for (int f = 0; f < 3; f++) {
                     real dq = d1_row_s<real>(sbp, &sv[b][f][ly + 3][0], lx + 3, gx, B.nq, ihq);
                     real dr = d1_col_s<real>(sbp, &sv[b][f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
-                    real ds = ihs * (c1 * (qv[4][f] - qv[2][f]) + c2 * (qv[5][f] - qv[1][f]) + c3 * (qv[6][f] - qv[0][f]));
+                    real ds;
+                    if constexpr (SB == 0) {
+                        ds = ihs * (c1 * (qv[4][f] - qv[2][f]) + c2 * (qv[5][f] - qv[1][f]) + c3 * (qv[6][f] - qv[0][f]));
+                    } else {
+                        const int r = wrow(z);
+                        real acc = 0;
+#pragma unroll
+                        for (int q = 0; q < 9; q++) acc = acc + sw(r, q) * qv[q][f];
+                        ds = ihs * acc;
+                    }
                     Ux[f] = cg[0] * dq + cg[1] * dr + cg[2] * ds;
                     Uy[f] = cg[3] * dq + cg[4] * dr + cg[5] * ds;
                     Uz[f] = cg[6] * dq + cg[7] * dr + cg[8] * ds;
@@ -1555,8 +1617,8 @@ __global__ void __launch_bounds__(TX * TY) k_volume_plane(Pair<real> P, TileList
     const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
     const bool inb = (gx < B.nq && gy < B.nr);
     const long NP = B.NP;
-    const long plane = (long)B.nq * B.nr;
-    const long p = (long)gx + (long)B.nq * gy + plane * z;
+    const long plane = (long)B.px * B.nr;
+    const long p = (long)gx + (long)B.px * gy + plane * z;
     const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
 
     auto fluxv = [&](const real sg[6], real J, real ax, real ay, real az, real fl[3]) {
@@ -1576,14 +1638,14 @@ __global__ void __launch_bounds__(TX * TY) k_volume_plane(Pair<real> P, TileList
         hcx = hx < 3 ? hx : TX + hx;
         hcy = hy;
         int ggx = xs - 3 + hcx, ggy = ys + hy;
-        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hp = (long)ggx + (long)B.nq * ggy + plane * z; }
+        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hp = (long)ggx + (long)B.px * ggy + plane * z; }
     } else if (tid < 6 * TY + 6 * TX) {
         int t2 = tid - 6 * TY;
         int hx = t2 % TX, hy = t2 / TX;
         hcy = hy < 3 ? hy : TY + hy;
         hcx = hx;
         int ggx = xs + hx, ggy = ys - 3 + hcy;
-        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hp = (long)ggx + (long)B.nq * ggy + plane * z; }
+        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hp = (long)ggx + (long)B.px * ggy + plane * z; }
     }
 
     real cv[3], cs[6], cg[10];
@@ -1948,7 +2010,7 @@ __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams
     const int nr = B.nr;
     const long FP = (long)nr * B.ns;
     const long fpi = (long)j + (long)nr * k;
-    const long pu = pidx(B1.nq, B1.nr, B1.nq - 1, j, k), pv = pidx(B2.nq, B2.nr, 0, j, k);
+    const long pu = pidx(B1.px, B1.nr, B1.nq - 1, j, k), pv = pidx(B2.px, B2.nr, 0, j, k);
     double u[9], v[9];
     for (int f = 0; f < 9; f++) { u[f] = (double)B1.Fin[f * B1.NP + pu]; v[f] = (double)B2.Fin[f * B2.NP + pv]; }
     const double *fr = B.frame[side];
@@ -2135,7 +2197,7 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
         i = 1 + (int)(r % (nq - 2)); j = 1 + (int)(r / (nq - 2));
     }
     const long NP = B.NP;
-    const long p = pidx(nq, nr, i, j, k);
+    const long p = pidx(B.px, nr, i, j, k);
     double u[9];
     for (int f = 0; f < 9; f++) u[f] = (double)B.Fin[f * NP + p];
     double sat[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -2216,7 +2278,7 @@ __global__ void k_plastic(Pair<real> P, PlasticArgs a)
     int w = a.x1[me] - a.x0[me], h = a.y1[me] - a.y0[me];
     int i = a.x0[me] + (int)(t % w), j = a.y0[me] + (int)((t / w) % h), k = a.z0[me] + (int)(t / ((long)w * h));
     const long NP = B.NP;
-    const long p = pidx(B.nq, B.nr, i, j, k);
+    const long p = pidx(B.px, B.nr, i, j, k);
     double s0[6];
     if (a.s0_kind == 1) {
         double y = B.Y[p];
@@ -2296,6 +2358,30 @@ __global__ void k_init_state(Pair<real> P, FaultParams f, Iface I0, Iface I1)
     double theta = f.rs_L / f.rs_V0 *
                    exp(((a * log(2.0 * sinh(T0l / (a * sn)))) - f.rs_f0 - a * log(f.rs_Vin / f.rs_V0)) / f.rs_b);
     (me == 0 ? I0 : I1).W[t] = f.rs_f0 + f.rs_b * log(f.rs_V0 / f.rs_L * theta);
+}
+
+// dense [nf][ns][nr][nq] FP64 -> padded [nf][ns][nr][px] volume array
+template <typename real>
+__global__ void k_pad(int nf, int nq, int nr, int ns, int px, const double *__restrict__ src, real *__restrict__ dst)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long rows = (long)nf * nr * ns;
+    if (t >= rows * px) return;
+    const long row = t / px;
+    const int i = (int)(t - row * px);
+    dst[t] = i < nq ? (real)src[row * nq + i] : (real)0;
+}
+
+// padded [nf][ns][nr][px] volume array -> dense [nf][ns][nr][nq] FP64
+template <typename real>
+__global__ void k_unpad(int nf, int nq, int nr, int ns, int px, const real *__restrict__ src, double *__restrict__ dst)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long rows = (long)nf * nr * ns;
+    if (t >= rows * nq) return;
+    const long row = t / nq;
+    const int i = (int)(t - row * nq);
+    dst[t] = (double)src[row * px + i];
 }
 
 // step time advance of a captured step graph: t = t + inc, the same FP64
