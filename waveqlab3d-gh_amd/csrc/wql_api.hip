@@ -728,11 +728,12 @@ size_t wql_estimate_bytes(const wql_problem *prob, const wql_block_desc blocks[2
         const wql_block_desc &b = blocks[ib];
         const size_t lane = 128 / r;
         size_t NP = (b.nq + lane - 1) / lane * lane * (size_t)b.nr * b.ns;  // padded q rows
+        size_t ND = (size_t)b.nq * b.nr * b.ns;                // dense
         size_t v = 27 * NP * r;                                // F x2, DF
         if (!b.cartesian) v += 10 * NP * r;                    // metrics + J
         if (b.M) v += 3 * NP * r;
-        if (prob && prob->plastic) v += 2 * NP * 8;
-        size_t init = 16 * NP * 8;                             // X, M64, G64 during wql_create
+        if (prob && prob->plastic) v += 3 * NP * 8;            // strain accumulators, y
+        size_t init = 16 * ND * 8;                             // dense X, M64, G64 during wql_create
         for (int s = 0; s < 6; s++)
             if (b.pml[s]) {
                 size_t n3[3] = {(size_t)b.nq, (size_t)b.nr, (size_t)b.ns};
@@ -740,7 +741,8 @@ size_t wql_estimate_bytes(const wql_problem *prob, const wql_block_desc blocks[2
                 v += 18 * n3[0] * n3[1] * n3[2] * r;
             }
         size_t faces = 2 * ((size_t)b.nr * b.ns + (size_t)b.nq * b.ns + (size_t)b.nq * b.nr) * 18 * 8;
-        tot += std::max(v, v / 2 + init) + faces + 25 * (size_t)b.nr * b.ns * 8;
+        // peak during wql_create: runtime arrays and the FP64 set-up arrays
+        tot += v + init + faces + 25 * (size_t)b.nr * b.ns * 8;
     }
     return tot;
 }

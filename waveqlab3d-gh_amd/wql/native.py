@@ -13,6 +13,9 @@ dp = C.POINTER(C.c_double)
 
 WQL_FP64, WQL_FP32 = 0, 1
 
+# error codes of include/wql.h
+WQL_E_ARG, WQL_E_NOMEM, WQL_E_HIP, WQL_E_NONFINITE, WQL_E_SOLVER, WQL_E_STATE = -1, -2, -3, -4, -5, -6
+
 EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wql_device_bytes',
            'wql_estimate_bytes', 'wql_init_state', 'wql_set_fields', 'wql_get_fields', 'wql_get_rates',
            'wql_set_pml', 'wql_get_pml', 'wql_get_pml_rates', 'wql_get_metrics', 'wql_get_frame',
