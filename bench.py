@@ -31,7 +31,9 @@ def kernel_name(prob, precision, variant):
     curv = 'true' if any(b.mesh != 'cartesian' for b in prob.blocks) else 'false'
     het = 'true' if any(b.material is not None for b in prob.blocks) else 'false'
     real = 'float' if precision == 'fp32' else 'double'
-    return f'{FAMILIES[v // 4]}<{real}, {tx}, {ty}, {curv}, {het}>'
+    fam = FAMILIES[v // 4]
+    extra = ', 0' if fam == 'k_volume_split' else ''  # interior (not s-closure) instantiation
+    return f'{fam}<{real}, {tx}, {ty}, {curv}, {het}{extra}>'
 
 
 def measured_traffic(workload, shape, kname):
@@ -112,6 +114,73 @@ def run(W, precision, steps, warmup, tile, zchunk, graph, order=None):
     return sec, kms, bytes_dev, shape
 
 
+def fp32_error(W, steps):
+    """FP32-volume against FP64 on the same workload and initial state after
+    `steps` steps: relative L2 of slip, of rupture time (nodes ruptured in
+    FP64) and of peak slip rate, on block 1's fault copy"""
+    import numpy as np
+    from wql.solver import Simulation
+    out = {}
+    for prec in ('fp64', 'fp32'):
+        with Simulation(W['problem'], precision=prec, coords=W['coords']) as sim:
+            if W.get('F0') is not None:
+                for ib in range(2):
+                    sim.set_fields(ib, W['F0'][ib])
+            if W.get('init_state'):
+                sim.init_state()
+            sim.step(steps)
+            sim.sync()
+            f = sim.fault(0)
+            out[prec] = dict(slip=f['S'][3], trup=f['trup'][0], psr=sim.fault_peak(0)[0])
+    a, b = out['fp64'], out['fp32']
+    rel = lambda x, y: float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))  # noqa: E731
+    rup = a['trup'] < 1e9
+    both = rup & (b['trup'] < 1e9)
+    return {'steps': steps, 'slip': rel(b['slip'], a['slip']),
+            'rupture_time': rel(b['trup'][both], a['trup'][both]) if both.any() else None,
+            'peak_slip_rate': rel(b['psr'], a['psr']),
+            'ruptured_nodes_fp64': int(rup.sum()), 'ruptured_nodes_fp32': int((b['trup'] < 1e9).sum()),
+            'max_slip_fp64': float(a['slip'].max())}
+
+
+def serving(n_scen, steps, rate, max_active, n_seq, seed=5):
+    """the serving stream: n_scen scenarios of the seeded size/friction
+    distribution, Poisson arrivals at `rate`/s, `steps` steps each, FP32
+    fields / FP64 friction through the continuous-batching server; beside
+    it the same scenarios run one after another in plain FP64 (first n_seq)"""
+    import numpy as np
+    from wql import workloads
+    from wql.serving import run_stream
+    from wql.solver import Simulation
+    scen = []
+    for k in range(n_scen):
+        sc = workloads.scenario(k, seed=seed)
+        sc['steps'] = steps
+        scen.append(sc)
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_scen))
+    arrivals -= arrivals[0]
+    # warm the library (first graph capture, code objects) outside the timing
+    run_stream(scen[:1], np.zeros(1), max_active=1, precision='fp32')
+    lat, wall, _ = run_stream(scen, arrivals, max_active=max_active, precision='fp32')
+    pts = sum(workloads.points(s['problem']) for s in scen) * 5 * steps
+    t0 = time.perf_counter()
+    for sc in scen[:n_seq]:
+        with Simulation(sc['problem'], precision='fp64', coords=sc['coords']) as sim:
+            sim.step(steps)
+            sim.sync()
+    seq = time.perf_counter() - t0
+    seq_rate = n_seq / seq
+    return {'scenarios': n_scen, 'steps_each': steps, 'arrival_rate_per_s': rate, 'max_active': max_active,
+            'sizes_per_block': [list(v) for v in workloads.SERVING_SIZES],
+            'p50_latency_s': round(float(np.percentile(lat, 50)), 4),
+            'p99_latency_s': round(float(np.percentile(lat, 99)), 4),
+            'scenarios_per_s': round(n_scen / wall, 4), 'wall_s': round(wall, 3),
+            'M_grid_pt_stages_per_s': round(pts / wall / 1e6, 2),
+            'sequential_fp64': {'scenarios': n_seq, 'scenarios_per_s': round(seq_rate, 4)},
+            'speedup_vs_sequential_fp64': round((n_scen / wall) / seq_rate, 3)}
+
+
 def cpu_baseline(W):
     """the oracle (C, 1 thread) on one LSRK step of a sub-volume of the same workload"""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
@@ -145,6 +214,12 @@ def main():
     ap.add_argument('--graph', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--err-steps', type=int, default=300, help='steps of the FP32-vs-FP64 slip comparison (0: off)')
+    ap.add_argument('--serve', type=int, default=64, help='scenarios in the serving stream (0: off)')
+    ap.add_argument('--serve-steps', type=int, default=100)
+    ap.add_argument('--serve-rate', type=float, default=20.0)
+    ap.add_argument('--serve-active', type=int, default=8)
+    ap.add_argument('--serve-seq', type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported '
@@ -193,6 +268,10 @@ def main():
                        'ms_per_step': round(res['fp64']['sec'] / a.steps * 1e3, 4),
                        'kernel_ms': round(res['fp64']['kms'], 5),
                        'speedup_fp32_over_fp64': round(r32['mpts'] / res['fp64']['mpts'], 3)}
+    if a.err_steps > 0 and not a.no_fp64:
+        out['fp32_error'] = fp32_error(W, a.err_steps)
+    if a.serve > 0:
+        out['serving'] = serving(a.serve, a.serve_steps, a.serve_rate, a.serve_active, min(a.serve_seq, a.serve))
     if not a.no_cpu:
         out['cpu_baseline'] = cpu_baseline(W)
     print(json.dumps(out), flush=True)

@@ -117,6 +117,12 @@ int wql_set_fault(wql_ctx *ctx, int block, const double *S, const double *W);
 int wql_get_fault(wql_ctx *ctx, int block, double *S, double *W, double *trup, double *Svel,
                   double *Uhat, double *Vhat);
 int wql_get_fault_rates(wql_ctx *ctx, int block, double *DS, double *DW);
+/* peak tangential slip rate per fault node over the run, sampled where the
+ * reference samples Svel (stage 1, CouplingForcing.f90:213-228) [nr*ns] */
+int wql_get_fault_peak(wql_ctx *ctx, int block, double *psr);
+/* fault-plane statistics on the device: {max slip, max slip rate, max peak
+ * slip rate, ruptured nodes (trup set), mean slip over them, earliest trup} */
+int wql_fault_stats(wql_ctx *ctx, int block, double stats[6]);
 
 /* One right-hand-side evaluation at time t with zero incoming rates:
  * what = 0: volume operator only (set_rates, domain.f90:908-924)
@@ -149,6 +155,31 @@ int wql_use_graph(wql_ctx *ctx, int on);
  * call when profiling is on (HIP events on the context's stream) */
 int wql_profile(wql_ctx *ctx, int on);
 double wql_volume_kernel_ms(wql_ctx *ctx);
+
+/* ---- serving: continuous batching of independent scenarios ------------
+ * No counterpart in the reference (one run per process there).  A server
+ * admits submitted scenarios first-fit by device footprint
+ * (wql_estimate_bytes) within its HBM budget and a cap on active
+ * scenarios, keeps each admitted scenario's state on the device, runs it on
+ * its own stream with a captured step graph in chunks of steps_per_launch,
+ * and retires it when its steps are done. */
+typedef struct wql_server wql_server;
+/* hbm_budget 0: 90% of the device memory free at creation */
+int wql_server_create(int max_active, size_t hbm_budget, int steps_per_launch, wql_server **out);
+int wql_server_destroy(wql_server *srv);
+/* copies the descriptors and arrays; *id identifies the scenario */
+int wql_server_submit(wql_server *srv, const wql_problem *prob, const wql_block_desc blocks[2], int nsteps,
+                      int *id);
+/* non-blocking progress: retires finished scenarios (ids into done[], at
+ * most max_done; the return value is their number), launches the next chunk
+ * of every active scenario whose chunk ended, admits queued ones */
+int wql_server_poll(wql_server *srv, int *done, int max_done);
+/* block 1's fault copy of a retired scenario: S[4][nr*ns], trup, peak slip
+ * rate [nr*ns], wql_fault_stats; frees the record (a failed scenario returns
+ * its error here) */
+int wql_server_result(wql_server *srv, int id, double *S, double *trup, double *psr, double stats[6]);
+/* {queued, active, device bytes in use, budget} */
+int wql_server_status(const wql_server *srv, long out[4]);
 
 #ifdef __cplusplus
 }

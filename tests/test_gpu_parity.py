@@ -225,3 +225,38 @@ def test_heterogeneous_rhs_vs_oracle(mesh):
             sim.eval_rates(0.3, what=1, stage=2)
             for ib in range(2):
                 assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= TOL, (tile, ib)
+
+
+@pytest.mark.parametrize('case', ['curv_sw', 'plastic_sw'])
+def test_fault_diagnostics(case):
+    """peak slip rate (running max of the stage-1 slip rate) and the device
+    fault-plane reductions against the same quantities formed on the host"""
+    _say(f'[gpu] fault diagnostics {case}')
+    z = G.load(case)
+    prob = G.problem(case)
+    args = z['traj_args']
+    F0 = H.pulse_init(prob, args)
+    with _sim(prob) as sim:
+        for ib in range(2):
+            sim.set_fields(ib, F0[ib])
+        sim.init_state()
+        peak = np.zeros((2,) + sim.fault(0)['trup'].shape)
+        for n in range(int(args[0])):
+            sim.step(1)
+            sim.sync()
+            for ib in range(2):
+                sv = sim.fault(ib)['Svel']
+                peak[ib] = np.maximum(peak[ib], np.sqrt(sv[1] ** 2 + sv[2] ** 2))
+        for ib in range(2):
+            assert np.array_equal(sim.fault_peak(ib), peak[ib])
+            f = sim.fault(ib)
+            st = sim.fault_stats(ib)
+            rup = f['trup'][0] < 1e9
+            assert st['max_slip'] == pytest.approx(max(0.0, f['S'][3].max()), rel=1e-15)
+            sr = np.sqrt(f['Svel'][1] ** 2 + f['Svel'][2] ** 2)
+            assert st['max_slip_rate'] == pytest.approx(sr.max(), rel=1e-15)
+            assert st['max_peak_slip_rate'] == pytest.approx(peak[ib].max(), rel=1e-15)
+            assert st['ruptured_nodes'] == rup.sum()
+            if rup.any():
+                assert st['mean_slip_ruptured'] == pytest.approx(f['S'][3][rup].mean(), rel=1e-12)
+                assert st['first_rupture_time'] == f['trup'][0][rup].min()

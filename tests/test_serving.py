@@ -1,0 +1,76 @@
+"""Serving loop (continuous batching): every scenario's result is the same
+alone and in the mix.  Kernels are deterministic and scenarios share no
+device state, so the bar is bit-for-bit equality of slip, rupture time and
+peak slip rate."""
+import sys
+
+import numpy as np
+import pytest
+
+from wql import workloads as Wk
+from wql.problem import Problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenario(k):
+    rng = np.random.default_rng([11, k])
+    sizes = [(25, 21, 21), (41, 25, 23), (33, 31, 29), (49, 21, 25)]
+    n = sizes[k % len(sizes)]
+    h = 0.1
+    blocks, coords = Wk._blocks(n, h, mesh='cartesian')
+    Ly, Lz = (n[1] - 1) * h, (n[2] - 1) * h
+    mu_s = rng.uniform(0.6, 0.7)
+    f = Wk.sw_fault(mu_s, rng.uniform(0.45, 0.55), rng.uniform(0.2, 0.6), yc=0.5 * Ly, zc=0.0, depth=Ly, width=Lz)
+    f.patches = [(0.5 * Ly, 0.0, 0.6, 0.6, 120.0 * mu_s - 70.0 + 0.5)]
+    return dict(problem=Problem(blocks=blocks, fault=f, cfl=0.5, name=f's{k}'), coords=coords, steps=23 + 4 * k)
+
+
+def _alone(sc, precision):
+    from wql.solver import Simulation
+    with Simulation(sc['problem'], precision=precision, coords=sc['coords']) as sim:
+        sim.step(sc['steps'])
+        sim.sync()
+        f = sim.fault(0)
+        return dict(S=f['S'], trup=f['trup'][0], psr=sim.fault_peak(0)[0], stats=sim.fault_stats(0))
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'fp64'])
+def test_alone_equals_in_the_mix(precision):
+    from wql.serving import run_stream
+    print(f'[gpu] serving mix {precision}', file=sys.stderr, flush=True)
+    scen = [_scenario(k) for k in range(6)]
+    arrivals = np.array([0.0, 0.0, 0.01, 0.02, 0.02, 0.05])
+    lat, wall, res = run_stream(scen, arrivals, max_active=3, steps_per_launch=7, precision=precision,
+                                keep_results=True)
+    assert len(res) == len(scen)
+    assert (lat > 0).all()
+    for k, sc in enumerate(scen):
+        a = _alone(sc, precision)
+        r = res[k]
+        assert np.array_equal(r['S'], a['S']), k
+        assert np.array_equal(r['trup'], a['trup']), k
+        assert np.array_equal(r['psr'], a['psr']), k
+        assert r['stats'] == a['stats'], k
+        assert a['stats']['max_slip'] > 0.0, k  # the nucleation patch ruptures
+
+
+def test_admission_respects_the_budget():
+    """a budget that fits one scenario at a time still serves them all, one by one"""
+    from wql.serving import Server
+    from wql.solver import estimate_bytes
+    print('[gpu] serving budget', file=sys.stderr, flush=True)
+    scen = [_scenario(2) for _ in range(3)]
+    need = estimate_bytes(scen[0]['problem'], 'fp32')
+    with Server(max_active=3, hbm_budget=int(need * 1.5), steps_per_launch=5) as srv:
+        ids = [srv.submit(s['problem'], s['coords'], s['steps'], 'fp32') for s in scen]
+        done = []
+        peak_active = 0
+        while len(done) < len(ids):
+            done += srv.poll()
+            st = srv.status()
+            peak_active = max(peak_active, st['active'])
+            assert st['bytes_used'] <= st['budget']
+        assert peak_active == 1
+        for sid in ids:
+            assert srv.result(sid)['stats']['max_slip'] > 0.0

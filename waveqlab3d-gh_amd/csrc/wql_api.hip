@@ -605,9 +605,9 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
     for (int ib = 0; ib < 2; ib++) {
         long FP = (long)c->b[ib].nr * c->b[ib].ns;
         Iface &I = c->I[ib];
-        double **arrs[8] = {&I.S, &I.DS, &I.W, &I.DW, &I.trup, &I.Svel, &I.Uhat, &I.Vhat};
-        int nf[8] = {4, 4, 1, 1, 1, 3, 6, 6};
-        for (int k = 0; k < 8; k++) {
+        double **arrs[9] = {&I.S, &I.DS, &I.W, &I.DW, &I.trup, &I.Svel, &I.Uhat, &I.Vhat, &I.psr};
+        int nf[9] = {4, 4, 1, 1, 1, 3, 6, 6, 1};
+        for (int k = 0; k < 9; k++) {
             if (int rc = dalloc(c, (void **)arrs[k], nf[k] * FP * sizeof(double))) return rc;
             double v = (k == 4) ? 1.0e9 : 0.0;  // trup = 1e9 (iface.f90:41)
             k_fill<double><<<(unsigned)((nf[k] * FP + 255) / 256), 256, 0, c->stream>>>(nf[k] * FP, *arrs[k], v);
@@ -905,6 +905,7 @@ int wql_get_frame(wql_ctx *c, int ib, int side, double *nml)
 int wql_set_fault(wql_ctx *c, int ib, const double *S, const double *W)
 {
     if (!c || ib < 0 || ib > 1) return fail(WQL_E_ARG, "bad argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
     long FP = (long)c->b[ib].nr * c->b[ib].ns;
     if (S) HIPCHK(hipMemcpy(c->I[ib].S, S, 4 * FP * sizeof(double), hipMemcpyHostToDevice));
     if (W) HIPCHK(hipMemcpy(c->I[ib].W, W, FP * sizeof(double), hipMemcpyHostToDevice));
@@ -924,6 +925,29 @@ int wql_get_fault(wql_ctx *c, int ib, double *S, double *W, double *trup, double
     if (Uhat) HIPCHK(hipMemcpy(Uhat, I.Uhat, 6 * FP * sizeof(double), hipMemcpyDeviceToHost));
     if (Vhat) HIPCHK(hipMemcpy(Vhat, I.Vhat, 6 * FP * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
+}
+
+int wql_get_fault_peak(wql_ctx *c, int ib, double *psr)
+{
+    if (!c || ib < 0 || ib > 1 || !psr) return fail(WQL_E_ARG, "bad argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    long FP = (long)c->b[ib].nr * c->b[ib].ns;
+    HIPCHK(hipMemcpy(psr, c->I[ib].psr, FP * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int wql_fault_stats(wql_ctx *c, int ib, double stats[6])
+{
+    if (!c || ib < 0 || ib > 1 || !stats) return fail(WQL_E_ARG, "bad argument");
+    long FP = (long)c->b[ib].nr * c->b[ib].ns;
+    double *d_out = nullptr;
+    if (hipMalloc(&d_out, 6 * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(stats)");
+    k_fault_stats<<<1, 1024, 0, c->stream>>>(c->I[ib], FP, d_out);
+    hipError_t e = hipMemcpyAsync(stats, d_out, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d_out);
+    if (e != hipSuccess) return fail(WQL_E_HIP, "fault stats: %s", hipGetErrorString(e));
+    return check_launch();
 }
 
 int wql_get_fault_rates(wql_ctx *c, int ib, double *DS, double *DW)
@@ -1094,3 +1118,5 @@ double wql_volume_kernel_ms(wql_ctx *c)
 }
 
 }  // extern "C"
+
+#include "wql_serve.hip"

@@ -73,6 +73,7 @@ struct Iface {
     double *trup;            // [nr*ns]
     double *Svel;            // [3][nr*ns]
     double *Uhat, *Vhat;     // [6][nr*ns]
+    double *psr;             // [nr*ns] peak tangential slip rate over the run (stage-1 samples)
 };
 
 __device__ __forceinline__ long pidx(int nq, int nr, int i, int j, int k)

@@ -2077,7 +2077,9 @@ __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams
         I.Uhat[4 * FP + fpi] = uh[4] + T0m; I.Vhat[4 * FP + fpi] = vh[4] + T0m;
         I.Uhat[5 * FP + fpi] = uh[5] + T0l; I.Vhat[5 * FP + fpi] = vh[5] + T0l;
         double s1 = I.Svel[FP + fpi], s2 = I.Svel[2 * FP + fpi];
-        if (sqrt(s1 * s1 + s2 * s2) > 1e-3) I.trup[fpi] = fmin(I.trup[fpi], t);
+        const double srate = sqrt(s1 * s1 + s2 * s2);
+        if (srate > 1e-3) I.trup[fpi] = fmin(I.trup[fpi], t);
+        I.psr[fpi] = fmax(I.psr[fpi], srate);
     }
     // rotate back (527-581) and SAT forcing (242-463)
     double uX[6], vX[6], uH[6], vH[6];
@@ -2358,6 +2360,63 @@ __global__ void k_init_state(Pair<real> P, FaultParams f, Iface I0, Iface I1)
     double theta = f.rs_L / f.rs_V0 *
                    exp(((a * log(2.0 * sinh(T0l / (a * sn)))) - f.rs_f0 - a * log(f.rs_Vin / f.rs_V0)) / f.rs_b);
     (me == 0 ? I0 : I1).W[t] = f.rs_f0 + f.rs_b * log(f.rs_V0 / f.rs_L * theta);
+}
+
+// ----------------------------------------------------------------------------
+// fault-plane statistics of one block's interface copy, one workgroup:
+// out = {max slip, max current slip rate, max peak slip rate,
+//        ruptured nodes (trup < 1e9), mean slip over ruptured nodes, earliest trup}
+// Each wave reduces with cross-lane shuffles (64 lanes), waves combine in LDS.
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(1024) k_fault_stats(Iface I, long FP, double *out)
+{
+    __shared__ double part[6][16];
+    double smax = 0.0, vmax = 0.0, pmax = 0.0, nrup = 0.0, srup = 0.0, tmin = 1.0e9;
+    for (long p = threadIdx.x; p < FP; p += blockDim.x) {
+        double sl = I.S[3 * FP + p];
+        double v1 = I.Svel[FP + p], v2 = I.Svel[2 * FP + p];
+        smax = fmax(smax, sl);
+        vmax = fmax(vmax, sqrt(v1 * v1 + v2 * v2));
+        pmax = fmax(pmax, I.psr[p]);
+        double tr = I.trup[p];
+        if (tr < 1.0e9) { nrup += 1.0; srup += sl; tmin = fmin(tmin, tr); }
+    }
+    smax = wave_max(smax); vmax = wave_max(vmax); pmax = wave_max(pmax);
+    nrup = wave_sum(nrup); srup = wave_sum(srup); tmin = wave_min(tmin);
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+    if (lane == 0) {
+        part[0][w] = smax; part[1][w] = vmax; part[2][w] = pmax;
+        part[3][w] = nrup; part[4][w] = srup; part[5][w] = tmin;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 1.0e9};
+        for (int k = 0; k < nw; k++) {
+            r[0] = fmax(r[0], part[0][k]); r[1] = fmax(r[1], part[1][k]); r[2] = fmax(r[2], part[2][k]);
+            r[3] += part[3][k]; r[4] += part[4][k]; r[5] = fmin(r[5], part[5][k]);
+        }
+        out[0] = r[0]; out[1] = r[1]; out[2] = r[2]; out[3] = r[3];
+        out[4] = r[3] > 0.0 ? r[4] / r[3] : 0.0;
+        out[5] = r[5];
+    }
 }
 
 // dense [nf][ns][nr][nq] FP64 -> padded [nf][ns][nr][px] volume array

@@ -1,0 +1,250 @@
+// wql_serve.hip -- continuous-batching loop over independent rupture
+// scenarios (part of libwql; included by wql_api.hip after the C ABI).
+//
+// A scenario is a full two-block problem with its own context: its state is
+// created on the device when it is admitted and stays there until it is
+// retired.  Admission is first-fit by device footprint (wql_estimate_bytes)
+// against the server's HBM budget and a cap on concurrently active
+// scenarios.  Every active scenario runs on its own stream with its LSRK
+// step captured once into a HIP graph; the server advances each one in
+// chunks of steps and tracks chunk completion with an event, so scenarios
+// that arrive while others run are admitted as soon as memory and a slot
+// free up, and their work overlaps on the device.  The server never blocks
+// on the device in wql_server_poll except to download a retired scenario's
+// fault arrays.
+
+#include <deque>
+#include <memory>
+#include <unordered_map>
+
+struct wql_scenario {
+    int id = 0;
+    int state = 0;  // 0 queued, 1 active, 2 finished (result ready), 3 failed
+    wql_problem prob;
+    wql_block_desc blocks[2];
+    std::vector<double> X[2], M[2];
+    int nsteps = 0, steps_done = 0, steps_inflight = 0;
+    size_t need = 0;
+    wql_ctx *ctx = nullptr;
+    hipEvent_t ev = nullptr;
+    std::vector<double> S, trup, psr;
+    double stats[6] = {0, 0, 0, 0, 0, 0};
+    int err = 0;
+    std::string msg;
+};
+
+struct wql_server {
+    int max_active = 8;
+    size_t budget = 0;
+    int chunk = 50;
+    int next_id = 1;
+    size_t used = 0;
+    std::deque<int> queue;
+    std::vector<int> active;
+    std::unordered_map<int, std::unique_ptr<wql_scenario>> all;
+};
+
+namespace {
+
+void scenario_release(wql_server *s, wql_scenario *sc)
+{
+    if (sc->ev) {
+        hipEventDestroy(sc->ev);
+        sc->ev = nullptr;
+    }
+    if (sc->ctx) {
+        wql_destroy(sc->ctx);
+        sc->ctx = nullptr;
+        s->used -= sc->need;
+    }
+    for (int ib = 0; ib < 2; ib++) {
+        std::vector<double>().swap(sc->X[ib]);
+        std::vector<double>().swap(sc->M[ib]);
+    }
+}
+
+int scenario_fail(wql_server *s, wql_scenario *sc, int rc)
+{
+    sc->err = rc;
+    sc->msg = wql_last_error();
+    sc->state = 3;
+    scenario_release(s, sc);
+    return rc;
+}
+
+// enqueue the next chunk of steps and the event that marks its end
+int scenario_launch(wql_scenario *sc, int chunk)
+{
+    int n = std::min(chunk, sc->nsteps - sc->steps_done);
+    if (int rc = wql_step(sc->ctx, n)) return rc;
+    if (hipEventRecord(sc->ev, sc->ctx->stream) != hipSuccess) return fail(WQL_E_HIP, "hipEventRecord");
+    sc->steps_inflight = n;
+    return 0;
+}
+
+int scenario_admit(wql_server *s, wql_scenario *sc)
+{
+    wql_ctx *ctx = nullptr;
+    if (int rc = wql_create(&sc->prob, sc->blocks, &ctx)) return rc;
+    sc->ctx = ctx;
+    s->used += sc->need;
+    for (int ib = 0; ib < 2; ib++) {  // host copies are no longer needed
+        std::vector<double>().swap(sc->X[ib]);
+        std::vector<double>().swap(sc->M[ib]);
+    }
+    if (int rc = wql_init_state(ctx)) return rc;
+    if (int rc = wql_use_graph(ctx, 1)) return rc;
+    if (hipEventCreateWithFlags(&sc->ev, hipEventDisableTiming) != hipSuccess) return fail(WQL_E_HIP, "hipEventCreate");
+    sc->state = 1;
+    return scenario_launch(sc, s->chunk);
+}
+
+int scenario_finish(wql_server *s, wql_scenario *sc)
+{
+    if (int rc = wql_sync(sc->ctx)) return rc;
+    const long FP = (long)sc->blocks[0].nr * sc->blocks[0].ns;
+    sc->S.resize(4 * FP);
+    sc->trup.resize(FP);
+    sc->psr.resize(FP);
+    if (int rc = wql_get_fault(sc->ctx, 0, sc->S.data(), nullptr, sc->trup.data(), nullptr, nullptr, nullptr))
+        return rc;
+    if (int rc = wql_get_fault_peak(sc->ctx, 0, sc->psr.data())) return rc;
+    if (int rc = wql_fault_stats(sc->ctx, 0, sc->stats)) return rc;
+    sc->state = 2;
+    scenario_release(s, sc);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wql_server_create(int max_active, size_t hbm_budget, int steps_per_launch, wql_server **out)
+{
+    if (!out || max_active < 1 || steps_per_launch < 1) return fail(WQL_E_ARG, "bad server argument");
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return fail(WQL_E_HIP, "hipMemGetInfo");
+    wql_server *s = new wql_server();
+    s->max_active = max_active;
+    // default budget: 90% of what is free now
+    s->budget = hbm_budget ? std::min(hbm_budget, free_b) : (size_t)(0.9 * (double)free_b);
+    s->chunk = steps_per_launch;
+    *out = s;
+    return 0;
+}
+
+int wql_server_destroy(wql_server *s)
+{
+    if (!s) return 0;
+    for (auto &kv : s->all) scenario_release(s, kv.second.get());
+    delete s;
+    return 0;
+}
+
+int wql_server_submit(wql_server *s, const wql_problem *prob, const wql_block_desc blocks[2], int nsteps, int *id)
+{
+    if (!s || !prob || !blocks || !id || nsteps < 1) return fail(WQL_E_ARG, "bad submit argument");
+    auto sc = std::make_unique<wql_scenario>();
+    sc->id = s->next_id++;
+    sc->prob = *prob;
+    sc->nsteps = nsteps;
+    for (int ib = 0; ib < 2; ib++) {
+        sc->blocks[ib] = blocks[ib];
+        if (int rc = valid_block(blocks[ib])) return rc;
+        const size_t nd = (size_t)blocks[ib].nq * blocks[ib].nr * blocks[ib].ns;
+        if (!blocks[ib].X) return fail(WQL_E_ARG, "block coordinates are required");
+        sc->X[ib].assign(blocks[ib].X, blocks[ib].X + 3 * nd);
+        sc->blocks[ib].X = sc->X[ib].data();
+        if (blocks[ib].M) {
+            sc->M[ib].assign(blocks[ib].M, blocks[ib].M + 3 * nd);
+            sc->blocks[ib].M = sc->M[ib].data();
+        }
+    }
+    sc->need = wql_estimate_bytes(prob, sc->blocks);
+    if (sc->need > s->budget) return fail(WQL_E_NOMEM, "scenario needs %zu bytes, budget %zu", sc->need, s->budget);
+    *id = sc->id;
+    s->queue.push_back(sc->id);
+    s->all[sc->id] = std::move(sc);
+    return 0;
+}
+
+int wql_server_poll(wql_server *s, int *done, int max_done)
+{
+    if (!s || (max_done > 0 && !done)) return fail(WQL_E_ARG, "bad poll argument");
+    int ndone = 0;
+    // progress of the active scenarios
+    std::vector<int> still;
+    for (int id : s->active) {
+        wql_scenario *sc = s->all[id].get();
+        hipError_t q = hipEventQuery(sc->ev);
+        if (q == hipErrorNotReady) {
+            still.push_back(id);
+            continue;
+        }
+        if (q != hipSuccess) {
+            fail(WQL_E_HIP, "scenario %d: %s", id, hipGetErrorString(q));
+            scenario_fail(s, sc, WQL_E_HIP);
+        } else {
+            sc->steps_done += sc->steps_inflight;
+            sc->steps_inflight = 0;
+            int rc = sc->steps_done >= sc->nsteps ? scenario_finish(s, sc) : scenario_launch(sc, s->chunk);
+            if (rc) scenario_fail(s, sc, rc);
+        }
+        if (sc->state == 1) {
+            still.push_back(id);
+        } else if (ndone < max_done) {
+            done[ndone++] = id;
+        }
+    }
+    s->active.swap(still);
+    // admission: first fit by footprint, in arrival order
+    for (auto it = s->queue.begin(); it != s->queue.end() && (int)s->active.size() < s->max_active;) {
+        wql_scenario *sc = s->all[*it].get();
+        if (s->used + sc->need > s->budget) {
+            ++it;
+            continue;
+        }
+        int rc = scenario_admit(s, sc);
+        if (rc) {
+            scenario_fail(s, sc, rc);
+            if (ndone < max_done) done[ndone++] = sc->id;
+        } else {
+            s->active.push_back(sc->id);
+        }
+        it = s->queue.erase(it);
+    }
+    return ndone;
+}
+
+int wql_server_result(wql_server *s, int id, double *S, double *trup, double *psr, double stats[6])
+{
+    if (!s) return fail(WQL_E_ARG, "null server");
+    auto f = s->all.find(id);
+    if (f == s->all.end()) return fail(WQL_E_ARG, "unknown scenario %d", id);
+    wql_scenario *sc = f->second.get();
+    if (sc->state == 3) {
+        int rc = sc->err;
+        std::string msg = sc->msg;
+        s->all.erase(f);
+        return fail(rc, "scenario %d failed: %s", id, msg.c_str());
+    }
+    if (sc->state != 2) return fail(WQL_E_STATE, "scenario %d is not finished", id);
+    if (S) memcpy(S, sc->S.data(), sc->S.size() * sizeof(double));
+    if (trup) memcpy(trup, sc->trup.data(), sc->trup.size() * sizeof(double));
+    if (psr) memcpy(psr, sc->psr.data(), sc->psr.size() * sizeof(double));
+    if (stats) memcpy(stats, sc->stats, sizeof sc->stats);
+    s->all.erase(f);
+    return 0;
+}
+
+int wql_server_status(const wql_server *s, long out[4])
+{
+    if (!s || !out) return fail(WQL_E_ARG, "bad argument");
+    out[0] = (long)s->queue.size();
+    out[1] = (long)s->active.size();
+    out[2] = (long)s->used;
+    out[3] = (long)s->budget;
+    return 0;
+}
+
+}  // extern "C"

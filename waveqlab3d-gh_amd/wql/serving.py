@@ -1,0 +1,104 @@
+"""Continuous batching of independent rupture scenarios on one GPU.
+
+`Server` wraps the library's serving loop (wql_server_* in include/wql.h):
+scenarios are admitted by device footprint as memory and slots free up,
+each keeps its state on the device and runs on its own stream with a
+captured step graph, and finished scenarios are retired so the next can be
+admitted.  `run_stream` drives a request stream with given arrival times
+and reports per-scenario latency (arrival -> result on the host).
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import native
+from .native import check
+from .solver import block_descs, problem_desc
+
+
+class Server:
+    def __init__(self, max_active=8, hbm_budget=0, steps_per_launch=50):
+        self.srv = C.c_void_p()
+        check(native.lib().wql_server_create(max_active, hbm_budget, steps_per_launch, C.byref(self.srv)))
+        self.shape = {}
+        self._done = (C.c_int * 4096)()
+
+    def close(self):
+        if self.srv:
+            native.lib().wql_server_destroy(self.srv)
+            self.srv = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, prob, coords, nsteps, precision='fp32'):
+        pd = problem_desc(prob, precision)
+        arr, keep = block_descs(prob, coords)
+        sid = C.c_int()
+        check(native.lib().wql_server_submit(self.srv, C.byref(pd), arr, nsteps, C.byref(sid)))
+        self.shape[sid.value] = (prob.blocks[0].n[1], prob.blocks[0].n[2])
+        return sid.value
+
+    def poll(self):
+        """ids of scenarios retired by this call (never blocks on the device
+        except to download a retired scenario's fault arrays)"""
+        n = native.lib().wql_server_poll(self.srv, self._done, len(self._done))
+        if n < 0:
+            check(n)
+        return [self._done[k] for k in range(n)]
+
+    def result(self, sid):
+        nr, ns = self.shape.pop(sid)
+        S = np.empty((4, ns, nr))
+        trup = np.empty((ns, nr))
+        psr = np.empty((ns, nr))
+        stats = np.empty(6)
+        p = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        check(native.lib().wql_server_result(self.srv, sid, p(S), p(trup), p(psr), p(stats)))
+        keys = ('max_slip', 'max_slip_rate', 'max_peak_slip_rate', 'ruptured_nodes', 'mean_slip_ruptured',
+                'first_rupture_time')
+        return dict(S=S, trup=trup, psr=psr, stats=dict(zip(keys, stats.tolist())))
+
+    def status(self):
+        out = (C.c_long * 4)()
+        check(native.lib().wql_server_status(self.srv, out))
+        return dict(queued=out[0], active=out[1], bytes_used=out[2], budget=out[3])
+
+
+def run_stream(scenarios, arrivals, max_active=8, steps_per_launch=50, precision='fp32', keep_results=False,
+               idle_sleep=2e-4):
+    """Serve `scenarios` (list of dicts with problem, coords, steps) arriving
+    at `arrivals` seconds after the start.  Returns per-scenario latency (s),
+    the wall time from first arrival to last result, and optionally results."""
+    order = np.argsort(arrivals)
+    lat = np.zeros(len(scenarios))
+    results = {}
+    t_start = time.perf_counter()
+    sub_t = {}
+    nxt = 0
+    ndone = 0
+    with Server(max_active=max_active, steps_per_launch=steps_per_launch) as srv:
+        while ndone < len(scenarios):
+            now = time.perf_counter() - t_start
+            while nxt < len(order) and arrivals[order[nxt]] <= now:
+                k = int(order[nxt])
+                sc = scenarios[k]
+                sid = srv.submit(sc['problem'], sc['coords'], sc['steps'], precision)
+                sub_t[sid] = (k, arrivals[k])
+                nxt += 1
+            finished = srv.poll()
+            for sid in finished:
+                k, ta = sub_t.pop(sid)
+                r = srv.result(sid)
+                lat[k] = time.perf_counter() - t_start - ta
+                ndone += 1
+                if keep_results:
+                    results[k] = r
+            if not finished and (nxt >= len(order) or arrivals[order[nxt]] > now):
+                time.sleep(idle_sleep)
+        wall = time.perf_counter() - t_start - float(np.min(arrivals))
+    return lat, wall, results

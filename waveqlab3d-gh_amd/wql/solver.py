@@ -196,6 +196,21 @@ class Simulation:
                                                          ('S', 'W', 'trup', 'Svel', 'Uhat', 'Vhat'))))
         return out
 
+    def fault_peak(self, ib):
+        """peak tangential slip rate per fault node [1][ns][nr]"""
+        nq, nr, ns = self.prob.blocks[ib].n
+        out = np.empty((1, ns, nr))
+        check(native.lib().wql_get_fault_peak(self.ctx, ib, _p(out)))
+        return out
+
+    def fault_stats(self, ib=0):
+        """device-side fault-plane statistics of one block's copy"""
+        out = np.empty(6)
+        check(native.lib().wql_fault_stats(self.ctx, ib, _p(out)))
+        keys = ('max_slip', 'max_slip_rate', 'max_peak_slip_rate', 'ruptured_nodes', 'mean_slip_ruptured',
+                'first_rupture_time')
+        return dict(zip(keys, out.tolist()))
+
     def fault_rates(self, ib):
         nq, nr, ns = self.prob.blocks[ib].n
         DS, DW = np.empty((4, ns, nr)), np.empty((1, ns, nr))
