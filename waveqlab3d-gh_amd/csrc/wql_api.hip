@@ -90,6 +90,8 @@ struct DevBlock {
     int px;                 // row pitch of the volume arrays
     long NP;                // px*nr*ns (elements per field on the device)
     long ND;                // nq*nr*ns (dense, as the host sees a field)
+    void *satb = nullptr;   // [9][nbnd] real: boundary penalties of the stage
+    long nbnd = 0;          // boundary nodes
     int cart;
     int bc[6], pml[6], npml;
     double lam, mu, rho;
@@ -167,6 +169,8 @@ Blk<real> blk_view(const wql_ctx *c, int ib, int cur)
     B.fx = d.fx;
     B.P = d.P;
     B.Y = d.Y;
+    B.satb = nullptr;
+    B.nbnd = d.nbnd;
     return B;
 }
 
@@ -370,6 +374,20 @@ void launch_band(const wql_ctx *c, Pair<real> &P, real A, real bdt)
     else k_volume_split<real, 32, 8, false, false, 1><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
 }
 
+// the boundary penalties are folded into the volume kernels when every
+// kernel of the stage adds them: the role-split march (interior), its
+// s-closure mode and the PML-slab kernel
+template <typename real>
+bool fused_faces(const wql_ctx *c)
+{
+    if (c->variant < 16) return false;
+    for (int ib = 0; ib < 2; ib++) {
+        if (!c->b[ib].satb) return false;
+        if ((double)c->b[ib].NP * 10.0 * sizeof(real) >= 2147483647.0) return false;
+    }
+    return true;
+}
+
 template <typename real>
 void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int stage, int rates_only,
                   int do_plastic, const double *tdev = nullptr, double tinc = 0.0)
@@ -377,6 +395,38 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
     Pair<real> P;
     P.b[0] = blk_view<real>(c, 0, cur);
     P.b[1] = blk_view<real>(c, 1, cur);
+    BoundaryArgs a;
+    memset(&a, 0, sizeof a);
+    a.fault = c->fault;
+    a.I[0] = c->I[0];
+    a.I[1] = c->I[1];
+    for (int ib = 0; ib < 2; ib++) a.nodes[ib] = c->b[ib].nbnd;
+    a.t = t_stage;
+    a.tdev = tdev;
+    a.tinc = tinc;
+    a.A[0] = a.A[1] = A;
+    a.bdt = bdt;
+    a.tau0 = TAU0;
+    a.stage = stage;
+    a.rates_only = rates_only;
+    a.flag = c->d_flag;
+    unsigned gb = (unsigned)((a.nodes[0] + a.nodes[1] + 127) / 128);
+    // fused boundary penalties: the role-split kernels (and the s-closure and
+    // PML-slab kernels launched with them) add each boundary node's penalty
+    // when they form DF, so the boundary kernel runs first and only writes
+    // the compact penalty array (plus the fault); PML penalties follow the
+    // volume kernels as before
+    const bool fused = fused_faces<real>(c);
+    bool any_pml = false;
+    for (int ib = 0; ib < 2; ib++)
+        for (int sd = 0; sd < 6; sd++) any_pml = any_pml || c->b[ib].pml[sd];
+    if (fused) {
+        a.mode = 1;
+        a.satb[0] = c->b[0].satb;
+        a.satb[1] = c->b[1].satb;
+        k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
+        for (int ib = 0; ib < 2; ib++) P.b[ib].satb = (const real *)c->b[ib].satb;
+    }
     // mixed grids: a Cartesian block inside a curvilinear pair still has no G
     TileList TL = tile_list(c);
     if (TL.total > 0) {
@@ -397,26 +447,13 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
         unsigned g = (unsigned)((BL.total + 255) / 256);
         k_volume_generic<real><<<g, 256, 0, c->stream>>>(P, BL, (real)A, (real)A, (real)bdt);
     }
-    BoundaryArgs a;
-    memset(&a, 0, sizeof a);
-    a.fault = c->fault;
-    a.I[0] = c->I[0];
-    a.I[1] = c->I[1];
-    for (int ib = 0; ib < 2; ib++) {
-        const DevBlock &d = c->b[ib];
-        a.nodes[ib] = 2L * d.nr * d.ns + 2L * (d.nq - 2) * d.ns + 2L * (d.nq - 2) * (d.nr - 2);
+    if (!fused) {
+        a.mode = 0;
+        k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
+    } else if (any_pml) {
+        a.mode = 2;
+        k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
     }
-    a.t = t_stage;
-    a.tdev = tdev;
-    a.tinc = tinc;
-    a.A[0] = a.A[1] = A;
-    a.bdt = bdt;
-    a.tau0 = TAU0;
-    a.stage = stage;
-    a.rates_only = rates_only;
-    a.flag = c->d_flag;
-    unsigned gb = (unsigned)((a.nodes[0] + a.nodes[1] + 127) / 128);
-    k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
     if (do_plastic && c->prob.plastic) {
         PlasticArgs pa;
         memset(&pa, 0, sizeof pa);
@@ -579,6 +616,8 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
             k_fill<real><<<(unsigned)((9 * NP + 255) / 256), 256, 0, c->stream>>>(9 * NP, (real *)d.F[k], (real)0);
         }
         if (int rc = dalloc(c, &d.DF, 9 * NP * sizeof(real))) return rc;
+        d.nbnd = 2L * d.nr * d.ns + 2L * (d.nq - 2) * d.ns + 2L * (d.nq - 2) * (d.nr - 2);
+        if (int rc = dalloc(c, &d.satb, 9 * d.nbnd * sizeof(real))) return rc;
         k_fill<real><<<(unsigned)((9 * NP + 255) / 256), 256, 0, c->stream>>>(9 * NP, (real *)d.DF, (real)0);
         for (int s = 0; s < 6; s++) {
             if (!d.pml[s]) continue;
@@ -763,10 +802,10 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
         return fail(WQL_E_ARG, "bad fault description");
     wql_ctx *c = new wql_ctx();
     c->prec = prob->precision;
-    // role-split march: FP32 32x8 tiles with 4-plane s chunks, FP64 64x8
-    // tiles with 16-plane chunks
-    c->variant = prob->precision == WQL_FP64 ? 17 : 16;
-    c->zchunk = prob->precision == WQL_FP64 ? 16 : 4;
+    // role-split march: FP32 32x8 tiles with 8-plane s chunks, FP64 32x16
+    // tiles with 12-plane chunks
+    c->variant = prob->precision == WQL_FP64 ? 19 : 16;
+    c->zchunk = prob->precision == WQL_FP64 ? 12 : 8;
     c->prob = *prob;
     c->dt = prob->dt;
     c->t = prob->t0;

@@ -64,7 +64,27 @@ struct Blk {
     const double *fx;        // [3][nr*ns] coordinates of the fault face (q-face carrying bc==0)
     double *P;               // [2][NP] plastic strain rate / accumulated plastic strain (FP64)
     const double *Y;         // [NP] y-coordinate (plastic depth-dependent background stress)
+    // boundary penalty of this stage per boundary node, added by the volume
+    // kernels when they form DF (nullptr: the boundary kernel applies it)
+    const real *satb;        // [9][nbnd]
+    long nbnd;               // boundary nodes: 2 nr ns + 2 (nq-2) ns + 2 (nq-2)(nr-2)
 };
+
+// index of node (i, j, k) in the boundary-node enumeration (q faces, then r
+// faces without q edges, then s faces without q/r edges), -1 inside
+template <typename real>
+__device__ __forceinline__ long bnd_index(const Blk<real> &B, int i, int j, int k)
+{
+    const int nq = B.nq, nr = B.nr, ns = B.ns;
+    const long nqf = 2L * nr * ns, nrf = 2L * (nq - 2) * ns;
+    if (i == 0) return (long)j + (long)nr * k;
+    if (i == nq - 1) return (long)nr * ns + j + (long)nr * k;
+    if (j == 0) return nqf + (i - 1) + (long)(nq - 2) * k;
+    if (j == nr - 1) return nqf + (long)(nq - 2) * ns + (i - 1) + (long)(nq - 2) * k;
+    if (k == 0) return nqf + nrf + (i - 1) + (long)(nq - 2) * (j - 1);
+    if (k == ns - 1) return nqf + nrf + (long)(nq - 2) * (nr - 2) + (i - 1) + (long)(nq - 2) * (j - 1);
+    return -1;
+}
 
 // One block's copy of the interface state (iface.f90:13-48), FP64.
 struct Iface {

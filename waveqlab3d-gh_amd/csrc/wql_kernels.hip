@@ -280,9 +280,11 @@ __device__ void volume_node_generic(const Blk<real> &B, int i, int j, int k, rea
             Q[f * SP + s] = q[f] + bdt * dq;
         }
     }
+    const long bi = B.satb ? bnd_index(B, i, j, k) : -1;
     for (int f = 0; f < NF; f++) {
         real df = A * B.DF[f * NP + p];
         df = df + DU[f];
+        if (bi >= 0) df = df + B.satb[f * B.nbnd + bi];
         B.DF[f * NP + p] = df;
         B.Fout[f * NP + p] = B.Fin[f * NP + p] + bdt * df;
     }
@@ -1202,8 +1204,13 @@ __device__ __forceinline__ real d1_col_s(const SbpLds<real> &S, const real *col,
 // of the fused kernel's state (one 7-plane s-queue of 3 values), which keeps
 // occupancy high; the metric and field lines both roles touch are fetched
 // from HBM once and hit L2 for the second role.
+// minimum waves per SIMD asked of the register allocator: three for FP32
+// (<= 168 VGPRs), two for FP64 (<= 256 registers incl. AGPRs)
+template <typename real>
+constexpr int split_min_waves() { return sizeof(real) == 4 ? 3 : 2; }
+
 template <typename real, int TX, int TY, bool CURV, bool HET, int SB>
-__global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList TL, real A0, real A1, real bdt)
+__global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_split(Pair<real> P, TileList TL, real A0, real A1, real bdt)
 {
     constexpr int WX = TX + 6, WY = TY + 6;
     constexpr unsigned RS = sizeof(real);
@@ -1245,6 +1252,25 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
     const unsigned NPb = (unsigned)B.NP * RS;
     const unsigned planeb = (unsigned)(B.px * B.nr) * RS;
     const unsigned colb = (unsigned)(gx + B.px * gy) * RS;
+    // boundary-node index of this column (fused boundary penalties): a q/r
+    // face column has index bbase + bstride*z; inner columns are boundary
+    // nodes only on the s faces (closure mode), at bsface (+ one s face)
+    int bbase = -1, bstride = 0, bsface = -1;
+    if (B.satb && own) {
+        const int nq = B.nq, nr = B.nr, ns = B.ns;
+        const int nqf = 2 * nr * ns, nrf = 2 * (nq - 2) * ns;
+        if (gx == 0 || gx == nq - 1) { bbase = (gx == 0 ? 0 : nr * ns) + gy; bstride = nr; }
+        else if (gy == 0 || gy == nr - 1) { bbase = nqf + (gy == 0 ? 0 : (nq - 2) * ns) + gx - 1; bstride = nq - 2; }
+        else bsface = nqf + nrf + (gx - 1) + (nq - 2) * (gy - 1);
+    }
+    auto bnd = [&](int z) -> int {
+        if (bbase >= 0) return bbase + bstride * z;
+        if (SB != 0 && bsface >= 0) {
+            if (z == 0) return bsface;
+            if (z == B.ns - 1) return bsface + (B.nq - 2) * (B.nr - 2);
+        }
+        return -1;
+    };
     const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
 
     const __amdgpu_buffer_rsrc_t rF = brsrc(B.Fin, 9u * NPb);
@@ -1400,6 +1426,7 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
             prefetch(nxt, z + 1);
             if (own) {
                 const unsigned o = colb + planeb * (unsigned)z;
+                const int bi = bnd(z);
                 real rho = HET ? bload<real>(rM, o, 2u * NPb) : (real)B.rho;
                 real rhoJ_inv = (real)1 / (rho * J);
 #pragma unroll
@@ -1417,6 +1444,7 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
                         c = ihs * acc;
                     }
                     real df = A * cdf[f] + (a + bb + c) * rhoJ_inv;
+                    if (bi >= 0) df = df + B.satb[(long)f * B.nbnd + bi];
                     bstore<real>(rD, o, f * NPb, df);
                     bstore<real>(rO, o, f * NPb, cv[f] + bdt * df);
                 }
@@ -1559,9 +1587,11 @@ __global__ void __launch_bounds__(TX * TY) k_volume_split(Pair<real> P, TileList
                 DU[3] = mu * (Uy[0] + Ux[1]);
                 DU[4] = mu * (Uz[0] + Ux[2]);
                 DU[5] = mu * (Uz[1] + Uy[2]);
+                const int bi = bnd(z);
 #pragma unroll
                 for (int f = 0; f < 6; f++) {
                     real df = A * cdf[f] + DU[f];
+                    if (bi >= 0) df = df + B.satb[(long)(3 + f) * B.nbnd + bi];
                     bstore<real>(rD, o, (3 + f) * NPb, df);
                     bstore<real>(rO, o, (3 + f) * NPb, cs[f] + bdt * df);
                 }
@@ -2166,6 +2196,12 @@ struct BoundaryArgs {
     double t, A[2], bdt, tau0;
     const double *tdev;   // non-null: stage time = *tdev + tinc (step time kept on the device)
     double tinc;
+    // 0: apply every boundary and fault penalty to DF / F_out (after the
+    //    volume kernels); 1: before the volume kernels, write each boundary
+    //    node's non-PML penalty to satb (the volume kernels add it) and run
+    //    the fault; 2: after the volume kernels, only the PML penalties
+    int mode;
+    void *satb[2];        // [9][nbnd] per block, in the stage's real type
     int stage;
     int rates_only;       // 1: RHS evaluation without the RK update (no F_out / S / W write)
     int *flag;
@@ -2210,6 +2246,7 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
         int dir = side / 2;
         int on = (side % 2 == 0) ? (pos[dir] == 0) : (pos[dir] == nn[dir] - 1);
         if (!on || B.bc[side] <= 0) continue;
+        if (a.mode == 2 && !B.pml[side]) continue;
         long fpi;
         long FP;
         if (dir == 0) { fpi = j + (long)nr * k; FP = (long)nr * ns; }
@@ -2225,8 +2262,9 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
         double ubc[9], Ux[9];
         bc_sat(u, fm[2 * FP + fpi], fm[FP + fpi], fm[fpi], l, m, n, norm, r, side % 2, ubc, Ux);
         double h = 1.0 / (double)(nn[dir] - 1);
-        for (int f = 0; f < 9; f++) sat[f] = sat[f] - a.tau0 / h * Ux[f];
-        if (B.pml[side]) {  // RHS_Interior.f90:1574-1700: SAT on the PML auxiliary rates
+        if (a.mode != 2)
+            for (int f = 0; f < 9; f++) sat[f] = sat[f] - a.tau0 / h * Ux[f];
+        if (B.pml[side] && a.mode != 1) {  // RHS_Interior.f90:1574-1700: SAT on the PML auxiliary rates
             int np = B.npml;
             int lo[3] = {0, 0, 0}, hi[3] = {nq - 1, nr - 1, ns - 1};
             if (side % 2 == 0) hi[dir] = np - 1; else lo[dir] = nn[dir] - np;
@@ -2243,12 +2281,20 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
     }
     // fault face (domain.f90:600-609): block 0's face q = nq-1, block 1's face q = 0
     int on_fault = (me == 0) ? (i == nq - 1 && B.bc[1] == 0) : (i == 0 && B.bc[0] == 0);
+    if (a.mode == 2) on_fault = 0;
     if (on_fault) {
         const double ts = a.tdev ? *a.tdev + a.tinc : a.t;
         int rc = fault_node<real>(P, a.I[me], a.fault, me, j, k, ts, a.stage, A, a.rates_only ? 0.0 : a.bdt,
                                   sat, a.tau0);
         if (rc) a.flag[0] = rc;
     }
+    if (a.mode == 1) {
+        real *sb = (real *)a.satb[me];
+        const long nb = a.nodes[me];
+        for (int f = 0; f < 9; f++) sb[f * nb + t] = (real)sat[f];
+        return;
+    }
+    if (a.mode == 2) return;
     for (int f = 0; f < 9; f++) {
         real df = B.DF[f * NP + p] + (real)sat[f];
         B.DF[f * NP + p] = df;
