@@ -20,19 +20,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, 'waveqlab3d-gh_amd'))
 
 HBM_PEAK_GBS = 8000.0
-FAMILIES = ['k_volume_main', 'k_volume_pipe', 'k_volume_plane', 'k_volume_buf', 'k_volume_split']
+FAMILIES = ['k_volume_main', 'k_volume_pipe', 'k_volume_plane', 'k_volume_buf', 'k_volume_split', 'k_volume_fused',
+            'k_volume_fused']
+NVARIANT = 28
 TILES = [(32, 8), (64, 8), (16, 16), (32, 16)]
 
 
 def kernel_name(prob, precision, variant):
     """template name of the main volume kernel a launch variant selects"""
-    v = variant % 20
+    v = variant % NVARIANT
     tx, ty = TILES[v % 4]
     curv = 'true' if any(b.mesh != 'cartesian' for b in prob.blocks) else 'false'
     het = 'true' if any(b.material is not None for b in prob.blocks) else 'false'
     real = 'float' if precision == 'fp32' else 'double'
     fam = FAMILIES[v // 4]
-    extra = ', 0' if fam == 'k_volume_split' else ''  # interior (not s-closure) instantiation
+    extra = ''
+    if fam == 'k_volume_split':
+        extra = ', 0'  # interior (not s-closure) instantiation
+    elif fam == 'k_volume_fused':
+        extra = ', true' if v < 24 else ', false'  # lead plane held in registers
     return f'{fam}<{real}, {tx}, {ty}, {curv}, {het}{extra}>'
 
 

@@ -31,7 +31,7 @@ def _nvariants():
 
 
 @pytest.mark.parametrize('case', G.RHS_CASES)
-@pytest.mark.parametrize('tile', list(range(40)))
+@pytest.mark.parametrize('tile', list(range(56)))
 def test_rhs_vs_golden(case, tile):
     _say(f'[gpu] rhs {case} tile={tile}')
     z = G.load(case)

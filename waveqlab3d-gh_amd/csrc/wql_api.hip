@@ -121,7 +121,7 @@ struct wql_ctx {
     // launch shape of the main volume kernel (set per precision in
     // wql_create: the fastest measured on the 2 x 201x101x101 curvilinear
     // problem)
-    int variant = 16, zchunk = 4, order = 0, group = 2;
+    int variant = 20, zchunk = 6, order = 0, group = 2;
     int use_graph = 0;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int *d_flag = nullptr;
@@ -188,11 +188,14 @@ void main_range(const DevBlock &d, int r[6])
 
 // variants 0-3: k_volume_main (s-march); 4-7: software-pipelined march;
 // 8-11: plane-parallel; 12-15: buffer-addressed pipelined march; 16-19:
-// role-split pipelined march.  Within each family: 32x8, 64x8, 16x16, 32x16
-// tiles.
-const int NVARIANT = 20;
-const int VARIANT_TX[NVARIANT] = {32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32};
-const int VARIANT_TY[NVARIANT] = {8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16};
+// role-split pipelined march; 20-23: fused single-pass march holding the
+// lead plane; 24-27: fused march re-reading the centre plane.  Within each
+// family: 32x8, 64x8, 16x16, 32x16 tiles.
+const int NVARIANT = 28;
+const int VARIANT_TX[NVARIANT] = {32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32, 32, 64,
+                                  16, 32, 32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32};
+const int VARIANT_TY[NVARIANT] = {8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16, 8, 8,
+                                  16, 16, 8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16};
 
 TileList tile_list(const wql_ctx *c)
 {
@@ -297,6 +300,22 @@ void launch_main_v(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0,
     bool buf_ok = true;
     for (int ib = 0; ib < 2; ib++)
         if ((double)c->b[ib].NP * 10.0 * sizeof(real) >= 2147483647.0) buf_ok = false;
+    if (c->variant >= 20 && buf_ok) {
+        // fused single-pass march; variants 20-23 hold the lead plane's
+        // stresses and J until they are the centre plane, 24-27 re-read them
+        if (c->variant < 24) {
+            if (curv && het) k_volume_fused<real, TX, TY, true, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+            else if (curv) k_volume_fused<real, TX, TY, true, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+            else if (het) k_volume_fused<real, TX, TY, false, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+            else k_volume_fused<real, TX, TY, false, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        } else {
+            if (curv && het) k_volume_fused<real, TX, TY, true, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+            else if (curv) k_volume_fused<real, TX, TY, true, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+            else if (het) k_volume_fused<real, TX, TY, false, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+            else k_volume_fused<real, TX, TY, false, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        }
+        return;
+    }
     if (c->variant >= 16 && buf_ok) {
         dim3 grd2((unsigned)(((2 * TL.total + 7) / 8) * 8));
         if (curv && het) k_volume_split<real, TX, TY, true, true, 0><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
@@ -802,10 +821,10 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
         return fail(WQL_E_ARG, "bad fault description");
     wql_ctx *c = new wql_ctx();
     c->prec = prob->precision;
-    // role-split march: FP32 32x8 tiles with 8-plane s chunks, FP64 32x16
-    // tiles with 12-plane chunks
-    c->variant = prob->precision == WQL_FP64 ? 19 : 16;
-    c->zchunk = prob->precision == WQL_FP64 ? 12 : 8;
+    // fused single-pass march on 32x8 tiles (measured fastest on MI355X for
+    // both precisions): 6-plane s chunks in FP32, 12-plane chunks in FP64
+    c->variant = 20;
+    c->zchunk = prob->precision == WQL_FP64 ? 12 : 6;
     c->prob = *prob;
     c->dt = prob->dt;
     c->t = prob->t0;

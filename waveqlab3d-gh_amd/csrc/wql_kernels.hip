@@ -885,6 +885,30 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, 
     }
 }
 
+// non-temporal forms (cache policy NT): for data no other workgroup reads in
+// this launch (the rates and the stage output), so that it does not evict the
+// lines the neighbouring tiles' halo loads hit in L2
+template <typename real>
+__device__ __forceinline__ real bload_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(real) == 4) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 2));
+    } else {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 2));
+    }
+}
+
+template <typename real>
+__device__ __forceinline__ void bstore_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, real v)
+{
+    if constexpr (sizeof(real) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 2);
+    } else {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)voff, (int)soff, 2);
+    }
+}
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void *p, unsigned bytes)
 {
     // raw (stride 0) buffer over [p, p + bytes): out-of-range loads return 0
@@ -1195,6 +1219,43 @@ __device__ __forceinline__ real d1_col_s(const SbpLds<real> &S, const real *col,
     return ih * s;
 }
 
+// Dense closure weights in LDS: row r of the 12 closure rows over the nine
+// nodes next to its face (c_sbw32 / c_sbw64), so a closure node is nine
+// independent LDS reads and FMAs with no per-lane tap loop.
+template <typename real>
+struct SbpDense {
+    real w[12][9];
+};
+
+template <typename real>
+__device__ __forceinline__ void sbd_to_lds(SbpDense<real> &S, int tid, int nthreads)
+{
+    for (int e = tid; e < 12 * 9; e += nthreads) {
+        if constexpr (sizeof(real) == 4) S.w[e / 9][e % 9] = c_sbw32[e / 9][e % 9];
+        else S.w[e / 9][e % 9] = c_sbw64[e / 9][e % 9];
+    }
+}
+
+// first derivative at local index l of an LDS line (element stride st) whose
+// global index is g on an n-node line; closure rows 0..5 use nodes 0..8,
+// rows n-6..n-1 nodes n-9..n-1
+template <typename real>
+__device__ __forceinline__ real d1_dense(const SbpDense<real> &S, const real *v, int st, int l, int g, int n, real ih)
+{
+    if (g >= 6 && g <= n - 7) {
+        return ih * ((real)0.75 * (v[(l + 1) * st] - v[(l - 1) * st]) +
+                     (real)(-0.15) * (v[(l + 2) * st] - v[(l - 2) * st]) +
+                     (real)(1.0 / 60.0) * (v[(l + 3) * st] - v[(l - 3) * st]));
+    }
+    const bool lo = g < 6;
+    const int r = lo ? g : g - n + 12;
+    const int b = lo ? l - g : l - g + n - 9;
+    real acc = 0;
+#pragma unroll
+    for (int t = 0; t < 9; t++) acc = acc + S.w[r][t] * v[(b + t) * st];
+    return ih * acc;
+}
+
 // ----------------------------------------------------------------------------
 // Role-split pipelined march.  The velocity rates depend on the stresses only
 // and the stress rates on the velocities only, so each tile is processed by
@@ -1221,7 +1282,7 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
     real(*sfq)[3][TY][WX] = reinterpret_cast<real(*)[3][TY][WX]>(lds);
     real(*sfr)[3][WY][TX] = reinterpret_cast<real(*)[3][WY][TX]>(lds + NQ);
     real(*sv)[3][WY][WX] = reinterpret_cast<real(*)[3][WY][WX]>(lds);
-    __shared__ SbpLds<real> sbp;
+    __shared__ SbpDense<real> sbd;
 
     const long nwg = (long)gridDim.x;
     const long per = (nwg + 7) / 8;
@@ -1242,8 +1303,10 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
     int xs, xc0, xc1, ys, yc0, yc1;
     tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
     tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
-    const int z0 = T.za0 + zc_i * T.zc;
-    const int z1 = min(z0 + T.zc, T.za1);
+    // chunks of balanced length (lengths differ by at most one plane)
+    const int span = T.za1 - T.za0;
+    const int z0 = T.za0 + (int)((long)zc_i * span / T.nzc);
+    const int z1 = T.za0 + (int)((long)(zc_i + 1) * span / T.nzc);
 
     const int lx = threadIdx.x, ly = threadIdx.y;
     const int gx = xs + lx, gy = ys + ly;
@@ -1282,7 +1345,7 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
 
     // halo node of this thread (x halo: 6*TY nodes, then y halo: 6*TX nodes)
     const int tid = ly * TX + lx;
-    sbp_to_lds<real>(sbp, tid, TX * TY);  // published by the first barrier of the march
+    sbd_to_lds<real>(sbd, tid, TX * TY);  // published by the first barrier of the march
     int hk = 0, hcx = 0, hcy = 0;
     unsigned hcolb = 0;
     if (tid < 6 * TY) {
@@ -1431,8 +1494,8 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
                 real rhoJ_inv = (real)1 / (rho * J);
 #pragma unroll
                 for (int f = 0; f < 3; f++) {
-                    real a = d1_row_s<real>(sbp, &sfq[b][f][ly][0], lx + 3, gx, B.nq, ihq);
-                    real bb = d1_col_s<real>(sbp, &sfr[b][f][0][lx], TX, ly + 3, gy, B.nr, ihr);
+                    real a = d1_dense<real>(sbd, &sfq[b][f][ly][0], 1, lx + 3, gx, B.nq, ihq);
+                    real bb = d1_dense<real>(sbd, &sfr[b][f][0][lx], TX, ly + 3, gy, B.nr, ihr);
                     real c;
                     if constexpr (SB == 0) {
                         c = ihs * (c1 * (qf[4][f] - qf[2][f]) + c2 * (qf[5][f] - qf[1][f]) + c3 * (qf[6][f] - qf[0][f]));
@@ -1560,8 +1623,8 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
                 real Ux[3], Uy[3], Uz[3];
 #pragma unroll
                 for (int f = 0; f < 3; f++) {
-                    real dq = d1_row_s<real>(sbp, &sv[b][f][ly + 3][0], lx + 3, gx, B.nq, ihq);
-                    real dr = d1_col_s<real>(sbp, &sv[b][f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
+                    real dq = d1_dense<real>(sbd, &sv[b][f][ly + 3][0], 1, lx + 3, gx, B.nq, ihq);
+                    real dr = d1_dense<real>(sbd, &sv[b][f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
                     real ds;
                     if constexpr (SB == 0) {
                         ds = ihs * (c1 * (qv[4][f] - qv[2][f]) + c2 * (qv[5][f] - qv[1][f]) + c3 * (qv[6][f] - qv[0][f]));
@@ -1606,6 +1669,323 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
         }
         if (z < z1) body(pa, pb, z);
     }
+}
+
+// ----------------------------------------------------------------------------
+// Fused single-pass march.  One workgroup per tile and s-chunk forms the
+// rates of all nine fields, and every value of the stage input is fetched
+// once per chunk: plane p enters as the lead of the s-stencil (stresses, J
+// and the s metrics -> s-flux queue; velocities -> velocity queue) and, with
+// HOLD, its stresses and J wait in a four-plane register queue until p is
+// the centre plane three iterations later, so the centre plane only adds
+// its q/r metrics and rates.  The q/r stencils read LDS tiles of the centre
+// plane's q-fluxes, r-fluxes and velocities (3-node halo), double buffered
+// with one barrier per plane.  The chunk's warm-up (lead planes z0-3 ..
+// z0+2) issues all of its loads together.  Arithmetic per node is the
+// role-split kernel's, in the same order.
+template <typename real>
+constexpr int fused_min_waves() { return sizeof(real) == 4 ? 2 : 1; }
+
+template <typename real, int TX, int TY, bool CURV, bool HET, bool HOLD>
+__global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
+    k_volume_fused(Pair<real> P, TileList TL, real A0, real A1, real bdt)
+{
+    constexpr int WX = TX + 6, WY = TY + 6;
+    constexpr unsigned RS = sizeof(real);
+    __shared__ real sfq[2][3][TY][WX];
+    __shared__ real sfr[2][3][WY][TX];
+    __shared__ real sv[2][3][WY][WX];
+    __shared__ SbpDense<real> sbd;
+
+    // consecutive logical tiles on one XCD
+    const long nwg = (long)gridDim.x;
+    const long per = (nwg + 7) / 8;
+    const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
+    if (logical >= TL.total) return;
+    int gi = 0;
+    for (int k = 1; k < TL.n; k++)
+        if (logical >= TL.g[k].first) gi = k;
+    const TileGeom &T = TL.g[gi];
+    const Blk<real> &B = P.b[T.blk];
+    const real A = T.blk == 0 ? A0 : A1;
+    long lid = logical - T.first;
+    int tx_i, ty_i, zc_i;
+    tile_coords(TL, T, lid, T.nzc, tx_i, ty_i, zc_i);
+
+    int xs, xc0, xc1, ys, yc0, yc1;
+    tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
+    tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
+    // chunks of balanced length (lengths differ by at most one plane)
+    const int span = T.za1 - T.za0;
+    const int z0 = T.za0 + (int)((long)zc_i * span / T.nzc);
+    const int z1 = T.za0 + (int)((long)(zc_i + 1) * span / T.nzc);
+
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int gx = xs + lx, gy = ys + ly;
+    const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
+    const bool inb = (gx < B.nq && gy < B.nr);
+    const unsigned NPb = (unsigned)B.NP * RS;
+    const unsigned planeb = (unsigned)(B.px * B.nr) * RS;
+    const unsigned colb = (unsigned)(gx + B.px * gy) * RS;
+    // boundary-node index of a q/r face column (fused boundary penalties);
+    // interior chunks hold no s-face node
+    int bbase = -1, bstride = 0;
+    if (B.satb && own) {
+        const int nq = B.nq, nr = B.nr, ns = B.ns;
+        const int nqf = 2 * nr * ns;
+        if (gx == 0 || gx == nq - 1) { bbase = (gx == 0 ? 0 : nr * ns) + gy; bstride = nr; }
+        else if (gy == 0 || gy == nr - 1) { bbase = nqf + (gy == 0 ? 0 : (nq - 2) * ns) + gx - 1; bstride = nq - 2; }
+    }
+    const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
+
+    const __amdgpu_buffer_rsrc_t rF = brsrc(B.Fin, 9u * NPb);
+    const __amdgpu_buffer_rsrc_t rD = brsrc(B.DF, 9u * NPb);
+    const __amdgpu_buffer_rsrc_t rO = brsrc(B.Fout, 9u * NPb);
+    const __amdgpu_buffer_rsrc_t rG = brsrc(CURV ? (const void *)B.G : (const void *)B.Fin, CURV ? 10u * NPb : 0u);
+    const __amdgpu_buffer_rsrc_t rM = brsrc(HET ? (const void *)B.M : (const void *)B.Fin, HET ? 3u * NPb : 0u);
+    auto G = [&](unsigned o, unsigned c) -> real { return CURV ? bload<real>(rG, o, c * NPb) : (real)B.gc[c]; };
+
+    // halo node of this thread (x halo: 6*TY nodes, then y halo: 6*TX nodes)
+    const int tid = ly * TX + lx;
+    sbd_to_lds<real>(sbd, tid, TX * TY);  // published by the first barrier of the march
+    int hk = 0, hcx = 0, hcy = 0;
+    unsigned hcolb = 0;
+    if (tid < 6 * TY) {
+        int hx = tid % 6, hy = tid / 6;
+        hcx = hx < 3 ? hx : TX + hx;
+        hcy = hy;
+        int ggx = xs - 3 + hcx, ggy = ys + hy;
+        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcolb = (unsigned)(ggx + B.px * ggy) * RS; }
+    } else if (tid < 6 * TY + 6 * TX) {
+        int t2 = tid - 6 * TY;
+        int hx = t2 % TX, hy = t2 / TX;
+        hcy = hy < 3 ? hy : TY + hy;
+        hcx = hx;
+        int ggx = xs + hx, ggy = ys - 3 + hcy;
+        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcolb = (unsigned)(ggx + B.px * ggy) * RS; }
+    }
+    const unsigned hax = hk == 2 ? 1u : 0u;
+
+    auto fluxv = [&](const real sg[6], real J, real ax, real ay, real az, real fl[3]) {
+        real Jx = J * ax, Jy = J * ay, Jz = J * az;
+        fl[0] = Jx * sg[0] + Jy * sg[3] + Jz * sg[4];
+        fl[1] = Jx * sg[3] + Jy * sg[1] + Jz * sg[5];
+        fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
+    };
+    const real c1 = (real)0.75, c2 = (real)(-0.15), c3 = (real)(1.0 / 60.0);
+
+    struct PF {
+        real ls[6], lg[4], lv[3];  // lead plane c+3: stresses; J, s_x s_y s_z; velocities
+        real cs[6], cJ;            // centre stresses and J (loaded when not held)
+        real gm[6];                // centre q_x q_y q_z r_x r_y r_z
+        real df[9], mat[3];        // centre rates; lambda mu rho (heterogeneous)
+        real hs[6], hg[4], hv[3];  // halo node: stresses; J + axis metrics; velocities
+    };
+    auto prefetch = [&](PF &R, int c) {
+        if (inb) {
+            const unsigned ol = colb + planeb * (unsigned)(c + 3);
+#pragma unroll
+            for (int f = 0; f < 6; f++) R.ls[f] = bload<real>(rF, ol, (3 + f) * NPb);
+            R.lg[0] = G(ol, 9); R.lg[1] = G(ol, 2); R.lg[2] = G(ol, 5); R.lg[3] = G(ol, 8);
+#pragma unroll
+            for (int f = 0; f < 3; f++) R.lv[f] = bload<real>(rF, ol, f * NPb);
+            const unsigned o = colb + planeb * (unsigned)c;
+            if (!HOLD) {
+#pragma unroll
+                for (int f = 0; f < 6; f++) R.cs[f] = bload<real>(rF, o, (3 + f) * NPb);
+                R.cJ = G(o, 9);
+            }
+            R.gm[0] = G(o, 0); R.gm[1] = G(o, 3); R.gm[2] = G(o, 6);
+            R.gm[3] = G(o, 1); R.gm[4] = G(o, 4); R.gm[5] = G(o, 7);
+        }
+        if (own) {
+            const unsigned o = colb + planeb * (unsigned)c;
+#pragma unroll
+            for (int f = 0; f < 9; f++) R.df[f] = bload_nt<real>(rD, o, f * NPb);
+            if (HET) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) R.mat[f] = bload<real>(rM, o, f * NPb);
+            }
+        }
+        if (hk) {
+            const unsigned o = hcolb + planeb * (unsigned)c;
+#pragma unroll
+            for (int f = 0; f < 6; f++) R.hs[f] = bload<real>(rF, o, (3 + f) * NPb);
+            R.hg[0] = G(o, 9); R.hg[1] = G(o, 0 + hax); R.hg[2] = G(o, 3 + hax); R.hg[3] = G(o, 6 + hax);
+#pragma unroll
+            for (int f = 0; f < 3; f++) R.hv[f] = bload<real>(rF, o, f * NPb);
+        }
+    };
+
+    // queues: fs/vq hold planes c-3..c+3, sm/hq planes c..c+3
+    real fs[7][3], vq[7][3], sm[4][3], hq[HOLD ? 4 : 1][7];
+#pragma unroll
+    for (int a = 0; a < 7; a++)
+#pragma unroll
+        for (int f = 0; f < 3; f++) { fs[a][f] = 0; vq[a][f] = 0; }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int f = 0; f < 3; f++) sm[a][f] = 0;
+#pragma unroll
+    for (int a = 0; a < (HOLD ? 4 : 1); a++)
+#pragma unroll
+        for (int f = 0; f < 7; f++) hq[a][f] = 0;
+    if (inb) {
+        // warm-up: lead planes z0-3 .. z0+2, every load in flight together
+        real ws[6][6], wg[6][4], wv[6][3];
+#pragma unroll
+        for (int a = 0; a < 6; a++) {
+            const unsigned o = colb + planeb * (unsigned)(z0 - 3 + a);
+#pragma unroll
+            for (int f = 0; f < 6; f++) ws[a][f] = bload<real>(rF, o, (3 + f) * NPb);
+            wg[a][0] = G(o, 9); wg[a][1] = G(o, 2); wg[a][2] = G(o, 5); wg[a][3] = G(o, 8);
+#pragma unroll
+            for (int f = 0; f < 3; f++) wv[a][f] = bload<real>(rF, o, f * NPb);
+        }
+#pragma unroll
+        for (int a = 0; a < 6; a++) {
+            fluxv(ws[a], wg[a][0], wg[a][1], wg[a][2], wg[a][3], fs[a + 1]);
+#pragma unroll
+            for (int f = 0; f < 3; f++) vq[a + 1][f] = wv[a][f];
+        }
+#pragma unroll
+        for (int a = 3; a < 6; a++) {
+#pragma unroll
+            for (int f = 0; f < 3; f++) sm[a - 2][f] = wg[a][1 + f];
+            if constexpr (HOLD) {
+#pragma unroll
+                for (int f = 0; f < 6; f++) hq[a - 2][f] = ws[a][f];
+                hq[a - 2][6] = wg[a][0];
+            }
+        }
+    }
+
+    auto body = [&](PF &cur, PF &nxt, int c) {
+        const int b = c & 1;
+        // plane c+3 enters the queues
+#pragma unroll
+        for (int a = 0; a < 6; a++)
+#pragma unroll
+            for (int f = 0; f < 3; f++) { fs[a][f] = fs[a + 1][f]; vq[a][f] = vq[a + 1][f]; }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+#pragma unroll
+            for (int f = 0; f < 3; f++) sm[a][f] = sm[a + 1][f];
+            if constexpr (HOLD) {
+#pragma unroll
+                for (int f = 0; f < 7; f++) hq[a][f] = hq[a + 1][f];
+            }
+        }
+        real sc[6], Jc = 0;
+        if (inb) {
+            fluxv(cur.ls, cur.lg[0], cur.lg[1], cur.lg[2], cur.lg[3], fs[6]);
+#pragma unroll
+            for (int f = 0; f < 3; f++) { vq[6][f] = cur.lv[f]; sm[3][f] = cur.lg[1 + f]; }
+            if constexpr (HOLD) {
+#pragma unroll
+                for (int f = 0; f < 6; f++) hq[3][f] = cur.ls[f];
+                hq[3][6] = cur.lg[0];
+            }
+#pragma unroll
+            for (int f = 0; f < 6; f++) sc[f] = HOLD ? hq[0][f] : cur.cs[f];
+            Jc = HOLD ? hq[0][6] : cur.cJ;
+            real fq[3], fr[3];
+            fluxv(sc, Jc, cur.gm[0], cur.gm[1], cur.gm[2], fq);
+            fluxv(sc, Jc, cur.gm[3], cur.gm[4], cur.gm[5], fr);
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                sfq[b][f][ly][lx + 3] = fq[f];
+                sfr[b][f][ly + 3][lx] = fr[f];
+                sv[b][f][ly + 3][lx + 3] = vq[3][f];
+            }
+        }
+        real cdf[9], cgm[6], cmat[3];
+        if (own) {
+#pragma unroll
+            for (int f = 0; f < 9; f++) cdf[f] = cur.df[f];
+#pragma unroll
+            for (int f = 0; f < 6; f++) cgm[f] = cur.gm[f];
+            if (HET) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) cmat[f] = cur.mat[f];
+            }
+        }
+        if (hk) {
+            real fl[3];
+            fluxv(cur.hs, cur.hg[0], cur.hg[1], cur.hg[2], cur.hg[3], fl);
+            if (hk == 1) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) { sfq[b][f][hcy][hcx] = fl[f]; sv[b][f][hcy + 3][hcx] = cur.hv[f]; }
+            } else {
+#pragma unroll
+                for (int f = 0; f < 3; f++) { sfr[b][f][hcy][hcx] = fl[f]; sv[b][f][hcy][hcx + 3] = cur.hv[f]; }
+            }
+        }
+        __syncthreads();
+        // unconditional: past the chunk's last plane this reads plane z1
+        // (and z1+3 for the lead), which lie inside the block because the
+        // march stops six planes short of the s faces; buffer loads return
+        // 0 for anything beyond the arrays
+        prefetch(nxt, c + 1);
+        if (own) {
+            const unsigned o = colb + planeb * (unsigned)c;
+            const int bi = bbase >= 0 ? bbase + bstride * c : -1;
+            // velocity rates: divergence of the stress fluxes / (rho J)
+            real rho = HET ? cmat[2] : (real)B.rho;
+            real rhoJ_inv = (real)1 / (rho * Jc);
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                real a = d1_dense<real>(sbd, &sfq[b][f][ly][0], 1, lx + 3, gx, B.nq, ihq);
+                real bb = d1_dense<real>(sbd, &sfr[b][f][0][lx], TX, ly + 3, gy, B.nr, ihr);
+                real cc = ihs * (c1 * (fs[4][f] - fs[2][f]) + c2 * (fs[5][f] - fs[1][f]) + c3 * (fs[6][f] - fs[0][f]));
+                real df = A * cdf[f] + (a + bb + cc) * rhoJ_inv;
+                if (bi >= 0) df = df + B.satb[(long)f * B.nbnd + bi];
+                bstore_nt<real>(rD, o, f * NPb, df);
+                bstore_nt<real>(rO, o, f * NPb, vq[3][f] + bdt * df);
+            }
+            // stress rates: Hooke's law on the velocity gradient
+            const real sx = CURV ? sm[0][0] : (real)B.gc[2];
+            const real sy = CURV ? sm[0][1] : (real)B.gc[5];
+            const real sz = CURV ? sm[0][2] : (real)B.gc[8];
+            real Ux[3], Uy[3], Uz[3];
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                real dq = d1_dense<real>(sbd, &sv[b][f][ly + 3][0], 1, lx + 3, gx, B.nq, ihq);
+                real dr = d1_dense<real>(sbd, &sv[b][f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
+                real ds = ihs * (c1 * (vq[4][f] - vq[2][f]) + c2 * (vq[5][f] - vq[1][f]) + c3 * (vq[6][f] - vq[0][f]));
+                Ux[f] = cgm[0] * dq + cgm[3] * dr + sx * ds;
+                Uy[f] = cgm[1] * dq + cgm[4] * dr + sy * ds;
+                Uz[f] = cgm[2] * dq + cgm[5] * dr + sz * ds;
+            }
+            real lam, mu;
+            if (HET) { lam = cmat[0]; mu = cmat[1]; }
+            else { lam = (real)B.lam; mu = (real)B.mu; }
+            real l2m = lam + (real)2 * mu;
+            real DU[6];
+            DU[0] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
+            DU[1] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
+            DU[2] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
+            DU[3] = mu * (Uy[0] + Ux[1]);
+            DU[4] = mu * (Uz[0] + Ux[2]);
+            DU[5] = mu * (Uz[1] + Uy[2]);
+#pragma unroll
+            for (int f = 0; f < 6; f++) {
+                real df = A * cdf[3 + f] + DU[f];
+                if (bi >= 0) df = df + B.satb[(long)(3 + f) * B.nbnd + bi];
+                bstore_nt<real>(rD, o, (3 + f) * NPb, df);
+                bstore_nt<real>(rO, o, (3 + f) * NPb, sc[f] + bdt * df);
+            }
+        }
+    };
+    PF pa, pb;
+    prefetch(pa, z0);
+    int z = z0;
+    for (; z + 1 < z1; z += 2) {
+        body(pa, pb, z);
+        body(pb, pa, z + 1);
+    }
+    if (z < z1) body(pa, pb, z);
 }
 
 // Plane-parallel variant: one thread per node and one workgroup per
