@@ -377,7 +377,11 @@ __device__ __forceinline__ void tile_coords(const TileList &TL, const TileGeom &
 template <int T>
 __device__ __forceinline__ void tile_range(int t, int nt, int a0, int a1, int nfull, int &start, int &c0, int &c1)
 {
-    // start: first node held by the tile; [c0,c1): nodes computed by it
+    // start: first node held by the tile; [c0,c1): nodes computed by it.
+    // The last tile owns the six closure rows at a face (cut) and must hold
+    // every tap they reach (nodes n-9..n-1, the first three through its
+    // halo): it starts at its natural position when that is at most n-6,
+    // else it is right-aligned at a1 - T.
     int cut = (a1 == nfull) ? nfull - 6 : a1;
     if (t < nt - 1) {
         start = a0 + t * T;
@@ -385,8 +389,9 @@ __device__ __forceinline__ void tile_range(int t, int nt, int a0, int a1, int nf
         c1 = min(start + T, cut);
         if (c1 < c0) c1 = c0;
     } else {
-        start = max(a0, a1 - T);
-        c0 = min(a0 + t * T, cut);
+        const int nat = a0 + t * T;
+        start = nat <= cut ? nat : max(a0, a1 - T);
+        c0 = min(nat, cut);
         if (c0 < a0) c0 = a0;
         c1 = a1;
     }
@@ -1296,6 +1301,8 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
     const TileGeom &T = TL.g[gi];
     const Blk<real> &B = P.b[T.blk];
     const real A = T.blk == 0 ? A0 : A1;
+    // the first RK stage has A = 0: its rates are not read (A * DF = 0)
+    const bool rdA = A != (real)0;
     long lid = logical - T.first;
     int tx_i, ty_i, zc_i;
     tile_coords(TL, T, lid, T.nzc, tx_i, ty_i, zc_i);
@@ -1407,7 +1414,7 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
             if (own) {
                 const unsigned o = colb + planeb * (unsigned)z;
 #pragma unroll
-                for (int f = 0; f < 3; f++) { R.v[f] = bload<real>(rF, o, f * NPb); R.df[f] = bload<real>(rD, o, f * NPb); }
+                for (int f = 0; f < 3; f++) { R.v[f] = bload<real>(rF, o, f * NPb); R.df[f] = rdA ? bload<real>(rD, o, f * NPb) : (real)0; }
             }
             if (hk) {
                 const unsigned o = hcolb + planeb * (unsigned)z;
@@ -1547,7 +1554,7 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
 #pragma unroll
                 for (int f = 0; f < 6; f++) {
                     R.s[f] = bload<real>(rF, o, (3 + f) * NPb);
-                    R.df[f] = bload<real>(rD, o, (3 + f) * NPb);
+                    R.df[f] = rdA ? bload<real>(rD, o, (3 + f) * NPb) : (real)0;
                 }
             }
             if (hk) {
@@ -1708,6 +1715,8 @@ __global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
     const TileGeom &T = TL.g[gi];
     const Blk<real> &B = P.b[T.blk];
     const real A = T.blk == 0 ? A0 : A1;
+    // the first RK stage has A = 0: its rates are not read (A * DF = 0)
+    const bool rdA = A != (real)0;
     long lid = logical - T.first;
     int tx_i, ty_i, zc_i;
     tile_coords(TL, T, lid, T.nzc, tx_i, ty_i, zc_i);
@@ -1801,7 +1810,7 @@ __global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
         if (own) {
             const unsigned o = colb + planeb * (unsigned)c;
 #pragma unroll
-            for (int f = 0; f < 9; f++) R.df[f] = bload_nt<real>(rD, o, f * NPb);
+            for (int f = 0; f < 9; f++) R.df[f] = rdA ? bload_nt<real>(rD, o, f * NPb) : (real)0;
             if (HET) {
 #pragma unroll
                 for (int f = 0; f < 3; f++) R.mat[f] = bload<real>(rM, o, f * NPb);
