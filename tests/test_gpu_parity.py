@@ -137,6 +137,31 @@ def test_rhs_vs_oracle_larger(n, mesh):
                 assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= TOL, (tile, ib)
 
 
+@pytest.mark.parametrize('mesh', ['cartesian', 'curvilinear'])
+def test_rhs_chunk_pairs_vs_oracle(mesh):
+    """tile order 3 (s-chunks in pairs, the lower chunk marched downwards):
+    the fused kernel at chunk lengths giving odd and even chunk counts
+    against the oracle"""
+    from wql import native
+    _say(f'[gpu] rhs-pairs {mesh}')
+    prob = G.problem('curv_sw' if mesh == 'curvilinear' else 'cart_sw')
+    for b in prob.blocks:
+        b.n = (41, 33, 40)
+    od, vol, _ = H.oracle_rhs(prob, 0.3)
+    for tile in (20, 22, 24):
+        for zc in (3, 4, 5, 7, 28):
+            with _sim(prob, tile=tile, zchunk=zc) as sim:
+                native.check(native.lib().wql_set_tile_order(sim.ctx, 3, 1))
+                H.load_pattern_state(sim, prob)
+                sim.eval_rates(0.3, what=0)
+                for ib in range(2):
+                    assert G.rel_l2(sim.rates(ib), vol[ib]) <= TOL, (tile, zc, ib)
+                H.load_pattern_state(sim, prob)
+                sim.eval_rates(0.3, what=1, stage=2)
+                for ib in range(2):
+                    assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= TOL, (tile, zc, ib)
+
+
 @pytest.mark.parametrize('case', ['curv_rs', 'pml_locked', 'plastic_sw'])
 def test_graph_replays_match_eager(case):
     """a captured step graph (device-side clock) gives the eager launches'

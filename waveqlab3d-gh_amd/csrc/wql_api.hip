@@ -121,7 +121,7 @@ struct wql_ctx {
     // launch shape of the main volume kernel (set per precision in
     // wql_create: the fastest measured on the 2 x 201x101x101 curvilinear
     // problem)
-    int variant = 20, zchunk = 6, order = 0, group = 2;
+    int variant = 24, zchunk = 6, order = 0, group = 2;
     int use_graph = 0;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int *d_flag = nullptr;
@@ -822,8 +822,10 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
     wql_ctx *c = new wql_ctx();
     c->prec = prob->precision;
     // fused single-pass march on 32x8 tiles (measured fastest on MI355X for
-    // both precisions): 6-plane s chunks in FP32, 12-plane chunks in FP64
-    c->variant = 20;
+    // both precisions): FP32 re-reads the centre plane at three waves per
+    // SIMD with 6-plane s chunks; FP64 holds the lead plane in registers,
+    // 12-plane chunks
+    c->variant = prob->precision == WQL_FP64 ? 20 : 24;
     c->zchunk = prob->precision == WQL_FP64 ? 12 : 6;
     c->prob = *prob;
     c->dt = prob->dt;
@@ -1137,7 +1139,7 @@ int wql_get_tile(const wql_ctx *c, int *variant, int *zchunk)
 
 int wql_set_tile_order(wql_ctx *c, int order, int group)
 {
-    if (!c || order < 0 || order > 2 || group < 1) return fail(WQL_E_ARG, "bad tile order");
+    if (!c || order < 0 || order > 3 || group < 1) return fail(WQL_E_ARG, "bad tile order");
     drop_graphs(c);
     c->order = order;
     c->group = group;

@@ -336,6 +336,10 @@ struct TileList {
     //    then r-tiles, then q-tiles.  Consecutive logical ids land on one
     //    XCD, so each XCD walks its group plane by plane and the s-neighbour
     //    planes and q/r halos of a tile are still in that XCD's L2.
+    // 3: s chunks in pairs (2m, 2m+1) of one tile at consecutive ids, then
+    //    q-tiles, r-tiles, pairs.  The fused march runs the lower chunk of a
+    //    pair downwards and the upper one upwards, so both start at their
+    //    shared boundary and load its warm-up planes together (one XCD, L2).
     int order;
     int gw;
 };
@@ -352,6 +356,20 @@ __device__ __forceinline__ void tile_coords(const TileList &TL, const TileGeom &
         tz = (int)(lid % nz);
         tx = (int)((lid / nz) % T.ntx);
         ty = (int)(lid / ((long)nz * T.ntx));
+    } else if (TL.order == 3) {
+        const long layer = (long)T.ntx * T.nty;
+        const long pr = lid / (2 * layer);
+        const long rem = lid - pr * 2 * layer;
+        if (2 * pr + 1 >= nz) {  // last chunk of an odd count: unpaired
+            tx = (int)(rem % T.ntx);
+            ty = (int)(rem / T.ntx);
+            tz = (int)(2 * pr);
+        } else {
+            const long t2 = rem >> 1;
+            tx = (int)(t2 % T.ntx);
+            ty = (int)(t2 / T.ntx);
+            tz = (int)(2 * pr + (rem & 1));
+        }
     } else {
         const int gw = TL.gw;
         const int ngx = (T.ntx + gw - 1) / gw;
@@ -1690,11 +1708,14 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
 // with one barrier per plane.  The chunk's warm-up (lead planes z0-3 ..
 // z0+2) issues all of its loads together.  Arithmetic per node is the
 // role-split kernel's, in the same order.
-template <typename real>
-constexpr int fused_min_waves() { return sizeof(real) == 4 ? 2 : 1; }
+// Occupancy asked of the register allocator: FP32 re-reading the centre
+// plane fits three waves per SIMD (<= 168 VGPRs), holding it needs two; FP64
+// runs one wave per SIMD with AGPRs (two would spill heavily).
+template <typename real, bool HOLD>
+constexpr int fused_min_waves() { return sizeof(real) == 4 ? (HOLD ? 2 : 3) : 1; }
 
 template <typename real, int TX, int TY, bool CURV, bool HET, bool HOLD>
-__global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
+__global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
     k_volume_fused(Pair<real> P, TileList TL, real A0, real A1, real bdt)
 {
     constexpr int WX = TX + 6, WY = TY + 6;
@@ -1728,6 +1749,13 @@ __global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
     const int span = T.za1 - T.za0;
     const int z0 = T.za0 + (int)((long)zc_i * span / T.nzc);
     const int z1 = T.za0 + (int)((long)(zc_i + 1) * span / T.nzc);
+    // march direction: the lower chunk of a pair (tile order 3) runs down
+    // from its top plane, every other chunk up from its bottom plane; the
+    // queues hold planes in march order, so the s-derivative takes the sign
+    // of the direction (an exact negation)
+    const int dir = (TL.order == 3 && (zc_i & 1) == 0 && zc_i + 1 < T.nzc) ? -1 : 1;
+    const int s0 = dir > 0 ? z0 : z1 - 1;
+    const real sgn = (real)dir;
 
     const int lx = threadIdx.x, ly = threadIdx.y;
     const int gx = xs + lx, gy = ys + ly;
@@ -1792,7 +1820,7 @@ __global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
     };
     auto prefetch = [&](PF &R, int c) {
         if (inb) {
-            const unsigned ol = colb + planeb * (unsigned)(c + 3);
+            const unsigned ol = colb + planeb * (unsigned)(c + 3 * dir);
 #pragma unroll
             for (int f = 0; f < 6; f++) R.ls[f] = bload<real>(rF, ol, (3 + f) * NPb);
             R.lg[0] = G(ol, 9); R.lg[1] = G(ol, 2); R.lg[2] = G(ol, 5); R.lg[3] = G(ol, 8);
@@ -1841,11 +1869,12 @@ __global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
 #pragma unroll
         for (int f = 0; f < 7; f++) hq[a][f] = 0;
     if (inb) {
-        // warm-up: lead planes z0-3 .. z0+2, every load in flight together
+        // warm-up: lead planes s0-3 .. s0+2 (march order), every load in
+        // flight together
         real ws[6][6], wg[6][4], wv[6][3];
 #pragma unroll
         for (int a = 0; a < 6; a++) {
-            const unsigned o = colb + planeb * (unsigned)(z0 - 3 + a);
+            const unsigned o = colb + planeb * (unsigned)(s0 + dir * (a - 3));
 #pragma unroll
             for (int f = 0; f < 6; f++) ws[a][f] = bload<real>(rF, o, (3 + f) * NPb);
             wg[a][0] = G(o, 9); wg[a][1] = G(o, 2); wg[a][2] = G(o, 5); wg[a][3] = G(o, 8);
@@ -1932,11 +1961,11 @@ __global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
             }
         }
         __syncthreads();
-        // unconditional: past the chunk's last plane this reads plane z1
-        // (and z1+3 for the lead), which lie inside the block because the
-        // march stops six planes short of the s faces; buffer loads return
-        // 0 for anything beyond the arrays
-        prefetch(nxt, c + 1);
+        // unconditional: past the chunk's last plane this reads one plane
+        // beyond the chunk (and three more for the lead), which lie inside
+        // the block because the march stops six planes short of the s faces;
+        // buffer loads return 0 for anything beyond the arrays
+        prefetch(nxt, c + dir);
         if (own) {
             const unsigned o = colb + planeb * (unsigned)c;
             const int bi = bbase >= 0 ? bbase + bstride * c : -1;
@@ -1947,7 +1976,7 @@ __global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
             for (int f = 0; f < 3; f++) {
                 real a = d1_dense<real>(sbd, &sfq[b][f][ly][0], 1, lx + 3, gx, B.nq, ihq);
                 real bb = d1_dense<real>(sbd, &sfr[b][f][0][lx], TX, ly + 3, gy, B.nr, ihr);
-                real cc = ihs * (c1 * (fs[4][f] - fs[2][f]) + c2 * (fs[5][f] - fs[1][f]) + c3 * (fs[6][f] - fs[0][f]));
+                real cc = sgn * (ihs * (c1 * (fs[4][f] - fs[2][f]) + c2 * (fs[5][f] - fs[1][f]) + c3 * (fs[6][f] - fs[0][f])));
                 real df = A * cdf[f] + (a + bb + cc) * rhoJ_inv;
                 if (bi >= 0) df = df + B.satb[(long)f * B.nbnd + bi];
                 bstore_nt<real>(rD, o, f * NPb, df);
@@ -1962,7 +1991,7 @@ __global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
             for (int f = 0; f < 3; f++) {
                 real dq = d1_dense<real>(sbd, &sv[b][f][ly + 3][0], 1, lx + 3, gx, B.nq, ihq);
                 real dr = d1_dense<real>(sbd, &sv[b][f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
-                real ds = ihs * (c1 * (vq[4][f] - vq[2][f]) + c2 * (vq[5][f] - vq[1][f]) + c3 * (vq[6][f] - vq[0][f]));
+                real ds = sgn * (ihs * (c1 * (vq[4][f] - vq[2][f]) + c2 * (vq[5][f] - vq[1][f]) + c3 * (vq[6][f] - vq[0][f])));
                 Ux[f] = cgm[0] * dq + cgm[3] * dr + sx * ds;
                 Uy[f] = cgm[1] * dq + cgm[4] * dr + sy * ds;
                 Uz[f] = cgm[2] * dq + cgm[5] * dr + sz * ds;
@@ -1988,13 +2017,16 @@ __global__ void __launch_bounds__(TX * TY, fused_min_waves<real>())
         }
     };
     PF pa, pb;
-    prefetch(pa, z0);
-    int z = z0;
-    for (; z + 1 < z1; z += 2) {
-        body(pa, pb, z);
-        body(pb, pa, z + 1);
+    prefetch(pa, s0);
+    const int nz = z1 - z0;
+    // one plane per iteration, not unrolled: unrolling by two keeps both
+    // prefetch buffers and two copies of the queues live and roughly
+    // doubles the register demand; the copies below are plain moves
+#pragma unroll 1
+    for (int m = 0; m < nz; m++) {
+        body(pa, pb, s0 + dir * m);
+        pa = pb;
     }
-    if (z < z1) body(pa, pb, z);
 }
 
 // Plane-parallel variant: one thread per node and one workgroup per
