@@ -123,8 +123,9 @@ def run(W, precision, steps, warmup, tile, zchunk, graph, order=None):
 
 def fp32_error(W, steps):
     """FP32-volume against FP64 on the same workload and initial state after
-    `steps` steps: relative L2 of slip, of rupture time (nodes ruptured in
-    FP64) and of peak slip rate, on block 1's fault copy"""
+    `steps` steps: relative L2 of block 1's fields, and of slip, rupture time
+    (nodes ruptured in FP64) and peak slip rate on block 1's fault copy
+    (None for a fault that does not slip, such as a locked interface)"""
     import numpy as np
     from wql.solver import Simulation
     out = {}
@@ -138,14 +139,16 @@ def fp32_error(W, steps):
             sim.step(steps)
             sim.sync()
             f = sim.fault(0)
-            out[prec] = dict(slip=f['S'][3], trup=f['trup'][0], psr=sim.fault_peak(0)[0])
+            out[prec] = dict(slip=f['S'][3], trup=f['trup'][0], psr=sim.fault_peak(0)[0], F=sim.fields(0))
     a, b = out['fp64'], out['fp32']
     rel = lambda x, y: float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))  # noqa: E731
     rup = a['trup'] < 1e9
     both = rup & (b['trup'] < 1e9)
-    return {'steps': steps, 'slip': rel(b['slip'], a['slip']),
+    slipping = float(np.abs(a['slip']).max()) > 1e-9
+    return {'steps': steps, 'fields_block1': rel(b['F'], a['F']),
+            'slip': rel(b['slip'], a['slip']) if slipping else None,
             'rupture_time': rel(b['trup'][both], a['trup'][both]) if both.any() else None,
-            'peak_slip_rate': rel(b['psr'], a['psr']),
+            'peak_slip_rate': rel(b['psr'], a['psr']) if slipping else None,
             'ruptured_nodes_fp64': int(rup.sum()), 'ruptured_nodes_fp32': int((b['trup'] < 1e9).sum()),
             'max_slip_fp64': float(a['slip'].max())}
 

@@ -225,7 +225,8 @@ TileList tile_list(const wql_ctx *c)
 }
 
 // s-band planes: the six closure planes at each s face that has no PML, as
-// one 6-plane chunk per tile (per_plane = 0) or one plane per tile
+// one 6-plane chunk per tile (per_plane = 0), one plane per tile
+// (per_plane = 1) or chunks of per_plane planes (per_plane > 1)
 TileList band_list(const wql_ctx *c, int per_plane)
 {
     TileList L;
@@ -247,8 +248,9 @@ TileList band_list(const wql_ctx *c, int per_plane)
             g.za0 = seg[k][0]; g.za1 = seg[k][1];
             g.ntx = (r[1] - r[0] + TX - 1) / TX;
             g.nty = (r[3] - r[2] + TY - 1) / TY;
-            g.zc = per_plane ? 1 : g.za1 - g.za0;
-            g.nzc = per_plane ? g.za1 - g.za0 : 1;
+            const int len = g.za1 - g.za0;
+            g.zc = per_plane == 0 ? len : per_plane;
+            g.nzc = (len + g.zc - 1) / g.zc;
             g.first = first;
             first += (long)g.ntx * g.nty * g.nzc;
         }

@@ -2313,8 +2313,9 @@ __device__ double regula_falsi(double V, double Phi, double eta, double sn, doub
 __device__ int hat_vars(const FaultParams &f, const double u[6], const double v[6], const double m1[3],
                         const double m2[3], const double x[3], const double nn[3], const double mm[3],
                         const double ll[3], double slip, double psi, double t, double uo[6], double vo[6],
-                        double &Vel)
+                        double &Vel, double T0[3])
 {
+    T0[0] = T0[1] = T0[2] = 0.0;
     double c1p = sqrt((2.0 * m1[1] + m1[0]) / m1[2]), c1s = sqrt(m1[1] / m1[2]);
     double c2p = sqrt((2.0 * m2[1] + m2[0]) / m2[2]), c2s = sqrt(m2[1] / m2[2]);
     double Z1s = m1[2] * c1s, Z2s = m2[2] * c2s, Z1p = m1[2] * c1p, Z2p = m2[2] * c2p;
@@ -2340,6 +2341,7 @@ __device__ int hat_vars(const FaultParams &f, const double u[6], const double v[
     double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
     double T0n, T0m, T0l;
     prestress(f, x, t, ll, mm, nn, T0n, T0m, T0l);
+    T0[0] = T0n; T0[1] = T0m; T0[2] = T0l;
     double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
     double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
     double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
@@ -2462,11 +2464,25 @@ __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams
     const long FP = (long)nr * B.ns;
     const long fpi = (long)j + (long)nr * k;
     const long pu = pidx(B1.px, B1.nr, B1.nq - 1, j, k), pv = pidx(B2.px, B2.nr, 0, j, k);
+    // every load of the node is issued here, before any store: each thread
+    // owns index fpi of every interface array, but the compiler cannot
+    // prove the arrays disjoint and would otherwise wait for each store
+    // before the next load
     double u[9], v[9];
     for (int f = 0; f < 9; f++) { u[f] = (double)B1.Fin[f * B1.NP + pu]; v[f] = (double)B2.Fin[f * B2.NP + pv]; }
     const double *fr = B.frame[side];
     double nn[3], mm[3], ll[3];
     for (int c = 0; c < 3; c++) { nn[c] = fr[c * FP + fpi]; mm[c] = fr[(3 + c) * FP + fpi]; ll[c] = fr[(6 + c) * FP + fpi]; }
+    const double *fm1 = B1.fmat[1], *fm2 = B2.fmat[0];
+    double mat1[3] = {fm1[fpi], fm1[FP + fpi], fm1[2 * FP + fpi]};
+    double mat2[3] = {fm2[fpi], fm2[FP + fpi], fm2[2 * FP + fpi]};
+    double x[3] = {B.fx[fpi], B.fx[FP + fpi], B.fx[2 * FP + fpi]};
+    const double *gm = B.fgm[side];
+    const double q_x = gm[fpi], q_y = gm[FP + fpi], q_z = gm[2 * FP + fpi];
+    double S0[4], DS0[4];
+    for (int c = 0; c < 4; c++) { S0[c] = I.S[c * FP + fpi]; DS0[c] = I.DS[c * FP + fpi]; }
+    const double psi = I.W[fpi], DW0 = I.DW[fpi];
+    const double trup0 = stage == 1 ? I.trup[fpi] : 0.0, psr0 = stage == 1 ? I.psr[fpi] : 0.0;
     // tractions and rotation into (n, m, l) (CouplingForcing.f90:105-191, 470-524)
     double ur[6], vr[6];
     ur[0] = u[0]; ur[1] = u[1]; ur[2] = u[2];
@@ -2488,49 +2504,46 @@ __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams
         d[4] = mm[0] * s[3] + mm[1] * s[4] + mm[2] * s[5];
         d[5] = ll[0] * s[3] + ll[1] * s[4] + ll[2] * s[5];
     }
-    const double *fm1 = B1.fmat[1], *fm2 = B2.fmat[0];
-    double mat1[3] = {fm1[fpi], fm1[FP + fpi], fm1[2 * FP + fpi]};
-    double mat2[3] = {fm2[fpi], fm2[FP + fpi], fm2[2 * FP + fpi]};
-    double x[3] = {B.fx[fpi], B.fx[FP + fpi], B.fx[2 * FP + fpi]};
-    double slip = I.S[3 * FP + fpi], psi = I.W[fpi];
-    double uh[6], vh[6], Vel;
-    int rc = hat_vars(fp, uL, vL, mat1, mat2, x, nn, mm, ll, slip, psi, t, uh, vh, Vel);
+    double slip = S0[3];
+    double uh[6], vh[6], Vel, T0[3];
+    int rc = hat_vars(fp, uL, vL, mat1, mat2, x, nn, mm, ll, slip, psi, t, uh, vh, Vel, T0);
     if (rc) return rc;
     // fault state rates (Interface_Condition.f90:431-436, 470-476) and RK update
     if (fp.law == 2) {
-        double dw = A * I.DW[fpi];
+        double dw = A * DW0;
         dw = dw + fp.rs_b * fp.rs_V0 / fp.rs_L * exp(-(psi - fp.rs_f0) / fp.rs_b) - Vel * fp.rs_b / fp.rs_L;
         I.DW[fpi] = dw;
         I.W[fpi] = psi + bdt * dw;
     } else {
-        double dw = A * I.DW[fpi];
+        double dw = A * DW0;
         I.DW[fpi] = dw;
         I.W[fpi] = psi + bdt * dw;
     }
     double slip_rate = sqrt((vh[1] - uh[1]) * (vh[1] - uh[1]) + (vh[2] - uh[2]) * (vh[2] - uh[2]));
     for (int c = 0; c < 4; c++) {
-        double ds = A * I.DS[c * FP + fpi];
+        double ds = A * DS0[c];
         ds = ds + (c < 3 ? vh[c] - uh[c] : slip_rate);
         I.DS[c * FP + fpi] = ds;
-        I.S[c * FP + fpi] = I.S[c * FP + fpi] + bdt * ds;
+        I.S[c * FP + fpi] = S0[c] + bdt * ds;
     }
     if (!isfinite(slip_rate) || !isfinite(uh[4]) || !isfinite(uh[5])) return 4;
     if (stage == 1) {  // 480-509, CouplingForcing.f90:213-228, 584-605
-        double T0n = 0.0, T0m = 0.0, T0l = 0.0;
-        if (fp.law != 0) prestress(fp, x, t, ll, mm, nn, T0n, T0m, T0l);
+        // prestress of this node and time, as formed in hat_vars (zero for
+        // the locked law)
+        const double T0n = T0[0], T0m = T0[1], T0l = T0[2];
+        double sv[3];
         for (int c = 0; c < 3; c++) {
-            double sv = vh[c] - uh[c];
-            I.Svel[c * FP + fpi] = sv;
-            I.Uhat[c * FP + fpi] = sv;
-            I.Vhat[c * FP + fpi] = sv;
+            sv[c] = vh[c] - uh[c];
+            I.Svel[c * FP + fpi] = sv[c];
+            I.Uhat[c * FP + fpi] = sv[c];
+            I.Vhat[c * FP + fpi] = sv[c];
         }
         I.Uhat[3 * FP + fpi] = uh[3] + T0n; I.Vhat[3 * FP + fpi] = vh[3] + T0n;
         I.Uhat[4 * FP + fpi] = uh[4] + T0m; I.Vhat[4 * FP + fpi] = vh[4] + T0m;
         I.Uhat[5 * FP + fpi] = uh[5] + T0l; I.Vhat[5 * FP + fpi] = vh[5] + T0l;
-        double s1 = I.Svel[FP + fpi], s2 = I.Svel[2 * FP + fpi];
-        const double srate = sqrt(s1 * s1 + s2 * s2);
-        if (srate > 1e-3) I.trup[fpi] = fmin(I.trup[fpi], t);
-        I.psr[fpi] = fmax(I.psr[fpi], srate);
+        const double srate = sqrt(sv[1] * sv[1] + sv[2] * sv[2]);
+        if (srate > 1e-3) I.trup[fpi] = fmin(trup0, t);
+        I.psr[fpi] = fmax(psr0, srate);
     }
     // rotate back (527-581) and SAT forcing (242-463)
     double uX[6], vX[6], uH[6], vH[6];
@@ -2541,8 +2554,6 @@ __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams
             dsts[g][c] = nn[c] * srcs[g][0] + mm[c] * srcs[g][1] + ll[c] * srcs[g][2];
             dsts[g][3 + c] = nn[c] * srcs[g][3] + mm[c] * srcs[g][4] + ll[c] * srcs[g][5];
         }
-    const double *gm = B.fgm[side];
-    double q_x = gm[fpi], q_y = gm[FP + fpi], q_z = gm[2 * FP + fpi];
     double norm = sqrt(q_x * q_x + q_y * q_y + q_z * q_z);
     const double *mat = me == 0 ? mat1 : mat2;
     double lam = mat[0], mu = mat[1], rho = mat[2];
@@ -2631,10 +2642,23 @@ struct BoundaryArgs {
 template <typename real>
 __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
 {
+    // dispatch order: the two fault faces first (block 0's +q face, block
+    // 1's -q face: their threads run the friction solve, the longest
+    // dependency chains), then the other boundary nodes of block 0 and of
+    // block 1.  t becomes the block-local boundary-node index.
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int me = 0;
-    if (t >= a.nodes[0]) { me = 1; t -= a.nodes[0]; }
-    if (t >= a.nodes[me]) return;
+    if (t >= a.nodes[0] + a.nodes[1]) return;
+    int me;
+    {
+        const long F0 = (long)P.b[0].nr * P.b[0].ns, F1 = (long)P.b[1].nr * P.b[1].ns;
+        if (t < F0) { me = 0; t = F0 + t; }
+        else if (t < F0 + F1) { me = 1; t = t - F0; }
+        else {
+            t -= F0 + F1;
+            if (t < a.nodes[0] - F0) { me = 0; t = t < F0 ? t : t + F0; }
+            else { me = 1; t = F1 + (t - (a.nodes[0] - F0)); }
+        }
+    }
     const Blk<real> &B = P.b[me];
     const int nq = B.nq, nr = B.nr, ns = B.ns;
     // unique enumeration: q-faces, then r-faces without q edges, then s-faces
