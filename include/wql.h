@@ -111,6 +111,13 @@ int wql_get_metrics(wql_ctx *ctx, int block, double *metric);
 /* face frame of one side [9][face nodes] = n, m, l (unit_normals.f90:8-168) */
 int wql_get_frame(wql_ctx *ctx, int block, int side, double *nml);
 
+/* The nine fields of both blocks on the fault: Uface is block 0's +q face,
+ * Vface block 1's -q face, each [9][ns][nr] in FP64.  These are the arrays
+ * write_fault_output hands to the fault writer, F(pq1,:,:,1:9) and
+ * F(mq2,:,:,1:9) (domain.f90:741-779, fault_output.f90:72-95).  Either
+ * pointer may be NULL. */
+int wql_get_fault_faces(wql_ctx *ctx, double *Uface, double *Vface);
+
 /* fault state of one block's copy (iface.f90:13-48): S[4], W[1], trup[1], Svel[3],
  * Uhat[6], Vhat[6] on [nr*ns] (q-face layout [f][s][r]); any pointer may be NULL */
 int wql_set_fault(wql_ctx *ctx, int block, const double *S, const double *W);

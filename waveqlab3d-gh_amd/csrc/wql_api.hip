@@ -989,6 +989,32 @@ int wql_get_fault(wql_ctx *c, int ib, double *S, double *W, double *trup, double
     return 0;
 }
 
+int wql_get_fault_faces(wql_ctx *c, double *Uface, double *Vface)
+{
+    if (!c) return fail(WQL_E_ARG, "bad argument");
+    double *dst[2] = {Uface, Vface};
+    for (int ib = 0; ib < 2; ib++) {
+        if (!dst[ib]) continue;
+        const DevBlock &d = c->b[ib];
+        const long n = 9L * d.nr * d.ns;
+        double *stage = nullptr;
+        if (hipMalloc(&stage, n * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(staging)");
+        // block 0's fault face is q = nq-1, block 1's is q = 0
+        const int i = ib == 0 ? d.nq - 1 : 0;
+        const unsigned g = (unsigned)((n + 255) / 256);
+        if (c->prec == WQL_FP64)
+            k_qface<double><<<g, 256, 0, c->stream>>>(i, d.nr, d.ns, d.px, d.NP, (const double *)d.F[c->cur], stage);
+        else
+            k_qface<float><<<g, 256, 0, c->stream>>>(i, d.nr, d.ns, d.px, d.NP, (const float *)d.F[c->cur], stage);
+        hipError_t e = hipMemcpyAsync(dst[ib], stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipFree(stage);
+        if (e != hipSuccess) return fail(WQL_E_HIP, "fault face download: %s", hipGetErrorString(e));
+        if (int rc = check_launch()) return rc;
+    }
+    return 0;
+}
+
 int wql_get_fault_peak(wql_ctx *c, int ib, double *psr)
 {
     if (!c || ib < 0 || ib > 1 || !psr) return fail(WQL_E_ARG, "bad argument");

@@ -2934,6 +2934,19 @@ __global__ void k_unpad(int nf, int nq, int nr, int ns, int px, const real *__re
     dst[t] = (double)src[row * px + i];
 }
 
+// the nine fields of the q-face i of a block in FP64, [field][s][r] (the
+// layout of the reference's fault output records)
+template <typename real>
+__global__ void k_qface(int i, int nr, int ns, int px, long NP, const real *__restrict__ F, double *__restrict__ out)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long FP = (long)nr * ns;
+    if (t >= 9 * FP) return;
+    const long f = t / FP, rem = t - f * FP;
+    const int j = (int)(rem % nr), k = (int)(rem / nr);
+    out[t] = (double)F[f * NP + i + (long)px * ((long)j + (long)nr * k)];
+}
+
 // step time advance of a captured step graph: t = t + inc, the same FP64
 // operation as the host-side clock
 __global__ void k_advance_time(double *t, double inc)

@@ -80,6 +80,13 @@ module wql_gpu
        integer(c_int), value :: block
        type(c_ptr), value :: S, W, trup, Svel, Uhat, Vhat
      end function wql_get_fault
+     ! fields of both blocks on the fault, [9][ns][nr] each (fault output
+     ! records Uface / Vface, domain.f90:741-779)
+     integer(c_int) function wql_get_fault_faces(ctx, Uface, Vface) bind(C, name='wql_get_fault_faces')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr), value :: Uface, Vface
+     end function wql_get_fault_faces
      integer(c_int) function wql_step(ctx, nsteps) bind(C, name='wql_step')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -127,6 +127,8 @@ class Problem:
     pl_eta: float = 0.0
     s0_kind: int = 0           # plastic background stress: 0 = fault.s0 everywhere, 1 = TPV26 depth profile
     name: str = 'default'
+    w_stride: int = 1          # fault output every w_stride steps (time_step.f90:137)
+    w_fault: bool = False      # write the fault output files (fault_output.f90)
 
     def dt(self):
         """Global time step (domain.f90:222-235, block.f90:437-452)."""
@@ -198,7 +200,8 @@ def from_namelist(path):
         plastic = 1 if model == 'scec' else 2
     prob = Problem(blocks=blocks, fault=fault_params(problem, coupling),
                    cfl=float(pl.get('cfl', 0.5)), t_final=float(pl.get('t_final', 0.0)),
-                   plastic=plastic, name=pl.get('name', 'default'))
+                   plastic=plastic, name=pl.get('name', 'default'), w_stride=int(pl.get('w_stride', 1)),
+                   w_fault=bool(pl.get('w_fault', False)))
     if plastic:
         f = fault_params(problem, 'slip-weakening_friction') if problem == 'TPV28' else prob.fault
         prob.s0_kind = 1 if problem in ('TPV26', 'TPV27') else 0

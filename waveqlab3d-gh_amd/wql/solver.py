@@ -196,6 +196,14 @@ class Simulation:
                                                          ('S', 'W', 'trup', 'Svel', 'Uhat', 'Vhat'))))
         return out
 
+    def fault_faces(self):
+        """fields on the fault: block 0's +q face and block 1's -q face,
+        each [9][ns][nr]"""
+        nr, ns = self.prob.blocks[0].n[1:]
+        U, V = np.empty((9, ns, nr)), np.empty((9, ns, nr))
+        check(native.lib().wql_get_fault_faces(self.ctx, _p(U), _p(V)))
+        return U, V
+
     def fault_peak(self, ib):
         """peak tangential slip rate per fault node [1][ns][nr]"""
         nq, nr, ns = self.prob.blocks[ib].n
