@@ -58,6 +58,8 @@ def lib():
         _lib.ora_d1.argtypes = [dp, C.c_int, C.c_int, C.c_double]
         _lib.ora_d1.restype = C.c_double
         _lib.ora_rk_coeffs.argtypes = [dp, dp, dp]
+        _lib.ora_plastic.argtypes = [C.POINTER(OraDomain)]
+        _lib.ora_initial_stress.argtypes = [C.POINTER(OraDomain), dp, dp]
         _lib.ora_slab_points.argtypes = [C.POINTER(OraBlock), C.c_int]
         _lib.ora_slab_points.restype = C.c_size_t
         _lib.ora_hat.argtypes = [C.POINTER(OraFault)] + [dp] * 6 + [C.c_double] * 3 + [dp, dp, dp]
@@ -151,6 +153,15 @@ class OracleDomain:
         rc = lib().ora_interface(C.byref(self.d), ib1, t, stage)
         if rc:
             raise RuntimeError(f'oracle interface error {rc}')
+
+    def plastic(self):
+        lib().ora_plastic(C.byref(self.d))
+
+    def initial_stress(self, x):
+        out = np.zeros(6)
+        lib().ora_initial_stress(C.byref(self.d), np.ascontiguousarray(x, dtype=np.float64).ctypes.data_as(
+            C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
 
     def step(self, n):
         rc = lib().ora_step(C.byref(self.d), n)

@@ -1025,6 +1025,18 @@ static void plastic_update(ora_domain *d, ora_block *b)
             }
 }
 
+void ora_plastic(ora_domain *d)
+{ /* the return map of the last RK stage on both blocks (domain.f90:986-993) */
+    if (!d->plastic) return;
+    plastic_update(d, &d->b[0]);
+    plastic_update(d, &d->b[1]);
+}
+
+void ora_initial_stress(const ora_domain *d, const double x[3], double s0[6])
+{ /* background stress tensor of the plastic update at one point */
+    s0_at(d, x, s0);
+}
+
 /* ------------------------------------------------------------------------ */
 void ora_rk_coeffs(double *A, double *B, double *C)
 { /* time_step.f90:53-62 */

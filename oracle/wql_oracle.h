@@ -89,6 +89,14 @@ void ora_init_state(ora_domain *d);
 double ora_d1(const double *u, int n, int i, double h);
 /* LSRK coefficients A[5], B[5], C[6] (time_step.f90:53-62) */
 void ora_rk_coeffs(double *A, double *B, double *C);
+/* the viscoplastic return map of the last RK stage on both blocks
+ * (plastic.f90:11-290; pinned by the return-map invariants in
+ * tests/test_oracle_plastic.py) */
+void ora_plastic(ora_domain *d);
+/* background stress tensor of the plastic update at point x
+ * (initial_stress_condition.f90:26-76; pinned by the depth-profile
+ * properties in tests/test_oracle_plastic.py) */
+void ora_initial_stress(const ora_domain *d, const double x[3], double s0[6]);
 /* pointwise friction solve (tests): slip-weakening / rate-and-state hat variables */
 int ora_hat(const ora_fault *f, const double u[6], const double v[6], const double mat1[3],
             const double mat2[3], const double x[3], const double nrm[9], double slip,

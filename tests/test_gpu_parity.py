@@ -285,3 +285,31 @@ def test_fault_diagnostics(case):
             if rup.any():
                 assert st['mean_slip_ruptured'] == pytest.approx(f['S'][3][rup].mean(), rel=1e-12)
                 assert st['first_rupture_time'] == f['trup'][0][rup].min()
+
+
+@pytest.mark.parametrize('s0_kind', [0, 1])
+def test_plastic_flow_model_vs_oracle(s0_kind):
+    """the default plasticity model (plastic_flow, plastic.f90:221-290) with
+    the constant (0) and the TPV26 depth (1) background stress: fields after
+    six LSRK steps against the oracle, whose return map and stress profile
+    are pinned by tests/test_oracle_plastic.py"""
+    _say(f'[gpu] plastic model 2 s0_kind={s0_kind}')
+    prob = G.problem('plastic_sw')
+    prob.plastic = 2
+    prob.pl_mu, prob.pl_beta, prob.pl_eta = 0.05, 0.3, 0.01
+    prob.s0_kind = s0_kind
+    args = G.load('plastic_sw')['traj_args']
+    F0 = H.pulse_init(prob, args)
+    od = O.OracleDomain(prob)
+    for ib in range(2):
+        od.blocks[ib]['F'][...] = F0[ib]
+    od.step(6)
+    with _sim(prob) as sim:
+        for ib in range(2):
+            sim.set_fields(ib, F0[ib])
+        sim.step(6)
+        sim.sync()
+        for ib in range(2):
+            assert G.rel_l2(sim.fields(ib), od.blocks[ib]['F']) <= TOL, ib
+    # the background stress yields: the plastic rate is nonzero somewhere
+    assert max(np.abs(od.blocks[ib]['P'][0]).max() for ib in range(2)) > 0.0
