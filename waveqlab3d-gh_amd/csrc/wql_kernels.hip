@@ -179,7 +179,11 @@ __device__ void volume_node_generic(const Blk<real> &B, int i, int j, int k, rea
         real ss[3][6];
         for (int fam = 0; fam < 3; fam++)
             for (int f = 0; f < 6; f++) ss[fam][f] = 0;
-        for (int t = 0; t < m; t++) {
+        // fixed trip count (closure rows have at most eight taps), so the
+        // tap tables stay in registers and the taps' loads issue together
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            if (t >= m) break;
             int ii = i, jj = j, kk = k;
             if (a == 0) ii = idx[t]; else if (a == 1) jj = idx[t]; else kk = idx[t];
             long q = pidx(B.px, B.nr, ii, jj, kk);
