@@ -112,12 +112,6 @@ struct DevBlock {
 struct wql_ctx {
     int prec;
     hipStream_t stream = nullptr;
-    // second stream of the stage: the s-closure band and PML-slab kernels run
-    // on it beside the main volume kernel (they write disjoint nodes and all
-    // three only read the stage input), forked after the boundary penalties
-    // and joined before the rest of the stage
-    hipStream_t side = nullptr;
-    hipEvent_t evf = nullptr, evj = nullptr;
     DevBlock b[2];
     Iface I[2];
     FaultParams fault;
@@ -371,7 +365,7 @@ void launch_main(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0, r
 }
 
 template <typename real>
-void launch_band(const wql_ctx *c, Pair<real> &P, real A, real bdt, hipStream_t st)
+void launch_band(const wql_ctx *c, Pair<real> &P, real A, real bdt)
 {
     // the six closure planes at each s face without PML: role-split march in
     // closure mode, one 6-plane chunk per face and tile (32 x 8 tiles)
@@ -386,19 +380,19 @@ void launch_band(const wql_ctx *c, Pair<real> &P, real A, real bdt, hipStream_t 
         TileList TP = band_list(c, 1);
         if (TP.total <= 0) return;
         dim3 grd((unsigned)(((TP.total + 7) / 8) * 8));
-        if (curv && het) k_volume_plane<real, 32, 8, true, true><<<grd, blk, 0, st>>>(P, TP, A, A, bdt);
-        else if (curv) k_volume_plane<real, 32, 8, true, false><<<grd, blk, 0, st>>>(P, TP, A, A, bdt);
-        else if (het) k_volume_plane<real, 32, 8, false, true><<<grd, blk, 0, st>>>(P, TP, A, A, bdt);
-        else k_volume_plane<real, 32, 8, false, false><<<grd, blk, 0, st>>>(P, TP, A, A, bdt);
+        if (curv && het) k_volume_plane<real, 32, 8, true, true><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
+        else if (curv) k_volume_plane<real, 32, 8, true, false><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
+        else if (het) k_volume_plane<real, 32, 8, false, true><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
+        else k_volume_plane<real, 32, 8, false, false><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
         return;
     }
     TileList TB = band_list(c, 0);
     if (TB.total <= 0) return;
     dim3 grd((unsigned)(((2 * TB.total + 7) / 8) * 8));
-    if (curv && het) k_volume_split<real, 32, 8, true, true, 1><<<grd, blk, 0, st>>>(P, TB, A, A, bdt);
-    else if (curv) k_volume_split<real, 32, 8, true, false, 1><<<grd, blk, 0, st>>>(P, TB, A, A, bdt);
-    else if (het) k_volume_split<real, 32, 8, false, true, 1><<<grd, blk, 0, st>>>(P, TB, A, A, bdt);
-    else k_volume_split<real, 32, 8, false, false, 1><<<grd, blk, 0, st>>>(P, TB, A, A, bdt);
+    if (curv && het) k_volume_split<real, 32, 8, true, true, 1><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+    else if (curv) k_volume_split<real, 32, 8, true, false, 1><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+    else if (het) k_volume_split<real, 32, 8, false, true, 1><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
+    else k_volume_split<real, 32, 8, false, false, 1><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
 }
 
 // the boundary penalties are folded into the volume kernels when every
@@ -454,21 +448,6 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
         k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
         for (int ib = 0; ib < 2; ib++) P.b[ib].satb = (const real *)c->b[ib].satb;
     }
-    // fork: the band and PML-slab kernels on the side stream (fused
-    // penalties only: the unfused path applies the penalties after all
-    // volume kernels)
-    hipStream_t vs = c->stream;
-    if (fused && c->side) {
-        hipEventRecord(c->evf, c->stream);
-        hipStreamWaitEvent(c->side, c->evf, 0);
-        vs = c->side;
-    }
-    launch_band<real>(c, P, (real)A, (real)bdt, vs);
-    BoxList BL = box_list(c);
-    if (BL.total > 0) {
-        unsigned g = (unsigned)((BL.total + 255) / 256);
-        k_volume_generic<real><<<g, 256, 0, vs>>>(P, BL, (real)A, (real)A, (real)bdt);
-    }
     // mixed grids: a Cartesian block inside a curvilinear pair still has no G
     TileList TL = tile_list(c);
     if (TL.total > 0) {
@@ -483,9 +462,11 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
             c->vol_launches++;
         }
     }
-    if (vs != c->stream) {  // join
-        hipEventRecord(c->evj, c->side);
-        hipStreamWaitEvent(c->stream, c->evj, 0);
+    launch_band<real>(c, P, (real)A, (real)bdt);
+    BoxList BL = box_list(c);
+    if (BL.total > 0) {
+        unsigned g = (unsigned)((BL.total + 255) / 256);
+        k_volume_generic<real><<<g, 256, 0, c->stream>>>(P, BL, (real)A, (real)A, (real)bdt);
     }
     if (!fused) {
         a.mode = 0;
@@ -854,12 +835,9 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
     static_assert(sizeof(FaultParams) == sizeof(wql_fault_params), "fault layout");
     memcpy(&c->fault, &prob->fault, sizeof(FaultParams));
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evf, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evj, hipEventDisableTiming);
     if (e != hipSuccess) {
-        wql_destroy(c);
-        return fail(WQL_E_HIP, "stream / event create: %s", hipGetErrorString(e));
+        delete c;
+        return fail(WQL_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     int rc = c->prec == WQL_FP64 ? create_typed<double>(c, blocks) : create_typed<float>(c, blocks);
     if (rc) {
@@ -883,15 +861,11 @@ int wql_destroy(wql_ctx *c)
 {
     if (!c) return 0;
     if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->side) hipStreamSynchronize(c->side);
     for (int k = 0; k < 2; k++)
         if (c->gexec[k]) hipGraphExecDestroy(c->gexec[k]);
     for (void *p : c->allocs) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->evf) hipEventDestroy(c->evf);
-    if (c->evj) hipEventDestroy(c->evj);
-    if (c->side) hipStreamDestroy(c->side);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -1109,12 +1083,12 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
         BoxList BL = box_list(c);
         if (c->prec == WQL_FP64) {
             if (TL.total > 0) launch_main<double>(c, Pd, TL, 0.0, 0.0, 0.0);
-            launch_band<double>(c, Pd, 0.0, 0.0, c->stream);
+            launch_band<double>(c, Pd, 0.0, 0.0);
             if (BL.total > 0)
                 k_volume_generic<double><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(Pd, BL, 0.0, 0.0, 0.0);
         } else {
             if (TL.total > 0) launch_main<float>(c, Pf, TL, 0.f, 0.f, 0.f);
-            launch_band<float>(c, Pf, 0.f, 0.f, c->stream);
+            launch_band<float>(c, Pf, 0.f, 0.f);
             if (BL.total > 0)
                 k_volume_generic<float><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(Pf, BL, 0.f, 0.f, 0.f);
         }
