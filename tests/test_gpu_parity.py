@@ -313,3 +313,29 @@ def test_plastic_flow_model_vs_oracle(s0_kind):
             assert G.rel_l2(sim.fields(ib), od.blocks[ib]['F']) <= TOL, ib
     # the background stress yields: the plastic rate is nonzero somewhere
     assert max(np.abs(od.blocks[ib]['P'][0]).max() for ib in range(2)) > 0.0
+
+
+@pytest.mark.parametrize('shapes', [((37, 21, 29), (13, 21, 29)), ((65, 22, 31), (38, 22, 31)),
+                                    ((13, 13, 13), (33, 13, 13))])
+def test_rhs_block_shapes_vs_oracle(shapes):
+    """blocks of different lengths along q, and q/r lengths whose last tile
+    sits at its natural position (q 38, r 22: the tile starts exactly six
+    nodes from the face) or is right-aligned (remainders 1 and 5), plus
+    single-tile blocks: the fused (default and lead-holding) and role-split
+    kernels against the oracle"""
+    _say(f'[gpu] rhs shapes {shapes}')
+    prob = G.problem('curv_sw')
+    for b, n in zip(prob.blocks, shapes):
+        b.n = n
+    od, vol, _ = H.oracle_rhs(prob, 0.3)
+    for tile in (None, 24, 16):
+        kw = {} if tile is None else dict(tile=tile, zchunk=5)
+        with _sim(prob, **kw) as sim:
+            H.load_pattern_state(sim, prob)
+            sim.eval_rates(0.3, what=0)
+            for ib in range(2):
+                assert G.rel_l2(sim.rates(ib), vol[ib]) <= TOL, (tile, ib)
+            H.load_pattern_state(sim, prob)
+            sim.eval_rates(0.3, what=1, stage=2)
+            for ib in range(2):
+                assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= TOL, (tile, ib)

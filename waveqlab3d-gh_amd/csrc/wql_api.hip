@@ -126,6 +126,9 @@ struct wql_ctx {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int *d_flag = nullptr;
     double *d_t = nullptr;
+    // persistent device staging of the small per-output downloads: both
+    // fault faces [9][nr ns] each, and the six fault statistics
+    double *d_faces = nullptr, *d_stats = nullptr;
     size_t bytes = 0;
     std::vector<void *> allocs;
     int profile = 0;
@@ -851,6 +854,15 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
         return fail(WQL_E_NOMEM, "hipMalloc(time)");
     }
     c->allocs.push_back(c->d_t);
+    {
+        const size_t nf = 9 * ((size_t)c->b[0].nr * c->b[0].ns + (size_t)c->b[1].nr * c->b[1].ns);
+        if (hipMalloc(&c->d_faces, nf * sizeof(double)) != hipSuccess ||
+            (c->allocs.push_back(c->d_faces), hipMalloc(&c->d_stats, 6 * sizeof(double)) != hipSuccess)) {
+            wql_destroy(c);
+            return fail(WQL_E_NOMEM, "hipMalloc(download staging)");
+        }
+        c->allocs.push_back(c->d_stats);
+    }
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
     *out = c;
@@ -997,8 +1009,7 @@ int wql_get_fault_faces(wql_ctx *c, double *Uface, double *Vface)
         if (!dst[ib]) continue;
         const DevBlock &d = c->b[ib];
         const long n = 9L * d.nr * d.ns;
-        double *stage = nullptr;
-        if (hipMalloc(&stage, n * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(staging)");
+        double *stage = c->d_faces + (ib == 0 ? 0 : 9L * c->b[0].nr * c->b[0].ns);
         // block 0's fault face is q = nq-1, block 1's is q = 0
         const int i = ib == 0 ? d.nq - 1 : 0;
         const unsigned g = (unsigned)((n + 255) / 256);
@@ -1008,7 +1019,6 @@ int wql_get_fault_faces(wql_ctx *c, double *Uface, double *Vface)
             k_qface<float><<<g, 256, 0, c->stream>>>(i, d.nr, d.ns, d.px, d.NP, (const float *)d.F[c->cur], stage);
         hipError_t e = hipMemcpyAsync(dst[ib], stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        hipFree(stage);
         if (e != hipSuccess) return fail(WQL_E_HIP, "fault face download: %s", hipGetErrorString(e));
         if (int rc = check_launch()) return rc;
     }
@@ -1028,12 +1038,10 @@ int wql_fault_stats(wql_ctx *c, int ib, double stats[6])
 {
     if (!c || ib < 0 || ib > 1 || !stats) return fail(WQL_E_ARG, "bad argument");
     long FP = (long)c->b[ib].nr * c->b[ib].ns;
-    double *d_out = nullptr;
-    if (hipMalloc(&d_out, 6 * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(stats)");
+    double *d_out = c->d_stats;
     k_fault_stats<<<1, 1024, 0, c->stream>>>(c->I[ib], FP, d_out);
     hipError_t e = hipMemcpyAsync(stats, d_out, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(d_out);
     if (e != hipSuccess) return fail(WQL_E_HIP, "fault stats: %s", hipGetErrorString(e));
     return check_launch();
 }
