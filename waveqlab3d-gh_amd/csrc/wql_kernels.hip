@@ -1818,7 +1818,7 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
     struct PF {
         real ls[6], lg[4], lv[3];  // lead plane c+3: stresses; J, s_x s_y s_z; velocities
         real cs[6], cJ;            // centre stresses and J (loaded when not held)
-        real gm[6];                // centre q_x q_y q_z r_x r_y r_z
+        real gm[6], gs[3];         // centre q_x q_y q_z r_x r_y r_z; s_x s_y s_z (not held)
         real df[9], mat[3];        // centre rates; lambda mu rho (heterogeneous)
         real hs[6], hg[4], hv[3];  // halo node: stresses; J + axis metrics; velocities
     };
@@ -1838,6 +1838,7 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
             }
             R.gm[0] = G(o, 0); R.gm[1] = G(o, 3); R.gm[2] = G(o, 6);
             R.gm[3] = G(o, 1); R.gm[4] = G(o, 4); R.gm[5] = G(o, 7);
+            if (!HOLD) { R.gs[0] = G(o, 2); R.gs[1] = G(o, 5); R.gs[2] = G(o, 8); }
         }
         if (own) {
             const unsigned o = colb + planeb * (unsigned)c;
@@ -1858,14 +1859,16 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
         }
     };
 
-    // queues: fs/vq hold planes c-3..c+3, sm/hq planes c..c+3
-    real fs[7][3], vq[7][3], sm[4][3], hq[HOLD ? 4 : 1][7];
+    // queues: fs/vq hold planes c-3..c+3, sm/hq planes c..c+3 (with HOLD;
+    // without it the centre plane re-reads its stresses, J and s-metrics,
+    // which keeps the kernel within 168 VGPRs)
+    real fs[7][3], vq[7][3], sm[HOLD ? 4 : 1][3], hq[HOLD ? 4 : 1][7];
 #pragma unroll
     for (int a = 0; a < 7; a++)
 #pragma unroll
         for (int f = 0; f < 3; f++) { fs[a][f] = 0; vq[a][f] = 0; }
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < (HOLD ? 4 : 1); a++)
 #pragma unroll
         for (int f = 0; f < 3; f++) sm[a][f] = 0;
 #pragma unroll
@@ -1893,9 +1896,9 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
         }
 #pragma unroll
         for (int a = 3; a < 6; a++) {
-#pragma unroll
-            for (int f = 0; f < 3; f++) sm[a - 2][f] = wg[a][1 + f];
             if constexpr (HOLD) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) sm[a - 2][f] = wg[a][1 + f];
 #pragma unroll
                 for (int f = 0; f < 6; f++) hq[a - 2][f] = ws[a][f];
                 hq[a - 2][6] = wg[a][0];
@@ -1912,9 +1915,9 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
             for (int f = 0; f < 3; f++) { fs[a][f] = fs[a + 1][f]; vq[a][f] = vq[a + 1][f]; }
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-#pragma unroll
-            for (int f = 0; f < 3; f++) sm[a][f] = sm[a + 1][f];
             if constexpr (HOLD) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) sm[a][f] = sm[a + 1][f];
 #pragma unroll
                 for (int f = 0; f < 7; f++) hq[a][f] = hq[a + 1][f];
             }
@@ -1923,8 +1926,10 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
         if (inb) {
             fluxv(cur.ls, cur.lg[0], cur.lg[1], cur.lg[2], cur.lg[3], fs[6]);
 #pragma unroll
-            for (int f = 0; f < 3; f++) { vq[6][f] = cur.lv[f]; sm[3][f] = cur.lg[1 + f]; }
+            for (int f = 0; f < 3; f++) vq[6][f] = cur.lv[f];
             if constexpr (HOLD) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) sm[3][f] = cur.lg[1 + f];
 #pragma unroll
                 for (int f = 0; f < 6; f++) hq[3][f] = cur.ls[f];
                 hq[3][6] = cur.lg[0];
@@ -1942,12 +1947,14 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
                 sv[b][f][ly + 3][lx + 3] = vq[3][f];
             }
         }
-        real cdf[9], cgm[6], cmat[3];
+        real cdf[9], cgm[6], csm[3], cmat[3];
         if (own) {
 #pragma unroll
             for (int f = 0; f < 9; f++) cdf[f] = cur.df[f];
 #pragma unroll
             for (int f = 0; f < 6; f++) cgm[f] = cur.gm[f];
+#pragma unroll
+            for (int f = 0; f < 3; f++) csm[f] = HOLD ? sm[0][f] : cur.gs[f];
             if (HET) {
 #pragma unroll
                 for (int f = 0; f < 3; f++) cmat[f] = cur.mat[f];
@@ -1987,9 +1994,9 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
                 bstore_nt<real>(rO, o, f * NPb, vq[3][f] + bdt * df);
             }
             // stress rates: Hooke's law on the velocity gradient
-            const real sx = CURV ? sm[0][0] : (real)B.gc[2];
-            const real sy = CURV ? sm[0][1] : (real)B.gc[5];
-            const real sz = CURV ? sm[0][2] : (real)B.gc[8];
+            const real sx = CURV ? csm[0] : (real)B.gc[2];
+            const real sy = CURV ? csm[1] : (real)B.gc[5];
+            const real sz = CURV ? csm[2] : (real)B.gc[8];
             real Ux[3], Uy[3], Uz[3];
 #pragma unroll
             for (int f = 0; f < 3; f++) {
