@@ -59,13 +59,16 @@ def test_rhs_vs_golden(case, tile):
             assert np.abs(DW - ref).max() <= TOL * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize('tile', [None, 24])
 @pytest.mark.parametrize('case', G.TRAJ_CASES)
-def test_trajectory_vs_golden(case):
-    _say(f'[gpu] traj {case}')
+def test_trajectory_vs_golden(case, tile):
+    """tile None: the precision's default launch shape; 24: the fused
+    variant that keeps the q-face nodes in compact buffers between stages"""
+    _say(f'[gpu] traj {case} tile={tile}')
     z = G.load(case)
     prob = G.problem(case)
     args = z['traj_args']
-    with _sim(prob) as sim:
+    with _sim(prob, tile=tile) as sim:
         assert sim.dt == pytest.approx(float(z['dt']), rel=1e-15)
         F0 = H.pulse_init(prob, args)
         for ib in range(2):
@@ -287,13 +290,14 @@ def test_fault_diagnostics(case):
                 assert st['first_rupture_time'] == f['trup'][0][rup].min()
 
 
+@pytest.mark.parametrize('tile', [None, 24])
 @pytest.mark.parametrize('s0_kind', [0, 1])
-def test_plastic_flow_model_vs_oracle(s0_kind):
+def test_plastic_flow_model_vs_oracle(s0_kind, tile):
     """the default plasticity model (plastic_flow, plastic.f90:221-290) with
     the constant (0) and the TPV26 depth (1) background stress: fields after
     six LSRK steps against the oracle, whose return map and stress profile
     are pinned by tests/test_oracle_plastic.py"""
-    _say(f'[gpu] plastic model 2 s0_kind={s0_kind}')
+    _say(f'[gpu] plastic model 2 s0_kind={s0_kind} tile={tile}')
     prob = G.problem('plastic_sw')
     prob.plastic = 2
     prob.pl_mu, prob.pl_beta, prob.pl_eta = 0.05, 0.3, 0.01
@@ -304,7 +308,7 @@ def test_plastic_flow_model_vs_oracle(s0_kind):
     for ib in range(2):
         od.blocks[ib]['F'][...] = F0[ib]
     od.step(6)
-    with _sim(prob) as sim:
+    with _sim(prob, tile=tile) as sim:
         for ib in range(2):
             sim.set_fields(ib, F0[ib])
         sim.step(6)

@@ -92,6 +92,7 @@ struct DevBlock {
     long ND;                // nq*nr*ns (dense, as the host sees a field)
     void *satb = nullptr;   // [9][nbnd] real: boundary penalties of the stage
     long nbnd = 0;          // boundary nodes
+    void *fq = nullptr;     // [2][9][nr ns] real: the nine fields on the q faces (q = 0, q = nq-1)
     int cart;
     int bc[6], pml[6], npml;
     double lam, mu, rho;
@@ -412,6 +413,32 @@ bool fused_faces(const wql_ctx *c)
     return true;
 }
 
+// the q-face buffers are in use with the fused penalties and the fused
+// volume variants that re-read the centre plane's metrics (24-27); every
+// kernel that writes the stage output of a q-face node then also writes the
+// buffer.  The metric-holding variants (20-23, the FP64 default) run at the
+// register limit: there the buffer's offset and the predicated stores cost
+// the volume kernel more (14 us a stage on the 2x201x101x101 rupture) than
+// the coalesced reads save the boundary kernel.
+template <typename real>
+bool fq_on(const wql_ctx *c)
+{
+    return c->variant >= 24 && fused_faces<real>(c);
+}
+
+// refresh the q-face buffers from the current fields (after host updates)
+template <typename real>
+void gather_fq(wql_ctx *c)
+{
+    if (!fq_on<real>(c)) return;
+    for (int ib = 0; ib < 2; ib++) {
+        const DevBlock &d = c->b[ib];
+        const long n = 18L * d.nr * d.ns;
+        k_fq_gather<real><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(d.nq, d.nr, d.ns, d.px, d.NP,
+                                                                              (const real *)d.F[c->cur], (real *)d.fq);
+    }
+}
+
 template <typename real>
 void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int stage, int rates_only,
                   int do_plastic, const double *tdev = nullptr, double tinc = 0.0)
@@ -444,11 +471,19 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
     bool any_pml = false;
     for (int ib = 0; ib < 2; ib++)
         for (int sd = 0; sd < 6; sd++) any_pml = any_pml || c->b[ib].pml[sd];
+    // q-face buffers (fused variants): the boundary kernel reads the stage
+    // input of the q-face nodes from them, coalesced, and the volume kernels
+    // write the stage output of those nodes into them
+    const bool fq = fq_on<real>(c);
+    if (fq)
+        for (int ib = 0; ib < 2; ib++) P.b[ib].FQ = (real *)c->b[ib].fq;
     if (fused) {
         a.mode = 1;
         a.satb[0] = c->b[0].satb;
         a.satb[1] = c->b[1].satb;
+        a.fq_read = fq ? 1 : 0;
         k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
+        a.fq_read = 0;
         for (int ib = 0; ib < 2; ib++) P.b[ib].satb = (const real *)c->b[ib].satb;
     }
     // mixed grids: a Cartesian block inside a curvilinear pair still has no G
@@ -642,6 +677,7 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
         if (int rc = dalloc(c, &d.DF, 9 * NP * sizeof(real))) return rc;
         d.nbnd = 2L * d.nr * d.ns + 2L * (d.nq - 2) * d.ns + 2L * (d.nq - 2) * (d.nr - 2);
         if (int rc = dalloc(c, &d.satb, 9 * d.nbnd * sizeof(real))) return rc;
+        if (int rc = dalloc(c, &d.fq, 18L * d.nr * d.ns * sizeof(real))) return rc;
         k_fill<real><<<(unsigned)((9 * NP + 255) / 256), 256, 0, c->stream>>>(9 * NP, (real *)d.DF, (real)0);
         for (int s = 0; s < 6; s++) {
             if (!d.pml[s]) continue;
@@ -1101,8 +1137,13 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
                 k_volume_generic<float><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(Pf, BL, 0.f, 0.f, 0.f);
         }
     } else {
-        if (c->prec == WQL_FP64) launch_stage<double>(c, c->cur, 0.0, 0.0, t, stage, 1, 0);
-        else launch_stage<float>(c, c->cur, 0.0, 0.0, t, stage, 1, 0);
+        if (c->prec == WQL_FP64) {
+            gather_fq<double>(c);
+            launch_stage<double>(c, c->cur, 0.0, 0.0, t, stage, 1, 0);
+        } else {
+            gather_fq<float>(c);
+            launch_stage<float>(c, c->cur, 0.0, 0.0, t, stage, 1, 0);
+        }
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return check_launch();
@@ -1111,6 +1152,10 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
 int wql_step(wql_ctx *c, int nsteps)
 {
     if (!c || nsteps < 0) return fail(WQL_E_ARG, "bad argument");
+    if (nsteps > 0) {
+        if (c->prec == WQL_FP64) gather_fq<double>(c);
+        else gather_fq<float>(c);
+    }
     const bool graph = c->use_graph && !c->profile;
     if (graph && nsteps > 0) {
         // the device clock starts from the host clock; one graph per

@@ -68,6 +68,10 @@ struct Blk {
     // kernels when they form DF (nullptr: the boundary kernel applies it)
     const real *satb;        // [9][nbnd]
     long nbnd;               // boundary nodes: 2 nr ns + 2 (nq-2) ns + 2 (nq-2)(nr-2)
+    // the nine fields on the q faces, [2][9][nr ns] (face q = 0, then q = nq-1;
+    // node j + nr k): written with the stage output by the volume kernels,
+    // read as the next stage's input by the boundary kernel (nullptr: unused)
+    real *FQ;
 };
 
 // index of node (i, j, k) in the boundary-node enumeration (q faces, then r

@@ -285,12 +285,17 @@ __device__ void volume_node_generic(const Blk<real> &B, int i, int j, int k, rea
         }
     }
     const long bi = B.satb ? bnd_index(B, i, j, k) : -1;
+    real *fq = nullptr;
+    if (B.FQ && (i == 0 || i == B.nq - 1))
+        fq = B.FQ + (i == 0 ? 0L : 9L * B.nr * B.ns) + j + (long)B.nr * k;
     for (int f = 0; f < NF; f++) {
         real df = A * B.DF[f * NP + p];
         df = df + DU[f];
         if (bi >= 0) df = df + B.satb[f * B.nbnd + bi];
         B.DF[f * NP + p] = df;
-        B.Fout[f * NP + p] = B.Fin[f * NP + p] + bdt * df;
+        const real fo = B.Fin[f * NP + p] + bdt * df;
+        B.Fout[f * NP + p] = fo;
+        if (fq) fq[(long)f * B.nr * B.ns] = fo;
     }
 }
 
@@ -1364,6 +1369,12 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
         return -1;
     };
     const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
+    // q-face buffer: an own column on a q face also writes its stage output
+    // there (byte offset of its node at plane 0; -1 elsewhere)
+    const unsigned FPqb = (unsigned)(B.nr * B.ns) * RS;
+    const __amdgpu_buffer_rsrc_t rQ = brsrc(B.FQ ? (const void *)B.FQ : (const void *)B.Fin, B.FQ ? 18u * FPqb : 0u);
+    int fqo = -1;
+    if (B.FQ && own && (gx == 0 || gx == B.nq - 1)) fqo = (int)((gx == 0 ? 0u : 9u * FPqb) + (unsigned)gy * RS);
 
     const __amdgpu_buffer_rsrc_t rF = brsrc(B.Fin, 9u * NPb);
     const __amdgpu_buffer_rsrc_t rD = brsrc(B.DF, 9u * NPb);
@@ -1538,7 +1549,9 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
                     real df = A * cdf[f] + (a + bb + c) * rhoJ_inv;
                     if (bi >= 0) df = df + B.satb[(long)f * B.nbnd + bi];
                     bstore<real>(rD, o, f * NPb, df);
-                    bstore<real>(rO, o, f * NPb, cv[f] + bdt * df);
+                    const real vo = cv[f] + bdt * df;
+                    bstore<real>(rO, o, f * NPb, vo);
+                    if (fqo >= 0) bstore<real>(rQ, (unsigned)fqo + (unsigned)(B.nr * z) * RS, f * FPqb, vo);
                 }
             }
         };
@@ -1685,7 +1698,9 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
                     real df = A * cdf[f] + DU[f];
                     if (bi >= 0) df = df + B.satb[(long)(3 + f) * B.nbnd + bi];
                     bstore<real>(rD, o, (3 + f) * NPb, df);
-                    bstore<real>(rO, o, (3 + f) * NPb, cs[f] + bdt * df);
+                    const real so = cs[f] + bdt * df;
+                    bstore<real>(rO, o, (3 + f) * NPb, so);
+                    if (fqo >= 0) bstore<real>(rQ, (unsigned)fqo + (unsigned)(B.nr * z) * RS, (3 + f) * FPqb, so);
                 }
             }
         };
@@ -1778,6 +1793,14 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
         else if (gy == 0 || gy == nr - 1) { bbase = nqf + (gy == 0 ? 0 : (nq - 2) * ns) + gx - 1; bstride = nq - 2; }
     }
     const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
+    // q-face buffer: an own column on a q face also writes its stage output
+    // there (byte offset of its node at plane 0; -1 elsewhere).  Compiled
+    // out of the metric-holding variants, which run at the register limit.
+    const unsigned FPqb = (unsigned)(B.nr * B.ns) * RS;
+    const __amdgpu_buffer_rsrc_t rQ = brsrc(B.FQ ? (const void *)B.FQ : (const void *)B.Fin, B.FQ ? 18u * FPqb : 0u);
+    int fqo = -1;
+    if (!HOLD && B.FQ && own && (gx == 0 || gx == B.nq - 1))
+        fqo = (int)((gx == 0 ? 0u : 9u * FPqb) + (unsigned)gy * RS);
 
     const __amdgpu_buffer_rsrc_t rF = brsrc(B.Fin, 9u * NPb);
     const __amdgpu_buffer_rsrc_t rD = brsrc(B.DF, 9u * NPb);
@@ -1991,7 +2014,9 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
                 real df = A * cdf[f] + (a + bb + cc) * rhoJ_inv;
                 if (bi >= 0) df = df + B.satb[(long)f * B.nbnd + bi];
                 bstore_nt<real>(rD, o, f * NPb, df);
-                bstore_nt<real>(rO, o, f * NPb, vq[3][f] + bdt * df);
+                const real vo = vq[3][f] + bdt * df;
+                bstore_nt<real>(rO, o, f * NPb, vo);
+                if (fqo >= 0) bstore<real>(rQ, (unsigned)fqo + (unsigned)(B.nr * c) * RS, f * FPqb, vo);
             }
             // stress rates: Hooke's law on the velocity gradient
             const real sx = CURV ? csm[0] : (real)B.gc[2];
@@ -2023,7 +2048,9 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
                 real df = A * cdf[3 + f] + DU[f];
                 if (bi >= 0) df = df + B.satb[(long)(3 + f) * B.nbnd + bi];
                 bstore_nt<real>(rD, o, (3 + f) * NPb, df);
-                bstore_nt<real>(rO, o, (3 + f) * NPb, sc[f] + bdt * df);
+                const real so = sc[f] + bdt * df;
+                bstore_nt<real>(rO, o, (3 + f) * NPb, so);
+                if (fqo >= 0) bstore<real>(rQ, (unsigned)fqo + (unsigned)(B.nr * c) * RS, (3 + f) * FPqb, so);
             }
         }
     };
@@ -2466,7 +2493,7 @@ __device__ void bc_sat(const double u[9], double rho, double mu, double lambda, 
 // Interface_RHS; adds -tau0/h_q * bc to sat[9]
 template <typename real>
 __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams &fp, int me, int j, int k,
-                          double t, int stage, double A, double bdt, double sat[9], double tau0)
+                          double t, int stage, double A, double bdt, double sat[9], double tau0, int fq_read)
 {
     const Blk<real> &B1 = P.b[0], &B2 = P.b[1];
     const Blk<real> &B = P.b[me];
@@ -2480,7 +2507,15 @@ __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams
     // prove the arrays disjoint and would otherwise wait for each store
     // before the next load
     double u[9], v[9];
-    for (int f = 0; f < 9; f++) { u[f] = (double)B1.Fin[f * B1.NP + pu]; v[f] = (double)B2.Fin[f * B2.NP + pv]; }
+    if (fq_read) {
+        // block 0's face q = nq-1 is the second half of its buffer, block
+        // 1's face q = 0 the first half of its buffer
+        const long FP1 = (long)B1.nr * B1.ns, FP2 = (long)B2.nr * B2.ns;
+        const real *fu = B1.FQ + 9 * FP1 + j + (long)B1.nr * k, *fv = B2.FQ + j + (long)B2.nr * k;
+        for (int f = 0; f < 9; f++) { u[f] = (double)fu[f * FP1]; v[f] = (double)fv[f * FP2]; }
+    } else {
+        for (int f = 0; f < 9; f++) { u[f] = (double)B1.Fin[f * B1.NP + pu]; v[f] = (double)B2.Fin[f * B2.NP + pv]; }
+    }
     const double *fr = B.frame[side];
     double nn[3], mm[3], ll[3];
     for (int c = 0; c < 3; c++) { nn[c] = fr[c * FP + fpi]; mm[c] = fr[(3 + c) * FP + fpi]; ll[c] = fr[(6 + c) * FP + fpi]; }
@@ -2647,6 +2682,7 @@ struct BoundaryArgs {
     void *satb[2];        // [9][nbnd] per block, in the stage's real type
     int stage;
     int rates_only;       // 1: RHS evaluation without the RK update (no F_out / S / W write)
+    int fq_read;          // 1: read the stage input of q-face nodes from the blocks' q-face buffers
     int *flag;
 };
 
@@ -2693,7 +2729,13 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
     const long NP = B.NP;
     const long p = pidx(B.px, nr, i, j, k);
     double u[9];
-    for (int f = 0; f < 9; f++) u[f] = (double)B.Fin[f * NP + p];
+    if (a.fq_read && (i == 0 || i == nq - 1)) {
+        const long FPq = (long)nr * ns;
+        const real *fq = B.FQ + (i == 0 ? 0 : 9 * FPq) + j + (long)nr * k;
+        for (int f = 0; f < 9; f++) u[f] = (double)fq[f * FPq];
+    } else {
+        for (int f = 0; f < 9; f++) u[f] = (double)B.Fin[f * NP + p];
+    }
     double sat[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int pos[3] = {i, j, k};
     const int nn[3] = {nq, nr, ns};
@@ -2741,7 +2783,7 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
     if (on_fault) {
         const double ts = a.tdev ? *a.tdev + a.tinc : a.t;
         int rc = fault_node<real>(P, a.I[me], a.fault, me, j, k, ts, a.stage, A, a.rates_only ? 0.0 : a.bdt,
-                                  sat, a.tau0);
+                                  sat, a.tau0, a.fq_read);
         if (rc) a.flag[0] = rc;
     }
     if (a.mode == 1) {
@@ -2835,6 +2877,10 @@ __global__ void k_plastic(Pair<real> P, PlasticArgs a)
         for (int c = 0; c < 6; c++) sa[c] = sigma * delta[c] + sd[c] * taun / (taun + a.dt * lambda * G);
     }
     for (int c = 0; c < 6; c++) B.Fout[(3 + c) * NP + p] = (real)(sa[c] - s0[c]);
+    if (B.FQ && (i == 0 || i == B.nq - 1)) {
+        real *fq = B.FQ + (i == 0 ? 0L : 9L * B.nr * B.ns) + j + (long)B.nr * k;
+        for (int c = 0; c < 6; c++) fq[(long)(3 + c) * B.nr * B.ns] = (real)(sa[c] - s0[c]);
+    }
 }
 
 // rate-and-state initial state (Interface_Condition.f90:2722-2806)
@@ -2943,6 +2989,20 @@ __global__ void k_unpad(int nf, int nq, int nr, int ns, int px, const real *__re
     const long row = t / nq;
     const int i = (int)(t - row * nq);
     dst[t] = (double)src[row * px + i];
+}
+
+// refresh the q-face buffer of a block from its fields: [2][9][nr ns]
+template <typename real>
+__global__ void k_fq_gather(int nq, int nr, int ns, int px, long NP, const real *__restrict__ F, real *__restrict__ FQ)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long FP = (long)nr * ns;
+    if (t >= 18 * FP) return;
+    const long sf = t / FP, rem = t - sf * FP;
+    const int side = (int)(sf / 9), f = (int)(sf % 9);
+    const int j = (int)(rem % nr), k = (int)(rem / nr);
+    const int i = side == 0 ? 0 : nq - 1;
+    FQ[t] = F[(long)f * NP + i + (long)px * ((long)j + (long)nr * k)];
 }
 
 // the nine fields of the q-face i of a block in FP64, [field][s][r] (the
