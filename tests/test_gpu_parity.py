@@ -165,6 +165,41 @@ def test_rhs_chunk_pairs_vs_oracle(mesh):
                     assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= TOL, (tile, zc, ib)
 
 
+@pytest.mark.parametrize('mesh', ['cartesian', 'curvilinear'])
+def test_rhs_short_chunks_vs_oracle(mesh):
+    """the one- and two-plane s chunks that the library picks for small
+    grids, for the fused variants of both precisions' defaults, against the
+    oracle"""
+    _say(f'[gpu] rhs-short-chunks {mesh}')
+    prob = G.problem('curv_sw' if mesh == 'curvilinear' else 'cart_sw')
+    for b in prob.blocks:
+        b.n = (31, 37, 29)
+    od, vol, _ = H.oracle_rhs(prob, 0.3)
+    for tile in (20, 24):
+        for zc in (1, 2):
+            with _sim(prob, tile=tile, zchunk=zc) as sim:
+                H.load_pattern_state(sim, prob)
+                sim.eval_rates(0.3, what=0)
+                for ib in range(2):
+                    assert G.rel_l2(sim.rates(ib), vol[ib]) <= TOL, (tile, zc, ib)
+                H.load_pattern_state(sim, prob)
+                sim.eval_rates(0.3, what=1, stage=2)
+                for ib in range(2):
+                    assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= TOL, (tile, zc, ib)
+
+
+def test_small_grid_default_chunks():
+    """the library shortens the default s chunks on a small grid (2x31x61x61)
+    and keeps them on the 2x201x101x101 rupture grid"""
+    from wql import workloads
+    for wl, fp32, fp64 in (('plain', 2, 4), ('rupture', 6, 12)):
+        W = workloads.WORKLOADS[wl](scale=1.0)
+        for prec, want in (('fp32', fp32), ('fp64', fp64)):
+            with _sim(W['problem'], precision=prec, coords=W['coords']) as sim:
+                v, zc, _, _ = sim.tile
+                assert zc == want, (wl, prec, v, zc)
+
+
 @pytest.mark.parametrize('case', ['curv_rs', 'pml_locked', 'plastic_sw'])
 def test_graph_replays_match_eager(case):
     """a captured step graph (device-side clock) gives the eager launches'

@@ -885,6 +885,19 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
         g_err = msg;
         return rc;
     }
+    // small grids: shorter s chunks until the main kernel has enough
+    // workgroups to keep every CU busy (1.5 per CU at three waves per SIMD
+    // in FP32, 0.75 per CU at one wave in FP64).  On 2x31x61x61 this picks
+    // 2-plane chunks in FP32 (0.430 -> 0.313 ms per step) and 4-plane chunks
+    // in FP64 (0.671 -> 0.445 ms); the large grids keep the defaults.
+    {
+        int dev = 0, ncu = 0;
+        hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+            ncu = 256;
+        const long want = c->prec == WQL_FP64 ? 3L * ncu / 4 : 3L * ncu / 2;
+        while (c->zchunk > 1 && tile_list(c).total < want) c->zchunk--;
+    }
     if (hipMalloc(&c->d_t, sizeof(double)) != hipSuccess) {
         wql_destroy(c);
         return fail(WQL_E_NOMEM, "hipMalloc(time)");
