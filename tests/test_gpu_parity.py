@@ -188,6 +188,38 @@ def test_rhs_short_chunks_vs_oracle(mesh):
                     assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= TOL, (tile, zc, ib)
 
 
+def test_fp32_constant_metrics_rupture_grid():
+    """the rupture grid (x affine in q, z affine in s, topography in y): in
+    FP32 the library folds q_x, s_x, q_y, s_y, q_z, s_z into constants (their
+    FP64 spread is SBP round-off), so each is constant in the FP32 metrics and
+    within 1e-10 of the metric scale of the FP64 metrics; the FP32 rates of
+    the fused kernel (variant 24, which skips those loads) and of the first
+    tiled kernel (variant 0, which reads every metric) stay within FP32
+    rounding of the oracle's rates"""
+    from wql import workloads
+    _say('[gpu] fp32 constant metrics')
+    W = workloads.WORKLOADS['rupture'](scale=0.3)
+    prob, coords = W['problem'], W['coords']
+    od = O.OracleDomain(prob, coords=coords)
+    H.load_pattern_state(H._NullSim(), prob, od)
+    for ib in range(2):
+        od.set_rates(ib)
+    with _sim(prob, precision='fp64', coords=coords) as sim:
+        G64 = [sim.metrics(ib) for ib in range(2)]
+    for tile in (24, 0):
+        with _sim(prob, precision='fp32', coords=coords, tile=tile) as sim:
+            for ib in range(2):
+                G32 = sim.metrics(ib)
+                scale = np.abs(G64[ib][:9]).max()
+                for c in (0, 2, 3, 5, 6, 8):
+                    assert np.unique(G32[c]).size == 1, (ib, c)
+                    assert np.abs(G32[c] - G64[ib][c]).max() <= 1e-10 * scale + 6e-8 * abs(G64[ib][c]).max(), (ib, c)
+            H.load_pattern_state(sim, prob)
+            sim.eval_rates(0.0, what=0)
+            for ib in range(2):
+                assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= 1e-5, (tile, ib)
+
+
 def test_small_grid_default_chunks():
     """the library shortens the default s chunks on a small grid (2x31x61x61)
     and keeps them on the 2x201x101x101 rupture grid"""

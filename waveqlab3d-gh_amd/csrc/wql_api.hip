@@ -97,6 +97,7 @@ struct DevBlock {
     int bc[6], pml[6], npml;
     double lam, mu, rho;
     double gc[10];
+    unsigned gconst = 0;    // bit c: metric field c of G is constant, value gc[c]
     void *F[2] = {nullptr, nullptr};
     void *DF = nullptr;
     void *G = nullptr;      // [10][NP] real (curvilinear)
@@ -162,6 +163,7 @@ Blk<real> blk_view(const wql_ctx *c, int ib, int cur)
     B.G = (const real *)d.G;
     B.M = (const real *)d.M;
     for (int k = 0; k < 10; k++) B.gc[k] = d.gc[k];
+    B.gconst = d.gconst;
     B.lam = d.lam; B.mu = d.mu; B.rho = d.rho;
     B.cart = d.cart;
     for (int s = 0; s < 6; s++) {
@@ -578,6 +580,53 @@ int check_launch()
     return 0;
 }
 
+// Metric fields that are constant over a curvilinear block.  Grids that are
+// affine along some axes (a planar fault, a flat bottom, topography only on
+// the free surface) have metric fields that are constant, or zero, up to the
+// round-off of the FP64 SBP derivatives (about 1e-14 of the metric scale).
+// A field whose values are all equal is marked constant in either precision.
+// In FP32 a field whose spread is within 1e-10 of the metric scale (the
+// largest |q_x| .. |s_z|; |J| for J) is set to its midpoint: a change far
+// below FP32 rounding (6e-8), written into G so that every kernel sees the
+// same values.  The fused volume kernel skips the loads of constant fields.
+template <typename real>
+int constant_metrics(wql_ctx *c, DevBlock &d, const double *G64)
+{
+    const long ND = d.ND;
+    double *rng = nullptr;
+    if (hipMalloc(&rng, 20 * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(metric range)");
+    double seed[20];
+    for (int k = 0; k < 10; k++) { seed[2 * k] = INFINITY; seed[2 * k + 1] = -INFINITY; }
+    double r[20];
+    hipError_t e = hipMemcpyAsync(rng, seed, sizeof seed, hipMemcpyHostToDevice, c->stream);
+    const unsigned g = (unsigned)std::min<long>(1024, (ND + 255) / 256);
+    if (e == hipSuccess)
+        for (int k = 0; k < 10; k++) k_range<<<g, 256, 0, c->stream>>>(ND, G64 + k * ND, rng + 2 * k);
+    if (e == hipSuccess) e = hipMemcpyAsync(r, rng, sizeof r, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(rng);
+    if (e != hipSuccess) return fail(WQL_E_HIP, "metric range: %s", hipGetErrorString(e));
+    if (int rc = check_launch()) return rc;
+    double scale = 0.0;
+    for (int k = 0; k < 9; k++) scale = std::max(scale, std::max(fabs(r[2 * k]), fabs(r[2 * k + 1])));
+    const double jscale = std::max(fabs(r[18]), fabs(r[19]));
+    d.gconst = 0;
+    for (int k = 0; k < 10; k++) {
+        const double lo = r[2 * k], hi = r[2 * k + 1];
+        if (!(lo <= hi)) continue;  // NaN: leave the field to the kernels
+        if (lo == hi) {
+            d.gconst |= 1u << k;
+            d.gc[k] = lo;
+        } else if (sizeof(real) == 4 && hi - lo <= 1e-10 * (k < 9 ? scale : jscale)) {
+            const real v = (real)(0.5 * (lo + hi));
+            d.gconst |= 1u << k;
+            d.gc[k] = (double)v;
+            k_fill<real><<<(unsigned)((d.NP + 255) / 256), 256, 0, c->stream>>>(d.NP, (real *)d.G + k * d.NP, v);
+        }
+    }
+    return check_launch();
+}
+
 template <typename real>
 int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
 {
@@ -659,6 +708,7 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
         if (!d.cart) {
             if (int rc = dalloc(c, &d.G, 10 * NP * sizeof(real))) return rc;
             k_pad<real><<<(unsigned)((10 * NP + 255) / 256), 256, 0, c->stream>>>(10, d.nq, d.nr, d.ns, d.px, G64, (real *)d.G);
+            if (int rc = constant_metrics<real>(c, d, G64)) return rc;
         } else {
             // metrics of a uniform block: the value at the block centre
             long pc = (long)(d.nq / 2) + (long)d.nq * ((d.nr / 2) + (long)d.nr * (d.ns / 2));

@@ -50,7 +50,8 @@ struct Blk {
     // volume arrays are [field][s][r][q] with q rows padded to px elements
     const real *G;           // [10][NP] metrics + J (curvilinear); nullptr for Cartesian
     const real *M;           // [3][NP] lambda mu rho (heterogeneous); nullptr if homogeneous
-    double gc[10];           // constant metrics + J (Cartesian)
+    double gc[10];           // constant metrics + J (Cartesian; constant fields of G)
+    unsigned gconst;         // bit c: metric field c of G holds gc[c] at every node
     double lam, mu, rho;     // homogeneous material
     int cart;                // 1: constant metrics
     int bc[6];

@@ -1807,7 +1807,27 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
     const __amdgpu_buffer_rsrc_t rO = brsrc(B.Fout, 9u * NPb);
     const __amdgpu_buffer_rsrc_t rG = brsrc(CURV ? (const void *)B.G : (const void *)B.Fin, CURV ? 10u * NPb : 0u);
     const __amdgpu_buffer_rsrc_t rM = brsrc(HET ? (const void *)B.M : (const void *)B.Fin, HET ? 3u * NPb : 0u);
-    auto G = [&](unsigned o, unsigned c) -> real { return CURV ? bload<real>(rG, o, c * NPb) : (real)B.gc[c]; };
+    // metric c of the node at byte offset o.  A field that is constant over
+    // the block (bit gconst) is loaded through a zero-length buffer (0, and no
+    // memory access); K(v, c) then selects its constant.  K is applied where
+    // a value is consumed, not at the prefetch, so that the loads of the next
+    // plane stay in flight under the current plane's work.
+    const __amdgpu_buffer_rsrc_t rG0 = brsrc(B.Fin, 0u);
+    auto G = [&](unsigned o, unsigned c) -> real {
+        if constexpr (CURV)
+            return bload<real>(((B.gconst >> c) & 1u) ? rG0 : rG, o, c * NPb);
+        else
+            return (real)B.gc[c];
+    };
+    auto K = [&](real v, unsigned c) -> real {
+        if constexpr (CURV)
+            return ((B.gconst >> c) & 1u) ? (real)B.gc[c] : v;
+        else
+            return v;
+    };
+    // a field index that differs between lanes (the halo threads' flux
+    // axis): a plain load
+    auto GH = [&](unsigned o, unsigned c) -> real { return CURV ? bload<real>(rG, o, c * NPb) : (real)B.gc[c]; };
 
     // halo node of this thread (x halo: 6*TY nodes, then y halo: 6*TX nodes)
     const int tid = ly * TX + lx;
@@ -1876,7 +1896,7 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
             const unsigned o = hcolb + planeb * (unsigned)c;
 #pragma unroll
             for (int f = 0; f < 6; f++) R.hs[f] = bload<real>(rF, o, (3 + f) * NPb);
-            R.hg[0] = G(o, 9); R.hg[1] = G(o, 0 + hax); R.hg[2] = G(o, 3 + hax); R.hg[3] = G(o, 6 + hax);
+            R.hg[0] = G(o, 9); R.hg[1] = GH(o, 0 + hax); R.hg[2] = GH(o, 3 + hax); R.hg[3] = GH(o, 6 + hax);
 #pragma unroll
             for (int f = 0; f < 3; f++) R.hv[f] = bload<real>(rF, o, f * NPb);
         }
@@ -1910,6 +1930,10 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
             wg[a][0] = G(o, 9); wg[a][1] = G(o, 2); wg[a][2] = G(o, 5); wg[a][3] = G(o, 8);
 #pragma unroll
             for (int f = 0; f < 3; f++) wv[a][f] = bload<real>(rF, o, f * NPb);
+        }
+#pragma unroll
+        for (int a = 0; a < 6; a++) {
+            wg[a][0] = K(wg[a][0], 9); wg[a][1] = K(wg[a][1], 2); wg[a][2] = K(wg[a][2], 5); wg[a][3] = K(wg[a][3], 8);
         }
 #pragma unroll
         for (int a = 0; a < 6; a++) {
@@ -1947,22 +1971,23 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
         }
         real sc[6], Jc = 0;
         if (inb) {
-            fluxv(cur.ls, cur.lg[0], cur.lg[1], cur.lg[2], cur.lg[3], fs[6]);
+            const real lg[4] = {K(cur.lg[0], 9), K(cur.lg[1], 2), K(cur.lg[2], 5), K(cur.lg[3], 8)};
+            fluxv(cur.ls, lg[0], lg[1], lg[2], lg[3], fs[6]);
 #pragma unroll
             for (int f = 0; f < 3; f++) vq[6][f] = cur.lv[f];
             if constexpr (HOLD) {
 #pragma unroll
-                for (int f = 0; f < 3; f++) sm[3][f] = cur.lg[1 + f];
+                for (int f = 0; f < 3; f++) sm[3][f] = lg[1 + f];
 #pragma unroll
                 for (int f = 0; f < 6; f++) hq[3][f] = cur.ls[f];
-                hq[3][6] = cur.lg[0];
+                hq[3][6] = lg[0];
             }
 #pragma unroll
             for (int f = 0; f < 6; f++) sc[f] = HOLD ? hq[0][f] : cur.cs[f];
-            Jc = HOLD ? hq[0][6] : cur.cJ;
+            Jc = HOLD ? hq[0][6] : K(cur.cJ, 9);
             real fq[3], fr[3];
-            fluxv(sc, Jc, cur.gm[0], cur.gm[1], cur.gm[2], fq);
-            fluxv(sc, Jc, cur.gm[3], cur.gm[4], cur.gm[5], fr);
+            fluxv(sc, Jc, K(cur.gm[0], 0), K(cur.gm[1], 3), K(cur.gm[2], 6), fq);
+            fluxv(sc, Jc, K(cur.gm[3], 1), K(cur.gm[4], 4), K(cur.gm[5], 7), fr);
 #pragma unroll
             for (int f = 0; f < 3; f++) {
                 sfq[b][f][ly][lx + 3] = fq[f];
@@ -1975,9 +2000,9 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
 #pragma unroll
             for (int f = 0; f < 9; f++) cdf[f] = cur.df[f];
 #pragma unroll
-            for (int f = 0; f < 6; f++) cgm[f] = cur.gm[f];
+            for (int f = 0; f < 6; f++) cgm[f] = K(cur.gm[f], f < 3 ? 3 * f : 3 * (f - 3) + 1);
 #pragma unroll
-            for (int f = 0; f < 3; f++) csm[f] = HOLD ? sm[0][f] : cur.gs[f];
+            for (int f = 0; f < 3; f++) csm[f] = HOLD ? sm[0][f] : K(cur.gs[f], 3 * f + 2);
             if (HET) {
 #pragma unroll
                 for (int f = 0; f < 3; f++) cmat[f] = cur.mat[f];
@@ -1985,7 +2010,7 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
         }
         if (hk) {
             real fl[3];
-            fluxv(cur.hs, cur.hg[0], cur.hg[1], cur.hg[2], cur.hg[3], fl);
+            fluxv(cur.hs, K(cur.hg[0], 9), cur.hg[1], cur.hg[2], cur.hg[3], fl);
             if (hk == 1) {
 #pragma unroll
                 for (int f = 0; f < 3; f++) { sfq[b][f][hcy][hcx] = fl[f]; sv[b][f][hcy + 3][hcx] = cur.hv[f]; }
@@ -2989,6 +3014,33 @@ __global__ void k_unpad(int nf, int nq, int nr, int ns, int px, const real *__re
     const long row = t / nq;
     const int i = (int)(t - row * nq);
     dst[t] = (double)src[row * px + i];
+}
+
+// min and max of n values (out[0] min, out[1] max; the caller seeds them
+// with +inf and -inf): the spread of a metric field at set-up
+__global__ void __launch_bounds__(256) k_range(long n, const double *__restrict__ a, double *out)
+{
+    __shared__ double smin[256], smax[256];
+    double lo = INFINITY, hi = -INFINITY;
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+        const double v = a[p];
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMin(&out[0], smin[0]);
+        atomicMax(&out[1], smax[0]);
+    }
 }
 
 // refresh the q-face buffer of a block from its fields: [2][9][nr ns]
