@@ -1811,16 +1811,21 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
     // the block (bit gconst) is loaded through a zero-length buffer (0, and no
     // memory access); K(v, c) then selects its constant.  K is applied where
     // a value is consumed, not at the prefetch, so that the loads of the next
-    // plane stay in flight under the current plane's work.
+    // plane stay in flight under the current plane's work.  The metric-holding
+    // variants, at the register limit, load every field (G holds the
+    // constants' values too).
+    constexpr bool KF = CURV && !HOLD;
     const __amdgpu_buffer_rsrc_t rG0 = brsrc(B.Fin, 0u);
     auto G = [&](unsigned o, unsigned c) -> real {
-        if constexpr (CURV)
+        if constexpr (KF)
             return bload<real>(((B.gconst >> c) & 1u) ? rG0 : rG, o, c * NPb);
+        else if constexpr (CURV)
+            return bload<real>(rG, o, c * NPb);
         else
             return (real)B.gc[c];
     };
     auto K = [&](real v, unsigned c) -> real {
-        if constexpr (CURV)
+        if constexpr (KF)
             return ((B.gconst >> c) & 1u) ? (real)B.gc[c] : v;
         else
             return v;
