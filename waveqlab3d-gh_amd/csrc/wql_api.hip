@@ -103,6 +103,7 @@ struct DevBlock {
     void *G = nullptr;      // [10][NP] real (curvilinear)
     void *M = nullptr;      // [3][NP] real (heterogeneous)
     void *Q[6] = {}, *DQ[6] = {};
+    void *pq[6] = {};       // [9][face nodes] real: PML face penalties of the stage
     long SP[6] = {};
     double *frame[6] = {}, *fgm[6] = {}, *fmat[6] = {};
     double *fx = nullptr;
@@ -479,6 +480,12 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
     const bool fq = fq_on<real>(c);
     if (fq)
         for (int ib = 0; ib < 2; ib++) P.b[ib].FQ = (real *)c->b[ib].fq;
+    // PML face penalties: with the fused penalties the boundary kernel writes
+    // them to compact per-face arrays and the PML-slab kernel adds them to
+    // DQ, so no second boundary launch follows the volume kernels
+    if (fused && any_pml)
+        for (int ib = 0; ib < 2; ib++)
+            for (int sd = 0; sd < 6; sd++) P.b[ib].PQ[sd] = (real *)c->b[ib].pq[sd];
     if (fused) {
         a.mode = 1;
         a.satb[0] = c->b[0].satb;
@@ -510,9 +517,6 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
     }
     if (!fused) {
         a.mode = 0;
-        k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
-    } else if (any_pml) {
-        a.mode = 2;
         k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
     }
     if (do_plastic && c->prob.plastic) {
@@ -737,6 +741,8 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
             d.SP[s] = n3[0] * n3[1] * n3[2];
             if (int rc = dalloc(c, &d.Q[s], 9 * d.SP[s] * sizeof(real))) return rc;
             if (int rc = dalloc(c, &d.DQ[s], 9 * d.SP[s] * sizeof(real))) return rc;
+            const long FPs = dir == 0 ? (long)d.nr * d.ns : dir == 1 ? (long)d.nq * d.ns : (long)d.nq * d.nr;
+            if (int rc = dalloc(c, &d.pq[s], 9 * FPs * sizeof(real))) return rc;
             unsigned gq = (unsigned)((9 * d.SP[s] + 255) / 256);
             k_fill<real><<<gq, 256, 0, c->stream>>>(9 * d.SP[s], (real *)d.Q[s], (real)0);
             k_fill<real><<<gq, 256, 0, c->stream>>>(9 * d.SP[s], (real *)d.DQ[s], (real)0);

@@ -58,6 +58,8 @@ struct Blk {
     int pml[6];
     int npml;
     real *Q[6], *DQ[6];      // PML slabs [9][slab nodes]
+    real *PQ[6];             // PML face penalties [9][face nodes] of the stage (written by the
+                             // boundary kernel, added to DQ by the PML-slab kernel); nullptr = unused
     // face data (FP64): per side, nodes in face layout (q: j + nr*k, r: i + nq*k, s: i + nq*j)
     const double *frame[6];  // [9][face] n m l
     const double *fgm[6];    // [3][face] metric components of the face axis (a_x, a_y, a_z)

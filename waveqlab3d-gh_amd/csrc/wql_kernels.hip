@@ -277,9 +277,21 @@ __device__ void volume_node_generic(const Blk<real> &B, int i, int j, int k, rea
             src[0] = Ux[3] / Jc; src[1] = Ux[6] / Jc; src[2] = Ux[7] / Jc;
             src[3] = Ux[0]; src[4] = 0; src[5] = 0; src[6] = Ux[1]; src[7] = Ux[2]; src[8] = 0;
         }
+        // this side's boundary-face penalty (written by the boundary kernel
+        // before this launch), for a node on that face
+        const real *pq = nullptr;
+        if (B.PQ[side] && B.bc[side] > 0 && pos[dir] == (side % 2 == 0 ? 0 : n - 1)) {
+            long fpi;
+            if (dir == 0) fpi = j + (long)B.nr * k;
+            else if (dir == 1) fpi = i + (long)B.nq * k;
+            else fpi = i + (long)B.nq * j;
+            pq = B.PQ[side] + fpi;
+        }
+        const long FPs = dir == 0 ? (long)B.nr * B.ns : dir == 1 ? (long)B.nq * B.ns : (long)B.nq * B.nr;
         for (int f = 0; f < NF; f++) {
             real dq = A * DQ[f * SP + s];
             dq = dq + src[f] - (d + cfs) * q[f];
+            if (pq) dq = dq + pq[f * FPs];
             DQ[f * SP + s] = dq;
             Q[f * SP + s] = q[f] + bdt * dq;
         }
@@ -2704,10 +2716,11 @@ struct BoundaryArgs {
     double t, A[2], bdt, tau0;
     const double *tdev;   // non-null: stage time = *tdev + tinc (step time kept on the device)
     double tinc;
-    // 0: apply every boundary and fault penalty to DF / F_out (after the
-    //    volume kernels); 1: before the volume kernels, write each boundary
-    //    node's non-PML penalty to satb (the volume kernels add it) and run
-    //    the fault; 2: after the volume kernels, only the PML penalties
+    // 0: apply every boundary and fault penalty to DF / F_out and the PML
+    //    faces' penalties to DQ / Q (after the volume kernels); 1: before the
+    //    volume kernels, write each boundary node's penalty to satb and the
+    //    PML faces' penalties to the blocks' PQ arrays (the volume and
+    //    PML-slab kernels add them) and run the fault
     int mode;
     void *satb[2];        // [9][nbnd] per block, in the stage's real type
     int stage;
@@ -2774,7 +2787,6 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
         int dir = side / 2;
         int on = (side % 2 == 0) ? (pos[dir] == 0) : (pos[dir] == nn[dir] - 1);
         if (!on || B.bc[side] <= 0) continue;
-        if (a.mode == 2 && !B.pml[side]) continue;
         long fpi;
         long FP;
         if (dir == 0) { fpi = j + (long)nr * k; FP = (long)nr * ns; }
@@ -2790,9 +2802,11 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
         double ubc[9], Ux[9];
         bc_sat(u, fm[2 * FP + fpi], fm[FP + fpi], fm[fpi], l, m, n, norm, r, side % 2, ubc, Ux);
         double h = 1.0 / (double)(nn[dir] - 1);
-        if (a.mode != 2)
-            for (int f = 0; f < 9; f++) sat[f] = sat[f] - a.tau0 / h * Ux[f];
-        if (B.pml[side] && a.mode != 1) {  // RHS_Interior.f90:1574-1700: SAT on the PML auxiliary rates
+        for (int f = 0; f < 9; f++) sat[f] = sat[f] - a.tau0 / h * Ux[f];
+        if (B.pml[side] && a.mode == 1) {
+            // the PML-slab kernel adds these to DQ (RHS_Interior.f90:1574-1700)
+            for (int f = 0; f < 9; f++) B.PQ[side][f * FP + fpi] = (real)(-(a.tau0 / h) * ubc[f]);
+        } else if (B.pml[side]) {  // RHS_Interior.f90:1574-1700: SAT on the PML auxiliary rates
             int np = B.npml;
             int lo[3] = {0, 0, 0}, hi[3] = {nq - 1, nr - 1, ns - 1};
             if (side % 2 == 0) hi[dir] = np - 1; else lo[dir] = nn[dir] - np;
@@ -2809,7 +2823,6 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
     }
     // fault face (domain.f90:600-609): block 0's face q = nq-1, block 1's face q = 0
     int on_fault = (me == 0) ? (i == nq - 1 && B.bc[1] == 0) : (i == 0 && B.bc[0] == 0);
-    if (a.mode == 2) on_fault = 0;
     if (on_fault) {
         const double ts = a.tdev ? *a.tdev + a.tinc : a.t;
         int rc = fault_node<real>(P, a.I[me], a.fault, me, j, k, ts, a.stage, A, a.rates_only ? 0.0 : a.bdt,
@@ -2822,7 +2835,6 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
         for (int f = 0; f < 9; f++) sb[f * nb + t] = (real)sat[f];
         return;
     }
-    if (a.mode == 2) return;
     for (int f = 0; f < 9; f++) {
         real df = B.DF[f * NP + p] + (real)sat[f];
         B.DF[f * NP + p] = df;
