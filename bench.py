@@ -172,7 +172,9 @@ def serving(n_scen, steps, rate, max_active, n_seq, seed=5):
     arrivals -= arrivals[0]
     # warm the library (first graph capture, code objects) outside the timing
     run_stream(scen[:1], np.zeros(1), max_active=1, precision='fp32')
-    lat, wall, _ = run_stream(scen, arrivals, max_active=max_active, precision='fp32')
+    # admission shortest job first with aging; arrival order beside it
+    lat, wall, _ = run_stream(scen, arrivals, max_active=max_active, precision='fp32', policy='sjf')
+    lat_f, wall_f, _ = run_stream(scen, arrivals, max_active=max_active, precision='fp32', policy='fifo')
     pts = sum(workloads.points(s['problem']) for s in scen) * 5 * steps
     t0 = time.perf_counter()
     for sc in scen[:n_seq]:
@@ -182,13 +184,16 @@ def serving(n_scen, steps, rate, max_active, n_seq, seed=5):
     seq = time.perf_counter() - t0
     seq_rate = n_seq / seq
     return {'scenarios': n_scen, 'steps_each': steps, 'arrival_rate_per_s': rate, 'max_active': max_active,
-            'sizes_per_block': [list(v) for v in workloads.SERVING_SIZES],
+            'sizes_per_block': [list(v) for v in workloads.SERVING_SIZES], 'policy': 'sjf',
             'p50_latency_s': round(float(np.percentile(lat, 50)), 4),
             'p99_latency_s': round(float(np.percentile(lat, 99)), 4),
             'scenarios_per_s': round(n_scen / wall, 4), 'wall_s': round(wall, 3),
             'M_grid_pt_stages_per_s': round(pts / wall / 1e6, 2),
             'sequential_fp64': {'scenarios': n_seq, 'scenarios_per_s': round(seq_rate, 4)},
-            'speedup_vs_sequential_fp64': round((n_scen / wall) / seq_rate, 3)}
+            'speedup_vs_sequential_fp64': round((n_scen / wall) / seq_rate, 3),
+            'fifo': {'p50_latency_s': round(float(np.percentile(lat_f, 50)), 4),
+                     'p99_latency_s': round(float(np.percentile(lat_f, 99)), 4),
+                     'scenarios_per_s': round(n_scen / wall_f, 4)}}
 
 
 def cpu_baseline(W):

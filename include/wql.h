@@ -187,6 +187,10 @@ int wql_server_poll(wql_server *srv, int *done, int max_done);
 int wql_server_result(wql_server *srv, int id, double *S, double *trup, double *psr, double stats[6]);
 /* {queued, active, device bytes in use, budget} */
 int wql_server_status(const wql_server *srv, long out[4]);
+/* admission order of queued scenarios: 0 arrival order (the default),
+ * 1 shortest job first with aging (key: nodes x steps / (1 + seconds
+ * waited)).  No counterpart in the reference. */
+int wql_server_set_policy(wql_server *srv, int policy);
 
 #ifdef __cplusplus
 }

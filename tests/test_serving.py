@@ -35,14 +35,15 @@ def _alone(sc, precision):
         return dict(S=f['S'], trup=f['trup'][0], psr=sim.fault_peak(0)[0], stats=sim.fault_stats(0))
 
 
+@pytest.mark.parametrize('policy', ['fifo', 'sjf'])
 @pytest.mark.parametrize('precision', ['fp32', 'fp64'])
-def test_alone_equals_in_the_mix(precision):
+def test_alone_equals_in_the_mix(precision, policy):
     from wql.serving import run_stream
-    print(f'[gpu] serving mix {precision}', file=sys.stderr, flush=True)
+    print(f'[gpu] serving mix {precision} {policy}', file=sys.stderr, flush=True)
     scen = [_scenario(k) for k in range(6)]
     arrivals = np.array([0.0, 0.0, 0.01, 0.02, 0.02, 0.05])
     lat, wall, res = run_stream(scen, arrivals, max_active=3, steps_per_launch=7, precision=precision,
-                                keep_results=True)
+                                keep_results=True, policy=policy)
     assert len(res) == len(scen)
     assert (lat > 0).all()
     for k, sc in enumerate(scen):
@@ -72,5 +73,24 @@ def test_admission_respects_the_budget():
             peak_active = max(peak_active, st['active'])
             assert st['bytes_used'] <= st['budget']
         assert peak_active == 1
+        for sid in ids:
+            assert srv.result(sid)['stats']['max_slip'] > 0.0
+
+
+def test_shortest_job_first_order():
+    """with one active slot and every scenario queued before the first poll,
+    the shortest-job-first policy runs them in increasing nodes x steps
+    (submitted largest first, so arrival order would be the reverse)"""
+    from wql.serving import Server
+    print('[gpu] serving sjf order', file=sys.stderr, flush=True)
+    scen = [_scenario(k) for k in (2, 1, 0)]
+    work = [2 * np.prod(s['problem'].blocks[0].n) * s['steps'] for s in scen]
+    assert work[0] > 1.4 * work[1] > 1.4 * 1.4 * work[2]
+    with Server(max_active=1, steps_per_launch=5, policy='sjf') as srv:
+        ids = [srv.submit(s['problem'], s['coords'], s['steps'], 'fp32') for s in scen]
+        done = []
+        while len(done) < len(ids):
+            done += srv.poll()
+        assert done == ids[::-1]
         for sid in ids:
             assert srv.result(sid)['stats']['max_slip'] > 0.0

@@ -11,8 +11,13 @@
 // that arrive while others run are admitted as soon as memory and a slot
 // free up, and their work overlaps on the device.  The server never blocks
 // on the device in wql_server_poll except to download a retired scenario's
-// fault arrays.
+// fault arrays.  Queued scenarios are taken in arrival order (policy 0) or
+// shortest job first with aging (policy 1: the key is nodes x steps /
+// (1 + seconds waited), so a large scenario overtakes fresh small ones after
+// waiting as many seconds as its work is a multiple of theirs).
 
+#include <algorithm>
+#include <chrono>
 #include <deque>
 #include <memory>
 #include <unordered_map>
@@ -25,6 +30,8 @@ struct wql_scenario {
     std::vector<double> X[2], M[2];
     int nsteps = 0, steps_done = 0, steps_inflight = 0;
     size_t need = 0;
+    double work = 0.0;      // nodes x steps
+    double t_submit = 0.0;  // host clock at submission (s)
     wql_ctx *ctx = nullptr;
     hipEvent_t ev = nullptr;
     std::vector<double> S, trup, psr;
@@ -38,6 +45,7 @@ struct wql_server {
     size_t budget = 0;
     int chunk = 50;
     int next_id = 1;
+    int policy = 0;  // 0 arrival order, 1 shortest job first with aging
     size_t used = 0;
     std::deque<int> queue;
     std::vector<int> active;
@@ -45,6 +53,11 @@ struct wql_server {
 };
 
 namespace {
+
+double host_seconds()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 void scenario_release(wql_server *s, wql_scenario *sc)
 {
@@ -133,6 +146,13 @@ int wql_server_create(int max_active, size_t hbm_budget, int steps_per_launch, w
     return 0;
 }
 
+int wql_server_set_policy(wql_server *s, int policy)
+{
+    if (!s || policy < 0 || policy > 1) return fail(WQL_E_ARG, "bad server policy");
+    s->policy = policy;
+    return 0;
+}
+
 int wql_server_destroy(wql_server *s)
 {
     if (!s) return 0;
@@ -161,6 +181,9 @@ int wql_server_submit(wql_server *s, const wql_problem *prob, const wql_block_de
         }
     }
     sc->need = wql_estimate_bytes(prob, sc->blocks);
+    sc->work = (double)nsteps * ((double)blocks[0].nq * blocks[0].nr * blocks[0].ns +
+                                 (double)blocks[1].nq * blocks[1].nr * blocks[1].ns);
+    sc->t_submit = host_seconds();
     if (sc->need > s->budget) return fail(WQL_E_NOMEM, "scenario needs %zu bytes, budget %zu", sc->need, s->budget);
     *id = sc->id;
     s->queue.push_back(sc->id);
@@ -197,7 +220,21 @@ int wql_server_poll(wql_server *s, int *done, int max_done)
         }
     }
     s->active.swap(still);
-    // admission: first fit by footprint, in arrival order
+    // admission: first fit by footprint, in arrival order or by the aged
+    // job-size key (stable, so equal keys keep arrival order)
+    if (s->policy == 1 && s->queue.size() > 1 && (int)s->active.size() < s->max_active) {
+        const double now = host_seconds();
+        std::vector<std::pair<double, int>> key;
+        key.reserve(s->queue.size());
+        for (int id : s->queue) {
+            const wql_scenario *q = s->all[id].get();
+            key.emplace_back(q->work / (1.0 + std::max(0.0, now - q->t_submit)), id);
+        }
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<double, int> &x, const std::pair<double, int> &y) { return x.first < y.first; });
+        s->queue.clear();
+        for (const auto &k : key) s->queue.push_back(k.second);
+    }
     for (auto it = s->queue.begin(); it != s->queue.end() && (int)s->active.size() < s->max_active;) {
         wql_scenario *sc = s->all[*it].get();
         if (s->used + sc->need > s->budget) {

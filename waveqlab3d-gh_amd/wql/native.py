@@ -21,7 +21,7 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_set_pml', 'wql_get_pml', 'wql_get_pml_rates', 'wql_get_metrics', 'wql_get_frame',
            'wql_set_fault', 'wql_get_fault', 'wql_get_fault_rates', 'wql_eval_rates', 'wql_step',
            'wql_sync', 'wql_time', 'wql_get_fault_peak', 'wql_fault_stats', 'wql_server_create', 'wql_server_destroy',
-           'wql_server_submit', 'wql_server_poll', 'wql_server_result', 'wql_server_status', 'wql_set_tile', 'wql_get_tile', 'wql_set_tile_order', 'wql_get_tile_order', 'wql_tile_variants', 'wql_use_graph', 'wql_profile',
+           'wql_server_submit', 'wql_server_poll', 'wql_server_result', 'wql_server_status', 'wql_server_set_policy', 'wql_set_tile', 'wql_get_tile', 'wql_set_tile_order', 'wql_get_tile_order', 'wql_tile_variants', 'wql_use_graph', 'wql_profile',
            'wql_volume_kernel_ms', 'wql_get_fault_faces']
 
 
@@ -88,6 +88,7 @@ def lib():
             'wql_server_poll': ([vp, C.POINTER(C.c_int), C.c_int], C.c_int),
             'wql_server_result': ([vp, C.c_int, dp, dp, dp, dp], C.c_int),
             'wql_server_status': ([vp, C.POINTER(C.c_long)], C.c_int),
+            'wql_server_set_policy': ([vp, C.c_int], C.c_int),
             'wql_fault_stats': ([vp, C.c_int, dp], C.c_int),
             'wql_set_tile': ([vp, C.c_int, C.c_int], C.c_int),
             'wql_get_tile': ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),

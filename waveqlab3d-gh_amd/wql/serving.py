@@ -18,9 +18,14 @@ from .solver import block_descs, problem_desc
 
 
 class Server:
-    def __init__(self, max_active=8, hbm_budget=0, steps_per_launch=50):
+    POLICIES = {'fifo': 0, 'sjf': 1}
+
+    def __init__(self, max_active=8, hbm_budget=0, steps_per_launch=50, policy='fifo'):
+        """policy: 'fifo' admits queued scenarios in arrival order, 'sjf'
+        shortest job first with aging (nodes x steps / (1 + seconds waited))"""
         self.srv = C.c_void_p()
         check(native.lib().wql_server_create(max_active, hbm_budget, steps_per_launch, C.byref(self.srv)))
+        check(native.lib().wql_server_set_policy(self.srv, self.POLICIES[policy]))
         self.shape = {}
         self._done = (C.c_int * 4096)()
 
@@ -70,7 +75,7 @@ class Server:
 
 
 def run_stream(scenarios, arrivals, max_active=8, steps_per_launch=50, precision='fp32', keep_results=False,
-               idle_sleep=2e-4):
+               idle_sleep=2e-4, policy='fifo'):
     """Serve `scenarios` (list of dicts with problem, coords, steps) arriving
     at `arrivals` seconds after the start.  Returns per-scenario latency (s),
     the wall time from first arrival to last result, and optionally results."""
@@ -81,7 +86,7 @@ def run_stream(scenarios, arrivals, max_active=8, steps_per_launch=50, precision
     sub_t = {}
     nxt = 0
     ndone = 0
-    with Server(max_active=max_active, steps_per_launch=steps_per_launch) as srv:
+    with Server(max_active=max_active, steps_per_launch=steps_per_launch, policy=policy) as srv:
         while ndone < len(scenarios):
             now = time.perf_counter() - t_start
             while nxt < len(order) and arrivals[order[nxt]] <= now:
