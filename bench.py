@@ -237,8 +237,9 @@ def main():
     ap.add_argument('--serve-seq', type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f'bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported '
-              '(the two-block problem would shard one block per GPU, see DESIGN.md)', file=sys.stderr)
+        print(f'bench.py: the library runs one problem on one GPU; --gpus {a.gpus} is not supported '
+              '(the two-block problem would shard one block per GPU, with the fault faces exchanged '
+              'once per RK stage)', file=sys.stderr)
         sys.exit(2)
     from wql import workloads
     W = workloads.WORKLOADS[a.workload](scale=a.scale)
