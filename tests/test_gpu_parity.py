@@ -386,6 +386,35 @@ def test_plastic_flow_model_vs_oracle(s0_kind, tile):
     assert max(np.abs(od.blocks[ib]['P'][0]).max() for ib in range(2)) > 0.0
 
 
+@pytest.mark.parametrize('tile', [None, 24])
+def test_host_update_between_steps(tile):
+    """fields set from the host between steps reach every kernel, including
+    the q-face buffers that variant 24 keeps between stages: two steps, the
+    fields halved from the host, two more steps, against the oracle doing the
+    same"""
+    _say(f'[gpu] host update between steps tile={tile}')
+    prob = G.problem('cart_sw')
+    args = G.load('cart_sw')['traj_args']
+    F0 = H.pulse_init(prob, args)
+    od = O.OracleDomain(prob)
+    for ib in range(2):
+        od.blocks[ib]['F'][...] = F0[ib]
+    with _sim(prob, tile=tile) as sim:
+        for ib in range(2):
+            sim.set_fields(ib, F0[ib])
+        sim.step(2)
+        od.step(2)
+        for ib in range(2):
+            F1 = 0.5 * od.blocks[ib]['F']
+            od.blocks[ib]['F'][...] = F1
+            sim.set_fields(ib, F1)
+        sim.step(2)
+        sim.sync()
+        od.step(2)
+        for ib in range(2):
+            assert G.rel_l2(sim.fields(ib), od.blocks[ib]['F']) <= TOL, ib
+
+
 @pytest.mark.parametrize('shapes', [((37, 21, 29), (13, 21, 29)), ((65, 22, 31), (38, 22, 31)),
                                     ((13, 13, 13), (33, 13, 13))])
 def test_rhs_block_shapes_vs_oracle(shapes):
