@@ -20,26 +20,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, 'waveqlab3d-gh_amd'))
 
 HBM_PEAK_GBS = 8000.0
-FAMILIES = ['k_volume_main', 'k_volume_pipe', 'k_volume_plane', 'k_volume_buf', 'k_volume_split', 'k_volume_fused',
-            'k_volume_fused']
-NVARIANT = 28
-TILES = [(32, 8), (64, 8), (16, 16), (32, 16)]
+# launch variants of the interior volume kernel (wql_api.hip): family, tile
+VARIANTS = [('fused', 32, 8), ('hold', 32, 8), ('fused', 64, 8), ('hold', 64, 8), ('fused', 32, 16),
+            ('hold', 32, 16), ('split', 32, 8)]
 
 
 def kernel_name(prob, precision, variant):
     """template name of the main volume kernel a launch variant selects"""
-    v = variant % NVARIANT
-    tx, ty = TILES[v % 4]
+    fam, tx, ty = VARIANTS[variant]
     curv = 'true' if any(b.mesh != 'cartesian' for b in prob.blocks) else 'false'
     het = 'true' if any(b.material is not None for b in prob.blocks) else 'false'
     real = 'float' if precision == 'fp32' else 'double'
-    fam = FAMILIES[v // 4]
-    extra = ''
-    if fam == 'k_volume_split':
-        extra = ', 0'  # interior (not s-closure) instantiation
-    elif fam == 'k_volume_fused':
-        extra = ', true' if v < 24 else ', false'  # lead plane held in registers
-    return f'{fam}<{real}, {tx}, {ty}, {curv}, {het}{extra}>'
+    if fam == 'split':
+        return f'k_volume_split<{real}, {tx}, {ty}, {curv}, {het}, 0>'  # interior (not s-closure) instantiation
+    hold = 'true' if fam == 'hold' else 'false'  # lead plane held in registers
+    return f'k_volume_fused<{real}, {tx}, {ty}, {curv}, {het}, {hold}>'
 
 
 def measured_traffic(workload, shape, kname):

@@ -124,7 +124,7 @@ struct wql_ctx {
     // launch shape of the main volume kernel (set per precision in
     // wql_create: the fastest measured on the 2 x 201x101x101 curvilinear
     // problem)
-    int variant = 24, zchunk = 6, order = 0, group = 2;
+    int variant = 0, zchunk = 6, order = 0, group = 2;
     int use_graph = 0;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int *d_flag = nullptr;
@@ -193,16 +193,20 @@ void main_range(const DevBlock &d, int r[6])
     r[4] = std::min(pz0, d.ns); r[5] = std::max(d.ns - pz1, r[4]);
 }
 
-// variants 0-3: k_volume_main (s-march); 4-7: software-pipelined march;
-// 8-11: plane-parallel; 12-15: buffer-addressed pipelined march; 16-19:
-// role-split pipelined march; 20-23: fused single-pass march holding the
-// lead plane; 24-27: fused march re-reading the centre plane.  Within each
-// family: 32x8, 64x8, 16x16, 32x16 tiles.
-const int NVARIANT = 28;
-const int VARIANT_TX[NVARIANT] = {32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32, 32, 64,
-                                  16, 32, 32, 64, 16, 32, 32, 64, 16, 32, 32, 64, 16, 32};
-const int VARIANT_TY[NVARIANT] = {8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16, 8, 8,
-                                  16, 16, 8, 8, 16, 16, 8, 8, 16, 16, 8, 8, 16, 16};
+// Launch variants of the interior volume kernel: the fused single-pass march
+// re-reading the centre plane's stresses and J (FP32 default) or holding the
+// lead plane in registers until it is the centre (FP64 default), on 32x8,
+// 64x8 and 32x16 tiles, and the role-split march (two workgroups per tile,
+// one per rate group) on 32x8 tiles.
+enum { FAM_FUSED = 0, FAM_HOLD = 1, FAM_SPLIT = 2 };
+const int NVARIANT = 7;
+const int VARIANT_FAM[NVARIANT] = {FAM_FUSED, FAM_HOLD, FAM_FUSED, FAM_HOLD, FAM_FUSED, FAM_HOLD, FAM_SPLIT};
+const int VARIANT_TX[NVARIANT] = {32, 32, 64, 64, 32, 32, 32};
+const int VARIANT_TY[NVARIANT] = {8, 8, 8, 8, 16, 16, 8};
+
+// the volume kernels address every array of a block with 32-bit buffer
+// offsets: the largest (metrics + J, 10 fields) must span < 2 GiB
+bool fits_buffers(long NP, size_t elem) { return (double)NP * 10.0 * (double)elem < 2147483647.0; }
 
 TileList tile_list(const wql_ctx *c)
 {
@@ -219,7 +223,7 @@ TileList tile_list(const wql_ctx *c)
         g.xa0 = r[0]; g.xa1 = r[1]; g.ya0 = r[2]; g.ya1 = r[3]; g.za0 = r[4]; g.za1 = r[5];
         g.ntx = (r[1] - r[0] + TX - 1) / TX;
         g.nty = (r[3] - r[2] + TY - 1) / TY;
-        g.zc = (c->variant >= 8 && c->variant < 12) ? 1 : std::max(1, c->zchunk);
+        g.zc = std::max(1, c->zchunk);
         g.nzc = (r[5] - r[4] + g.zc - 1) / g.zc;
         g.first = first;
         first += (long)g.ntx * g.nty * g.nzc;
@@ -303,72 +307,35 @@ void launch_main_v(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0,
 {
     dim3 blk(TX, TY);
     dim3 grd((unsigned)(((TL.total + 7) / 8) * 8));
-    bool curv = !c->b[0].cart || !c->b[1].cart;
-    bool het = c->b[0].M || c->b[1].M;
-    // buffer addressing needs 32-bit byte offsets: every array under 2 GiB
-    bool buf_ok = true;
-    for (int ib = 0; ib < 2; ib++)
-        if ((double)c->b[ib].NP * 10.0 * sizeof(real) >= 2147483647.0) buf_ok = false;
-    if (c->variant >= 20 && buf_ok) {
-        // fused single-pass march; variants 20-23 hold the lead plane's
-        // stresses and J until they are the centre plane, 24-27 re-read them
-        if (c->variant < 24) {
-            if (curv && het) k_volume_fused<real, TX, TY, true, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-            else if (curv) k_volume_fused<real, TX, TY, true, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-            else if (het) k_volume_fused<real, TX, TY, false, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-            else k_volume_fused<real, TX, TY, false, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        } else {
-            if (curv && het) k_volume_fused<real, TX, TY, true, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-            else if (curv) k_volume_fused<real, TX, TY, true, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-            else if (het) k_volume_fused<real, TX, TY, false, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-            else k_volume_fused<real, TX, TY, false, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        }
-        return;
-    }
-    if (c->variant >= 16 && buf_ok) {
+    const bool curv = !c->b[0].cart || !c->b[1].cart;
+    const bool het = c->b[0].M || c->b[1].M;
+    const int fam = VARIANT_FAM[c->variant];
+    if (fam == FAM_HOLD) {
+        if (curv && het) k_volume_fused<real, TX, TY, true, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (curv) k_volume_fused<real, TX, TY, true, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (het) k_volume_fused<real, TX, TY, false, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else k_volume_fused<real, TX, TY, false, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+    } else if (fam == FAM_FUSED) {
+        if (curv && het) k_volume_fused<real, TX, TY, true, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (curv) k_volume_fused<real, TX, TY, true, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else if (het) k_volume_fused<real, TX, TY, false, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+        else k_volume_fused<real, TX, TY, false, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
+    } else {
         dim3 grd2((unsigned)(((2 * TL.total + 7) / 8) * 8));
         if (curv && het) k_volume_split<real, TX, TY, true, true, 0><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
         else if (curv) k_volume_split<real, TX, TY, true, false, 0><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
         else if (het) k_volume_split<real, TX, TY, false, true, 0><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
         else k_volume_split<real, TX, TY, false, false, 0><<<grd2, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        return;
     }
-    if (c->variant >= 12 && buf_ok) {
-        if (curv && het) k_volume_buf<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else if (curv) k_volume_buf<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else if (het) k_volume_buf<real, TX, TY, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else k_volume_buf<real, TX, TY, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        return;
-    }
-    if (c->variant >= 8) {
-        if (curv && het) k_volume_plane<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else if (curv) k_volume_plane<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else if (het) k_volume_plane<real, TX, TY, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else k_volume_plane<real, TX, TY, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        return;
-    }
-    if (c->variant >= 4 && (c->variant < 8 || c->variant >= 12)) {
-        if (curv && het) k_volume_pipe<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else if (curv) k_volume_pipe<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else if (het) k_volume_pipe<real, TX, TY, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        else k_volume_pipe<real, TX, TY, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-        return;
-    }
-    if (curv && het) k_volume_main<real, TX, TY, true, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-    else if (curv) k_volume_main<real, TX, TY, true, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-    else if (het) k_volume_main<real, TX, TY, false, true><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
-    else k_volume_main<real, TX, TY, false, false><<<grd, blk, 0, c->stream>>>(P, TL, A0, A1, bdt);
 }
 
 template <typename real>
 void launch_main(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0, real A1, real bdt)
 {
-    switch (c->variant % 4) {
-    case 1: launch_main_v<real, 64, 8>(c, P, TL, A0, A1, bdt); break;
-    case 2: launch_main_v<real, 16, 16>(c, P, TL, A0, A1, bdt); break;
-    case 3: launch_main_v<real, 32, 16>(c, P, TL, A0, A1, bdt); break;
-    default: launch_main_v<real, 32, 8>(c, P, TL, A0, A1, bdt); break;
-    }
+    const int tx = VARIANT_TX[c->variant], ty = VARIANT_TY[c->variant];
+    if (tx == 64) launch_main_v<real, 64, 8>(c, P, TL, A0, A1, bdt);
+    else if (ty == 16) launch_main_v<real, 32, 16>(c, P, TL, A0, A1, bdt);
+    else launch_main_v<real, 32, 8>(c, P, TL, A0, A1, bdt);
 }
 
 template <typename real>
@@ -376,23 +343,9 @@ void launch_band(const wql_ctx *c, Pair<real> &P, real A, real bdt)
 {
     // the six closure planes at each s face without PML: role-split march in
     // closure mode, one 6-plane chunk per face and tile (32 x 8 tiles)
-    bool curv = !c->b[0].cart || !c->b[1].cart;
-    bool het = c->b[0].M || c->b[1].M;
+    const bool curv = !c->b[0].cart || !c->b[1].cart;
+    const bool het = c->b[0].M || c->b[1].M;
     dim3 blk(32, 8);
-    bool buf_ok = true;
-    for (int ib = 0; ib < 2; ib++)
-        if ((double)c->b[ib].NP * 10.0 * sizeof(real) >= 2147483647.0) buf_ok = false;
-    if (!buf_ok) {
-        // arrays beyond 32-bit buffer offsets: plane-parallel kernel per plane
-        TileList TP = band_list(c, 1);
-        if (TP.total <= 0) return;
-        dim3 grd((unsigned)(((TP.total + 7) / 8) * 8));
-        if (curv && het) k_volume_plane<real, 32, 8, true, true><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
-        else if (curv) k_volume_plane<real, 32, 8, true, false><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
-        else if (het) k_volume_plane<real, 32, 8, false, true><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
-        else k_volume_plane<real, 32, 8, false, false><<<grd, blk, 0, c->stream>>>(P, TP, A, A, bdt);
-        return;
-    }
     TileList TB = band_list(c, 0);
     if (TB.total <= 0) return;
     dim3 grd((unsigned)(((2 * TB.total + 7) / 8) * 8));
@@ -402,31 +355,28 @@ void launch_band(const wql_ctx *c, Pair<real> &P, real A, real bdt)
     else k_volume_split<real, 32, 8, false, false, 1><<<grd, blk, 0, c->stream>>>(P, TB, A, A, bdt);
 }
 
-// the boundary penalties are folded into the volume kernels when every
-// kernel of the stage adds them: the role-split march (interior), its
-// s-closure mode and the PML-slab kernel
+// the boundary penalties are folded into the volume kernels: every kernel
+// of the stage (the interior march, its s-closure mode and the PML-slab
+// kernel) adds a boundary node's penalty when it forms DF
 template <typename real>
 bool fused_faces(const wql_ctx *c)
 {
-    if (c->variant < 16) return false;
-    for (int ib = 0; ib < 2; ib++) {
+    for (int ib = 0; ib < 2; ib++)
         if (!c->b[ib].satb) return false;
-        if ((double)c->b[ib].NP * 10.0 * sizeof(real) >= 2147483647.0) return false;
-    }
     return true;
 }
 
 // the q-face buffers are in use with the fused penalties and the fused
-// volume variants that re-read the centre plane's metrics (24-27); every
+// volume variants that re-read the centre plane's metrics; every
 // kernel that writes the stage output of a q-face node then also writes the
-// buffer.  The metric-holding variants (20-23, the FP64 default) run at the
+// buffer.  The metric-holding variants (the FP64 default) run at the
 // register limit: there the buffer's offset and the predicated stores cost
 // the volume kernel more (14 us a stage on the 2x201x101x101 rupture) than
 // the coalesced reads save the boundary kernel.
 template <typename real>
 bool fq_on(const wql_ctx *c)
 {
-    return c->variant >= 24 && fused_faces<real>(c);
+    return VARIANT_FAM[c->variant] == FAM_FUSED && fused_faces<real>(c);
 }
 
 // refresh the q-face buffers from the current fields (after host updates)
@@ -652,6 +602,14 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
         HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_sbw32), w32, sizeof w32));
         table_done = true;
     }
+    // A pair is curvilinear when either block is, and heterogeneous when
+    // either block carries M: the volume kernels are instantiated once for
+    // the pair, so a Cartesian block beside a curvilinear one keeps its
+    // (uniform) metrics in G like a curvilinear block, and a homogeneous
+    // block beside a heterogeneous one stores its constant lambda, mu, rho
+    // in M.
+    const bool pair_curv = !blocks[0].cartesian || !blocks[1].cartesian;
+    const bool pair_het = blocks[0].M || blocks[1].M;
     int *d_err = nullptr;
     if (int rc = dalloc(c, (void **)&d_err, sizeof(int))) return rc;
     HIPCHK(hipMemset(d_err, 0, sizeof(int)));
@@ -665,7 +623,7 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
         d.px = (bd.nq + lane_elems - 1) / lane_elems * lane_elems;
         d.NP = (long)d.px * bd.nr * bd.ns;
         d.ND = (long)bd.nq * bd.nr * bd.ns;
-        d.cart = bd.cartesian ? 1 : 0;
+        d.cart = pair_curv ? 0 : 1;
         for (int s = 0; s < 6; s++) { d.bc[s] = bd.bc[s]; d.pml[s] = bd.pml[s] ? 1 : 0; }
         d.npml = bd.npml;
         const long NP = d.NP, ND = d.ND;
@@ -720,7 +678,7 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
             for (int k = 0; k < 10; k++)
                 HIPCHK(hipMemcpy(&d.gc[k], G64 + k * ND + pc, sizeof(double), hipMemcpyDeviceToHost));
         }
-        if (bd.M) {
+        if (pair_het) {
             if (int rc = dalloc(c, &d.M, 3 * NP * sizeof(real))) return rc;
             k_pad<real><<<(unsigned)((3 * NP + 255) / 256), 256, 0, c->stream>>>(3, d.nq, d.nr, d.ns, d.px, M64, (real *)d.M);
         }
@@ -874,7 +832,7 @@ extern "C" {
 
 const char *wql_last_error(void) { return g_err.c_str(); }
 int wql_abi_version(void) { return WQL_ABI_VERSION; }
-int wql_tile_variants(void) { return 2 * NVARIANT; }
+int wql_tile_variants(void) { return NVARIANT; }
 
 size_t wql_estimate_bytes(const wql_problem *prob, const wql_block_desc blocks[2])
 {
@@ -885,8 +843,8 @@ size_t wql_estimate_bytes(const wql_problem *prob, const wql_block_desc blocks[2
         size_t NP = (b.nq + lane - 1) / lane * lane * (size_t)b.nr * b.ns;  // padded q rows
         size_t ND = (size_t)b.nq * b.nr * b.ns;                // dense
         size_t v = 27 * NP * r;                                // F x2, DF
-        if (!b.cartesian) v += 10 * NP * r;                    // metrics + J
-        if (b.M) v += 3 * NP * r;
+        if (!blocks[0].cartesian || !blocks[1].cartesian) v += 10 * NP * r;  // metrics + J (pair curvilinear)
+        if (blocks[0].M || blocks[1].M) v += 3 * NP * r;       // lambda mu rho (pair heterogeneous)
         if (prob && prob->plastic) v += 3 * NP * 8;            // strain accumulators, y
         size_t init = 16 * ND * 8;                             // dense X, M64, G64 during wql_create
         for (int s = 0; s < 6; s++)
@@ -916,13 +874,22 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
         return fail(WQL_E_ARG, "the fault joins block 0 side Rx and block 1 side Lx (bc code 0)");
     if (prob->fault.law < 0 || prob->fault.law > 2 || prob->fault.npatch < 0 || prob->fault.npatch > 4)
         return fail(WQL_E_ARG, "bad fault description");
+    for (int ib = 0; ib < 2; ib++) {
+        const size_t elem = prob->precision == WQL_FP32 ? 4 : 8;
+        const long lane = 128 / (long)elem;
+        const long NP = (blocks[ib].nq + lane - 1) / lane * lane * (long)blocks[ib].nr * blocks[ib].ns;
+        if (!fits_buffers(NP, elem))
+            return fail(WQL_E_ARG, "block %d (%d x %d x %d) exceeds the 32-bit buffer offsets of the volume kernels "
+                        "(10 fields x padded nodes x %zu bytes must stay below 2 GiB)",
+                        ib, blocks[ib].nq, blocks[ib].nr, blocks[ib].ns, elem);
+    }
     wql_ctx *c = new wql_ctx();
     c->prec = prob->precision;
     // fused single-pass march on 32x8 tiles (measured fastest on MI355X for
     // both precisions): FP32 re-reads the centre plane at three waves per
     // SIMD with 6-plane s chunks; FP64 holds the lead plane in registers,
     // 12-plane chunks
-    c->variant = prob->precision == WQL_FP64 ? 20 : 24;
+    c->variant = prob->precision == WQL_FP64 ? 1 : 0;
     c->zchunk = prob->precision == WQL_FP64 ? 12 : 6;
     c->prob = *prob;
     c->dt = prob->dt;
@@ -1270,10 +1237,8 @@ int wql_sync(wql_ctx *c)
 
 int wql_set_tile(wql_ctx *c, int variant, int zchunk)
 {
-    if (!c || variant < 0 || variant >= 2 * NVARIANT || zchunk < 1) return fail(WQL_E_ARG, "bad tile variant");
+    if (!c || variant < 0 || variant >= NVARIANT || zchunk < 1) return fail(WQL_E_ARG, "bad tile variant");
     drop_graphs(c);
-    c->order = variant >= NVARIANT ? 1 : 0;
-    variant %= NVARIANT;
     c->variant = variant;
     c->zchunk = zchunk;
     return 0;
@@ -1282,7 +1247,7 @@ int wql_set_tile(wql_ctx *c, int variant, int zchunk)
 int wql_get_tile(const wql_ctx *c, int *variant, int *zchunk)
 {
     if (!c || !variant || !zchunk) return fail(WQL_E_ARG, "null argument");
-    *variant = c->variant + (c->order == 1 ? NVARIANT : 0);
+    *variant = c->variant;
     *zchunk = c->zchunk;
     return 0;
 }

@@ -1,16 +1,18 @@
 // wql_kernels.hip -- gfx950 kernels of the WaveQLab3D time-stepping path.
 //
 // Per RK stage (time_step.f90:109-147) the host enqueues:
-//   k_volume_main    interior + x/y boundary rows, LDS-tiled x/y stencils with
-//                    a z-marching register queue; fused DF = A*DF + L(F),
-//                    F_out = F + B*dt*DF                     (HBM-bound)
-//   k_volume_generic the z boundary planes and PML slabs, per node in the
-//                    reference's boundary-band form (RHS_Interior.f90:221-268)
-//                    with the PML auxiliary update (1782-2201)
 //   k_boundary       every outer-face node of both blocks: characteristic SAT
-//                    terms (1516-1703) and the fault SAT + friction solve
-//                    (1706-1779, CouplingForcing.f90, Interface_Condition.f90),
-//                    then the final F_out / PML Q / slip / state update
+//                    penalties (1516-1703) into a compact per-node array, and
+//                    the fault SAT + friction solve (1706-1779,
+//                    CouplingForcing.f90, Interface_Condition.f90)
+//   k_volume_fused   interior + q/r closure rows: LDS-tiled q/r stencils with
+//                    an s-marching register queue; fused DF = A*DF + L(F) +
+//                    penalty, F_out = F + B*dt*DF               (HBM-bound)
+//   k_volume_split   the six s-closure planes at each s face (role-split
+//                    march in closure mode)
+//   k_volume_generic the PML slabs, per node in the reference's boundary-band
+//                    form (RHS_Interior.f90:221-268) with the PML auxiliary
+//                    update (1782-2201)
 //   k_plastic        last stage only: viscoplastic return map (plastic.f90)
 #include "wql_device.h"
 
@@ -436,478 +438,12 @@ __device__ __forceinline__ void tile_range(int t, int nt, int a0, int a1, int nf
     }
 }
 
-template <typename real>
-__device__ __forceinline__ real d1_row(const real *row, int lx, int gx, int n, real ih)
-{
-    // row: LDS row, lx: local index of the node in row, gx: global index
-    if (gx >= 6 && gx <= n - 7) {
-        return ih * ((real)0.75 * (row[lx + 1] - row[lx - 1]) + (real)(-0.15) * (row[lx + 2] - row[lx - 2]) +
-                     (real)(1.0 / 60.0) * (row[lx + 3] - row[lx - 3]));
-    }
-    const SbpRow &R = (gx < 6) ? c_sbp.left[gx] : c_sbp.right[gx - (n - 6)];
-    real s = 0;
-    for (int t = 0; t < R.n; t++) s = s + (real)R.c[t] * row[lx + R.off[t]];
-    return ih * s;
-}
-
-template <typename real>
-__device__ __forceinline__ real d1_col(const real *col, int stride, int ly, int gy, int n, real ih)
-{
-    if (gy >= 6 && gy <= n - 7) {
-        return ih * ((real)0.75 * (col[(ly + 1) * stride] - col[(ly - 1) * stride]) +
-                     (real)(-0.15) * (col[(ly + 2) * stride] - col[(ly - 2) * stride]) +
-                     (real)(1.0 / 60.0) * (col[(ly + 3) * stride] - col[(ly - 3) * stride]));
-    }
-    const SbpRow &R = (gy < 6) ? c_sbp.left[gy] : c_sbp.right[gy - (n - 6)];
-    real s = 0;
-    for (int t = 0; t < R.n; t++) s = s + (real)R.c[t] * col[(ly + R.off[t]) * stride];
-    return ih * s;
-}
-
-template <typename real, bool CURV>
-__device__ __forceinline__ void load_node(const Blk<real> &B, long p, real v[3], real sg[6])
-{
-    const long NP = B.NP;
-#pragma unroll
-    for (int f = 0; f < 3; f++) v[f] = B.Fin[f * NP + p];
-#pragma unroll
-    for (int f = 0; f < 6; f++) sg[f] = B.Fin[(3 + f) * NP + p];
-}
-
-// flux along axis a at node p: J (a_x s_x. + a_y s_y. + a_z s_z.)
-template <typename real, bool CURV>
-__device__ __forceinline__ void flux(const Blk<real> &B, long p, int a, const real sg[6], real fl[3])
-{
-    real J, ax, ay, az;
-    if (CURV) {
-        const long NP = B.NP;
-        J = B.G[9 * NP + p];
-        ax = B.G[(0 + a) * NP + p];
-        ay = B.G[(3 + a) * NP + p];
-        az = B.G[(6 + a) * NP + p];
-    } else {
-        J = (real)B.gc[9]; ax = (real)B.gc[0 + a]; ay = (real)B.gc[3 + a]; az = (real)B.gc[6 + a];
-    }
-    real Jx = J * ax, Jy = J * ay, Jz = J * az;
-    // s_x. = (sxx, sxy, sxz), s_y. = (sxy, syy, syz), s_z. = (sxz, syz, szz)
-    fl[0] = Jx * sg[0] + Jy * sg[3] + Jz * sg[4];
-    fl[1] = Jx * sg[3] + Jy * sg[1] + Jz * sg[5];
-    fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
-}
-
-template <typename real, int TX, int TY, bool CURV, bool HET>
-__global__ void __launch_bounds__(TX * TY) k_volume_main(Pair<real> P, TileList TL, real A0, real A1, real bdt)
-{
-    constexpr int WX = TX + 6, WY = TY + 6;
-    __shared__ real sv[3][WY][WX];   // velocities on the cross-shaped halo tile
-    __shared__ real sfq[3][TY][WX];  // q-fluxes on x-extended rows
-    __shared__ real sfr[3][WY][TX];  // r-fluxes on y-extended columns
-
-    // XCD-aware ordering: hardware deals consecutive workgroups round-robin to
-    // the 8 XCDs; give each XCD a contiguous range of logical tiles so that
-    // neighbouring tiles (which share halo rows) share an L2.
-    const long nwg = (long)gridDim.x;
-    const long per = (nwg + 7) / 8;
-    const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
-    if (logical >= TL.total) return;
-    int gi = 0;
-    for (int k = 1; k < TL.n; k++)
-        if (logical >= TL.g[k].first) gi = k;
-    const TileGeom &T = TL.g[gi];
-    const Blk<real> &B = P.b[T.blk];
-    const real A = T.blk == 0 ? A0 : A1;
-    long lid = logical - T.first;
-    int tx_i, ty_i, zc_i;
-    tile_coords(TL, T, lid, T.nzc, tx_i, ty_i, zc_i);
-
-    int xs, xc0, xc1, ys, yc0, yc1;
-    tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
-    tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
-    const int z0 = T.za0 + zc_i * T.zc;
-    const int z1 = min(z0 + T.zc, T.za1);
-
-    const int lx = threadIdx.x, ly = threadIdx.y;
-    const int gx = xs + lx, gy = ys + ly;
-    const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
-    const bool inb = (gx < B.nq && gy < B.nr);
-    const long NP = B.NP;
-    const long plane = (long)B.px * B.nr;
-    const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
-
-    // s-direction register queue: planes z-3 .. z+3 of this column
-    real qv[7][3], qf[7][3];
-#pragma unroll
-    for (int a = 0; a < 7; a++)
-#pragma unroll
-        for (int f = 0; f < 3; f++) { qv[a][f] = 0; qf[a][f] = 0; }
-
-    auto push = [&](int z) {
-#pragma unroll
-        for (int a = 0; a < 6; a++)
-#pragma unroll
-            for (int f = 0; f < 3; f++) { qv[a][f] = qv[a + 1][f]; qf[a][f] = qf[a + 1][f]; }
-        if (inb) {
-            long p = (long)gx + (long)B.px * gy + plane * z;
-            real v[3], sg[6], fl[3];
-            load_node<real, CURV>(B, p, v, sg);
-            flux<real, CURV>(B, p, 2, sg, fl);
-#pragma unroll
-            for (int f = 0; f < 3; f++) { qv[6][f] = v[f]; qf[6][f] = fl[f]; }
-        }
-    };
-
-    for (int z = z0 - 3; z < z0 + 3; z++) push(z);
-
-    const int nthr = TX * TY;
-    const int tid = ly * TX + lx;
-    for (int z = z0; z < z1; z++) {
-        push(z + 3);
-        __syncthreads();
-        // fill LDS for plane z: x-extended rows (center + x halo)
-        for (int c = tid; c < WX * TY; c += nthr) {
-            int cx = c % WX, cy = c / WX;
-            int ggx = xs - 3 + cx, ggy = ys + cy;
-            if (ggx < 0 || ggx >= B.nq || ggy >= B.nr) continue;
-            long p = (long)ggx + (long)B.px * ggy + plane * z;
-            real v[3], sg[6], fl[3];
-            load_node<real, CURV>(B, p, v, sg);
-            flux<real, CURV>(B, p, 0, sg, fl);
-#pragma unroll
-            for (int f = 0; f < 3; f++) { sv[f][cy + 3][cx] = v[f]; sfq[f][cy][cx] = fl[f]; }
-            if (cx >= 3 && cx < TX + 3) {
-                flux<real, CURV>(B, p, 1, sg, fl);
-#pragma unroll
-                for (int f = 0; f < 3; f++) sfr[f][cy + 3][cx - 3] = fl[f];
-            }
-        }
-        // y halo rows
-        for (int c = tid; c < TX * 6; c += nthr) {
-            int cx = c % TX, hy = c / TX;
-            int cy = hy < 3 ? hy : TY + hy;  // 0,1,2 and TY+3..TY+5
-            int ggx = xs + cx, ggy = ys - 3 + cy;
-            if (ggx >= B.nq || ggy < 0 || ggy >= B.nr) continue;
-            long p = (long)ggx + (long)B.px * ggy + plane * z;
-            real v[3], sg[6], fl[3];
-            load_node<real, CURV>(B, p, v, sg);
-            flux<real, CURV>(B, p, 1, sg, fl);
-#pragma unroll
-            for (int f = 0; f < 3; f++) { sv[f][cy][cx + 3] = v[f]; sfr[f][cy][cx] = fl[f]; }
-        }
-        __syncthreads();
-        if (!own) continue;
-        const long p = (long)gx + (long)B.px * gy + plane * z;
-        // derivatives along q and r from LDS, along s from the queue
-        real dvq[3], dvr[3], dvs[3], div[3];
-#pragma unroll
-        for (int f = 0; f < 3; f++) {
-            dvq[f] = d1_row<real>(&sv[f][ly + 3][0], lx + 3, gx, B.nq, ihq);
-            dvr[f] = d1_col<real>(&sv[f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
-            dvs[f] = ihs * ((real)0.75 * (qv[4][f] - qv[2][f]) + (real)(-0.15) * (qv[5][f] - qv[1][f]) +
-                            (real)(1.0 / 60.0) * (qv[6][f] - qv[0][f]));
-            real a = d1_row<real>(&sfq[f][ly][0], lx + 3, gx, B.nq, ihq);
-            real b = d1_col<real>(&sfr[f][0][lx], TX, ly + 3, gy, B.nr, ihr);
-            real c = ihs * ((real)0.75 * (qf[4][f] - qf[2][f]) + (real)(-0.15) * (qf[5][f] - qf[1][f]) +
-                            (real)(1.0 / 60.0) * (qf[6][f] - qf[0][f]));
-            div[f] = a + b + c;
-        }
-        real g[10];
-        if (CURV) {
-#pragma unroll
-            for (int c = 0; c < 10; c++) g[c] = B.G[c * NP + p];
-        } else {
-#pragma unroll
-            for (int c = 0; c < 10; c++) g[c] = (real)B.gc[c];
-        }
-        real lam, mu, rho;
-        if (HET) { lam = B.M[p]; mu = B.M[NP + p]; rho = B.M[2 * NP + p]; }
-        else { lam = (real)B.lam; mu = (real)B.mu; rho = (real)B.rho; }
-        // physical velocity gradients (chain rule with the centre metrics)
-        real Ux[3], Uy[3], Uz[3];
-#pragma unroll
-        for (int f = 0; f < 3; f++) {
-            Ux[f] = g[0] * dvq[f] + g[1] * dvr[f] + g[2] * dvs[f];
-            Uy[f] = g[3] * dvq[f] + g[4] * dvr[f] + g[5] * dvs[f];
-            Uz[f] = g[6] * dvq[f] + g[7] * dvr[f] + g[8] * dvs[f];
-        }
-        real rhoJ_inv = (real)1 / (rho * g[9]);
-        real l2m = lam + (real)2 * mu;
-        real DU[NF];
-        DU[0] = div[0] * rhoJ_inv;
-        DU[1] = div[1] * rhoJ_inv;
-        DU[2] = div[2] * rhoJ_inv;
-        DU[3] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
-        DU[4] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
-        DU[5] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
-        DU[6] = mu * (Uy[0] + Ux[1]);
-        DU[7] = mu * (Uz[0] + Ux[2]);
-        DU[8] = mu * (Uz[1] + Uy[2]);
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-            real df = A * B.DF[f * NP + p] + DU[f];
-            B.DF[f * NP + p] = df;
-            real fin = (f < 3) ? sv[f][ly + 3][lx + 3] : B.Fin[f * NP + p];
-            B.Fout[f * NP + p] = fin + bdt * df;
-        }
-    }
-}
-
-// Software-pipelined variant of the main kernel.  Each thread owns its
-// centre node and at most one halo node of the plane, so the loads of plane
-// z+1 (centre, halo, s-queue lead z+4, DF) are all issued before the
-// arithmetic of plane z and are in flight while it runs.  LDS is double
-// buffered: one barrier per plane.
-template <typename real, bool CURV>
-struct PlanePrefetch {
-    real v[3], s[6], g[10];   // centre node: velocity, stress, metrics + J
-    real hv[3], hs[6], hg[4]; // halo node: velocity, stress, J + axis metrics
-    real lv[3], ls[6], lg[4]; // s-queue lead (plane z+3): velocity, stress, J + s metrics
-    real df[9];               // centre rates
-};
-
-template <typename real, int TX, int TY, bool CURV, bool HET>
-__global__ void __launch_bounds__(TX * TY) k_volume_pipe(Pair<real> P, TileList TL, real A0, real A1, real bdt)
-{
-    constexpr int WX = TX + 6, WY = TY + 6;
-    __shared__ real sv[2][3][WY][WX];
-    __shared__ real sfq[2][3][TY][WX];
-    __shared__ real sfr[2][3][WY][TX];
-
-    const long nwg = (long)gridDim.x;
-    const long per = (nwg + 7) / 8;
-    const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
-    if (logical >= TL.total) return;
-    int gi = 0;
-    for (int k = 1; k < TL.n; k++)
-        if (logical >= TL.g[k].first) gi = k;
-    const TileGeom &T = TL.g[gi];
-    const Blk<real> &B = P.b[T.blk];
-    const real A = T.blk == 0 ? A0 : A1;
-    long lid = logical - T.first;
-    int tx_i, ty_i, zc_i;
-    tile_coords(TL, T, lid, T.nzc, tx_i, ty_i, zc_i);
-
-    int xs, xc0, xc1, ys, yc0, yc1;
-    tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
-    tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
-    const int z0 = T.za0 + zc_i * T.zc;
-    const int z1 = min(z0 + T.zc, T.za1);
-
-    const int lx = threadIdx.x, ly = threadIdx.y;
-    const int gx = xs + lx, gy = ys + ly;
-    const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
-    const bool inb = (gx < B.nq && gy < B.nr);
-    const long NP = B.NP;
-    const long plane = (long)B.px * B.nr;
-    const long pcol = (long)gx + (long)B.px * gy;
-    const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
-
-    // this thread's halo node (x halo: 6*TY nodes, then y halo: 6*TX nodes)
-    const int tid = ly * TX + lx;
-    int hk = 0, hcx = 0, hcy = 0;
-    long hcol = 0;
-    if (tid < 6 * TY) {
-        int hx = tid % 6, hy = tid / 6;
-        hcx = hx < 3 ? hx : TX + hx;
-        hcy = hy;
-        int ggx = xs - 3 + hcx, ggy = ys + hy;
-        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcol = (long)ggx + (long)B.px * ggy; }
-    } else if (tid < 6 * TY + 6 * TX) {
-        int t2 = tid - 6 * TY;
-        int hx = t2 % TX, hy = t2 / TX;
-        hcy = hy < 3 ? hy : TY + hy;
-        hcx = hx;
-        int ggx = xs + hx, ggy = ys - 3 + hcy;
-        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcol = (long)ggx + (long)B.px * ggy; }
-    }
-    const int hax = hk == 2 ? 1 : 0;  // metric axis of the halo flux
-
-    auto gload = [&](long p, int c) -> real { return CURV ? B.G[(long)c * NP + p] : (real)B.gc[c]; };
-
-    auto prefetch = [&](PlanePrefetch<real, CURV> &R, int z) {
-        if (inb) {
-            long p = pcol + plane * z;
-#pragma unroll
-            for (int f = 0; f < 3; f++) R.v[f] = B.Fin[f * NP + p];
-#pragma unroll
-            for (int f = 0; f < 6; f++) R.s[f] = B.Fin[(3 + f) * NP + p];
-            if (CURV) {
-#pragma unroll
-                for (int c = 0; c < 10; c++) R.g[c] = B.G[(long)c * NP + p];
-            }
-            long pl = pcol + plane * (z + 3);
-#pragma unroll
-            for (int f = 0; f < 3; f++) R.lv[f] = B.Fin[f * NP + pl];
-#pragma unroll
-            for (int f = 0; f < 6; f++) R.ls[f] = B.Fin[(3 + f) * NP + pl];
-            if (CURV) {
-                R.lg[0] = B.G[9 * NP + pl];
-                R.lg[1] = B.G[2 * NP + pl];
-                R.lg[2] = B.G[5 * NP + pl];
-                R.lg[3] = B.G[8 * NP + pl];
-            }
-        }
-        if (own) {
-            long p = pcol + plane * z;
-#pragma unroll
-            for (int f = 0; f < 9; f++) R.df[f] = B.DF[f * NP + p];
-        }
-        if (hk) {
-            long p = hcol + plane * z;
-#pragma unroll
-            for (int f = 0; f < 3; f++) R.hv[f] = B.Fin[f * NP + p];
-#pragma unroll
-            for (int f = 0; f < 6; f++) R.hs[f] = B.Fin[(3 + f) * NP + p];
-            if (CURV) {
-                R.hg[0] = B.G[9 * NP + p];
-                R.hg[1] = B.G[(0 + hax) * NP + p];
-                R.hg[2] = B.G[(3 + hax) * NP + p];
-                R.hg[3] = B.G[(6 + hax) * NP + p];
-            }
-        }
-    };
-    auto fluxv = [&](const real sg[6], real J, real ax, real ay, real az, real fl[3]) {
-        real Jx = J * ax, Jy = J * ay, Jz = J * az;
-        fl[0] = Jx * sg[0] + Jy * sg[3] + Jz * sg[4];
-        fl[1] = Jx * sg[3] + Jy * sg[1] + Jz * sg[5];
-        fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
-    };
-
-    real qv[7][3], qf[7][3];
-#pragma unroll
-    for (int a = 0; a < 7; a++)
-#pragma unroll
-        for (int f = 0; f < 3; f++) { qv[a][f] = 0; qf[a][f] = 0; }
-    // warm the s-queue with planes z0-3 .. z0+2
-    if (inb) {
-#pragma unroll
-        for (int a = 1; a < 7; a++) {
-            long p = pcol + plane * (z0 - 4 + a);
-            real sg[6];
-#pragma unroll
-            for (int f = 0; f < 3; f++) qv[a][f] = B.Fin[f * NP + p];
-#pragma unroll
-            for (int f = 0; f < 6; f++) sg[f] = B.Fin[(3 + f) * NP + p];
-            fluxv(sg, gload(p, 9), gload(p, 2), gload(p, 5), gload(p, 8), qf[a]);
-        }
-    }
-
-    PlanePrefetch<real, CURV> cur, nxt;
-    prefetch(cur, z0);
-    for (int z = z0; z < z1; z++) {
-        const int b = z & 1;
-        // (1) publish plane z: centre and halo into LDS buffer b, lead into the queue
-        real cv[3], cs[6], cg[10], cdf[9];
-        if (inb) {
-            real J, g9[9];
-            if (CURV) {
-#pragma unroll
-                for (int c = 0; c < 10; c++) cg[c] = cur.g[c];
-            } else {
-#pragma unroll
-                for (int c = 0; c < 10; c++) cg[c] = (real)B.gc[c];
-            }
-            J = cg[9];
-            (void)g9;
-            real fq[3], fr[3];
-            fluxv(cur.s, J, cg[0], cg[3], cg[6], fq);
-            fluxv(cur.s, J, cg[1], cg[4], cg[7], fr);
-#pragma unroll
-            for (int f = 0; f < 3; f++) {
-                sv[b][f][ly + 3][lx + 3] = cur.v[f];
-                sfq[b][f][ly][lx + 3] = fq[f];
-                sfr[b][f][ly + 3][lx] = fr[f];
-                cv[f] = cur.v[f];
-            }
-#pragma unroll
-            for (int f = 0; f < 6; f++) cs[f] = cur.s[f];
-#pragma unroll
-            for (int a = 0; a < 6; a++)
-#pragma unroll
-                for (int f = 0; f < 3; f++) { qv[a][f] = qv[a + 1][f]; qf[a][f] = qf[a + 1][f]; }
-            real lJ, lx_, ly_, lz_;
-            if (CURV) { lJ = cur.lg[0]; lx_ = cur.lg[1]; ly_ = cur.lg[2]; lz_ = cur.lg[3]; }
-            else { lJ = (real)B.gc[9]; lx_ = (real)B.gc[2]; ly_ = (real)B.gc[5]; lz_ = (real)B.gc[8]; }
-#pragma unroll
-            for (int f = 0; f < 3; f++) qv[6][f] = cur.lv[f];
-            fluxv(cur.ls, lJ, lx_, ly_, lz_, qf[6]);
-        }
-        if (own) {
-#pragma unroll
-            for (int f = 0; f < 9; f++) cdf[f] = cur.df[f];
-        }
-        if (hk) {
-            real hJ, hx_, hy_, hz_;
-            if (CURV) { hJ = cur.hg[0]; hx_ = cur.hg[1]; hy_ = cur.hg[2]; hz_ = cur.hg[3]; }
-            else { hJ = (real)B.gc[9]; hx_ = (real)B.gc[0 + hax]; hy_ = (real)B.gc[3 + hax]; hz_ = (real)B.gc[6 + hax]; }
-            real fl[3];
-            fluxv(cur.hs, hJ, hx_, hy_, hz_, fl);
-            if (hk == 1) {
-#pragma unroll
-                for (int f = 0; f < 3; f++) { sv[b][f][hcy + 3][hcx] = cur.hv[f]; sfq[b][f][hcy][hcx] = fl[f]; }
-            } else {
-#pragma unroll
-                for (int f = 0; f < 3; f++) { sv[b][f][hcy][hcx + 3] = cur.hv[f]; sfr[b][f][hcy][hcx] = fl[f]; }
-            }
-        }
-        __syncthreads();
-        // (2) loads of plane z+1 go out now and overlap the arithmetic below
-        if (z + 1 < z1) prefetch(nxt, z + 1);
-        // (3) plane z
-        if (own) {
-            const long p = pcol + plane * z;
-            real dvq[3], dvr[3], dvs[3], div[3];
-#pragma unroll
-            for (int f = 0; f < 3; f++) {
-                dvq[f] = d1_row<real>(&sv[b][f][ly + 3][0], lx + 3, gx, B.nq, ihq);
-                dvr[f] = d1_col<real>(&sv[b][f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
-                dvs[f] = ihs * ((real)0.75 * (qv[4][f] - qv[2][f]) + (real)(-0.15) * (qv[5][f] - qv[1][f]) +
-                                (real)(1.0 / 60.0) * (qv[6][f] - qv[0][f]));
-                real a = d1_row<real>(&sfq[b][f][ly][0], lx + 3, gx, B.nq, ihq);
-                real bb = d1_col<real>(&sfr[b][f][0][lx], TX, ly + 3, gy, B.nr, ihr);
-                real c = ihs * ((real)0.75 * (qf[4][f] - qf[2][f]) + (real)(-0.15) * (qf[5][f] - qf[1][f]) +
-                                (real)(1.0 / 60.0) * (qf[6][f] - qf[0][f]));
-                div[f] = a + bb + c;
-            }
-            real lam, mu, rho;
-            if (HET) { lam = B.M[p]; mu = B.M[NP + p]; rho = B.M[2 * NP + p]; }
-            else { lam = (real)B.lam; mu = (real)B.mu; rho = (real)B.rho; }
-            real Ux[3], Uy[3], Uz[3];
-#pragma unroll
-            for (int f = 0; f < 3; f++) {
-                Ux[f] = cg[0] * dvq[f] + cg[1] * dvr[f] + cg[2] * dvs[f];
-                Uy[f] = cg[3] * dvq[f] + cg[4] * dvr[f] + cg[5] * dvs[f];
-                Uz[f] = cg[6] * dvq[f] + cg[7] * dvr[f] + cg[8] * dvs[f];
-            }
-            real rhoJ_inv = (real)1 / (rho * cg[9]);
-            real l2m = lam + (real)2 * mu;
-            real DU[NF];
-            DU[0] = div[0] * rhoJ_inv;
-            DU[1] = div[1] * rhoJ_inv;
-            DU[2] = div[2] * rhoJ_inv;
-            DU[3] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
-            DU[4] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
-            DU[5] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
-            DU[6] = mu * (Uy[0] + Ux[1]);
-            DU[7] = mu * (Uz[0] + Ux[2]);
-            DU[8] = mu * (Uz[1] + Uy[2]);
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                real df = A * cdf[f] + DU[f];
-                B.DF[f * NP + p] = df;
-                real fin = f < 3 ? cv[f] : cs[f - 3];
-                B.Fout[f * NP + p] = fin + bdt * df;
-            }
-        }
-        cur = nxt;
-    }
-}
-
 // ----------------------------------------------------------------------------
-// Buffer-addressed pipelined march.  Same algorithm and tiling as
-// k_volume_pipe; every global access is a raw buffer load/store whose
-// per-thread byte offset (one VGPR per node column) is shared by all fields
-// and whose field offset rides in an SGPR, so no 64-bit address arithmetic is
-// spent per load.  Used when every array of a block spans < 2 GiB.
+// Raw buffer loads and stores of the tiled volume kernels: the per-thread
+// byte offset (one VGPR per node column) is shared by all fields and the
+// field offset rides in an SGPR, so no 64-bit address arithmetic is spent
+// per access.  A block's arrays must each span < 2 GiB (checked by
+// wql_create).
 template <typename real>
 __device__ __forceinline__ real bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
@@ -957,310 +493,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void *p, unsigned 
 {
     // raw (stride 0) buffer over [p, p + bytes): out-of-range loads return 0
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
-}
-
-template <typename real, int TX, int TY, bool CURV, bool HET>
-__global__ void __launch_bounds__(TX * TY) k_volume_buf(Pair<real> P, TileList TL, real A0, real A1, real bdt)
-{
-    constexpr int WX = TX + 6, WY = TY + 6;
-    constexpr unsigned RS = sizeof(real);
-    __shared__ real sv[2][3][WY][WX];
-    __shared__ real sfq[2][3][TY][WX];
-    __shared__ real sfr[2][3][WY][TX];
-
-    const long nwg = (long)gridDim.x;
-    const long per = (nwg + 7) / 8;
-    const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
-    if (logical >= TL.total) return;
-    int gi = 0;
-    for (int k = 1; k < TL.n; k++)
-        if (logical >= TL.g[k].first) gi = k;
-    const TileGeom &T = TL.g[gi];
-    const Blk<real> &B = P.b[T.blk];
-    const real A = T.blk == 0 ? A0 : A1;
-    long lid = logical - T.first;
-    int tx_i, ty_i, zc_i;
-    tile_coords(TL, T, lid, T.nzc, tx_i, ty_i, zc_i);
-
-    int xs, xc0, xc1, ys, yc0, yc1;
-    tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
-    tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
-    const int z0 = T.za0 + zc_i * T.zc;
-    const int z1 = min(z0 + T.zc, T.za1);
-
-    const int lx = threadIdx.x, ly = threadIdx.y;
-    const int gx = xs + lx, gy = ys + ly;
-    const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
-    const bool inb = (gx < B.nq && gy < B.nr);
-    const unsigned NPb = (unsigned)B.NP * RS;           // bytes per field
-    const unsigned planeb = (unsigned)(B.px * B.nr) * RS;
-    const unsigned colb = (unsigned)(gx + B.px * gy) * RS;
-    const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
-
-    const __amdgpu_buffer_rsrc_t rF = brsrc(B.Fin, 9u * NPb);
-    const __amdgpu_buffer_rsrc_t rD = brsrc(B.DF, 9u * NPb);
-    const __amdgpu_buffer_rsrc_t rO = brsrc(B.Fout, 9u * NPb);
-    const __amdgpu_buffer_rsrc_t rG = brsrc(CURV ? (const void *)B.G : (const void *)B.Fin, CURV ? 10u * NPb : 0u);
-    const __amdgpu_buffer_rsrc_t rM = brsrc(HET ? (const void *)B.M : (const void *)B.Fin, HET ? 3u * NPb : 0u);
-
-    // this thread's halo node (x halo: 6*TY nodes, then y halo: 6*TX nodes)
-    const int tid = ly * TX + lx;
-    int hk = 0, hcx = 0, hcy = 0;
-    unsigned hcolb = 0;
-    if (tid < 6 * TY) {
-        int hx = tid % 6, hy = tid / 6;
-        hcx = hx < 3 ? hx : TX + hx;
-        hcy = hy;
-        int ggx = xs - 3 + hcx, ggy = ys + hy;
-        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hcolb = (unsigned)(ggx + B.px * ggy) * RS; }
-    } else if (tid < 6 * TY + 6 * TX) {
-        int t2 = tid - 6 * TY;
-        int hx = t2 % TX, hy = t2 / TX;
-        hcy = hy < 3 ? hy : TY + hy;
-        hcx = hx;
-        int ggx = xs + hx, ggy = ys - 3 + hcy;
-        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hcolb = (unsigned)(ggx + B.px * ggy) * RS; }
-    }
-    const unsigned hax = hk == 2 ? 1u : 0u;  // metric axis of the halo flux
-
-    auto fluxv = [&](const real sg[6], real J, real ax, real ay, real az, real fl[3]) {
-        real Jx = J * ax, Jy = J * ay, Jz = J * az;
-        fl[0] = Jx * sg[0] + Jy * sg[3] + Jz * sg[4];
-        fl[1] = Jx * sg[3] + Jy * sg[1] + Jz * sg[5];
-        fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
-    };
-    auto gconst = [&](int c) -> real { return (real)B.gc[c]; };
-
-    struct PF {
-        real v[3], s[6], g[10];
-        real hv[3], hs[6], hg[4];
-        real lv[3], ls[6], lg[4];
-        real df[9];
-    };
-    auto prefetch = [&](PF &R, int z) {
-        if (inb) {
-            const unsigned o = colb + planeb * (unsigned)z;
-#pragma unroll
-            for (int f = 0; f < 3; f++) R.v[f] = bload<real>(rF, o, f * NPb);
-#pragma unroll
-            for (int f = 0; f < 6; f++) R.s[f] = bload<real>(rF, o, (3 + f) * NPb);
-            if (CURV) {
-#pragma unroll
-                for (int c = 0; c < 10; c++) R.g[c] = bload<real>(rG, o, c * NPb);
-            }
-            const unsigned ol = o + 3u * planeb;
-#pragma unroll
-            for (int f = 0; f < 3; f++) R.lv[f] = bload<real>(rF, ol, f * NPb);
-#pragma unroll
-            for (int f = 0; f < 6; f++) R.ls[f] = bload<real>(rF, ol, (3 + f) * NPb);
-            if (CURV) {
-                R.lg[0] = bload<real>(rG, ol, 9u * NPb);
-                R.lg[1] = bload<real>(rG, ol, 2u * NPb);
-                R.lg[2] = bload<real>(rG, ol, 5u * NPb);
-                R.lg[3] = bload<real>(rG, ol, 8u * NPb);
-            }
-        }
-        if (own) {
-            const unsigned o = colb + planeb * (unsigned)z;
-#pragma unroll
-            for (int f = 0; f < 9; f++) R.df[f] = bload<real>(rD, o, f * NPb);
-        }
-        if (hk) {
-            const unsigned o = hcolb + planeb * (unsigned)z;
-#pragma unroll
-            for (int f = 0; f < 3; f++) R.hv[f] = bload<real>(rF, o, f * NPb);
-#pragma unroll
-            for (int f = 0; f < 6; f++) R.hs[f] = bload<real>(rF, o, (3 + f) * NPb);
-            if (CURV) {
-                R.hg[0] = bload<real>(rG, o, 9u * NPb);
-                R.hg[1] = bload<real>(rG, o, (0u + hax) * NPb);
-                R.hg[2] = bload<real>(rG, o, (3u + hax) * NPb);
-                R.hg[3] = bload<real>(rG, o, (6u + hax) * NPb);
-            }
-        }
-    };
-
-    real qv[7][3], qf[7][3];
-#pragma unroll
-    for (int a = 0; a < 7; a++)
-#pragma unroll
-        for (int f = 0; f < 3; f++) { qv[a][f] = 0; qf[a][f] = 0; }
-    // warm the s-queue with planes z0-3 .. z0+2
-    if (inb) {
-#pragma unroll
-        for (int a = 1; a < 7; a++) {
-            const unsigned o = colb + planeb * (unsigned)(z0 - 4 + a);
-            real sg[6];
-#pragma unroll
-            for (int f = 0; f < 3; f++) qv[a][f] = bload<real>(rF, o, f * NPb);
-#pragma unroll
-            for (int f = 0; f < 6; f++) sg[f] = bload<real>(rF, o, (3 + f) * NPb);
-            if (CURV)
-                fluxv(sg, bload<real>(rG, o, 9u * NPb), bload<real>(rG, o, 2u * NPb), bload<real>(rG, o, 5u * NPb),
-                      bload<real>(rG, o, 8u * NPb), qf[a]);
-            else
-                fluxv(sg, gconst(9), gconst(2), gconst(5), gconst(8), qf[a]);
-        }
-    }
-
-    auto body = [&](PF &cur, PF &nxt, int z) {
-        const int b = z & 1;
-        // (1) publish plane z: centre and halo into LDS buffer b, lead into the queue
-        real cv[3], cs[6], cg[10], cdf[9];
-        if (inb) {
-            if (CURV) {
-#pragma unroll
-                for (int c = 0; c < 10; c++) cg[c] = cur.g[c];
-            } else {
-#pragma unroll
-                for (int c = 0; c < 10; c++) cg[c] = gconst(c);
-            }
-            real fq[3], fr[3];
-            fluxv(cur.s, cg[9], cg[0], cg[3], cg[6], fq);
-            fluxv(cur.s, cg[9], cg[1], cg[4], cg[7], fr);
-#pragma unroll
-            for (int f = 0; f < 3; f++) {
-                sv[b][f][ly + 3][lx + 3] = cur.v[f];
-                sfq[b][f][ly][lx + 3] = fq[f];
-                sfr[b][f][ly + 3][lx] = fr[f];
-                cv[f] = cur.v[f];
-            }
-#pragma unroll
-            for (int f = 0; f < 6; f++) cs[f] = cur.s[f];
-#pragma unroll
-            for (int a = 0; a < 6; a++)
-#pragma unroll
-                for (int f = 0; f < 3; f++) { qv[a][f] = qv[a + 1][f]; qf[a][f] = qf[a + 1][f]; }
-#pragma unroll
-            for (int f = 0; f < 3; f++) qv[6][f] = cur.lv[f];
-            if (CURV) fluxv(cur.ls, cur.lg[0], cur.lg[1], cur.lg[2], cur.lg[3], qf[6]);
-            else fluxv(cur.ls, gconst(9), gconst(2), gconst(5), gconst(8), qf[6]);
-        }
-        if (own) {
-#pragma unroll
-            for (int f = 0; f < 9; f++) cdf[f] = cur.df[f];
-        }
-        if (hk) {
-            real fl[3];
-            if (CURV) fluxv(cur.hs, cur.hg[0], cur.hg[1], cur.hg[2], cur.hg[3], fl);
-            else fluxv(cur.hs, gconst(9), gconst(0 + hax), gconst(3 + hax), gconst(6 + hax), fl);
-            if (hk == 1) {
-#pragma unroll
-                for (int f = 0; f < 3; f++) { sv[b][f][hcy + 3][hcx] = cur.hv[f]; sfq[b][f][hcy][hcx] = fl[f]; }
-            } else {
-#pragma unroll
-                for (int f = 0; f < 3; f++) { sv[b][f][hcy][hcx + 3] = cur.hv[f]; sfr[b][f][hcy][hcx] = fl[f]; }
-            }
-        }
-        __syncthreads();
-        // (2) loads of plane z+1 go out now and overlap the arithmetic below
-        if (z + 1 < z1) prefetch(nxt, z + 1);
-        // (3) plane z
-        if (own) {
-            const unsigned o = colb + planeb * (unsigned)z;
-            real dvq[3], dvr[3], dvs[3], div[3];
-#pragma unroll
-            for (int f = 0; f < 3; f++) {
-                dvq[f] = d1_row<real>(&sv[b][f][ly + 3][0], lx + 3, gx, B.nq, ihq);
-                dvr[f] = d1_col<real>(&sv[b][f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
-                dvs[f] = ihs * ((real)0.75 * (qv[4][f] - qv[2][f]) + (real)(-0.15) * (qv[5][f] - qv[1][f]) +
-                                (real)(1.0 / 60.0) * (qv[6][f] - qv[0][f]));
-                real a = d1_row<real>(&sfq[b][f][ly][0], lx + 3, gx, B.nq, ihq);
-                real bb = d1_col<real>(&sfr[b][f][0][lx], TX, ly + 3, gy, B.nr, ihr);
-                real c = ihs * ((real)0.75 * (qf[4][f] - qf[2][f]) + (real)(-0.15) * (qf[5][f] - qf[1][f]) +
-                                (real)(1.0 / 60.0) * (qf[6][f] - qf[0][f]));
-                div[f] = a + bb + c;
-            }
-            real lam, mu, rho;
-            if (HET) { lam = bload<real>(rM, o, 0u); mu = bload<real>(rM, o, NPb); rho = bload<real>(rM, o, 2u * NPb); }
-            else { lam = (real)B.lam; mu = (real)B.mu; rho = (real)B.rho; }
-            real Ux[3], Uy[3], Uz[3];
-#pragma unroll
-            for (int f = 0; f < 3; f++) {
-                Ux[f] = cg[0] * dvq[f] + cg[1] * dvr[f] + cg[2] * dvs[f];
-                Uy[f] = cg[3] * dvq[f] + cg[4] * dvr[f] + cg[5] * dvs[f];
-                Uz[f] = cg[6] * dvq[f] + cg[7] * dvr[f] + cg[8] * dvs[f];
-            }
-            real rhoJ_inv = (real)1 / (rho * cg[9]);
-            real l2m = lam + (real)2 * mu;
-            real DU[NF];
-            DU[0] = div[0] * rhoJ_inv;
-            DU[1] = div[1] * rhoJ_inv;
-            DU[2] = div[2] * rhoJ_inv;
-            DU[3] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
-            DU[4] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
-            DU[5] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
-            DU[6] = mu * (Uy[0] + Ux[1]);
-            DU[7] = mu * (Uz[0] + Ux[2]);
-            DU[8] = mu * (Uz[1] + Uy[2]);
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                real df = A * cdf[f] + DU[f];
-                bstore<real>(rD, o, f * NPb, df);
-                real fin = f < 3 ? cv[f] : cs[f - 3];
-                bstore<real>(rO, o, f * NPb, fin + bdt * df);
-            }
-        }
-    };
-
-    PF pa, pb;
-    prefetch(pa, z0);
-    int z = z0;
-    for (; z + 1 < z1; z += 2) {
-        body(pa, pb, z);
-        body(pb, pa, z + 1);
-    }
-    if (z < z1) body(pa, pb, z);
-}
-
-// Closure table staged in LDS in the working precision.  Reads inside the
-// s-march sit behind its barriers, so the compiler keeps them in the loop
-// instead of holding per-lane copies of the table in registers.
-template <typename real>
-struct SbpLds {
-    real c[12][8];
-    int off[12][8];
-    int n[12];
-};
-
-template <typename real>
-__device__ __forceinline__ void sbp_to_lds(SbpLds<real> &S, int tid, int nthreads)
-{
-    for (int e = tid; e < 12 * 8; e += nthreads) {
-        const int r = e / 8, t = e % 8;
-        const SbpRow &R = r < 6 ? c_sbp.left[r] : c_sbp.right[r - 6];
-        S.c[r][t] = t < R.n ? (real)R.c[t] : (real)0;
-        S.off[r][t] = t < R.n ? R.off[t] : 0;
-        if (t == 0) S.n[r] = R.n;
-    }
-}
-
-template <typename real>
-__device__ __forceinline__ real d1_row_s(const SbpLds<real> &S, const real *row, int lx, int gx, int n, real ih)
-{
-    if (gx >= 6 && gx <= n - 7) {
-        return ih * ((real)0.75 * (row[lx + 1] - row[lx - 1]) + (real)(-0.15) * (row[lx + 2] - row[lx - 2]) +
-                     (real)(1.0 / 60.0) * (row[lx + 3] - row[lx - 3]));
-    }
-    const int r = gx < 6 ? gx : 6 + gx - (n - 6);
-    real s = 0;
-    for (int t = 0; t < S.n[r]; t++) s = s + S.c[r][t] * row[lx + S.off[r][t]];
-    return ih * s;
-}
-
-template <typename real>
-__device__ __forceinline__ real d1_col_s(const SbpLds<real> &S, const real *col, int stride, int ly, int gy, int n,
-                                         real ih)
-{
-    if (gy >= 6 && gy <= n - 7) {
-        return ih * ((real)0.75 * (col[(ly + 1) * stride] - col[(ly - 1) * stride]) +
-                     (real)(-0.15) * (col[(ly + 2) * stride] - col[(ly - 2) * stride]) +
-                     (real)(1.0 / 60.0) * (col[(ly + 3) * stride] - col[(ly - 3) * stride]));
-    }
-    const int r = gy < 6 ? gy : 6 + gy - (n - 6);
-    real s = 0;
-    for (int t = 0; t < S.n[r]; t++) s = s + S.c[r][t] * col[(ly + S.off[r][t]) * stride];
-    return ih * s;
 }
 
 // Dense closure weights in LDS: row r of the 12 closure rows over the nine
@@ -2106,189 +1338,6 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
     for (int m = 0; m < nz; m++) {
         body(pa, pb, s0 + dir * m);
         pa = pb;
-    }
-}
-
-// Plane-parallel variant: one thread per node and one workgroup per
-// (tile, s-plane).  The q/r stencils read an LDS tile (3-node halo, boundary
-// closure rows included), the s stencil reads the six s-neighbours straight
-// from memory; those loads hit L2 because the workgroups of consecutive
-// planes of a tile column are dispatched together to one XCD.  No loop-carried
-// state, so occupancy and memory-level parallelism are high.
-template <typename real, int TX, int TY, bool CURV, bool HET>
-__global__ void __launch_bounds__(TX * TY) k_volume_plane(Pair<real> P, TileList TL, real A0, real A1, real bdt)
-{
-    constexpr int WX = TX + 6, WY = TY + 6;
-    __shared__ real sv[3][WY][WX];
-    __shared__ real sfq[3][TY][WX];
-    __shared__ real sfr[3][WY][TX];
-
-    const long nwg = (long)gridDim.x;
-    const long per = (nwg + 7) / 8;
-    const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
-    if (logical >= TL.total) return;
-    int gi = 0;
-    for (int k = 1; k < TL.n; k++)
-        if (logical >= TL.g[k].first) gi = k;
-    const TileGeom &T = TL.g[gi];
-    const Blk<real> &B = P.b[T.blk];
-    const real A = T.blk == 0 ? A0 : A1;
-    long lid = logical - T.first;
-    const int nzp = T.za1 - T.za0;
-    int tx_i, ty_i, zl;
-    tile_coords(TL, T, lid, nzp, tx_i, ty_i, zl);
-    const int z = T.za0 + zl;
-
-    int xs, xc0, xc1, ys, yc0, yc1;
-    tile_range<TX>(tx_i, T.ntx, T.xa0, T.xa1, B.nq, xs, xc0, xc1);
-    tile_range<TY>(ty_i, T.nty, T.ya0, T.ya1, B.nr, ys, yc0, yc1);
-
-    const int lx = threadIdx.x, ly = threadIdx.y;
-    const int gx = xs + lx, gy = ys + ly;
-    const bool own = (gx >= xc0 && gx < xc1 && gy >= yc0 && gy < yc1);
-    const bool inb = (gx < B.nq && gy < B.nr);
-    const long NP = B.NP;
-    const long plane = (long)B.px * B.nr;
-    const long p = (long)gx + (long)B.px * gy + plane * z;
-    const real ihq = (real)(B.nq - 1), ihr = (real)(B.nr - 1), ihs = (real)(B.ns - 1);
-
-    auto fluxv = [&](const real sg[6], real J, real ax, real ay, real az, real fl[3]) {
-        real Jx = J * ax, Jy = J * ay, Jz = J * az;
-        fl[0] = Jx * sg[0] + Jy * sg[3] + Jz * sg[4];
-        fl[1] = Jx * sg[3] + Jy * sg[1] + Jz * sg[5];
-        fl[2] = Jx * sg[4] + Jy * sg[5] + Jz * sg[2];
-    };
-    auto gl = [&](long q, int c) -> real { return CURV ? B.G[(long)c * NP + q] : (real)B.gc[c]; };
-
-    // halo node of this thread (x halo 6*TY nodes, then y halo 6*TX nodes)
-    const int tid = ly * TX + lx;
-    int hk = 0, hcx = 0, hcy = 0;
-    long hp = 0;
-    if (tid < 6 * TY) {
-        int hx = tid % 6, hy = tid / 6;
-        hcx = hx < 3 ? hx : TX + hx;
-        hcy = hy;
-        int ggx = xs - 3 + hcx, ggy = ys + hy;
-        if (ggx >= 0 && ggx < B.nq && ggy < B.nr) { hk = 1; hp = (long)ggx + (long)B.px * ggy + plane * z; }
-    } else if (tid < 6 * TY + 6 * TX) {
-        int t2 = tid - 6 * TY;
-        int hx = t2 % TX, hy = t2 / TX;
-        hcy = hy < 3 ? hy : TY + hy;
-        hcx = hx;
-        int ggx = xs + hx, ggy = ys - 3 + hcy;
-        if (ggx < B.nq && ggy >= 0 && ggy < B.nr) { hk = 2; hp = (long)ggx + (long)B.px * ggy + plane * z; }
-    }
-
-    real cv[3], cs[6], cg[10];
-    if (inb) {
-#pragma unroll
-        for (int f = 0; f < 3; f++) cv[f] = B.Fin[f * NP + p];
-#pragma unroll
-        for (int f = 0; f < 6; f++) cs[f] = B.Fin[(3 + f) * NP + p];
-#pragma unroll
-        for (int c = 0; c < 10; c++) cg[c] = gl(p, c);
-        real fq[3], fr[3];
-        fluxv(cs, cg[9], cg[0], cg[3], cg[6], fq);
-        fluxv(cs, cg[9], cg[1], cg[4], cg[7], fr);
-#pragma unroll
-        for (int f = 0; f < 3; f++) {
-            sv[f][ly + 3][lx + 3] = cv[f];
-            sfq[f][ly][lx + 3] = fq[f];
-            sfr[f][ly + 3][lx] = fr[f];
-        }
-    }
-    if (hk) {
-        const int ax = hk == 2 ? 1 : 0;
-        real hv[3], hs[6], fl[3];
-#pragma unroll
-        for (int f = 0; f < 3; f++) hv[f] = B.Fin[f * NP + hp];
-#pragma unroll
-        for (int f = 0; f < 6; f++) hs[f] = B.Fin[(3 + f) * NP + hp];
-        fluxv(hs, gl(hp, 9), gl(hp, 0 + ax), gl(hp, 3 + ax), gl(hp, 6 + ax), fl);
-        if (hk == 1) {
-#pragma unroll
-            for (int f = 0; f < 3; f++) { sv[f][hcy + 3][hcx] = hv[f]; sfq[f][hcy][hcx] = fl[f]; }
-        } else {
-#pragma unroll
-            for (int f = 0; f < 3; f++) { sv[f][hcy][hcx + 3] = hv[f]; sfr[f][hcy][hcx] = fl[f]; }
-        }
-    }
-    // s derivatives from the s-neighbours read straight from memory: central
-    // rows as +-k pairs, boundary closure rows (planes within six nodes of an
-    // s face) through the closure table
-    real dvs[3] = {0, 0, 0}, dfs[3] = {0, 0, 0};
-    if (own && (z < 6 || z > B.ns - 7)) {
-        int idx[8]; double cz[8];
-        int m = sbp_taps(z, B.ns, idx, cz);
-        for (int t = 0; t < m; t++) {
-            long pt = p + plane * (idx[t] - z);
-            real st[6], ft[3];
-#pragma unroll
-            for (int f = 0; f < 6; f++) st[f] = B.Fin[(3 + f) * NP + pt];
-            fluxv(st, gl(pt, 9), gl(pt, 2), gl(pt, 5), gl(pt, 8), ft);
-#pragma unroll
-            for (int f = 0; f < 3; f++) {
-                dvs[f] += (real)cz[t] * B.Fin[f * NP + pt];
-                dfs[f] += (real)cz[t] * ft[f];
-            }
-        }
-    } else if (own) {
-        const real cc[3] = {(real)0.75, (real)(-0.15), (real)(1.0 / 60.0)};
-#pragma unroll
-        for (int k = 1; k <= 3; k++) {
-            long pp = p + plane * k, pm = p - plane * k;
-            real sp[6], sm[6], fp[3], fm[3];
-#pragma unroll
-            for (int f = 0; f < 6; f++) { sp[f] = B.Fin[(3 + f) * NP + pp]; sm[f] = B.Fin[(3 + f) * NP + pm]; }
-            fluxv(sp, gl(pp, 9), gl(pp, 2), gl(pp, 5), gl(pp, 8), fp);
-            fluxv(sm, gl(pm, 9), gl(pm, 2), gl(pm, 5), gl(pm, 8), fm);
-#pragma unroll
-            for (int f = 0; f < 3; f++) {
-                dvs[f] += cc[k - 1] * (B.Fin[f * NP + pp] - B.Fin[f * NP + pm]);
-                dfs[f] += cc[k - 1] * (fp[f] - fm[f]);
-            }
-        }
-    }
-    __syncthreads();
-    if (!own) return;
-    real dvq[3], dvr[3], div[3];
-#pragma unroll
-    for (int f = 0; f < 3; f++) {
-        dvq[f] = d1_row<real>(&sv[f][ly + 3][0], lx + 3, gx, B.nq, ihq);
-        dvr[f] = d1_col<real>(&sv[f][0][lx + 3], WX, ly + 3, gy, B.nr, ihr);
-        real a = d1_row<real>(&sfq[f][ly][0], lx + 3, gx, B.nq, ihq);
-        real b = d1_col<real>(&sfr[f][0][lx], TX, ly + 3, gy, B.nr, ihr);
-        div[f] = a + b + ihs * dfs[f];
-        dvs[f] = ihs * dvs[f];
-    }
-    real lam, mu, rho;
-    if (HET) { lam = B.M[p]; mu = B.M[NP + p]; rho = B.M[2 * NP + p]; }
-    else { lam = (real)B.lam; mu = (real)B.mu; rho = (real)B.rho; }
-    real Ux[3], Uy[3], Uz[3];
-#pragma unroll
-    for (int f = 0; f < 3; f++) {
-        Ux[f] = cg[0] * dvq[f] + cg[1] * dvr[f] + cg[2] * dvs[f];
-        Uy[f] = cg[3] * dvq[f] + cg[4] * dvr[f] + cg[5] * dvs[f];
-        Uz[f] = cg[6] * dvq[f] + cg[7] * dvr[f] + cg[8] * dvs[f];
-    }
-    real rhoJ_inv = (real)1 / (rho * cg[9]);
-    real l2m = lam + (real)2 * mu;
-    real DU[NF];
-    DU[0] = div[0] * rhoJ_inv;
-    DU[1] = div[1] * rhoJ_inv;
-    DU[2] = div[2] * rhoJ_inv;
-    DU[3] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
-    DU[4] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
-    DU[5] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
-    DU[6] = mu * (Uy[0] + Ux[1]);
-    DU[7] = mu * (Uz[0] + Ux[2]);
-    DU[8] = mu * (Uz[1] + Uy[2]);
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-        real df = A * B.DF[f * NP + p] + DU[f];
-        B.DF[f * NP + p] = df;
-        real fin = f < 3 ? cv[f] : cs[f - 3];
-        B.Fout[f * NP + p] = fin + bdt * df;
     }
 }
 

@@ -79,7 +79,8 @@ def estimate_bytes(prob, precision='fp64'):
 
 
 class Simulation:
-    def __init__(self, prob, precision='fp64', tile=None, zchunk=None, coords=None, graph=False, dt=None):
+    def __init__(self, prob, precision='fp64', tile=None, zchunk=None, coords=None, graph=False, dt=None,
+                 order=None):
         L = native.lib()
         self.prob = prob
         self.precision = precision
@@ -88,8 +89,9 @@ class Simulation:
         arr, keep = block_descs(prob, coords)
         check(L.wql_create(C.byref(pd), arr, C.byref(self.ctx)))
         self.dt = pd.dt
-        order = None
-        if tile is None and zchunk is None:
+        if order is not None:
+            order = (order, 2)
+        if tile is None and zchunk is None and order is None:
             from . import autotune
             best = autotune.lookup(prob, precision)
             if best is not None:
