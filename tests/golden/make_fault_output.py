@@ -13,6 +13,7 @@ are copied unchanged next to the input file that produced them.
 import os
 import shutil
 import subprocess
+import tempfile
 import sys
 
 from make_golden import CASES, HERE, MPIEXEC, ROOT, namelist
@@ -53,7 +54,7 @@ def run(name, case, t_final, w_stride, work):
 if __name__ == '__main__':
     if not os.path.exists(REF_MAIN):
         sys.exit('build the reference first: make -C oracle ref')
-    work = os.path.join(ROOT, '_work', 'golden_runs')
+    work = tempfile.mkdtemp(prefix='wql_golden_')
     for nm in sys.argv[1:] or list(RUNS):
         case, t_final, w_stride = RUNS[nm]
         run(nm, case, t_final, w_stride, os.path.join(work, nm))

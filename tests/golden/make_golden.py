@@ -15,7 +15,7 @@ unset `btp` fields uninitialised, domain.f90:99) and one golden_driver mode:
 The .npz fixtures hold float64 arrays in C order (field, s, r, q) for volume
 arrays and (field, s, r) for fault arrays.
 """
-import os, subprocess, sys, shutil
+import os, subprocess, sys, shutil, tempfile
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -167,7 +167,7 @@ if __name__ == '__main__':
     if not os.path.exists(REF):
         sys.exit('build the reference first: make -C oracle ref')
     os.makedirs(os.path.join(HERE, 'inputs'), exist_ok=True)
-    work = os.path.join(ROOT, '_work', 'golden_runs')
+    work = tempfile.mkdtemp(prefix='wql_golden_')
     names = sys.argv[1:] or list(CASES)
     for nm in names:
         kw, runs = CASES[nm]

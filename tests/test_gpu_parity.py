@@ -443,3 +443,46 @@ def test_rhs_block_shapes_vs_oracle(shapes):
             sim.eval_rates(0.3, what=1, stage=2)
             for ib in range(2):
                 assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= TOL, (tile, ib)
+
+
+@pytest.mark.parametrize('prec', ['fp64', 'fp32'])
+def test_mixed_pair_vs_oracle(prec):
+    """a Cartesian block beside a curvilinear one, with node-wise material
+    on the curvilinear block only: the pair runs the curvilinear,
+    heterogeneous kernels, so the library keeps uniform metrics in G and
+    constant lambda, mu, rho in M for the other block.  Rates (every launch
+    variant in FP64) and a 6-step trajectory against the oracle: relative L2
+    <= 1e-12 in FP64, <= 1e-5 in FP32."""
+    _say(f'[gpu] mixed pair {prec}')
+    prob = G.problem('curv_sw')
+    prob.blocks[0].mesh = 'cartesian'
+    rng = np.random.default_rng(11)
+    M = prob.blocks[1].lame()
+    M *= 1.0 + 0.05 * rng.standard_normal(M.shape[1:])[None]
+    prob.blocks[1].material = M
+    tol = TOL if prec == 'fp64' else 1e-5
+    od, vol, _ = H.oracle_rhs(prob, 0.3)
+    for tile in (range(7) if prec == 'fp64' else (None,)):
+        with _sim(prob, precision=prec, tile=tile) as sim:
+            H.load_pattern_state(sim, prob)
+            sim.eval_rates(0.3, what=0)
+            for ib in range(2):
+                assert G.rel_l2(sim.rates(ib), vol[ib]) <= tol, (tile, ib)
+            H.load_pattern_state(sim, prob)
+            sim.eval_rates(0.3, what=1, stage=2)
+            for ib in range(2):
+                assert G.rel_l2(sim.rates(ib), od.blocks[ib]['DF']) <= tol, (tile, ib)
+    od = O.OracleDomain(prob)
+    F0 = H.pulse_init(prob, [0, -1.0, 7.5, 0.0, 0.4, 1.0])
+    with _sim(prob, precision=prec) as sim:
+        for ib in range(2):
+            sim.set_fields(ib, F0[ib])
+            od.blocks[ib]['F'][...] = F0[ib]
+        sim.step(6)
+        sim.sync()
+        od.step(6)
+        for ib in range(2):
+            assert G.rel_l2(sim.fields(ib), od.blocks[ib]['F']) <= tol, ib
+            f = sim.fault(ib)
+            ref = od.blocks[ib]['iface']['S']
+            assert np.abs(f['S'] - ref).max() <= tol * max(1.0, np.abs(ref).max()), ib
