@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define WQL_ABI_VERSION 1
+#define WQL_ABI_VERSION 2
 
 enum {
     WQL_OK = 0,
@@ -54,16 +54,37 @@ typedef struct {
  * made explicit).  Same meaning as the oracle's ora_fault. */
 typedef struct {
     int law;               /* WQL_LOCKED / WQL_SLIP_WEAKENING / WQL_RATE_STATE */
-    int stress_mode;       /* 0: tractions T0 + boxcar patches; 1: tensor s0 on the face frame */
+    int stress_mode;       /* 0: tractions T0 + boxcar patches; 1: tensor s0 on the face frame;
+                              2: TPV26/27 depth-dependent tensor on the face frame
+                              (prestress 2008-2048, initial_stress_condition.f90:26-76) */
     double T0[3];
     double s0[6];
     int npatch;
     double patch[4][5];    /* (yc, zc, wy, wz, amplitude) added to T0_l */
-    int nuc;               /* 0 none, 1 cosine disk (yc,zc,r1,width,amp), 2 smooth bump (yc,zc,R,amp,T) */
+    int nuc;               /* 0 none, 1 cosine disk (yc,zc,r1,width,amp), 2 smooth bump (yc,zc,R,amp,T),
+                              3 Gaussian pulse (yc,zc,r0,c,-): T0_l += 2 c T0_n exp(-(r/r0)^2/2) for t > 0
+                              ('rate-weakening', 2387-2420) */
     double nucp[5];
     double fs, fd, Dc;
     double bar_wy, bar_wz, bar_big;
+    /* rate and state: a = rs_a + rs_da (1 - B(y) B(z)), B the smooth boxcar of
+     * half-width rs_w and ramp rs_wr (386-416) */
     double rs_a, rs_da, rs_w, rs_wr, rs_b, rs_L, rs_V0, rs_f0, rs_Vin;
+    /* ---- ABI 2 ---- */
+    int rs_law;            /* state evolution: 0 aging law (TPV101/102, 431-436), 1 slip law with
+                              flash weakening ('rate-weakening', 438-444), 2 KDN (446-449) */
+    double rs_fw;          /* flash weakening: Fss <- fw + (Fss - fw) / (1 + (V/Vw)^8)^(1/8) */
+    double rs_Vw, rs_dVw;  /* Vw = rs_Vw + rs_dVw (1 - B(y) B(z)); rs_Vw <= 0: no flash weakening */
+    int rs_init;           /* wql_init_state: 0 steady state from the prestress and rs_Vin
+                              (2756-2819), 1 constant state rs_W0, zero background velocity (2820-2850) */
+    double rs_W0;
+    int sw_law;            /* slip-weakening strength: 0 sigma_n (fs - (fs-fd) min(D,Dc)/Dc)
+                              (1283-1299); 1 TPV26/27: C0 + sigma_n (fs - (fs-fd) max(min(D/Dc,1), f2))
+                              with cohesion C0 and forced rupture f2 (1376-1458) */
+    double coh[3];         /* C0 = coh[0] + coh[1] (coh[2] - y) for y <= coh[2], else coh[0] (2491-2530) */
+    double frc[5];         /* forced rupture (yc, zc, r_crit, Vs, t0) (2625-2660) */
+    int rs_solver;         /* slip-rate solve: 0 regula falsi, the reference's iteration step for step
+                              (1170-1239); 1 bracketed Newton, |step| <= 1e-12 |V| + 1e-300 */
 } wql_fault_params;
 
 typedef struct {
@@ -124,6 +145,14 @@ int wql_set_fault(wql_ctx *ctx, int block, const double *S, const double *W);
 int wql_get_fault(wql_ctx *ctx, int block, double *S, double *W, double *trup, double *Svel,
                   double *Uhat, double *Vhat);
 int wql_get_fault_rates(wql_ctx *ctx, int block, double *DS, double *DW);
+/* plastic state of one block [2][ns][nr][nq] in FP64, the reference's
+ * P%P(:,:,:,1:2) (datatypes.f90 block_plastic, plastic_material.f90:19-60):
+ * field 0 the plastic strain rate lambda of the last update, field 1 the
+ * accumulated plastic strain gammap (plastic.f90:269-270).  Model 1
+ * ('scec', plastic_flow2) has no accumulator in the reference; there the
+ * library keeps lambda = (1 - r) tau / (G dt) and gammap += dt lambda.
+ * WQL_E_STATE if the problem is elastic. */
+int wql_get_plastic(wql_ctx *ctx, int block, double *P);
 /* peak tangential slip rate per fault node over the run, sampled where the
  * reference samples Svel (stage 1, CouplingForcing.f90:213-228) [nr*ns] */
 int wql_get_fault_peak(wql_ctx *ctx, int block, double *psr);

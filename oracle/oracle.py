@@ -20,7 +20,10 @@ class OraFault(C.Structure):
                 ('bar_wy', C.c_double), ('bar_wz', C.c_double), ('bar_big', C.c_double),
                 ('rs_a', C.c_double), ('rs_da', C.c_double), ('rs_w', C.c_double), ('rs_wr', C.c_double),
                 ('rs_b', C.c_double), ('rs_L', C.c_double), ('rs_V0', C.c_double), ('rs_f0', C.c_double),
-                ('rs_Vin', C.c_double)]
+                ('rs_Vin', C.c_double),
+                ('rs_law', C.c_int), ('rs_fw', C.c_double), ('rs_Vw', C.c_double), ('rs_dVw', C.c_double),
+                ('rs_init', C.c_int), ('rs_W0', C.c_double), ('sw_law', C.c_int), ('coh', C.c_double * 3),
+                ('frc', C.c_double * 5), ('rs_solver', C.c_int)]
 
 
 class OraBlock(C.Structure):
@@ -81,8 +84,11 @@ def fault_struct(f):
     s.nuc = f.nuc
     s.nucp[:] = f.nucp
     for k in ('fs', 'fd', 'Dc', 'bar_wy', 'bar_wz', 'bar_big', 'rs_a', 'rs_da', 'rs_w', 'rs_wr',
-              'rs_b', 'rs_L', 'rs_V0', 'rs_f0', 'rs_Vin'):
+              'rs_b', 'rs_L', 'rs_V0', 'rs_f0', 'rs_Vin', 'rs_law', 'rs_fw', 'rs_Vw', 'rs_dVw', 'rs_init',
+              'rs_W0', 'sw_law', 'rs_solver'):
         setattr(s, k, getattr(f, k))
+    s.coh[:] = f.coh
+    s.frc[:] = f.frc
     return s
 
 

@@ -572,8 +572,27 @@ static double smooth_boxcar(double x, double w, double wr) /* 2459-2476 */
     return 0.0;
 }
 
+/* TPV26/27 background stress tensor at depth y (initial_stress_condition.f90:26-76) */
+static void tpv26_stress(double y, double s0[6])
+{
+    double Omega = 0.0;
+    if (y <= 15.0) Omega = 1.0;
+    else if (y >= 15.0 && y <= 20.0) Omega = (20.0 - y) / 5.0;
+    else if (y >= 20.0) Omega = 0.0;
+    double b11 = 0.926793, b33 = 1.073206, b13 = -0.169029;
+    double Pf = 9.8 * y;
+    double sigma22 = -2.67 * 9.8 * y;
+    s0[0] = Omega * (b33 * (sigma22 + Pf) - Pf) + (1.0 - Omega) * sigma22 + Pf;
+    s0[1] = sigma22 + Pf;
+    s0[2] = Omega * (b11 * (sigma22 + Pf) - Pf) + (1.0 - Omega) * sigma22 + Pf;
+    s0[3] = 0.0;
+    s0[4] = Omega * (b13 * (sigma22 + Pf));
+    s0[5] = 0.0;
+}
+
 /* prestress 1802-2435: TPV5 (1822-1830) as stress_mode 0 with patches,
- * TPV28 (2049-2079) / TPV101-102 (2324-2385) as stress_mode 1 */
+ * TPV28 (2049-2079) / TPV101-102 (2324-2385) / 'rate-weakening' (2387-2420)
+ * as stress_mode 1, TPV26/27 (2008-2048) as stress_mode 2 */
 static void prestress(const ora_fault *f, const double x[3], double t, const double l[3],
                       const double m[3], const double n[3], double *Tn, double *Tm, double *Tl)
 {
@@ -584,7 +603,10 @@ static void prestress(const ora_fault *f, const double x[3], double t, const dou
             *Tl = *Tl + P[4] * boxcar(x[1] - P[0], P[2]) * boxcar(x[2] - P[1], P[3]);
         }
     } else {
-        const double *s = f->s0;
+        double st[6];
+        if (f->stress_mode == 2) tpv26_stress(x[1], st);
+        else for (int c = 0; c < 6; c++) st[c] = f->s0[c];
+        const double *s = st;
         double Tx = n[0] * s[0] + n[1] * s[3] + n[2] * s[4];
         double Ty = n[0] * s[3] + n[1] * s[1] + n[2] * s[5];
         double Tz = n[0] * s[4] + n[1] * s[5] + n[2] * s[2];
@@ -608,12 +630,41 @@ static void prestress(const ora_fault *f, const double x[3], double t, const dou
         if (t > 0.0 && t < T) Gv = exp(pow(t - T, 2) / (t * (t - 2.0 * T)));
         if (t >= T) Gv = 1.0;
         *Tl = *Tl + f->nucp[3] * Fv * Gv;
+    } else if (f->nuc == 3) { /* 'rate-weakening' 2405-2420 */
+        double tau_pulse = f->nucp[3] * *Tn;
+        double r0 = f->nucp[2];
+        double r = sqrt(pow(x[1] - f->nucp[0], 2) + pow(x[2] - f->nucp[1], 2));
+        double Fv = exp(-(pow(r / r0, 2)) / 2.0);
+        double Gv = 0.0;
+        if (t > 0.0) Gv = 1.0;
+        *Tl = *Tl + 2.0 * tau_pulse * Fv * Gv;
     }
 }
 
-/* Tau_strength TPV5 1283-1299 / TPV28 1459-1474 */
-static double tau_strength(const ora_fault *f, double sigma_n, double D, const double x[3])
+/* Tau_strength TPV5 1283-1299 / TPV28 1459-1474 (sw_law 0); TPV26/27
+ * 1376-1458 with Frictional_Cohesion 2491-2530 and Time_of_Forced_Rupture
+ * 2625-2660 (sw_law 1) */
+static double tau_strength(const ora_fault *f, double sigma_n, double D, const double x[3], double t)
 {
+    if (f->sw_law == 1) {
+        double fs = f->fs, fd = f->fd, Dc = f->Dc, t0 = f->frc[4], Vs = f->frc[3];
+        double r = sqrt(pow(x[1] - f->frc[0], 2) + pow(x[2] - f->frc[1], 2));
+        double depth = x[1], C0, tfr, f1, f2 = 0.0;
+        if (depth <= f->coh[2]) C0 = f->coh[0] + f->coh[1] * (f->coh[2] - depth);
+        else C0 = f->coh[0];
+        double r_crit = f->frc[2];
+        if (r < r_crit - 1.0e-9)
+            tfr = r / (0.7 * Vs) + 0.081 * r_crit / (0.7 * Vs) * (1.0 / (1.0 - pow(r / r_crit, 2)) - 1.0);
+        else
+            tfr = 1.0e9;
+        f1 = fmin(D / Dc, 1.0);
+        if (t < tfr) f2 = 0.0;
+        else if (t >= tfr && t < tfr + t0) f2 = (t - tfr) / t0;
+        else if (t >= tfr + t0) f2 = 1.0;
+        fs = fs + f->bar_big * (1.0 - boxcar(x[1], f->bar_wy) * boxcar(x[2], f->bar_wz));
+        double fr = fs - (fs - fd) * fmax(f1, f2);
+        return C0 + fr * sigma_n;
+    }
     double fs = f->fs + f->bar_big * (1.0 - boxcar(x[1], f->bar_wy) * boxcar(x[2], f->bar_wz));
     double fr = fs - (fs - f->fd) * fmin(D, f->Dc) / f->Dc;
     return sigma_n * fr;
@@ -714,7 +765,7 @@ int ora_hat(const ora_fault *f, const double u[6], const double v[6], const doub
     double vv_m, vv_l, tau_m, tau_l, Tu_m_hat, Tu_l_hat, Tv_m_hat, Tv_l_hat;
     if (f->law == ORA_SLIP_WEAKENING) {
         double tau_lock = sqrt(pow(T0_l + phi_l, 2) + pow(T0_m + phi_m, 2));
-        double tau_str = tau_strength(f, sigma_n, slip, x);
+        double tau_str = tau_strength(f, sigma_n, slip, x, t);
         if (tau_lock > tau_str) {
             solve_sw(&vv_m, &vv_l, Vel, &tau_m, &tau_l, T0_m + phi_m, T0_l + phi_l, eta_s, tau_str);
             if (*Vel < 0.0) return -2;
@@ -816,12 +867,28 @@ int ora_interface(ora_domain *d, int ib, double t, int stage)
             double psi = I->W[fp];
             int rc = ora_hat(&d->fault, uL, vL, mat1, mat2, x, frame, slip, psi, t, uh, vh, &Vel);
             if (rc) return rc;
-            if (d->fault.law == ORA_RATE_STATE) { /* 431-436 */
+            double slip_rate = sqrt(pow(vh[1] - uh[1], 2) + pow(vh[2] - uh[2], 2)); /* 425, 470 */
+            if (d->fault.law == ORA_RATE_STATE) { /* 427-450 */
                 const ora_fault *f = &d->fault;
-                I->DW[fp] = I->DW[fp] + f->rs_b * f->rs_V0 / f->rs_L * exp(-(I->W[fp] - f->rs_f0) / f->rs_b)
-                            - Vel * f->rs_b / f->rs_L;
+                double W = I->W[fp], L0 = f->rs_L, f0 = f->rs_f0, b = f->rs_b, V0 = f->rs_V0;
+                if (f->rs_law == 0) { /* TPV101/102 431-436 */
+                    I->DW[fp] = I->DW[fp] + b * V0 / L0 * exp(-(W - f0) / b) - Vel * b / L0;
+                } else {
+                    double a = rs_a(f, x);
+                    double Fss = f0 - (b - a) * log(Vel / V0);
+                    if (f->rs_law == 1) { /* 'rate-weakening' 438-444 */
+                        if (f->rs_Vw > 0.0) {
+                            double V_w = f->rs_Vw + f->rs_dVw * (1.0 - smooth_boxcar(x[1], f->rs_w, f->rs_wr)
+                                                                       * smooth_boxcar(x[2], f->rs_w, f->rs_wr));
+                            double q = Vel / V_w, q8 = (q * q) * (q * q) * ((q * q) * (q * q));
+                            Fss = f->rs_fw + (Fss - f->rs_fw) / pow(1.0 + q8, 0.125);
+                        }
+                        I->DW[fp] = I->DW[fp] - Vel / L0 * (a * asinh(Vel / (2.0 * V0) * exp(W / a)) - Fss);
+                    } else { /* 'KDN' 446-449 */
+                        I->DW[fp] = I->DW[fp] - (1.0 / L0) * slip_rate * (asinh(Vel / (2.0 * V0) * exp(W / a)) - Fss);
+                    }
+                }
             }
-            double slip_rate = sqrt(pow(vh[1] - uh[1], 2) + pow(vh[2] - uh[2], 2)); /* 470 */
             for (int c = 0; c < 3; c++) I->DS[c * FP + fp] = I->DS[c * FP + fp] + vh[c] - uh[c];
             I->DS[3 * FP + fp] = I->DS[3 * FP + fp] + slip_rate;
             if (stage == 1) { /* 480-509 and CouplingForcing.f90:213-228, 584-605 */
@@ -950,23 +1017,8 @@ int ora_interface(ora_domain *d, int ib, double t, int stage)
  * initial stress: initial_stress_condition.f90:8-120 */
 static void s0_at(const ora_domain *d, const double x[3], double s0[6])
 {
-    if (d->s0_kind == 1) { /* TPV26 26-50 */
-        double Omega = 0.0;
-        if (x[1] <= 15.0) Omega = 1.0;
-        else if (x[1] >= 15.0 && x[1] <= 20.0) Omega = (20.0 - x[1]) / 5.0;
-        else if (x[1] >= 20.0) Omega = 0.0;
-        double b11 = 0.926793, b33 = 1.073206, b13 = -0.169029;
-        double Pf = 9.8 * x[1];
-        double sigma22 = -2.67 * 9.8 * x[1];
-        s0[0] = Omega * (b33 * (sigma22 + Pf) - Pf) + (1.0 - Omega) * sigma22 + Pf;
-        s0[1] = sigma22 + Pf;
-        s0[2] = Omega * (b11 * (sigma22 + Pf) - Pf) + (1.0 - Omega) * sigma22 + Pf;
-        s0[3] = 0.0;
-        s0[4] = Omega * (b13 * (sigma22 + Pf));
-        s0[5] = 0.0;
-    } else {
-        for (int c = 0; c < 6; c++) s0[c] = d->fault.s0[c];
-    }
+    if (d->s0_kind == 1) tpv26_stress(x[1], s0); /* TPV26 26-50 */
+    else for (int c = 0; c < 6; c++) s0[c] = d->fault.s0[c];
 }
 
 static void invariants(const double sa[6], double *tau, double *sigma)
@@ -1003,10 +1055,18 @@ static void plastic_update(ora_domain *d, ora_block *b)
                     double c = d->pl_c, nu = d->pl_nu, Tv = d->pl_Tv;
                     double Ys = fmax(0.0, c * cos(atan(nu)) - (sigma) * sin(atan(nu)));
                     double Yf = tau - Ys;
-                    if (Yf <= 0.0) continue;
+                    /* no accumulator in the reference's plastic_flow2 (plastic.f90:153-217);
+                     * the build keeps one in P: lambda = dgamma / dt, gammap += dgamma with
+                     * dgamma = (1 - r) tau / G, the deviator's shear-stress drop over G */
+                    if (Yf <= 0.0) { if (b->P) b->P[p] = 0.0; continue; }
                     double r = exp(-d->dt / Tv) + (1.0 - exp(-d->dt / Tv)) * Ys / tau;
                     for (int c2 = 0; c2 < 6; c2++) sd[c2] = sa[c2] - sigma * delta[c2];
                     for (int c2 = 0; c2 < 6; c2++) sa[c2] = sigma * delta[c2] + r * sd[c2];
+                    if (b->P) {
+                        double G = b->M[NP + p], dg = (1.0 - r) * tau / G;
+                        b->P[p] = dg / d->dt;
+                        b->P[NP + p] = b->P[NP + p] + dg;
+                    }
                 } else {
                     double mu = d->pl_mu, beta = d->pl_beta, eta = d->pl_eta;
                     double G = b->M[NP + p], K = b->M[p] + 2.0 / 3.0 * b->M[NP + p];
@@ -1059,7 +1119,8 @@ void ora_rk_coeffs(double *A, double *B, double *C)
 }
 
 void ora_init_state(ora_domain *d)
-{ /* Init_Vel_State 2722-2806 (TPV101/102 branch) */
+{ /* Init_Vel_State 2722-2850: TPV101/102 branch (steady state, rs_init 0),
+   * 'rate-weakening' branch (constant state, rs_init 1) */
     const ora_fault *f = &d->fault;
     if (f->law != ORA_RATE_STATE) return;
     for (int ib = 1; ib <= 2; ib++) {
@@ -1077,6 +1138,10 @@ void ora_init_state(ora_domain *d)
                 const double *nm = B->nml[side];
                 double n[3], m[3], l[3];
                 for (int c = 0; c < 3; c++) { n[c] = nm[c * FP + fp]; m[c] = nm[(3 + c) * FP + fp]; l[c] = nm[(6 + c) * FP + fp]; }
+                if (f->rs_init == 1) { /* 2820-2850 */
+                    d->I[ib - 1].W[fp] = f->rs_W0;
+                    continue;
+                }
                 double T0n, T0m, T0l;
                 prestress(f, x, 0.0, l, m, n, &T0n, &T0m, &T0l);
                 double a = rs_a(f, x);

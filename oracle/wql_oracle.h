@@ -27,16 +27,26 @@ enum { ORA_LOCKED = 0, ORA_SLIP_WEAKENING = 1, ORA_RATE_STATE = 2 };
 /* fault law and prestress description (same meaning as wql_fault_params) */
 typedef struct {
     int law;            /* ORA_LOCKED / ORA_SLIP_WEAKENING / ORA_RATE_STATE */
-    int stress_mode;    /* 0: constant tractions T0 + boxcar patches; 1: tensor s0 on the face frame */
+    int stress_mode;    /* 0: constant tractions T0 + boxcar patches; 1: tensor s0 on the face frame;
+                           2: TPV26/27 depth tensor on the face frame */
     double T0[3];       /* (n, m, l) background tractions, stress_mode 0 */
     double s0[6];       /* sxx syy szz sxy sxz syz, stress_mode 1 */
     int npatch;         /* boxcar patches added to T0_l: (yc, zc, wy, wz, amp) */
     double patch[4][5];
-    int nuc;            /* 0 none, 1 cosine-tapered disk (yc, zc, r1, width, amp), 2 smooth bump (yc, zc, R, amp, T) */
+    int nuc;            /* 0 none, 1 cosine-tapered disk (yc, zc, r1, width, amp), 2 smooth bump (yc, zc, R, amp, T),
+                           3 Gaussian pulse scaled by T0_n (yc, zc, r0, c, -) */
     double nucp[5];
     double fs, fd, Dc;  /* linear slip weakening */
     double bar_wy, bar_wz, bar_big; /* fs += bar_big*(1 - boxcar(y,bar_wy)*boxcar(z,bar_wz)) */
     double rs_a, rs_da, rs_w, rs_wr, rs_b, rs_L, rs_V0, rs_f0, rs_Vin;
+    int rs_law;         /* state evolution: 0 aging (TPV101/102), 1 slip law + flash weakening, 2 KDN */
+    double rs_fw, rs_Vw, rs_dVw; /* flash weakening (rs_Vw <= 0: off) */
+    int rs_init;        /* initial state: 0 steady state from the prestress, 1 constant rs_W0 */
+    double rs_W0;
+    int sw_law;         /* slip-weakening strength: 0 linear, 1 TPV26/27 cohesion + forced rupture */
+    double coh[3];      /* cohesion (C0 at depth, slope, depth of the ramp) */
+    double frc[5];      /* forced rupture (yc, zc, r_crit, Vs, t0) */
+    int rs_solver;      /* ignored: the oracle always runs the reference's regula falsi */
 } ora_fault;
 
 typedef struct {

@@ -32,7 +32,7 @@ BASE_BLOCKS = dict(
 def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
              plastic_model='default', pml_l=((False,) * 3,) * 2, pml_r=((False,) * 3,) * 2,
              npml=0, lc=(1.0, 0.0), rc=(0.0, 1.0), profile='analytical_test_problem',
-             rho_s_p=(2.67, 3.464, 6.0), t_final=1.0):
+             rho_s_p=(2.67, 3.464, 6.0), t_final=1.0, aqrs=None, bqrs=None, mu_beta_eta=(0.0, 0.0, 0.0)):
     T = lambda b: 'T' if b else 'F'
     s = ["&problem_list",
          f" name = 'data/{name}',", f" problem = '{problem}',", f" response = '{response}',",
@@ -40,13 +40,14 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
          " CFL = 0.5d0,", f" coupling = '{coupling}',", f" type_of_mesh = '{mesh}',",
          " w_stride = 1,", " w_fault = F /", "", "&block_list"]
     for b in range(2):
-        a, c = BASE_BLOCKS['aqrs'][b], BASE_BLOCKS['bqrs'][b]
+        a = (aqrs or BASE_BLOCKS['aqrs'])[b]
+        c = (bqrs or BASE_BLOCKS['bqrs'])[b]
         i = b + 1
         s += [f"btp({i})%nqrs = {n[0]}, {n[1]}, {n[2]},",
               f"btp({i})%aqrs = {a[0]}d0, {a[1]}d0, {a[2]}d0,",
               f"btp({i})%bqrs = {c[0]}d0, {c[1]}d0, {c[2]}d0,",
               f"btp({i})%rho_s_p = {rho_s_p[0]}d0, {rho_s_p[1]}d0, {rho_s_p[2]}d0,",
-              "btp(%d)%%mu_beta_eta = 0d0, 0d0, 0d0," % i,
+              f"btp({i})%mu_beta_eta = {mu_beta_eta[0]}d0, {mu_beta_eta[1]}d0, {mu_beta_eta[2]}d0,",
               f"btp({i})%lqrs = {lq[b][0]}, {lq[b][1]}, {lq[b][2]},",
               f"btp({i})%rqrs = {rq[b][0]}, {rq[b][1]}, {rq[b][2]},",
               f"btp({i})%lc = {lc[b]}d0,", f"btp({i})%rc = {rc[b]}d0,",
@@ -90,6 +91,28 @@ CASES = {
     'locked_pulse': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=N_STD,
                           lq=LQ_STD, rq=RQ_STD),
                      [('traj', ['10', '-0.8', '7.2', '0.3', '0.4', '1.0'])]),
+    # slip law with flash weakening, Gaussian prestress pulse, constant initial
+    # state, and the default plasticity model (plastic_flow, which accumulates
+    # gammap); mu = 0.35 puts the background stress just past yield everywhere
+    'rw_plastic': (dict(problem='rate-weakening', coupling='rate-and-state_friction', mesh='curvilinear',
+                        n=N_STD, lq=LQ_STD, rq=RQ_STD, response='plastic', plastic_model='default',
+                        mu_beta_eta=(0.35, 0.15, 0.05)),
+                   [('rhs', ['0.3']), ('traj', ['16'])]),
+    # TPV26: depth-dependent prestress tensor on the fault, cohesion and forced
+    # rupture in the strength; the blocks span depths 2-24 km (cohesion ramp
+    # above 5 km, the stress taper between 15 and 20 km) and reach the forced
+    # rupture centre (y, z) = (10, -5)
+    'tpv26_sw': (dict(problem='TPV26', coupling='slip-weakening_friction', mesh='cartesian',
+                      n=N_STD, lq=LQ_STD, rq=RQ_STD,
+                      aqrs=[(-6.0, 2.0, -9.0), (0.0, 2.0, -9.0)], bqrs=[(0.0, 24.0, 3.0), (6.0, 24.0, 3.0)]),
+                 [('rhs', ['0.3']), ('traj', ['30'])]),
+    # TPV26 with the 'scec' viscoplastic return map: a 2 m/s velocity pulse
+    # drives the stresses past the Drucker-Prager yield over the depth range,
+    # so the return map runs on the TPV26 background tensor
+    'tpv26_plastic': (dict(problem='TPV26', coupling='slip-weakening_friction', mesh='cartesian',
+                           n=N_STD, lq=LQ_STD, rq=RQ_STD, response='plastic', plastic_model='scec',
+                           aqrs=[(-6.0, 2.0, -9.0), (0.0, 2.0, -9.0)], bqrs=[(0.0, 24.0, 3.0), (6.0, 24.0, 3.0)]),
+                      [('traj', ['20', '-1.0', '13.0', '-3.0', '4.0', '2.0'])]),
 }
 
 

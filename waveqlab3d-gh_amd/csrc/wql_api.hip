@@ -872,8 +872,18 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
         return fail(WQL_E_ARG, "blocks must share the fault face size (nr, ns)");
     if (blocks[0].bc[1] != 0 || blocks[1].bc[0] != 0)
         return fail(WQL_E_ARG, "the fault joins block 0 side Rx and block 1 side Lx (bc code 0)");
-    if (prob->fault.law < 0 || prob->fault.law > 2 || prob->fault.npatch < 0 || prob->fault.npatch > 4)
-        return fail(WQL_E_ARG, "bad fault description");
+    {
+        const wql_fault_params &f = prob->fault;
+        if (f.law < 0 || f.law > 2 || f.npatch < 0 || f.npatch > 4 || f.stress_mode < 0 || f.stress_mode > 2 ||
+            f.nuc < 0 || f.nuc > 3 || f.rs_law < 0 || f.rs_law > 2 || f.rs_init < 0 || f.rs_init > 1 ||
+            f.sw_law < 0 || f.sw_law > 1 || f.rs_solver < 0 || f.rs_solver > 1)
+            return fail(WQL_E_ARG, "bad fault description");
+        if (f.law == WQL_SLIP_WEAKENING && !(f.Dc > 0)) return fail(WQL_E_ARG, "slip weakening needs Dc > 0");
+        if (f.law == WQL_RATE_STATE && !(f.rs_b > 0 && f.rs_L > 0 && f.rs_V0 > 0 && f.rs_a > 0))
+            return fail(WQL_E_ARG, "rate and state needs a, b, L, V0 > 0");
+    }
+    if (prob->plastic < 0 || prob->plastic > 2 || prob->s0_kind < 0 || prob->s0_kind > 1)
+        return fail(WQL_E_ARG, "bad plasticity description");
     for (int ib = 0; ib < 2; ib++) {
         const size_t elem = prob->precision == WQL_FP32 ? 4 : 8;
         const long lane = 128 / (long)elem;
@@ -1126,6 +1136,14 @@ int wql_get_fault_rates(wql_ctx *c, int ib, double *DS, double *DW)
     if (DS) HIPCHK(hipMemcpy(DS, c->I[ib].DS, 4 * FP * sizeof(double), hipMemcpyDeviceToHost));
     if (DW) HIPCHK(hipMemcpy(DW, c->I[ib].DW, FP * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
+}
+
+int wql_get_plastic(wql_ctx *c, int ib, double *P)
+{
+    if (!c || ib < 0 || ib > 1 || !P) return fail(WQL_E_ARG, "bad argument");
+    const DevBlock &d = c->b[ib];
+    if (!d.P) return fail(WQL_E_STATE, "elastic problem: no plastic state");
+    return copy_vol_t<double>(c, d, d.P, P, 2);
 }
 
 int wql_eval_rates(wql_ctx *c, double t, int what, int stage)

@@ -36,6 +36,14 @@ struct FaultParams {
     double fs, fd, Dc;
     double bar_wy, bar_wz, bar_big;
     double rs_a, rs_da, rs_w, rs_wr, rs_b, rs_L, rs_V0, rs_f0, rs_Vin;
+    int rs_law;              // 0 aging, 1 slip law + flash weakening, 2 KDN
+    double rs_fw, rs_Vw, rs_dVw;
+    int rs_init;             // 0 steady state from the prestress, 1 constant rs_W0
+    double rs_W0;
+    int sw_law;              // 0 linear slip weakening, 1 TPV26/27 cohesion + forced rupture
+    double coh[3];
+    double frc[5];
+    int rs_solver;           // 0 regula falsi (reference), 1 bracketed Newton
 };
 
 // One block as the kernels see it.

@@ -1365,6 +1365,25 @@ __device__ __forceinline__ double smooth_boxcar(double x, double w, double wr)  
 }
 
 // prestress (Interface_Condition.f90:1802-2435)
+// TPV26/27 background stress at depth y (initial_stress_condition.f90:26-76):
+// effective vertical stress -(rho - rho_w) g y, the tectonic ratios b11, b33,
+// b13 above 15 km tapered linearly to isotropic at 20 km
+__device__ void tpv26_stress(double y, double s0[6])
+{
+    double Omega = 0.0;
+    if (y <= 15.0) Omega = 1.0;
+    else if (y >= 15.0 && y <= 20.0) Omega = (20.0 - y) / 5.0;
+    else if (y >= 20.0) Omega = 0.0;
+    const double b11 = 0.926793, b33 = 1.073206, b13 = -0.169029;
+    const double Pf = 9.8 * y, s22 = -2.67 * 9.8 * y;
+    s0[0] = Omega * (b33 * (s22 + Pf) - Pf) + (1.0 - Omega) * s22 + Pf;
+    s0[1] = s22 + Pf;
+    s0[2] = Omega * (b11 * (s22 + Pf) - Pf) + (1.0 - Omega) * s22 + Pf;
+    s0[3] = 0.0;
+    s0[4] = Omega * (b13 * (s22 + Pf));
+    s0[5] = 0.0;
+}
+
 __device__ void prestress(const FaultParams &f, const double x[3], double t, const double l[3], const double m[3],
                           const double n[3], double &Tn, double &Tm, double &Tl)
 {
@@ -1373,7 +1392,10 @@ __device__ void prestress(const FaultParams &f, const double x[3], double t, con
         for (int k = 0; k < f.npatch; k++)
             Tl = Tl + f.patch[k][4] * boxcar(x[1] - f.patch[k][0], f.patch[k][2]) * boxcar(x[2] - f.patch[k][1], f.patch[k][3]);
     } else {
-        const double *s = f.s0;
+        double st[6];
+        if (f.stress_mode == 2) tpv26_stress(x[1], st);
+        else for (int c = 0; c < 6; c++) st[c] = f.s0[c];
+        const double *s = st;
         double Tx = n[0] * s[0] + n[1] * s[3] + n[2] * s[4];
         double Ty = n[0] * s[3] + n[1] * s[1] + n[2] * s[5];
         double Tz = n[0] * s[4] + n[1] * s[5] + n[2] * s[2];
@@ -1398,7 +1420,52 @@ __device__ void prestress(const FaultParams &f, const double x[3], double t, con
         if (t > 0.0 && t < T) Gv = exp((t - T) * (t - T) / (t * (t - 2.0 * T)));
         if (t >= T) Gv = 1.0;
         Tl = Tl + f.nucp[3] * Fv * Gv;
+    } else if (f.nuc == 3) {  // 'rate-weakening' (2387-2420): Gaussian pulse scaled by the normal traction
+        const double tau_pulse = f.nucp[3] * Tn;
+        const double r0 = f.nucp[2];
+        double dy = x[1] - f.nucp[0], dz = x[2] - f.nucp[1];
+        double r = sqrt(dy * dy + dz * dz);
+        double Fv = exp(-((r / r0) * (r / r0)) / 2.0);
+        double Gv = t > 0.0 ? 1.0 : 0.0;
+        Tl = Tl + 2.0 * tau_pulse * Fv * Gv;
     }
+}
+
+// direct-effect parameter a at fault point x (Interface_Condition.f90:386-416)
+__device__ __forceinline__ double rs_a_at(const FaultParams &f, const double x[3])
+{
+    return f.rs_a + f.rs_da * (1.0 - smooth_boxcar(x[1], f.rs_w, f.rs_wr) * smooth_boxcar(x[2], f.rs_w, f.rs_wr));
+}
+
+// shear strength of the slip-weakening laws (Tau_strength, 1268-1797)
+__device__ double tau_strength(const FaultParams &f, double sigma_n, double D, const double x[3], double t)
+{
+    if (f.sw_law == 1) {  // TPV26/27 (1376-1458)
+        const double yc = f.frc[0], zc = f.frc[1], r_crit = f.frc[2], Vs = f.frc[3], t0 = f.frc[4];
+        const double dy = x[1] - yc, dz = x[2] - zc;
+        const double r = sqrt(dy * dy + dz * dz);
+        // Frictional_Cohesion (2491-2530), depth = y
+        const double C0 = x[1] <= f.coh[2] ? f.coh[0] + f.coh[1] * (f.coh[2] - x[1]) : f.coh[0];
+        // Time_of_Forced_Rupture (2625-2660)
+        double tf;
+        if (r < r_crit - 1.0e-9) {
+            const double q = r / r_crit;
+            tf = r / (0.7 * Vs) + 0.081 * r_crit / (0.7 * Vs) * (1.0 / (1.0 - q * q) - 1.0);
+        } else {
+            tf = 1.0e9;
+        }
+        const double f1 = fmin(D / f.Dc, 1.0);
+        double f2;
+        if (t < tf) f2 = 0.0;
+        else if (t < tf + t0) f2 = (t - tf) / t0;
+        else f2 = 1.0;
+        const double fs = f.fs + f.bar_big * (1.0 - boxcar(x[1], f.bar_wy) * boxcar(x[2], f.bar_wz));
+        const double fr = fs - (fs - f.fd) * fmax(f1, f2);
+        return C0 + fr * sigma_n;
+    }
+    const double fs = f.fs + f.bar_big * (1.0 - boxcar(x[1], f.bar_wy) * boxcar(x[2], f.bar_wz));
+    const double fr = fs - (fs - f.fd) * fmin(D, f.Dc) / f.Dc;
+    return sigma_n * fr;
 }
 
 __device__ __forceinline__ double rs_f(double V, double Phi, double eta, double sn, double psi, double V0, double a)
@@ -1433,6 +1500,38 @@ __device__ double regula_falsi(double V, double Phi, double eta, double sn, doub
         fv = rs_f(V, Phi, eta, sn, psi, V0, a);
         err = fabs(V - V1);
         k = k + 1;
+    }
+    return V;
+}
+
+// Fast slip-rate solve: the root of the same f(V) (1241-1265) by Newton's
+// method inside the bracket [0, Phi/eta] that regula falsi starts from.  f is
+// increasing in V; every iterate updates the bracket by the sign of f.  A
+// Newton step in V that leaves the bracket is retried in ln V (where f is
+// nearly linear once the asinh argument is large), and bisected if that
+// leaves it too.  Stops when a step moves V by at most 1e-13 V.
+__device__ double newton_rs(double V, double Phi, double eta, double sn, double psi, double V0, double a)
+{
+    double lo = 0.0, hi = Phi / eta;
+    if (!(hi > 0.0)) return 0.0;
+    if (!(V > lo && V < hi)) V = 0.5 * hi;
+    const double e = exp(psi / a) / (2.0 * V0);
+    const double c = a * sn / eta;
+    const double rhs = Phi / eta;
+    for (int k = 0; k < 100; k++) {
+        const double z = V * e;
+        const double fv = V + c * asinh(z) - rhs;
+        if (fv == 0.0) return V;
+        if (fv < 0.0) lo = V;
+        else hi = V;
+        const double df = 1.0 + c * e / sqrt(1.0 + z * z);
+        double Vn = V - fv / df;
+        if (!(Vn > lo && Vn < hi)) {
+            Vn = V * exp(-fv / (V * df));
+            if (!(Vn > lo && Vn < hi)) Vn = lo > 0.0 ? sqrt(lo * hi) : 0.5 * hi;
+        }
+        if (fabs(Vn - V) <= 1.0e-13 * Vn) return Vn;
+        V = Vn;
     }
     return V;
 }
@@ -1480,9 +1579,7 @@ __device__ int hat_vars(const FaultParams &f, const double u[6], const double v[
     double vv_m, vv_l, Tu_m, Tu_l, Tv_m, Tv_l;
     if (f.law == 1) {
         double tau_lock = sqrt((T0l + phi_l) * (T0l + phi_l) + (T0m + phi_m) * (T0m + phi_m));
-        double fs = f.fs + f.bar_big * (1.0 - boxcar(x[1], f.bar_wy) * boxcar(x[2], f.bar_wz));
-        double fr = fs - (fs - f.fd) * fmin(slip, f.Dc) / f.Dc;
-        double tau_str = sigma_n * fr;
+        double tau_str = tau_strength(f, sigma_n, slip, x, t);
         if (tau_lock > tau_str) {
             double phi_1 = T0m + phi_m, phi_2 = T0l + phi_l;
             double Phi = sqrt(phi_1 * phi_1 + phi_2 * phi_2);
@@ -1499,7 +1596,7 @@ __device__ int hat_vars(const FaultParams &f, const double u[6], const double v[
             vv_m = 0.0; vv_l = 0.0; Vel = 0.0;
         }
     } else {
-        double a = f.rs_a + f.rs_da * (1.0 - smooth_boxcar(x[1], f.rs_w, f.rs_wr) * smooth_boxcar(x[2], f.rs_w, f.rs_wr));
+        double a = rs_a_at(f, x);
         double V0 = f.rs_V0;
         vv_m = v[1] - u[1];
         vv_l = v[2] - u[2];
@@ -1507,7 +1604,8 @@ __device__ int hat_vars(const FaultParams &f, const double u[6], const double v[
         double Phi = sqrt(phi_1 * phi_1 + phi_2 * phi_2);
         double V = sqrt(vv_m * vv_m + vv_l * vv_l);
         if (V > Phi) V = 0.5 * Phi / eta_s;
-        V = regula_falsi(V, Phi, eta_s, sigma_n, psi, V0, a);
+        V = f.rs_solver == 1 ? newton_rs(V, Phi, eta_s, sigma_n, psi, V0, a)
+                             : regula_falsi(V, Phi, eta_s, sigma_n, psi, V0, a);
         if (V < 0.0) return 3;
         Vel = V;
         double fv = sigma_n * a * asinh(0.5 * V / V0 * exp(psi / a));
@@ -1584,7 +1682,7 @@ __device__ void bc_sat(const double u[9], double rho, double mu, double lambda, 
 // Interface_RHS; adds -tau0/h_q * bc to sat[9]
 template <typename real>
 __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams &fp, int me, int j, int k,
-                          double t, int stage, double A, double bdt, double sat[9], double tau0, int fq_read)
+                          double t, int snap, double A, double bdt, double sat[9], double tau0, int fq_read)
 {
     const Blk<real> &B1 = P.b[0], &B2 = P.b[1];
     const Blk<real> &B = P.b[me];
@@ -1619,7 +1717,7 @@ __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams
     double S0[4], DS0[4];
     for (int c = 0; c < 4; c++) { S0[c] = I.S[c * FP + fpi]; DS0[c] = I.DS[c * FP + fpi]; }
     const double psi = I.W[fpi], DW0 = I.DW[fpi];
-    const double trup0 = stage == 1 ? I.trup[fpi] : 0.0, psr0 = stage == 1 ? I.psr[fpi] : 0.0;
+    const double trup0 = snap ? I.trup[fpi] : 0.0, psr0 = snap ? I.psr[fpi] : 0.0;
     // tractions and rotation into (n, m, l) (CouplingForcing.f90:105-191, 470-524)
     double ur[6], vr[6];
     ur[0] = u[0]; ur[1] = u[1]; ur[2] = u[2];
@@ -1646,17 +1744,32 @@ __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams
     int rc = hat_vars(fp, uL, vL, mat1, mat2, x, nn, mm, ll, slip, psi, t, uh, vh, Vel, T0);
     if (rc) return rc;
     // fault state rates (Interface_Condition.f90:431-436, 470-476) and RK update
-    if (fp.law == 2) {
+    double slip_rate = sqrt((vh[1] - uh[1]) * (vh[1] - uh[1]) + (vh[2] - uh[2]) * (vh[2] - uh[2]));
+    {
         double dw = A * DW0;
-        dw = dw + fp.rs_b * fp.rs_V0 / fp.rs_L * exp(-(psi - fp.rs_f0) / fp.rs_b) - Vel * fp.rs_b / fp.rs_L;
-        I.DW[fpi] = dw;
-        I.W[fpi] = psi + bdt * dw;
-    } else {
-        double dw = A * DW0;
+        if (fp.law == 2) {
+            const double b = fp.rs_b, V0 = fp.rs_V0, L0 = fp.rs_L, f0 = fp.rs_f0;
+            if (fp.rs_law == 0) {  // aging law (431-436)
+                dw = dw + b * V0 / L0 * exp(-(psi - f0) / b) - Vel * b / L0;
+            } else {
+                const double a = rs_a_at(fp, x);
+                double Fss = f0 - (b - a) * log(Vel / V0);
+                if (fp.rs_law == 1) {  // slip law with flash weakening (438-444)
+                    if (fp.rs_Vw > 0.0) {
+                        const double Vw = fp.rs_Vw + fp.rs_dVw * (1.0 - smooth_boxcar(x[1], fp.rs_w, fp.rs_wr) *
+                                                                            smooth_boxcar(x[2], fp.rs_w, fp.rs_wr));
+                        const double q = Vel / Vw, q2 = q * q, q4 = q2 * q2, q8 = q4 * q4;
+                        Fss = fp.rs_fw + (Fss - fp.rs_fw) / pow(1.0 + q8, 0.125);
+                    }
+                    dw = dw - Vel / L0 * (a * asinh(Vel / (2.0 * V0) * exp(psi / a)) - Fss);
+                } else {  // KDN (446-449): slip_rate weighting, asinh without a
+                    dw = dw - (1.0 / L0) * slip_rate * (asinh(Vel / (2.0 * V0) * exp(psi / a)) - Fss);
+                }
+            }
+        }
         I.DW[fpi] = dw;
         I.W[fpi] = psi + bdt * dw;
     }
-    double slip_rate = sqrt((vh[1] - uh[1]) * (vh[1] - uh[1]) + (vh[2] - uh[2]) * (vh[2] - uh[2]));
     for (int c = 0; c < 4; c++) {
         double ds = A * DS0[c];
         ds = ds + (c < 3 ? vh[c] - uh[c] : slip_rate);
@@ -1664,7 +1777,7 @@ __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams
         I.S[c * FP + fpi] = S0[c] + bdt * ds;
     }
     if (!isfinite(slip_rate) || !isfinite(uh[4]) || !isfinite(uh[5])) return 4;
-    if (stage == 1) {  // 480-509, CouplingForcing.f90:213-228, 584-605
+    if (snap) {  // stage 1: 480-509, CouplingForcing.f90:213-228, 584-605
         // prestress of this node and time, as formed in hat_vars (zero for
         // the locked law)
         const double T0n = T0[0], T0m = T0[1], T0l = T0[2];
@@ -1773,7 +1886,8 @@ struct BoundaryArgs {
     int mode;
     void *satb[2];        // [9][nbnd] per block, in the stage's real type
     int stage;
-    int rates_only;       // 1: RHS evaluation without the RK update (no F_out / S / W write)
+    int rates_only;       // 1: RHS evaluation without the RK update (no F_out / S / W write, and no
+                          //    stage-1 snapshot of Svel / Uhat / Vhat / trup / peak slip rate)
     int fq_read;          // 1: read the stage input of q-face nodes from the blocks' q-face buffers
     int *flag;
 };
@@ -1874,7 +1988,10 @@ __global__ void __launch_bounds__(128) k_boundary(Pair<real> P, BoundaryArgs a)
     int on_fault = (me == 0) ? (i == nq - 1 && B.bc[1] == 0) : (i == 0 && B.bc[0] == 0);
     if (on_fault) {
         const double ts = a.tdev ? *a.tdev + a.tinc : a.t;
-        int rc = fault_node<real>(P, a.I[me], a.fault, me, j, k, ts, a.stage, A, a.rates_only ? 0.0 : a.bdt,
+        // the stage-1 snapshot (slip rate, hat variables, rupture time, peak
+        // slip rate) belongs to a step; a rates-only evaluation leaves it
+        const int snap = a.stage == 1 && !a.rates_only;
+        int rc = fault_node<real>(P, a.I[me], a.fault, me, j, k, ts, snap, A, a.rates_only ? 0.0 : a.bdt,
                                   sat, a.tau0, a.fq_read);
         if (rc) a.flag[0] = rc;
     }
@@ -1918,18 +2035,7 @@ __global__ void k_plastic(Pair<real> P, PlasticArgs a)
     const long p = pidx(B.px, B.nr, i, j, k);
     double s0[6];
     if (a.s0_kind == 1) {
-        double y = B.Y[p];
-        double Omega = 0.0;
-        if (y <= 15.0) Omega = 1.0;
-        else if (y >= 15.0 && y <= 20.0) Omega = (20.0 - y) / 5.0;
-        double b11 = 0.926793, b33 = 1.073206, b13 = -0.169029;
-        double Pf = 9.8 * y, s22 = -2.67 * 9.8 * y;
-        s0[0] = Omega * (b33 * (s22 + Pf) - Pf) + (1.0 - Omega) * s22 + Pf;
-        s0[1] = s22 + Pf;
-        s0[2] = Omega * (b11 * (s22 + Pf) - Pf) + (1.0 - Omega) * s22 + Pf;
-        s0[3] = 0.0;
-        s0[4] = Omega * (b13 * (s22 + Pf));
-        s0[5] = 0.0;
+        tpv26_stress(B.Y[p], s0);
     } else {
         for (int c = 0; c < 6; c++) s0[c] = a.s0[c];
     }
@@ -1942,7 +2048,7 @@ __global__ void k_plastic(Pair<real> P, PlasticArgs a)
     const double delta[6] = {1, 1, 1, 0, 0, 0};
     if (a.model == 1) {
         double Ys = fmax(0.0, a.c * cos(atan(a.nu)) - sigma * sin(atan(a.nu)));
-        if (tau - Ys <= 0.0) return;
+        if (tau - Ys <= 0.0) { if (B.P) B.P[p] = 0.0; return; }
         double r = exp(-a.dt / a.Tv) + (1.0 - exp(-a.dt / a.Tv)) * Ys / tau;
         for (int c = 0; c < 6; c++) sa[c] = sigma * delta[c] + r * (sa[c] - sigma * delta[c]);
         // plastic strain accumulator (FP64): equivalent plastic strain increment
@@ -1992,9 +2098,13 @@ __global__ void k_init_state(Pair<real> P, FaultParams f, Iface I0, Iface I1)
         n[c] = fr[c * FP + t]; m[c] = fr[(3 + c) * FP + t]; l[c] = fr[(6 + c) * FP + t];
         x[c] = B.fx[c * FP + t];
     }
+    if (f.rs_init == 1) {  // constant state ('rate-weakening', 2820-2850)
+        (me == 0 ? I0 : I1).W[t] = f.rs_W0;
+        return;
+    }
     double T0n, T0m, T0l;
     prestress(f, x, 0.0, l, m, n, T0n, T0m, T0l);
-    double a = f.rs_a + f.rs_da * (1.0 - smooth_boxcar(x[1], f.rs_w, f.rs_wr) * smooth_boxcar(x[2], f.rs_w, f.rs_wr));
+    double a = rs_a_at(f, x);
     double sn = -T0n;
     double theta = f.rs_L / f.rs_V0 *
                    exp(((a * log(2.0 * sinh(T0l / (a * sn)))) - f.rs_f0 - a * log(f.rs_Vin / f.rs_V0)) / f.rs_b);

@@ -33,6 +33,13 @@ module wql_gpu
      real(c_double) :: nucp(5)
      real(c_double) :: fs, fd, Dc, bar_wy, bar_wz, bar_big
      real(c_double) :: rs_a, rs_da, rs_w, rs_wr, rs_b, rs_L, rs_V0, rs_f0, rs_Vin
+     integer(c_int) :: rs_law = 0           ! 0 aging, 1 slip law + flash weakening, 2 KDN
+     real(c_double) :: rs_fw = 0, rs_Vw = 0, rs_dVw = 0
+     integer(c_int) :: rs_init = 0          ! 0 steady state from the prestress, 1 constant rs_W0
+     real(c_double) :: rs_W0 = 0
+     integer(c_int) :: sw_law = 0           ! 0 linear slip weakening, 1 TPV26/27
+     real(c_double) :: coh(3) = 0, frc(5) = 0
+     integer(c_int) :: rs_solver = 0        ! 0 regula falsi, 1 bracketed Newton
   end type wql_fault_params
 
   type, bind(C) :: wql_problem

@@ -22,7 +22,7 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_set_fault', 'wql_get_fault', 'wql_get_fault_rates', 'wql_eval_rates', 'wql_step',
            'wql_sync', 'wql_time', 'wql_get_fault_peak', 'wql_fault_stats', 'wql_server_create', 'wql_server_destroy',
            'wql_server_submit', 'wql_server_poll', 'wql_server_result', 'wql_server_status', 'wql_server_set_policy', 'wql_set_tile', 'wql_get_tile', 'wql_set_tile_order', 'wql_get_tile_order', 'wql_tile_variants', 'wql_use_graph', 'wql_profile',
-           'wql_volume_kernel_ms', 'wql_get_fault_faces']
+           'wql_volume_kernel_ms', 'wql_get_fault_faces', 'wql_get_plastic']
 
 
 class FaultParams(C.Structure):
@@ -32,7 +32,10 @@ class FaultParams(C.Structure):
                 ('bar_wy', C.c_double), ('bar_wz', C.c_double), ('bar_big', C.c_double),
                 ('rs_a', C.c_double), ('rs_da', C.c_double), ('rs_w', C.c_double), ('rs_wr', C.c_double),
                 ('rs_b', C.c_double), ('rs_L', C.c_double), ('rs_V0', C.c_double), ('rs_f0', C.c_double),
-                ('rs_Vin', C.c_double)]
+                ('rs_Vin', C.c_double),
+                ('rs_law', C.c_int), ('rs_fw', C.c_double), ('rs_Vw', C.c_double), ('rs_dVw', C.c_double),
+                ('rs_init', C.c_int), ('rs_W0', C.c_double), ('sw_law', C.c_int), ('coh', C.c_double * 3),
+                ('frc', C.c_double * 5), ('rs_solver', C.c_int)]
 
 
 class BlockDesc(C.Structure):
@@ -76,6 +79,7 @@ def lib():
             'wql_set_fault': ([vp, C.c_int, dp, dp], C.c_int),
             'wql_get_fault': ([vp, C.c_int, dp, dp, dp, dp, dp, dp], C.c_int),
             'wql_get_fault_rates': ([vp, C.c_int, dp, dp], C.c_int),
+            'wql_get_plastic': ([vp, C.c_int, dp], C.c_int),
             'wql_eval_rates': ([vp, C.c_double, C.c_int, C.c_int], C.c_int),
             'wql_step': ([vp, C.c_int], C.c_int),
             'wql_sync': ([vp], C.c_int),

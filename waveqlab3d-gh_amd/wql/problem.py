@@ -87,6 +87,16 @@ class Fault:
     rs_V0: float = 1.0
     rs_f0: float = 0.0
     rs_Vin: float = 0.0
+    rs_law: int = 0            # state evolution: 0 aging, 1 slip law + flash weakening, 2 KDN
+    rs_fw: float = 0.0         # flash-weakening residual friction
+    rs_Vw: float = 0.0         # flash-weakening velocity (<= 0: off) ...
+    rs_dVw: float = 0.0        # ... + rs_dVw (1 - B(y) B(z)) outside the velocity-weakening box
+    rs_init: int = 0           # initial state: 0 steady state from the prestress, 1 constant rs_W0
+    rs_W0: float = 0.0
+    sw_law: int = 0            # slip-weakening strength: 0 linear, 1 TPV26/27 cohesion + forced rupture
+    coh: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    frc: Tuple[float, ...] = (0.0, 0.0, 1.0, 1.0, 1.0)
+    rs_solver: int = 0         # 0 regula falsi (the reference's), 1 bracketed Newton
 
 
 def fault_params(problem: str, coupling: str) -> Fault:
@@ -103,6 +113,32 @@ def fault_params(problem: str, coupling: str) -> Fault:
             return Fault(law=SLIP_WEAKENING, stress_mode=1, s0=(-60.0, 0.0, 60.0, 0.0, 29.38, 0.0),
                          nuc=1, nucp=(7.5, 0.0, 1.4, 0.6, 11.6),
                          fs=0.677, fd=0.373, Dc=0.4, bar_wy=15.0, bar_wz=18.0, bar_big=1.0e10)
+        if problem in ('TPV26', 'TPV27'):
+            # 1376-1458 (strength with cohesion and forced rupture), 2008-2048
+            # (depth-dependent prestress, initial_stress_condition.f90:26-76)
+            return Fault(law=SLIP_WEAKENING, stress_mode=2, fs=0.18, fd=0.12, Dc=0.3,
+                         bar_wy=20.0, bar_wz=20.0, bar_big=1.0e10, sw_law=1,
+                         coh=(0.4, 0.72, 5.0), frc=(10.0, -5.0, 4.0, 3.464, 0.5))
+    if coupling == 'rate-and-state_friction' and problem == 'rate-weakening':
+        # 410-416 (a box), 438-444 (slip law with flash weakening),
+        # 2387-2420 (prestress + pulse), 2820-2850 (constant state),
+        # initial_stress_condition.f90:184-192 (tensor, with its pi constant)
+        pi = 3.14159265359
+        psi = 50.0 / 180.0 * pi
+        s1, s5 = -126.0, 42.0
+        s3 = (1.0 - 2.0 * s5 / (s1 * math.tan(2.0 * psi))) * s1
+        return Fault(law=RATE_STATE, stress_mode=1, s0=(s1, 0.5 * (s1 + s3), s3, 0.0, s5, 0.0),
+                     nuc=3, nucp=(8.0, 0.0, 3.0 * 1.5 * 0.3, -0.2429, 0.0),
+                     rs_a=0.016, rs_da=0.016, rs_w=20.0, rs_wr=2.5, rs_b=0.02, rs_L=2.0 * 0.2572,
+                     rs_V0=1.0e-6, rs_f0=0.7, rs_Vin=0.0, rs_law=1, rs_fw=0.13, rs_Vw=0.17, rs_dVw=5.0,
+                     rs_init=1, rs_W0=0.4367)
+    if coupling == 'rate-and-state_friction' and problem == 'KDN':
+        # the reference keys friction (402-408) and the state law (446-449)
+        # for 'KDN' but has no prestress or initial state for it: its run
+        # stops in prestress (2425-2427).  The law is reachable here through
+        # an explicit Fault(rs_law=2, ...).
+        raise ValueError("problem 'KDN' has no prestress in the reference (Interface_Condition.f90:2425-2427); "
+                         "give an explicit Fault(law=RATE_STATE, rs_law=2, ...) instead")
     if coupling == 'rate-and-state_friction' and problem in ('TPV101', 'TPV102'):
         # 386-400, 433-436, 2324-2385, 2756-2780, initial_stress_condition.f90:114-120
         return Fault(law=RATE_STATE, stress_mode=1, s0=(-120.0, 0.0, -120.0, 0.0, 75.0, 0.0),
@@ -202,6 +238,12 @@ def from_namelist(path):
                    cfl=float(pl.get('cfl', 0.5)), t_final=float(pl.get('t_final', 0.0)),
                    plastic=plastic, name=pl.get('name', 'default'), w_stride=int(pl.get('w_stride', 1)),
                    w_fault=bool(pl.get('w_fault', False)))
+    if plastic == 2:
+        # per-block mu_beta_eta (plastic_material.f90:38-44); one set for both blocks here
+        mbe = [tuple(float(v) for v in bl.get(f'btp({i})%mu_beta_eta', (0.0, 0.0, 0.0))) for i in (1, 2)]
+        if mbe[0] != mbe[1]:
+            raise ValueError('the two blocks give different mu_beta_eta; the library takes one set')
+        prob.pl_mu, prob.pl_beta, prob.pl_eta = mbe[0]
     if plastic:
         f = fault_params(problem, 'slip-weakening_friction') if problem == 'TPV28' else prob.fault
         prob.s0_kind = 1 if problem in ('TPV26', 'TPV27') else 0

@@ -27,8 +27,11 @@ def fault_struct(f):
     s.nuc = f.nuc
     s.nucp[:] = f.nucp
     for k in ('fs', 'fd', 'Dc', 'bar_wy', 'bar_wz', 'bar_big', 'rs_a', 'rs_da', 'rs_w', 'rs_wr',
-              'rs_b', 'rs_L', 'rs_V0', 'rs_f0', 'rs_Vin'):
+              'rs_b', 'rs_L', 'rs_V0', 'rs_f0', 'rs_Vin', 'rs_law', 'rs_fw', 'rs_Vw', 'rs_dVw', 'rs_init',
+              'rs_W0', 'sw_law', 'rs_solver'):
         setattr(s, k, getattr(f, k))
+    s.coh[:] = f.coh
+    s.frc[:] = f.frc
     return s
 
 
@@ -220,6 +223,13 @@ class Simulation:
         keys = ('max_slip', 'max_slip_rate', 'max_peak_slip_rate', 'ruptured_nodes', 'mean_slip_ruptured',
                 'first_rupture_time')
         return dict(zip(keys, out.tolist()))
+
+    def plastic(self, ib):
+        """plastic state [2][ns][nr][nq]: strain rate lambda of the last
+        update, accumulated plastic strain gammap (the reference's P%P)"""
+        P = np.empty(self._shape(ib, 2))
+        check(native.lib().wql_get_plastic(self.ctx, ib, _p(P)))
+        return P
 
     def fault_rates(self, ib):
         nq, nr, ns = self.prob.blocks[ib].n
