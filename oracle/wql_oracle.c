@@ -361,6 +361,8 @@ void ora_set_rates(ora_block *b)
     for (int s = 0; s < 3; s++) { cx[s] = c3[s] / hx; cy[s] = c3[s] / hy; cz[s] = c3[s] / hz; }
     const double *F = b->F, *G = b->metric, *M = b->M;
     double *DF = b->DF;
+    /* every node writes only its own rates (and PML auxiliary rates) */
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
@@ -716,6 +718,34 @@ static double regula_falsi(double V, double Phi, double eta, double sigma_n, dou
     return V;
 }
 
+/* the library's fast slip-rate mode (rs_solver 1, no counterpart in the
+ * reference): Newton's method on the same f(V) inside the bracket
+ * [0, Phi/eta], retried in ln V and then bisected when a step leaves the
+ * bracket; stops when a step moves V by at most 1e-13 V */
+static double newton_rs(double V, double Phi, double eta, double sigma_n, double psi, double V0, double a)
+{
+    double lo = 0.0, hi = Phi / eta;
+    if (!(hi > 0.0)) return 0.0;
+    if (!(V > lo && V < hi)) V = 0.5 * hi;
+    double e = exp(psi / a) / (2.0 * V0), c = a * sigma_n / eta, rhs = Phi / eta;
+    for (int k = 0; k < 100; k++) {
+        double z = V * e;
+        double fv = V + c * asinh(z) - rhs;
+        if (fv == 0.0) return V;
+        if (fv < 0.0) lo = V;
+        else hi = V;
+        double df = 1.0 + c * e / sqrt(1.0 + z * z);
+        double Vn = V - fv / df;
+        if (!(Vn > lo && Vn < hi)) {
+            Vn = V * exp(-fv / (V * df));
+            if (!(Vn > lo && Vn < hi)) Vn = lo > 0.0 ? sqrt(lo * hi) : 0.5 * hi;
+        }
+        if (fabs(Vn - V) <= 1.0e-13 * Vn) return Vn;
+        V = Vn;
+    }
+    return V;
+}
+
 static double rs_a(const ora_fault *f, const double x[3])
 { /* 386-400 */
     return f->rs_a + f->rs_da * (1.0 - smooth_boxcar(x[1], f->rs_w, f->rs_wr) * smooth_boxcar(x[2], f->rs_w, f->rs_wr));
@@ -785,7 +815,8 @@ int ora_hat(const ora_fault *f, const double u[6], const double v[6], const doub
         double Phi = sqrt(phi_1 * phi_1 + phi_2 * phi_2);
         double V = sqrt(vv_m * vv_m + vv_l * vv_l);
         if (V > Phi) V = 0.5 * Phi / eta_s;
-        V = regula_falsi(V, Phi, eta_s, sigma_n, psi, V0, a);
+        V = f->rs_solver == 1 ? newton_rs(V, Phi, eta_s, sigma_n, psi, V0, a)
+                              : regula_falsi(V, Phi, eta_s, sigma_n, psi, V0, a);
         if (V < 0.0) return -2;
         *Vel = V;
         double fv = sigma_n * a * asinh(0.5 * V / V0 * exp(psi / a));
@@ -824,6 +855,8 @@ int ora_interface(ora_domain *d, int ib, double t, int stage)
     size_t NP1 = NPTS(B1), NP2 = NPTS(B2), NP = NPTS(B);
     const double *nml = B->nml[side];
     double hx = 1.0 / (double)(B->nq - 1);
+    int err = 0;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < ns; k++)
         for (int j = 0; j < nr; j++) {
             size_t fp = (size_t)j + (size_t)nr * k;
@@ -866,7 +899,11 @@ int ora_interface(ora_domain *d, int ib, double t, int stage)
             double slip = I->S[3 * FP + fp];
             double psi = I->W[fp];
             int rc = ora_hat(&d->fault, uL, vL, mat1, mat2, x, frame, slip, psi, t, uh, vh, &Vel);
-            if (rc) return rc;
+            if (rc) {
+#pragma omp atomic write
+                err = rc;
+                continue;
+            }
             double slip_rate = sqrt(pow(vh[1] - uh[1], 2) + pow(vh[2] - uh[2], 2)); /* 425, 470 */
             if (d->fault.law == ORA_RATE_STATE) { /* 427-450 */
                 const ora_fault *f = &d->fault;
@@ -1008,7 +1045,7 @@ int ora_interface(ora_domain *d, int ib, double t, int stage)
             }
             for (int f = 0; f < 9; f++) B->DF[f * NP + pe] = B->DF[f * NP + pe] - TAU0 / hx * bc[f];
         }
-    return 0;
+    return err;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1042,6 +1079,7 @@ static void plastic_update(ora_domain *d, ora_block *b)
     if (b->pml[0]) ix = np + 1;
     if (b->pml[1]) fx = nx - np;
     const double delta[6] = {1, 1, 1, 0, 0, 0};
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = iz - 1; k < fz; k++)
         for (int j = iy - 1; j < fy; j++)
             for (int i = ix - 1; i < fx; i++) {
@@ -1166,6 +1204,7 @@ int ora_step(ora_domain *d, int nsteps)
             for (int ib = 0; ib < 2; ib++) {
                 ora_block *b = &d->b[ib];
                 size_t NP = NPTS(b);
+#pragma omp parallel for schedule(static)
                 for (size_t p = 0; p < 9 * NP; p++) b->DF[p] = a * b->DF[p];
                 for (int s = 0; s < 6; s++)
                     if (b->pml[s]) {
@@ -1189,6 +1228,7 @@ int ora_step(ora_domain *d, int nsteps)
             for (int ib = 0; ib < 2; ib++) {
                 ora_block *b = &d->b[ib];
                 size_t NP = NPTS(b);
+#pragma omp parallel for schedule(static)
                 for (size_t p = 0; p < 9 * NP; p++) b->F[p] = b->F[p] + bdt * b->DF[p];
                 for (int s = 0; s < 6; s++)
                     if (b->pml[s]) {

@@ -46,7 +46,7 @@ typedef struct {
     int sw_law;         /* slip-weakening strength: 0 linear, 1 TPV26/27 cohesion + forced rupture */
     double coh[3];      /* cohesion (C0 at depth, slope, depth of the ramp) */
     double frc[5];      /* forced rupture (yc, zc, r_crit, Vs, t0) */
-    int rs_solver;      /* ignored: the oracle always runs the reference's regula falsi */
+    int rs_solver;      /* 0 the reference's regula falsi, 1 the library's bracketed Newton */
 } ora_fault;
 
 typedef struct {

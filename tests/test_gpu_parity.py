@@ -291,7 +291,8 @@ def test_graph_replays_match_eager(case):
             sim.close()
 
 
-@pytest.mark.parametrize('case', ['curv_sw', 'curv_rs_long', 'pml_locked', 'plastic_sw'])
+@pytest.mark.parametrize('case', ['curv_sw', 'curv_rs_long', 'pml_locked', 'plastic_sw', 'rw_plastic', 'tpv26_sw',
+                                  'tpv26_plastic'])
 def test_fp32_volume_tracks_fp64(case):
     """FP32 volume fields with the FP64 fault stay close to the FP64 path:
     relative L2 of the fields and of the slip <= 1e-5 at the trajectory's end
@@ -511,3 +512,37 @@ def test_mixed_pair_vs_oracle(prec):
             f = sim.fault(ib)
             ref = od.blocks[ib]['iface']['S']
             assert np.abs(f['S'] - ref).max() <= tol * max(1.0, np.abs(ref).max()), ib
+
+
+@pytest.mark.parametrize('case', ['curv_rs', 'curv_rs_long', 'rw_plastic'])
+def test_newton_mode_vs_oracle(case):
+    """the fast slip-rate solve (rs_solver 1, bracketed Newton) against the
+    oracle's same solve at the FP64 bar, and against the reference build's
+    regula-falsi trajectory within 1e-6 during rupture (the reference's own
+    stopping rule leaves it ~3e-7 from the root)"""
+    _say(f'[gpu] newton {case}')
+    z = G.load(case)
+    prob = G.problem(case)
+    prob.fault.rs_solver = 1
+    args = z['traj_args']
+    od = O.OracleDomain(prob)
+    F0 = H.pulse_init(prob, args)
+    for ib in range(2):
+        od.blocks[ib]['F'][...] = F0[ib]
+    od.init_state()
+    od.step(int(args[0]))
+    with _sim(prob) as sim:
+        for ib in range(2):
+            sim.set_fields(ib, F0[ib])
+        sim.init_state()
+        sim.step(int(args[0]))
+        sim.sync()
+        for ib in range(2):
+            F = sim.fields(ib)
+            assert G.rel_l2(F, od.blocks[ib]['F']) <= TOL, f'F block {ib + 1}'
+            f = sim.fault(ib)
+            for key in ('S', 'W', 'Svel'):
+                ref = od.blocks[ib]['iface'][key]
+                assert np.abs(f[key] - ref).max() <= TOL * max(1.0, np.abs(ref).max()), key
+            if case != 'curv_rs':
+                assert G.rel_l2(F, z[f'b{ib + 1}_F']) <= 1e-6

@@ -130,3 +130,36 @@ def test_tpv26_background_stress_profile():
     assert np.allclose(d2, d2[0], rtol=1e-9, atol=1e-9)
     # the shallow tensor is anisotropic (the taper matters)
     assert abs(S[0, 0] - S[0, 1]) > 1e-3 and abs(S[0, 4]) > 1e-3
+
+
+def test_scec_accumulator_is_the_shear_stress_drop():
+    """model 1 ('scec', plastic_flow2): the reference keeps no plastic strain
+    for it; the library's FP64 accumulator grows by the drop of the
+    deviatoric stress invariant over the shear modulus, (tau_trial -
+    tau_returned) / G, recomputed here from the returned stress, and lambda
+    is that increment over dt; nodes below yield keep gammap and get
+    lambda = 0"""
+    prob, od = _domain(1)
+    rng = np.random.default_rng(5)
+    s0 = np.array(prob.fault.s0, dtype=float)
+    for ib in range(2):
+        F = od.blocks[ib]['F']
+        F[3:] = rng.normal(0.0, 40.0, F[3:].shape)
+        od.blocks[ib]['P'][1] = rng.uniform(0.0, 1.0, F.shape[1:])
+    trial = [od.blocks[ib]['F'][3:].copy() for ib in range(2)]
+    gam0 = [od.blocks[ib]['P'][1].copy() for ib in range(2)]
+    od.plastic()
+    dt = prob.dt()
+    nyield = 0
+    for ib in range(2):
+        Gm = od.blocks[ib]['M'][1]
+        tau_t = _inv(trial[ib] + s0[:, None, None, None])[0]
+        tau_n = _inv(od.blocks[ib]['F'][3:] + s0[:, None, None, None])[0]
+        lam, gam = od.blocks[ib]['P']
+        moved = tau_n < tau_t * (1 - 1e-15)
+        nyield += int(moved.sum())
+        dg = (tau_t - tau_n) / Gm
+        assert np.allclose(gam[moved] - gam0[ib][moved], dg[moved], rtol=1e-9, atol=0)
+        assert np.allclose(lam[moved] * dt, dg[moved], rtol=1e-9, atol=0)
+        assert np.array_equal(gam[~moved], gam0[ib][~moved]) and np.all(lam[~moved] == 0.0)
+    assert nyield > 100

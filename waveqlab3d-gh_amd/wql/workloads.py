@@ -8,10 +8,12 @@ physics and seeds; scale=1 gives the full workload.
            interface), Gaussian velocity pulse, free surface + absorbing SAT
   rupture  two curvilinear 201x101x101 blocks, planar vertical fault, linear
            slip weakening, Gaussian-random-field topography on the free surface
-  rough    two 401x201x201 blocks, self-similar rough fault, rate and state,
-           10-node PML on every outer side except the free surface
+  rough    two 401x201x201 blocks, self-similar rough fault, rate and state
+           (slip law, FP64 Newton solve), 10-node PML on every outer side
+           except the free surface
   plastic  two Cartesian 301x151x151 blocks, von Karman heterogeneous medium,
-           depth-dependent background stress, Drucker-Prager viscoplasticity
+           depth-dependent background stress on the fault and in the
+           Drucker-Prager viscoplasticity, TPV26-style slip weakening
   serving  per-scenario two-block problems drawn from {61^3, 121x61x61,
            201x101x101, 241x121x121} with seed-drawn friction
 """
@@ -158,10 +160,12 @@ def rough(scale=1.0, seed=3):
     fault *= np.sin(np.pi * y / Ly)[:, None] ** 2
     blocks, coords = _blocks(n, h, fault=fault, pml=True, npml=10)
     Lz = (n[2] - 1) * h
+    # TPV101 friction constants with the slip law (rs_law 1, flash weakening
+    # off) and the FP64 Newton slip-rate solve
     f = Fault(law=RATE_STATE, stress_mode=1, s0=(-120.0, 0.0, -120.0, 0.0, 75.0, 0.0), nuc=2,
               nucp=(Ly / 2, 0.0, min(3.0, 0.3 * Lz), 25.0, 1.0),
               rs_a=0.008, rs_da=0.008, rs_w=0.4 * Lz, rs_wr=0.1 * Lz, rs_b=0.012, rs_L=0.02,
-              rs_V0=1.0e-6, rs_f0=0.6, rs_Vin=1.0e-12)
+              rs_V0=1.0e-6, rs_f0=0.6, rs_Vin=1.0e-12, rs_law=1, rs_solver=1)
     prob = Problem(blocks=blocks, fault=f, cfl=0.5, name='rough')
     return dict(problem=prob, coords=coords, F0=None, steps=3000, precision='fp32', init_state=True)
 
@@ -184,8 +188,12 @@ def plastic(scale=1.0, seed=4):
         b.material = M
         b.mesh = 'cartesian'
     Ly, Lz = (n[1] - 1) * h, (n[2] - 1) * h
-    f = sw_fault(depth=Ly, width=Lz)
-    f.stress_mode = 0
+    # TPV26-style fault: the fault resolves the same depth-dependent stress
+    # tensor as the plastic background (stress_mode 2), TPV26 friction with
+    # cohesion, and forced rupture nucleating at the fault centre
+    f = Fault(law=SLIP_WEAKENING, stress_mode=2, fs=0.18, fd=0.12, Dc=0.3,
+              bar_wy=0.8 * Ly, bar_wz=0.4 * Lz, bar_big=1.0e10, sw_law=1, coh=(0.4, 0.72, 5.0),
+              frc=(0.5 * Ly, 0.0, min(4.0, 0.3 * min(Ly, Lz)), CS, 0.5))
     prob = Problem(blocks=blocks, fault=f, cfl=0.5, plastic=1, pl_c=1.0, pl_nu=0.75, pl_Tv=0.05,
                    s0_kind=1, name='plastic')
     # the time step uses the fastest material
