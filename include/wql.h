@@ -123,6 +123,14 @@ int wql_init_state(wql_ctx *ctx);
 int wql_set_fields(wql_ctx *ctx, int block, const double *F);
 int wql_get_fields(wql_ctx *ctx, int block, double *F);
 int wql_get_rates(wql_ctx *ctx, int block, double *DF);
+/* wql_get_fields without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  It stages through a fixed
+ * per-GPU device array of 256 MiB that the library holds until the process
+ * ends, so a block of at most 3,728,270 nodes (9 x 8-byte fields); larger
+ * blocks get WQL_E_ARG.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (from several threads) race. */
+int wql_get_fields_noalloc(wql_ctx *ctx, int block, double *F);
 /* PML auxiliary fields of one slab [9][slab nodes] (pml.f90:11-168) */
 int wql_set_pml(wql_ctx *ctx, int block, int side, const double *Q);
 int wql_get_pml(wql_ctx *ctx, int block, int side, double *Q);

@@ -8,12 +8,26 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace wql;
 
+// Staging memory of the allocation-free volume download
+// (wql_get_fields_noalloc): a file-scope device array of 256 MiB, of which
+// every GPU the library runs on holds its own copy until the process ends.
+// In host code `scratch` is only a symbol: its address on a device comes
+// from hipGetSymbolAddress, taken once per device (wql_create, with the
+// device current) and kept in g_scratch.
+constexpr size_t SCRATCH_DOUBLES = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
 namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[MAX_DEVICES] = {};
 
 thread_local std::string g_err;
 
@@ -132,6 +146,11 @@ struct wql_ctx {
     // persistent device staging of the small per-output downloads: both
     // fault faces [9][nr ns] each, and the six fault statistics
     double *d_faces = nullptr, *d_stats = nullptr;
+    // dense FP64 staging of the volume uploads/downloads, allocated on first
+    // use and grown to the largest transfer (freed by wql_destroy)
+    double *d_stage = nullptr;
+    size_t stage_elems = 0;
+    int dev = 0;            // device the context lives on
     size_t bytes = 0;
     std::vector<void *> allocs;
     int profile = 0;
@@ -768,19 +787,49 @@ int put_field(wql_ctx *c, void *dev, const double *host, long n)
 // volume arrays between the dense host layout [nf][ns][nr][nq] (FP64) and
 // the padded device layout [nf][ns][nr][px]: one contiguous copy through a
 // dense FP64 staging buffer and a (un)padding kernel on the context's stream
+// the context's dense staging buffer, at least n doubles (allocated once
+// and grown; a transfer then costs no hipMalloc / hipFree, which would
+// synchronise the device)
+int staging(wql_ctx *c, size_t n, double **out)
+{
+    if (c->stage_elems < n) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->d_stage) {
+            hipFree(c->d_stage);
+            c->bytes -= c->stage_elems * sizeof(double);
+            c->d_stage = nullptr;
+            c->stage_elems = 0;
+        }
+        if (hipMalloc(&c->d_stage, n * sizeof(double)) != hipSuccess) {
+            c->d_stage = nullptr;
+            return fail(WQL_E_NOMEM, "hipMalloc(staging, %zu bytes)", n * sizeof(double));
+        }
+        c->stage_elems = n;
+        c->bytes += n * sizeof(double);
+    }
+    *out = c->d_stage;
+    return 0;
+}
+
+// padded device volume -> dense FP64 device buffer -> host
 template <typename real>
-int copy_vol_t(wql_ctx *c, const DevBlock &d, const void *dev, double *host, int nf)
+int download_vol(wql_ctx *c, const DevBlock &d, const void *dev, double *host, int nf, double *stage)
 {
     const long n = (long)nf * d.ND;
-    double *stage = nullptr;
-    if (hipMalloc(&stage, n * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(staging)");
     k_unpad<real><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(nf, d.nq, d.nr, d.ns, d.px, (const real *)dev,
                                                                        stage);
     hipError_t e = hipMemcpyAsync(host, stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(stage);
     if (e != hipSuccess) return fail(WQL_E_HIP, "volume download: %s", hipGetErrorString(e));
     return check_launch();
+}
+
+template <typename real>
+int copy_vol_t(wql_ctx *c, const DevBlock &d, const void *dev, double *host, int nf)
+{
+    double *stage = nullptr;
+    if (int rc = staging(c, (size_t)nf * d.ND, &stage)) return rc;
+    return download_vol<real>(c, d, dev, host, nf, stage);
 }
 
 template <typename real>
@@ -788,7 +837,7 @@ int put_vol_t(wql_ctx *c, const DevBlock &d, void *dev, const double *host, int 
 {
     const long n = (long)nf * d.ND;
     double *stage = nullptr;
-    if (hipMalloc(&stage, n * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(staging)");
+    if (int rc = staging(c, (size_t)n, &stage)) return rc;
     hipError_t e = hipMemcpyAsync(stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
         const long np = (long)nf * d.nr * d.ns * d.px;
@@ -796,7 +845,6 @@ int put_vol_t(wql_ctx *c, const DevBlock &d, void *dev, const double *host, int 
                                                                          (real *)dev);
         e = hipStreamSynchronize(c->stream);
     }
-    hipFree(stage);
     if (e != hipSuccess) return fail(WQL_E_HIP, "volume upload: %s", hipGetErrorString(e));
     return check_launch();
 }
@@ -809,6 +857,26 @@ int copy_vol(wql_ctx *c, const DevBlock &d, const void *dev, double *host, int n
 int put_vol(wql_ctx *c, const DevBlock &d, void *dev, const double *host, int nf)
 {
     return c->prec == WQL_FP64 ? put_vol_t<double>(c, d, dev, host, nf) : put_vol_t<float>(c, d, dev, host, nf);
+}
+
+// device address of `scratch` on device dev (the current device), once
+int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(WQL_E_ARG, "device %d beyond %d", dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[dev]) return 0;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(WQL_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[dev] = (double *)p;
+    return 0;
+}
+
+double *scratch_ptr(int dev)
+{
+    if (dev < 0 || dev >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch[dev];
 }
 
 int valid_block(const wql_block_desc &b)
@@ -895,6 +963,14 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
     }
     wql_ctx *c = new wql_ctx();
     c->prec = prob->precision;
+    if (hipGetDevice(&c->dev) != hipSuccess) {
+        delete c;
+        return fail(WQL_E_HIP, "hipGetDevice");
+    }
+    if (int rc = scratch_setup(c->dev)) {
+        delete c;
+        return rc;
+    }
     // fused single-pass march on 32x8 tiles (measured fastest on MI355X for
     // both precisions): FP32 re-reads the centre plane at three waves per
     // SIMD with 6-plane s chunks; FP64 holds the lead plane in registers,
@@ -958,6 +1034,7 @@ int wql_destroy(wql_ctx *c)
     for (int k = 0; k < 2; k++)
         if (c->gexec[k]) hipGraphExecDestroy(c->gexec[k]);
     for (void *p : c->allocs) hipFree(p);
+    if (c->d_stage) hipFree(c->d_stage);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1009,6 +1086,20 @@ int wql_get_fields(wql_ctx *c, int ib, double *F)
 {
     if (!c || ib < 0 || ib > 1 || !F) return fail(WQL_E_ARG, "bad argument");
     return copy_vol(c, c->b[ib], c->b[ib].F[c->cur], F, 9);
+}
+
+int wql_get_fields_noalloc(wql_ctx *c, int ib, double *F)
+{
+    if (!c || ib < 0 || ib > 1 || !F) return fail(WQL_E_ARG, "bad argument");
+    const DevBlock &d = c->b[ib];
+    const size_t n = 9 * (size_t)d.ND;
+    if (n > SCRATCH_DOUBLES)
+        return fail(WQL_E_ARG, "block %d: %zu bytes of fields exceed the %zu bytes of the allocation-free staging",
+                    ib, n * sizeof(double), SCRATCH_DOUBLES * sizeof(double));
+    double *stage = scratch_ptr(c->dev);
+    if (!stage) return fail(WQL_E_STATE, "no staging address for device %d", c->dev);
+    return c->prec == WQL_FP64 ? download_vol<double>(c, d, d.F[c->cur], F, 9, stage)
+                               : download_vol<float>(c, d, d.F[c->cur], F, 9, stage);
 }
 
 int wql_get_rates(wql_ctx *c, int ib, double *DF)

@@ -22,7 +22,8 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_set_fault', 'wql_get_fault', 'wql_get_fault_rates', 'wql_eval_rates', 'wql_step',
            'wql_sync', 'wql_time', 'wql_get_fault_peak', 'wql_fault_stats', 'wql_server_create', 'wql_server_destroy',
            'wql_server_submit', 'wql_server_poll', 'wql_server_result', 'wql_server_status', 'wql_server_set_policy', 'wql_set_tile', 'wql_get_tile', 'wql_set_tile_order', 'wql_get_tile_order', 'wql_tile_variants', 'wql_use_graph', 'wql_profile',
-           'wql_volume_kernel_ms', 'wql_get_fault_faces', 'wql_get_plastic']
+           'wql_volume_kernel_ms', 'wql_get_fault_faces', 'wql_get_plastic',
+           'wql_get_fields_noalloc']
 
 
 class FaultParams(C.Structure):
@@ -80,6 +81,7 @@ def lib():
             'wql_get_fault': ([vp, C.c_int, dp, dp, dp, dp, dp, dp], C.c_int),
             'wql_get_fault_rates': ([vp, C.c_int, dp, dp], C.c_int),
             'wql_get_plastic': ([vp, C.c_int, dp], C.c_int),
+            'wql_get_fields_noalloc': ([vp, C.c_int, dp], C.c_int),
             'wql_eval_rates': ([vp, C.c_double, C.c_int, C.c_int], C.c_int),
             'wql_step': ([vp, C.c_int], C.c_int),
             'wql_sync': ([vp], C.c_int),
