@@ -191,8 +191,9 @@ def serving(n_scen, steps, rate, max_active, n_seq, seed=5):
                      'scenarios_per_s': round(n_scen / wall_f, 4)}}
 
 
-def cpu_baseline(W):
-    """the oracle (C, 1 thread) on one LSRK step of a sub-volume of the same workload"""
+def cpu_baseline(W, min_seconds=12.0, max_steps=40):
+    """the oracle (C, OpenMP over the host's cores) on full LSRK steps of the
+    same workload: steps until at least min_seconds of wall time"""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle as O
     from wql import workloads
@@ -200,14 +201,23 @@ def cpu_baseline(W):
     sub = workloads.WORKLOADS.get(name, workloads.rupture)(scale=1.0)
     prob = sub['problem']
     od = O.OracleDomain(prob, coords=sub['coords'])
-    nsteps = 3
+    if sub.get('F0') is not None:
+        for ib in range(2):
+            od.blocks[ib]['F'][...] = sub['F0'][ib]
+    if sub.get('init_state'):
+        od.init_state()
+    od.step(1)  # first touch of the arrays outside the timing
+    nsteps = 0
     t0 = time.perf_counter()
-    od.step(nsteps)
+    while nsteps < max_steps and (time.perf_counter() - t0) < min_seconds:
+        od.step(1)
+        nsteps += 1
     sec = (time.perf_counter() - t0) / nsteps
     nodes = workloads.points(prob)
-    return dict(value=nodes * 5 / sec / 1e6, unit='M grid-pt RK stages/s', cores=1, kind='port',
-                sample=f'oracle C restatement (1 thread), {nsteps} LSRK steps of the full {name} workload '
-                       f'({"x".join(str(v) for v in prob.blocks[0].n)} per block, {nodes} nodes), '
+    threads = int(os.environ.get('OMP_NUM_THREADS', '0') or 0) or os.cpu_count()
+    return dict(value=nodes * 5 / sec / 1e6, unit='M grid-pt RK stages/s', cores=threads, kind='port',
+                sample=f'oracle C restatement (OpenMP, {threads} threads), {nsteps} LSRK steps of the full {name} '
+                       f'workload ({"x".join(str(v) for v in prob.blocks[0].n)} per block, {nodes} nodes), '
                        f'{sec * nsteps:.1f} s')
 
 
@@ -224,7 +234,9 @@ def main():
     ap.add_argument('--graph', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
-    ap.add_argument('--err-steps', type=int, default=300, help='steps of the FP32-vs-FP64 slip comparison (0: off)')
+    ap.add_argument('--err-steps', type=int, default=None,
+                    help="steps of the FP32-vs-FP64 slip comparison (default: the workload's own step count, "
+                         "1500 for the rupture workload; 0: off)")
     ap.add_argument('--serve', type=int, default=64, help='scenarios in the serving stream (0: off)')
     ap.add_argument('--serve-steps', type=int, default=100)
     ap.add_argument('--serve-rate', type=float, default=20.0)
@@ -279,8 +291,9 @@ def main():
                        'ms_per_step': round(res['fp64']['sec'] / a.steps * 1e3, 4),
                        'kernel_ms': round(res['fp64']['kms'], 5),
                        'speedup_fp32_over_fp64': round(r32['mpts'] / res['fp64']['mpts'], 3)}
-    if a.err_steps > 0 and not a.no_fp64:
-        out['fp32_error'] = fp32_error(W, a.err_steps)
+    err_steps = W['steps'] if a.err_steps is None else a.err_steps
+    if err_steps > 0 and not a.no_fp64:
+        out['fp32_error'] = fp32_error(W, err_steps)
     if a.serve > 0:
         out['serving'] = serving(a.serve, a.serve_steps, a.serve_rate, a.serve_active, min(a.serve_seq, a.serve))
     if not a.no_cpu:

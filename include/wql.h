@@ -99,7 +99,39 @@ typedef struct {
     wql_fault_params fault;
 } wql_problem;
 
+/* The fault law, prestress and plasticity settings the reference keys by
+ * name: &problem_list's problem, coupling, response and plastic_model, and
+ * btp%mu_beta_eta (Interface_Condition.f90:350-450, 1268-2476, 2722-2850;
+ * initial_stress_condition.f90; plastic.f90:174-182; plastic_material.f90:
+ * 38-44) for TPV5, TPV28, TPV26/27 (slip weakening), TPV101/102 and
+ * 'rate-weakening' (rate and state), 'locked' and the 'scec' / default
+ * plasticity models.  Fills out->fault and the plastic fields; keeps
+ * out->precision, dt and t0.  WQL_E_ARG for keys outside that set (among
+ * them 'KDN', which the reference cannot run: it has no prestress). */
+int wql_problem_keys(const char *problem, const char *coupling, const char *response, const char *plastic_model,
+                     const double mu_beta_eta[3], wql_problem *out);
+
 typedef struct wql_ctx wql_ctx;
+
+/* Node coordinates of one block, generated on the device: the uniform grid
+ * of cartesian_grid_3d (grid.f90:213-240) or the transfinite interpolation
+ * of Curve_Grid_3D3 (grid.f90:471-1907; faces Gen_Surf_Mesh 2418-2455,
+ * volume Gen3_Curve_Mesh 2290-2416) with planar q faces or the
+ * 'analytical_test_problem' bump (775-831), plus optional offset fields of
+ * one q face (a fault surface) and of the r = 0 face (topography), the
+ * surfaces the reference reads from files through the same interpolation. */
+typedef struct {
+    int nq, nr, ns;
+    double a[3], b[3];     /* btp%aqrs, btp%bqrs: corners of the block */
+    int mesh;              /* 0 uniform ('cartesian'), 1 transfinite ('curvilinear') */
+    int profile;           /* q faces: 0 planar, 1 'analytical_test_problem' weighted by lc / rc */
+    double lc, rc;         /* btp%lc, btp%rc */
+    const double *fault_x; /* optional [ns][nr] x offsets of the q face fault_side (mesh 1, profile 0) */
+    int fault_side;        /* 0: q = 0 face, 1: q = nq-1 face */
+    const double *top_y;   /* optional [ns][nq] y offsets of the r = 0 face (mesh 1) */
+} wql_grid_desc;
+/* X[3][ns][nr][nq] (the reference's X(q,r,s,1:3)) */
+int wql_grid_block(const wql_grid_desc *grid, double *X);
 
 /* replaces init_domain's device-relevant part (domain.f90:31-564): uploads
  * geometry/material, computes metrics and face frames on the device, zeroes
@@ -177,6 +209,13 @@ int wql_eval_rates(wql_ctx *ctx, double t, int what, int stage);
 /* nsteps LSRK(5,4) steps (time_step.f90:68-150); enqueued on the context's
  * stream, returns without waiting. */
 int wql_step(wql_ctx *ctx, int nsteps);
+/* one RK stage of the current step (the body of time_step_RK's stage loop,
+ * time_step.f90:109-147), for stage-by-stage comparison: stages 1..5 in
+ * order (WQL_E_STATE otherwise); after stage 5 the step is complete and
+ * wql_time advances by dt.  Between stages wql_get_fields / wql_get_rates
+ * return the stage's output and rates.  wql_step refuses to start while a
+ * step is part done. */
+int wql_stage(wql_ctx *ctx, int stage);
 /* wait for the context's stream; reports NaN/Inf found in the fault state */
 int wql_sync(wql_ctx *ctx);
 double wql_time(const wql_ctx *ctx);

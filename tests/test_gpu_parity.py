@@ -546,3 +546,34 @@ def test_newton_mode_vs_oracle(case):
                 assert np.abs(f[key] - ref).max() <= TOL * max(1.0, np.abs(ref).max()), key
             if case != 'curv_rs':
                 assert G.rel_l2(F, z[f'b{ib + 1}_F']) <= 1e-6
+
+
+def test_stage_by_stage_equals_step():
+    """wql_stage(1..5) reproduces wql_step bit for bit; stages out of order
+    and a step started while one is part done are refused"""
+    from wql import native
+    _say('[gpu] stage by stage')
+    z = G.load('curv_rs_long')
+    prob = G.problem('curv_rs_long')
+    F0 = H.pulse_init(prob, z['traj_args'])
+    L = native.lib()
+    with _sim(prob) as a, _sim(prob) as b:
+        for sim in (a, b):
+            for ib in range(2):
+                sim.set_fields(ib, F0[ib])
+            sim.init_state()
+        a.step(3)
+        a.sync()
+        assert L.wql_stage(b.ctx, 2) == native.WQL_E_STATE
+        for _ in range(3):
+            for s in range(1, 6):
+                b.stage(s)
+                if s == 2:
+                    assert L.wql_step(b.ctx, 1) == native.WQL_E_STATE
+        b.sync()
+        assert a.t == b.t
+        for ib in range(2):
+            assert np.array_equal(a.fields(ib), b.fields(ib)), ib
+            fa, fb = a.fault(ib), b.fault(ib)
+            for key in ('S', 'W', 'Svel', 'trup'):
+                assert np.array_equal(fa[key], fb[key]), key

@@ -2,6 +2,7 @@
 // lifetime, uploads/downloads, and the per-stage launch sequence.
 #include "wql.h"
 #include "wql_kernels.hip"
+#include "wql_grid.hip"
 
 #include <algorithm>
 #include <cmath>
@@ -140,6 +141,7 @@ struct wql_ctx {
     // problem)
     int variant = 0, zchunk = 6, order = 0, group = 2;
     int use_graph = 0;
+    int stage_next = 1;     // next stage of a step run stage by stage (wql_stage)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int *d_flag = nullptr;
     double *d_t = nullptr;
@@ -1229,6 +1231,169 @@ int wql_get_fault_rates(wql_ctx *c, int ib, double *DS, double *DW)
     return 0;
 }
 
+int wql_problem_keys(const char *problem, const char *coupling, const char *response, const char *plastic_model,
+                     const double mu_beta_eta[3], wql_problem *out)
+{
+    if (!problem || !coupling || !out) return fail(WQL_E_ARG, "null argument");
+    const std::string pk = problem, ck = coupling, rk = response ? response : "elastic",
+                      mk = plastic_model ? plastic_model : "default";
+    wql_problem p;
+    memset(&p, 0, sizeof p);
+    p.precision = out->precision;
+    p.dt = out->dt;
+    p.t0 = out->t0;
+    // plasticity constants of plastic_flow2 (plastic.f90:174-182)
+    p.pl_c = 1.36; p.pl_nu = 0.1934; p.pl_Tv = 0.03;
+    wql_fault_params &f = p.fault;
+    f.Dc = 1.0; f.bar_wy = 1e30; f.bar_wz = 1e30; f.rs_wr = 1.0; f.rs_L = 1.0; f.rs_V0 = 1.0;
+    f.frc[2] = f.frc[3] = f.frc[4] = 1.0;
+    auto patch = [&](int k, double yc, double zc, double wy, double wz, double amp) {
+        f.patch[k][0] = yc; f.patch[k][1] = zc; f.patch[k][2] = wy; f.patch[k][3] = wz; f.patch[k][4] = amp;
+    };
+    if (ck == "locked") {
+        f.law = WQL_LOCKED;
+    } else if (ck == "slip-weakening_friction" && pk == "TPV5") {  // Interface_Condition.f90:1283-1299, 1822-1830
+        f.law = WQL_SLIP_WEAKENING;
+        f.stress_mode = 0;
+        f.T0[0] = -120.0; f.T0[1] = 0.0; f.T0[2] = 70.0;
+        f.npatch = 3;
+        patch(0, 7.5, -7.5, 1.5, 1.5, 8.0);
+        patch(1, 7.5, 0.0, 1.5, 1.5, 11.6);
+        patch(2, 7.5, 7.5, 1.5, 1.5, -8.0);
+        f.fs = 0.677; f.fd = 0.525; f.Dc = 0.4;
+        f.bar_wy = 15.0; f.bar_wz = 15.0; f.bar_big = 1.0e10;
+    } else if (ck == "slip-weakening_friction" && pk == "TPV28") {  // 1459-1474, 2049-2079
+        f.law = WQL_SLIP_WEAKENING;
+        f.stress_mode = 1;
+        const double s0[6] = {-60.0, 0.0, 60.0, 0.0, 29.38, 0.0};  // initial_stress_condition.f90:78-84
+        memcpy(f.s0, s0, sizeof s0);
+        f.nuc = 1;
+        const double np[5] = {7.5, 0.0, 1.4, 0.6, 11.6};
+        memcpy(f.nucp, np, sizeof np);
+        f.fs = 0.677; f.fd = 0.373; f.Dc = 0.4;
+        f.bar_wy = 15.0; f.bar_wz = 18.0; f.bar_big = 1.0e10;
+    } else if (ck == "slip-weakening_friction" && (pk == "TPV26" || pk == "TPV27")) {  // 1376-1458, 2008-2048
+        f.law = WQL_SLIP_WEAKENING;
+        f.stress_mode = 2;
+        f.fs = 0.18; f.fd = 0.12; f.Dc = 0.3;
+        f.bar_wy = 20.0; f.bar_wz = 20.0; f.bar_big = 1.0e10;
+        f.sw_law = 1;
+        f.coh[0] = 0.4; f.coh[1] = 0.72; f.coh[2] = 5.0;
+        f.frc[0] = 10.0; f.frc[1] = -5.0; f.frc[2] = 4.0; f.frc[3] = 3.464; f.frc[4] = 0.5;
+    } else if (ck == "rate-and-state_friction" && (pk == "TPV101" || pk == "TPV102")) {
+        // 386-400, 433-436, 2324-2385, 2756-2819, initial_stress_condition.f90:114-120
+        f.law = WQL_RATE_STATE;
+        f.stress_mode = 1;
+        const double s0[6] = {-120.0, 0.0, -120.0, 0.0, 75.0, 0.0};
+        memcpy(f.s0, s0, sizeof s0);
+        f.nuc = 2;
+        const double np[5] = {7.5, 0.0, 3.0, 25.0, 1.0};
+        memcpy(f.nucp, np, sizeof np);
+        f.rs_a = 0.008; f.rs_da = 0.008; f.rs_w = 15.0; f.rs_wr = 3.0; f.rs_b = 0.012; f.rs_L = 0.02;
+        f.rs_V0 = 1.0e-6; f.rs_f0 = 0.6; f.rs_Vin = 1.0e-12;
+    } else if (ck == "rate-and-state_friction" && pk == "rate-weakening") {
+        // 410-416, 438-444, 2387-2420, 2820-2850, initial_stress_condition.f90:184-192 (with its pi constant)
+        f.law = WQL_RATE_STATE;
+        f.stress_mode = 1;
+        const double pi = 3.14159265359, psi = 50.0 / 180.0 * pi;
+        const double s1 = -126.0, s5 = 42.0;
+        const double s3 = (1.0 - 2.0 * s5 / (s1 * tan(2.0 * psi))) * s1;
+        f.s0[0] = s1; f.s0[1] = 0.5 * (s1 + s3); f.s0[2] = s3; f.s0[3] = 0.0; f.s0[4] = s5; f.s0[5] = 0.0;
+        f.nuc = 3;
+        f.nucp[0] = 8.0; f.nucp[1] = 0.0; f.nucp[2] = 3.0 * 1.5 * 0.3; f.nucp[3] = -0.2429; f.nucp[4] = 0.0;
+        f.rs_a = 0.016; f.rs_da = 0.016; f.rs_w = 20.0; f.rs_wr = 2.5; f.rs_b = 0.02; f.rs_L = 2.0 * 0.2572;
+        f.rs_V0 = 1.0e-6; f.rs_f0 = 0.7; f.rs_Vin = 0.0;
+        f.rs_law = 1; f.rs_fw = 0.13; f.rs_Vw = 0.17; f.rs_dVw = 5.0;
+        f.rs_init = 1; f.rs_W0 = 0.4367;
+    } else if (ck == "rate-and-state_friction" && pk == "KDN") {
+        return fail(WQL_E_ARG, "problem 'KDN' has no prestress in the reference (Interface_Condition.f90:2425-2427); "
+                               "describe the fault explicitly (law WQL_RATE_STATE, rs_law 2)");
+    } else {
+        return fail(WQL_E_ARG, "unsupported problem/coupling: %s/%s", problem, coupling);
+    }
+    if (rk == "plastic") {
+        p.plastic = mk == "scec" ? 1 : 2;
+        // the plastic background stress is the problem's initial stress
+        // tensor whatever the coupling (initial_stress_condition.f90:8-254)
+        if (pk == "TPV26" || pk == "TPV27") {
+            p.s0_kind = 1;
+        } else if (pk == "TPV28") {
+            const double s0[6] = {-60.0, 0.0, 60.0, 0.0, 29.38, 0.0};
+            memcpy(f.s0, s0, sizeof s0);
+        } else if (pk == "TPV101" || pk == "TPV102") {
+            const double s0[6] = {-120.0, 0.0, -120.0, 0.0, 75.0, 0.0};
+            memcpy(f.s0, s0, sizeof s0);
+        } else if (pk == "rate-weakening") {
+            const double pi = 3.14159265359, psi = 50.0 / 180.0 * pi;
+            const double s1 = -126.0, s5 = 42.0;
+            const double s3 = (1.0 - 2.0 * s5 / (s1 * tan(2.0 * psi))) * s1;
+            f.s0[0] = s1; f.s0[1] = 0.5 * (s1 + s3); f.s0[2] = s3; f.s0[3] = 0.0; f.s0[4] = s5; f.s0[5] = 0.0;
+        } else {
+            return fail(WQL_E_ARG, "no initial stress tensor for problem '%s' (initial_stress_condition.f90:250)",
+                        problem);
+        }
+        if (pk == "TPV30") { p.pl_c = 1.18; p.pl_nu = 0.1680; p.pl_Tv = 0.05; }
+        if (p.plastic == 2 && mu_beta_eta) {  // plastic_material.f90:38-44
+            p.pl_mu = mu_beta_eta[0]; p.pl_beta = mu_beta_eta[1]; p.pl_eta = mu_beta_eta[2];
+        }
+    } else if (rk != "elastic") {
+        return fail(WQL_E_ARG, "unsupported response '%s'", rk.c_str());
+    }
+    *out = p;
+    return 0;
+}
+
+int wql_grid_block(const wql_grid_desc *gd, double *X)
+{
+    if (!gd || !X) return fail(WQL_E_ARG, "null argument");
+    if (gd->nq < 2 || gd->nr < 2 || gd->ns < 2) return fail(WQL_E_ARG, "a grid needs two nodes per direction");
+    if (gd->mesh < 0 || gd->mesh > 1 || gd->profile < 0 || gd->profile > 1 || gd->fault_side < 0 ||
+        gd->fault_side > 1)
+        return fail(WQL_E_ARG, "bad grid description");
+    const long N = (long)gd->nq * gd->nr * gd->ns;
+    const long FQ = (long)gd->nr * gd->ns, FS = (long)gd->nq * gd->nr, FR = (long)gd->nq * gd->ns;
+    const size_t work = (size_t)(3 * N + 4 * FQ + 4 * FS + 4 * FR + FQ + FR);
+    double *d = nullptr;
+    if (hipMalloc(&d, work * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(grid)");
+    GridArgs g;
+    memset(&g, 0, sizeof g);
+    g.nq = gd->nq; g.nr = gd->nr; g.ns = gd->ns;
+    for (int k = 0; k < 3; k++) { g.a[k] = gd->a[k]; g.b[k] = gd->b[k]; }
+    g.lc = gd->lc; g.rc = gd->rc;
+    g.profile = gd->profile;
+    g.fault_side = gd->fault_side;
+    g.X = d;
+    g.qf = d + 3 * N;
+    g.sf = g.qf + 4 * FQ;
+    g.rf = g.sf + 4 * FS;
+    double *fx = g.rf + 4 * FR, *ty = fx + FQ;
+    hipError_t e = hipSuccess;
+    if (gd->mesh == 1 && gd->fault_x) {
+        e = hipMemcpy(fx, gd->fault_x, FQ * sizeof(double), hipMemcpyHostToDevice);
+        g.fault_x = fx;
+    }
+    if (e == hipSuccess && gd->mesh == 1 && gd->top_y) {
+        e = hipMemcpy(ty, gd->top_y, FR * sizeof(double), hipMemcpyHostToDevice);
+        g.top_y = ty;
+    }
+    if (e == hipSuccess) {
+        if (gd->mesh == 0) {
+            k_grid_cartesian<<<(unsigned)((3 * N + 255) / 256), 256>>>(g.nq, g.nr, g.ns, g.a[0], g.a[1], g.a[2], g.b[0],
+                                                                        g.b[1], g.b[2], g.X);
+        } else {
+            k_grid_qfaces<<<(unsigned)((FQ + 255) / 256), 256>>>(g);
+            k_grid_sfaces<<<(unsigned)((2 * FS + 255) / 256), 256>>>(g);
+            k_grid_rfaces<<<(unsigned)((2 * FR + 255) / 256), 256>>>(g);
+            k_grid_volume<<<(unsigned)((3 * N + 255) / 256), 256>>>(g);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(X, g.X, 3 * N * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(WQL_E_HIP, "grid: %s", hipGetErrorString(e));
+    return 0;
+}
+
 int wql_get_plastic(wql_ctx *c, int ib, double *P)
 {
     if (!c || ib < 0 || ib > 1 || !P) return fail(WQL_E_ARG, "bad argument");
@@ -1294,9 +1459,34 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
     return check_launch();
 }
 
+int wql_stage(wql_ctx *c, int stage)
+{
+    if (!c || stage < 1 || stage > 5) return fail(WQL_E_ARG, "bad argument");
+    if (stage != c->stage_next)
+        return fail(WQL_E_STATE, "RK stage %d requested, stage %d of the step is next", stage, c->stage_next);
+    const int s = stage - 1;
+    if (c->prec == WQL_FP64) {
+        gather_fq<double>(c);
+        launch_stage<double>(c, c->cur, RK_A[s], RK_B[s] * c->dt, c->t + RK_C[s] * c->dt, stage, 0, stage == 5);
+    } else {
+        gather_fq<float>(c);
+        launch_stage<float>(c, c->cur, RK_A[s], RK_B[s] * c->dt, c->t + RK_C[s] * c->dt, stage, 0, stage == 5);
+    }
+    c->cur = 1 - c->cur;
+    if (stage == 5) {
+        c->t = c->t + RK_C[5] * c->dt;
+        c->stage_next = 1;
+    } else {
+        c->stage_next = stage + 1;
+    }
+    return check_launch();
+}
+
 int wql_step(wql_ctx *c, int nsteps)
 {
     if (!c || nsteps < 0) return fail(WQL_E_ARG, "bad argument");
+    if (c->stage_next != 1)
+        return fail(WQL_E_STATE, "a step run by wql_stage is at stage %d; finish it first", c->stage_next);
     if (nsteps > 0) {
         if (c->prec == WQL_FP64) gather_fq<double>(c);
         else gather_fq<float>(c);

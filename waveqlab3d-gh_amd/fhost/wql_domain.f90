@@ -1,0 +1,232 @@
+!> wql_domain -- the reference's domain / time-stepping API for the Fortran
+!> host, over libwql: `init_domain(D, infile)`, `time_step_RK(D, dt, RK, n)`
+!> and `close_domain(D)` keep the call shape of domain.f90:31-584 and
+!> time_step.f90:20-150, so the reference's main program (main.f90:58-96)
+!> runs unchanged on top of them.  The whole block state lives on the GPU;
+!> `D%B(ib)%X` keeps the node coordinates on the host, and the fields reach
+!> the host only through `get_fields`.
+module wql_domain
+  use iso_c_binding
+  use wql_gpu
+  use wql_input
+  implicit none
+  private
+  public :: domain_type, block_type, RK_type, init_RK, init_domain, time_step_RK, close_domain, &
+            get_fields, get_fault, wp
+
+  integer, parameter :: wp = c_double
+
+  !> Carpenter-Kennedy LSRK(5,4) (time_step.f90:20-65); the library holds
+  !> the same table and uses its own copy
+  type :: RK_type
+     integer :: nstage = 5
+     real(kind = wp) :: A(5), B(5), C(0:5)
+  end type RK_type
+
+  type :: block_type
+     integer :: nq = 0, nr = 0, ns = 0
+     real(kind = wp), allocatable :: X(:,:,:,:)   ! X(q, r, s, 1:3)
+  end type block_type
+
+  type :: domain_type
+     type(problem_params) :: p                   ! the input namelists
+     character(len=256) :: name, problem, coupling, response
+     real(kind = wp) :: t = 0, dt = 0, t_final = 0, CFL = 0.5_wp
+     integer :: nt = 0, w_stride = 1
+     logical :: w_fault = .false.
+     type(block_type) :: B(2)
+     type(wql_problem) :: prob
+     type(wql_block_desc) :: desc(2)
+     type(c_ptr) :: ctx = c_null_ptr
+     integer :: fault_units(8) = -1              ! the eight fault output files
+  end type domain_type
+
+  character(len=5), parameter :: EXTS(8) = ['Uface', 'Vface', 'S    ', 'Uhat ', 'Vhat ', 'Svel ', 'trup ', 'state']
+
+contains
+
+  subroutine init_RK(RK)
+    type(RK_type), intent(out) :: RK
+    RK%nstage = 5
+    RK%A = [0.0_wp, -567301805773.0_wp/1357537059087.0_wp, -2404267990393.0_wp/2016746695238.0_wp, &
+            -3550918686646.0_wp/2091501179385.0_wp, -1275806237668.0_wp/842570457699.0_wp]
+    RK%B = [1432997174477.0_wp/9575080441755.0_wp, 5161836677717.0_wp/13612068292357.0_wp, &
+            1720146321549.0_wp/2090206949498.0_wp, 3134564353537.0_wp/4481467310338.0_wp, &
+            2277821191437.0_wp/14882151754819.0_wp]
+    RK%C = [0.0_wp, 1432997174477.0_wp/9575080441755.0_wp, 2526269341429.0_wp/6820363962896.0_wp, &
+            2006345519317.0_wp/3224310063776.0_wp, 2802321613138.0_wp/2924317926251.0_wp, 1.0_wp]
+  end subroutine init_RK
+
+  !> block_time_step (block.f90:437-452)
+  real(kind = wp) function block_time_step(b, CFL)
+    type(block_temp_parameters), intent(in) :: b
+    real(kind = wp), intent(in) :: CFL
+    real(kind = wp) :: h(3)
+    integer :: d
+    do d = 1, 3
+       h(d) = (b%bqrs(d) - b%aqrs(d)) / real(b%nqrs(d) - 1, wp)
+    end do
+    block_time_step = CFL * minval(h) / sqrt(b%rho_s_p(2)**2 + b%rho_s_p(3)**2)
+  end function block_time_step
+
+  !> init_domain (domain.f90:31-564): namelists, time step, grids, the
+  !> device context; precision is WQL_FP64 (default) or WQL_FP32
+  subroutine init_domain(D, ifname, precision)
+    type(domain_type), intent(inout), target :: D
+    character(*), intent(in) :: ifname
+    integer(c_int), intent(in), optional :: precision
+    type(wql_grid_desc) :: g
+    integer :: ib, k
+    real(kind = wp) :: dt(2)
+
+    call read_input(ifname, D%p)
+    D%name = D%p%name; D%problem = D%p%problem; D%coupling = D%p%coupling; D%response = D%p%response
+    D%t_final = D%p%t_final; D%CFL = D%p%CFL; D%w_stride = D%p%w_stride; D%w_fault = D%p%w_fault
+    if (D%p%nblocks /= 2) error stop 'init_domain: the path runs two blocks joined by one interface'
+    D%t = 0.0_wp
+    do ib = 1, 2
+       dt(ib) = block_time_step(D%p%btp(ib), D%CFL)
+    end do
+    D%dt = min(dt(1), dt(2))
+    D%nt = floor(D%t_final / D%dt)                       ! domain.f90:494
+
+    do ib = 1, 2
+       associate(b => D%p%btp(ib))
+         D%B(ib)%nq = b%nqrs(1); D%B(ib)%nr = b%nqrs(2); D%B(ib)%ns = b%nqrs(3)
+         allocate(D%B(ib)%X(b%nqrs(1), b%nqrs(2), b%nqrs(3), 3))
+         g%nq = b%nqrs(1); g%nr = b%nqrs(2); g%ns = b%nqrs(3)
+         g%a = b%aqrs; g%b = b%bqrs; g%lc = b%lc; g%rc = b%rc
+         if (trim(D%p%type_of_mesh) == 'cartesian') then
+            g%mesh = 0
+         else if (trim(D%p%type_of_mesh) == 'curvilinear') then
+            g%mesh = 1
+         else
+            error stop 'init_domain: type_of_mesh must be cartesian or curvilinear'
+         end if
+         g%profile = 0
+         if (trim(b%profile_type) == 'analytical_test_problem') g%profile = 1
+         if (g%mesh == 1 .and. g%profile == 0 .and. trim(b%profile_type) /= 'none' .and. &
+             trim(b%profile_type) /= 'flat') error stop 'init_domain: fault profile not supported by this host'
+         if (trim(b%topography_type) /= 'none') error stop 'init_domain: topography from files is not read by this host'
+         call wql_check(wql_grid_block(g, D%B(ib)%X), 'wql_grid_block')
+         D%desc(ib)%nq = g%nq; D%desc(ib)%nr = g%nr; D%desc(ib)%ns = g%ns
+         ! boundary codes Lx Rx Ly Ry Lz Rz and PML sides (block.f90:27-269)
+         D%desc(ib)%bc = [b%lqrs(1), b%rqrs(1), b%lqrs(2), b%rqrs(2), b%lqrs(3), b%rqrs(3)]
+         do k = 1, 3
+            D%desc(ib)%pml(2 * k - 1) = merge(1, 0, b%pml_lqrs(k))
+            D%desc(ib)%pml(2 * k) = merge(1, 0, b%pml_rqrs(k))
+         end do
+         D%desc(ib)%npml = b%npml
+         D%desc(ib)%cartesian = merge(1, 0, g%mesh == 0)
+         D%desc(ib)%X = c_loc(D%B(ib)%X)
+         D%desc(ib)%M = c_null_ptr
+         D%desc(ib)%rho_s_p = b%rho_s_p
+       end associate
+    end do
+    if (any(D%p%btp(1)%mu_beta_eta /= D%p%btp(2)%mu_beta_eta)) &
+         error stop 'init_domain: both blocks must give the same mu_beta_eta'
+
+    D%prob%precision = WQL_FP64
+    if (present(precision)) D%prob%precision = precision
+    D%prob%dt = D%dt
+    D%prob%t0 = D%t
+    call wql_check(wql_problem_keys(c_string(D%problem), c_string(D%coupling), c_string(D%p%response), &
+                                    c_string(D%p%plastic_model), D%p%btp(1)%mu_beta_eta, D%prob), &
+                   'wql_problem_keys')
+    call wql_check(wql_create(D%prob, D%desc, D%ctx), 'wql_create')
+    ! Init_Vel_State (domain.f90:526-529)
+    call wql_check(wql_init_state(D%ctx), 'wql_init_state')
+    if (D%w_fault) call open_fault_output(D)
+  end subroutine init_domain
+
+  !> time_step_RK (time_step.f90:68-150): one LSRK(5,4) step on the device;
+  !> the fault output of write_output (time_step.f90:137) every w_stride
+  !> steps: the start-of-step records before the step, the records of its
+  !> first stage after it (the library keeps those on the device)
+  subroutine time_step_RK(D, dt, RK, n)
+    type(domain_type), intent(inout) :: D
+    real(kind = wp), intent(in) :: dt
+    type(RK_type), intent(in) :: RK
+    integer, intent(in) :: n
+    logical :: out
+    if (abs(dt - D%dt) > 1e-15_wp * D%dt .or. RK%nstage /= 5) error stop 'time_step_RK: dt and RK are fixed at init_domain'
+    out = D%w_fault .and. mod(n - 1, D%w_stride) == 0
+    if (out) call write_fault_start(D)
+    call wql_check(wql_step(D%ctx, 1_c_int), 'wql_step')
+    if (out) call write_fault_stage1(D)
+    D%t = wql_time(D%ctx)
+  end subroutine time_step_RK
+
+  !> close_domain (domain.f90:567-584)
+  subroutine close_domain(D)
+    type(domain_type), intent(inout) :: D
+    integer :: k
+    if (c_associated(D%ctx)) then
+       call wql_check(wql_sync(D%ctx), 'wql_sync')
+       call wql_check(wql_destroy(D%ctx), 'wql_destroy')
+       D%ctx = c_null_ptr
+    end if
+    do k = 1, 8
+       if (D%fault_units(k) >= 0) close(D%fault_units(k))
+       D%fault_units(k) = -1
+    end do
+  end subroutine close_domain
+
+  !> the fields of block ib (1 or 2), F(q, r, s, 1:9)
+  subroutine get_fields(D, ib, F)
+    type(domain_type), intent(inout) :: D
+    integer, intent(in) :: ib
+    real(kind = wp), intent(out) :: F(:,:,:,:)
+    call wql_check(wql_get_fields(D%ctx, int(ib - 1, c_int), F), 'wql_get_fields')
+  end subroutine get_fields
+
+  !> block ib's copy of the interface state: S(nr, ns, 4), W, trup (nr, ns, 1), Svel(nr, ns, 3)
+  subroutine get_fault(D, ib, S, W, trup, Svel)
+    type(domain_type), intent(inout) :: D
+    integer, intent(in) :: ib
+    real(kind = wp), intent(out), target :: S(:,:,:), W(:,:,:), trup(:,:,:), Svel(:,:,:)
+    call wql_check(wql_get_fault(D%ctx, int(ib - 1, c_int), c_loc(S), c_loc(W), c_loc(trup), c_loc(Svel), &
+                                 c_null_ptr, c_null_ptr), 'wql_get_fault')
+  end subroutine get_fault
+
+  ! ---- fault output (fault_output.f90:13-135): raw FP32 records ------------
+  subroutine open_fault_output(D)
+    type(domain_type), intent(inout) :: D
+    integer :: k, stat
+    do k = 1, 8
+       open(newunit = D%fault_units(k), file = trim(D%name) // '_interface.' // trim(EXTS(k)), &
+            access = 'stream', form = 'unformatted', status = 'replace', action = 'write', iostat = stat)
+       if (stat /= 0) error stop 'init_fault_output: cannot open the fault output files'
+    end do
+  end subroutine open_fault_output
+
+  subroutine write_fault_start(D)
+    type(domain_type), intent(inout) :: D
+    real(kind = wp), allocatable, target :: U(:,:,:), V(:,:,:), S(:,:,:), W(:,:,:)
+    integer :: nr, ns
+    nr = D%B(1)%nr; ns = D%B(1)%ns
+    allocate(U(nr, ns, 9), V(nr, ns, 9), S(nr, ns, 4), W(nr, ns, 1))
+    call wql_check(wql_get_fault_faces(D%ctx, c_loc(U), c_loc(V)), 'wql_get_fault_faces')
+    call wql_check(wql_get_fault(D%ctx, 0_c_int, c_loc(S), c_loc(W), c_null_ptr, c_null_ptr, c_null_ptr, &
+                                 c_null_ptr), 'wql_get_fault')
+    write(D%fault_units(1)) real(U, 4)
+    write(D%fault_units(2)) real(V, 4)
+    write(D%fault_units(3)) real(S, 4)
+    write(D%fault_units(8)) real(W, 4)
+  end subroutine write_fault_start
+
+  subroutine write_fault_stage1(D)
+    type(domain_type), intent(inout) :: D
+    real(kind = wp), allocatable, target :: Uh(:,:,:), Vh(:,:,:), tr(:,:,:)
+    integer :: nr, ns
+    nr = D%B(1)%nr; ns = D%B(1)%ns
+    allocate(Uh(nr, ns, 6), Vh(nr, ns, 6), tr(nr, ns, 1))
+    call wql_check(wql_get_fault(D%ctx, 0_c_int, c_null_ptr, c_null_ptr, c_loc(tr), c_null_ptr, c_loc(Uh), &
+                                 c_loc(Vh)), 'wql_get_fault')
+    write(D%fault_units(4)) real(Uh, 4)
+    write(D%fault_units(5)) real(Vh, 4)
+    write(D%fault_units(6)) real(Uh(:, :, 1:3), 4)
+    write(D%fault_units(7)) real(tr, 4)
+  end subroutine write_fault_stage1
+
+end module wql_domain

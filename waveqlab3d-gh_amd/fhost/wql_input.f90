@@ -1,0 +1,166 @@
+!> wql_input -- the reference's input-file API for the Fortran host: the
+!> &problem_list and &block_list namelists with the variables, types and
+!> defaults of init_domain (domain.f90:99-211, datatypes.f90:255-266),
+!> including the `&block_list = 'file'` indirection, which reads the block
+!> list from a second file named relative to the input file
+!> (input_utils.f90:9-94).
+module wql_input
+  use iso_c_binding, only : c_double
+  implicit none
+  private
+  public :: problem_params, block_temp_parameters, read_input
+
+  integer, parameter :: wp = c_double
+
+  !> block parameters of &block_list, component for component the
+  !> reference's block_temp_parameters (datatypes.f90:255-266)
+  type :: block_temp_parameters
+     integer :: nqrs(3) = 0
+     real(kind = wp) :: aqrs(3) = 0, bqrs(3) = 0
+     real(kind = wp) :: lc = 0, rc = 0
+     real(kind = wp) :: rho_s_p(3) = [2.67_wp, 3.464_wp, 6.0_wp]
+     real(kind = wp) :: mu_beta_eta(3) = 0
+     integer :: lqrs(3) = 1, rqrs(3) = 1
+     character(len=256) :: profile_type = 'none', profile_path = 'No-path', material_path(3) = 'none'
+     character(len=256) :: topography_type = 'none', topography_path = 'No-path'
+     logical :: pml_lqrs(3) = .false., pml_rqrs(3) = .false.
+     integer :: npml = 0, faultsize(2) = 0
+  end type block_temp_parameters
+
+  !> the &problem_list variables with init_domain's defaults (domain.f90:119-141)
+  type :: problem_params
+     character(len=256) :: name = 'default', problem = 'TPV5', response = 'elastic', plastic_model = 'default'
+     character(len=256) :: coupling = 'locked', fd_type = 'traditional', mesh_source = 'compute'
+     character(len=256) :: type_of_mesh = 'cartesian', material_source = 'hardcode'
+     integer :: nblocks = 2, nt = 0, order = 5, w_stride = 1
+     real(kind = wp) :: CFL = 0.5_wp, t_final = 0.0_wp, topo = 1.0_wp
+     logical :: interpol = .false., w_fault = .true., use_topography = .false., mollify_source = .false.
+     type(block_temp_parameters) :: btp(2)
+  end type problem_params
+
+contains
+
+  !> read &problem_list and &block_list from the file `path`
+  subroutine read_input(path, p)
+    character(*), intent(in) :: path
+    type(problem_params), intent(out) :: p
+    character(len=256) :: name, problem, response, plastic_model, coupling, fd_type, mesh_source, &
+                          type_of_mesh, material_source
+    integer :: nblocks, nt, order, w_stride
+    real(kind = wp) :: CFL, t_final, topo
+    logical :: interpol, w_fault, use_topography, mollify_source
+    type(block_temp_parameters) :: btp(2)
+    namelist /problem_list/ name, problem, response, plastic_model, nblocks, &
+                            nt, CFL, coupling, fd_type, order, t_final, &
+                            mesh_source, type_of_mesh, material_source, &
+                            interpol, w_stride, w_fault, use_topography, topo, mollify_source
+    namelist /block_list/ btp
+    integer :: u, stat
+    character(len=512) :: msg
+    character(len=1024) :: block_file
+    logical :: has_file
+
+    name = p%name; problem = p%problem; response = p%response; plastic_model = p%plastic_model
+    coupling = p%coupling; fd_type = p%fd_type; mesh_source = p%mesh_source; type_of_mesh = p%type_of_mesh
+    material_source = p%material_source; nblocks = p%nblocks; nt = p%nt; order = p%order
+    w_stride = p%w_stride; CFL = p%CFL; t_final = p%t_final; topo = p%topo; interpol = p%interpol
+    w_fault = p%w_fault; use_topography = p%use_topography; mollify_source = p%mollify_source
+    btp = p%btp
+
+    ! scanned before the input is opened for the namelist reads: a file is
+    ! connected to one unit at a time
+    call find_block_list_file(path, block_file, has_file)
+    open(newunit = u, file = trim(path), status = 'old', action = 'read', iostat = stat)
+    if (stat /= 0) error stop ':: Cannot open input file'
+    read(u, nml = problem_list, iostat = stat, iomsg = msg)
+    if (stat > 0) then
+       write(*, '(a)') trim(msg)
+       error stop 'error reading namelist problem_list'
+    end if
+    if (has_file) then
+       close(u)
+       open(newunit = u, file = trim(block_file), status = 'old', action = 'read', iostat = stat)
+       if (stat /= 0) error stop 'error opening block list file'
+    else
+       rewind(u)
+    end if
+    read(u, nml = block_list, iostat = stat, iomsg = msg)
+    if (stat > 0) then
+       write(*, '(a)') trim(msg)
+       error stop 'error reading namelist block_list'
+    end if
+    close(u)
+
+    p%name = name; p%problem = problem; p%response = response; p%plastic_model = plastic_model
+    p%coupling = coupling; p%fd_type = fd_type; p%mesh_source = mesh_source; p%type_of_mesh = type_of_mesh
+    p%material_source = material_source; p%nblocks = nblocks; p%nt = nt; p%order = order
+    p%w_stride = w_stride; p%CFL = CFL; p%t_final = t_final; p%topo = topo; p%interpol = interpol
+    p%w_fault = w_fault; p%use_topography = use_topography; p%mollify_source = mollify_source
+    p%btp = btp
+  end subroutine read_input
+
+  !> the first `&block_list` line of the input: `&block_list = 'file'`
+  !> names the block-list file, relative to the input file's directory
+  !> unless absolute (input_utils.f90:45-94)
+  subroutine find_block_list_file(path, block_file, has_file)
+    character(*), intent(in) :: path
+    character(len=*), intent(out) :: block_file
+    logical, intent(out) :: has_file
+    character(len=1024) :: line, low, rest
+    integer :: u, stat, pos, peq, q1, q2, k
+    character :: quote
+
+    has_file = .false.
+    block_file = ''
+    open(newunit = u, file = trim(path), status = 'old', action = 'read', iostat = stat)
+    if (stat /= 0) return
+    do
+       read(u, '(a)', iostat = stat) line
+       if (stat /= 0) exit
+       k = index(line, '!')
+       if (k > 0) line = line(1:k - 1)
+       line = adjustl(line)
+       if (len_trim(line) == 0) cycle
+       low = lower(line)
+       pos = index(low, '&block_list')
+       if (pos == 0) cycle
+       rest = line(pos + len('&block_list'):)
+       peq = index(rest, '=')
+       if (peq == 0) exit
+       ! an assignment right after the group name (not btp(...)%... = )
+       if (len_trim(rest(1:peq - 1)) /= 0) exit
+       rest = adjustl(rest(peq + 1:))
+       quote = rest(1:1)
+       if (quote /= "'" .and. quote /= '"') exit
+       q1 = 2
+       q2 = index(rest(q1:), quote)
+       if (q2 == 0) exit
+       block_file = rest(q1:q1 + q2 - 2)
+       if (block_file(1:1) /= '/') block_file = trim(parent_dir(path)) // trim(block_file)
+       has_file = len_trim(block_file) > 0
+       exit
+    end do
+    close(u)
+  end subroutine find_block_list_file
+
+  function parent_dir(path) result(dir)
+    character(*), intent(in) :: path
+    character(len=len(path)) :: dir
+    integer :: k
+    k = index(path, '/', back = .true.)
+    dir = ''
+    if (k > 0) dir = path(1:k)
+  end function parent_dir
+
+  function lower(s) result(t)
+    character(*), intent(in) :: s
+    character(len=len(s)) :: t
+    integer :: i, c
+    t = s
+    do i = 1, len(s)
+       c = iachar(s(i:i))
+       if (c >= iachar('A') .and. c <= iachar('Z')) t(i:i) = achar(c + 32)
+    end do
+  end function lower
+
+end module wql_input

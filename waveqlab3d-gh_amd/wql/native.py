@@ -23,7 +23,8 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_sync', 'wql_time', 'wql_get_fault_peak', 'wql_fault_stats', 'wql_server_create', 'wql_server_destroy',
            'wql_server_submit', 'wql_server_poll', 'wql_server_result', 'wql_server_status', 'wql_server_set_policy', 'wql_set_tile', 'wql_get_tile', 'wql_set_tile_order', 'wql_get_tile_order', 'wql_tile_variants', 'wql_use_graph', 'wql_profile',
            'wql_volume_kernel_ms', 'wql_get_fault_faces', 'wql_get_plastic',
-           'wql_get_fields_noalloc']
+           'wql_get_fields_noalloc', 'wql_problem_keys', 'wql_grid_block',
+           'wql_stage']
 
 
 class FaultParams(C.Structure):
@@ -37,6 +38,12 @@ class FaultParams(C.Structure):
                 ('rs_law', C.c_int), ('rs_fw', C.c_double), ('rs_Vw', C.c_double), ('rs_dVw', C.c_double),
                 ('rs_init', C.c_int), ('rs_W0', C.c_double), ('sw_law', C.c_int), ('coh', C.c_double * 3),
                 ('frc', C.c_double * 5), ('rs_solver', C.c_int)]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [('nq', C.c_int), ('nr', C.c_int), ('ns', C.c_int), ('a', C.c_double * 3), ('b', C.c_double * 3),
+                ('mesh', C.c_int), ('profile', C.c_int), ('lc', C.c_double), ('rc', C.c_double), ('fault_x', dp),
+                ('fault_side', C.c_int), ('top_y', dp)]
 
 
 class BlockDesc(C.Structure):
@@ -82,8 +89,12 @@ def lib():
             'wql_get_fault_rates': ([vp, C.c_int, dp, dp], C.c_int),
             'wql_get_plastic': ([vp, C.c_int, dp], C.c_int),
             'wql_get_fields_noalloc': ([vp, C.c_int, dp], C.c_int),
+            'wql_problem_keys': ([C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, dp, C.POINTER(ProblemDesc)],
+                                 C.c_int),
+            'wql_grid_block': ([C.POINTER(GridDesc), dp], C.c_int),
             'wql_eval_rates': ([vp, C.c_double, C.c_int, C.c_int], C.c_int),
             'wql_step': ([vp, C.c_int], C.c_int),
+            'wql_stage': ([vp, C.c_int], C.c_int),
             'wql_sync': ([vp], C.c_int),
             'wql_time': ([vp], C.c_double),
             'wql_get_fault_peak': ([vp, C.c_int, dp], C.c_int),

@@ -247,6 +247,10 @@ class Simulation:
     def step(self, n=1):
         check(native.lib().wql_step(self.ctx, n))
 
+    def stage(self, s):
+        """RK stage s (1..5) of the current step"""
+        check(native.lib().wql_stage(self.ctx, s))
+
     def sync(self):
         check(native.lib().wql_sync(self.ctx))
 
@@ -262,3 +266,29 @@ class Simulation:
 
     def volume_kernel_ms(self):
         return native.lib().wql_volume_kernel_ms(self.ctx)
+
+
+def device_grid(block):
+    """node coordinates X[3][ns][nr][nq] of a Block generated on the device
+    by wql_grid_block (the same grid as block.coords())"""
+    g = native.GridDesc()
+    g.nq, g.nr, g.ns = block.n
+    g.a[:] = block.a
+    g.b[:] = block.b
+    g.mesh = 0 if block.mesh == 'cartesian' else 1
+    g.profile = 1 if block.profile == 'analytical_test_problem' else 0
+    g.lc, g.rc = block.lc, block.rc
+    keep = []
+    if block.fault_offset is not None:
+        fx = np.ascontiguousarray(np.asarray(block.fault_offset, dtype=np.float64).T)  # [ns][nr]
+        keep.append(fx)
+        g.fault_x = _p(fx)
+        g.fault_side = 1 if block.bc[1] == 0 else 0
+    if block.top_offset is not None:
+        ty = np.ascontiguousarray(np.asarray(block.top_offset, dtype=np.float64).T)  # [ns][nq]
+        keep.append(ty)
+        g.top_y = _p(ty)
+    nq, nr, ns = block.n
+    X = np.empty((3, ns, nr, nq))
+    check(native.lib().wql_grid_block(C.byref(g), _p(X)))
+    return X
