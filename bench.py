@@ -148,7 +148,7 @@ def fp32_error(W, steps):
             'max_slip_fp64': float(a['slip'].max())}
 
 
-def serving(n_scen, steps, rate, max_active, n_seq, seed=5):
+def serving(n_scen, steps, rate, max_active, n_seq, seed=5, fifo=True):
     """the serving stream: n_scen scenarios of the seeded size/friction
     distribution, Poisson arrivals at `rate`/s, `steps` steps each, FP32
     fields / FP64 friction through the continuous-batching server; beside
@@ -167,15 +167,26 @@ def serving(n_scen, steps, rate, max_active, n_seq, seed=5):
     arrivals -= arrivals[0]
     # warm the library (first graph capture, code objects) outside the timing
     run_stream(scen[:1], np.zeros(1), max_active=1, precision='fp32')
+
+    def progress(tag):
+        def cb(n):
+            if n % 16 == 0 or n == n_scen:
+                print(f'[bench] serving {tag}: {n}/{n_scen} scenarios retired', file=sys.stderr, flush=True)
+        return cb
     # admission shortest job first with aging; arrival order beside it
-    lat, wall, _ = run_stream(scen, arrivals, max_active=max_active, precision='fp32', policy='sjf')
-    lat_f, wall_f, _ = run_stream(scen, arrivals, max_active=max_active, precision='fp32', policy='fifo')
+    pool = {}
+    lat, wall, _ = run_stream(scen, arrivals, max_active=max_active, precision='fp32', policy='sjf', stats=pool,
+                              progress=progress('sjf'))
+    lat_f, wall_f, _ = (None, None, None) if not fifo else \
+        run_stream(scen, arrivals, max_active=max_active, precision='fp32', policy='fifo', progress=progress('fifo'))
     pts = sum(workloads.points(s['problem']) for s in scen) * 5 * steps
     t0 = time.perf_counter()
-    for sc in scen[:n_seq]:
+    for k, sc in enumerate(scen[:n_seq]):
         with Simulation(sc['problem'], precision='fp64', coords=sc['coords']) as sim:
             sim.step(steps)
             sim.sync()
+        if (k + 1) % 8 == 0:
+            print(f'[bench] sequential fp64: {k + 1}/{n_seq}', file=sys.stderr, flush=True)
     seq = time.perf_counter() - t0
     seq_rate = n_seq / seq
     return {'scenarios': n_scen, 'steps_each': steps, 'arrival_rate_per_s': rate, 'max_active': max_active,
@@ -186,9 +197,10 @@ def serving(n_scen, steps, rate, max_active, n_seq, seed=5):
             'M_grid_pt_stages_per_s': round(pts / wall / 1e6, 2),
             'sequential_fp64': {'scenarios': n_seq, 'scenarios_per_s': round(seq_rate, 4)},
             'speedup_vs_sequential_fp64': round((n_scen / wall) / seq_rate, 3),
-            'fifo': {'p50_latency_s': round(float(np.percentile(lat_f, 50)), 4),
-                     'p99_latency_s': round(float(np.percentile(lat_f, 99)), 4),
-                     'scenarios_per_s': round(n_scen / wall_f, 4)}}
+            'context_pool': pool,
+            'fifo': None if lat_f is None else {'p50_latency_s': round(float(np.percentile(lat_f, 50)), 4),
+                                                'p99_latency_s': round(float(np.percentile(lat_f, 99)), 4),
+                                                'scenarios_per_s': round(n_scen / wall_f, 4)}}
 
 
 def cpu_baseline(W, min_seconds=12.0, max_steps=40):
@@ -242,6 +254,9 @@ def main():
     ap.add_argument('--serve-rate', type=float, default=20.0)
     ap.add_argument('--serve-active', type=int, default=8)
     ap.add_argument('--serve-seq', type=int, default=16)
+    ap.add_argument('--serve-full', action='store_true',
+                    help='the serving stream as stated: 256 scenarios x 1000 steps, Poisson arrivals at 20/s, '
+                         'against 32 sequential plain FP64 runs (several minutes)')
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py: the library runs one problem on one GPU; --gpus {a.gpus} is not supported '
@@ -294,7 +309,9 @@ def main():
     err_steps = W['steps'] if a.err_steps is None else a.err_steps
     if err_steps > 0 and not a.no_fp64:
         out['fp32_error'] = fp32_error(W, err_steps)
-    if a.serve > 0:
+    if a.serve_full:
+        out['serving'] = serving(256, 1000, 20.0, a.serve_active, 32, fifo=False)
+    elif a.serve > 0:
         out['serving'] = serving(a.serve, a.serve_steps, a.serve_rate, a.serve_active, min(a.serve_seq, a.serve))
     if not a.no_cpu:
         out['cpu_baseline'] = cpu_baseline(W)

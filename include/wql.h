@@ -253,16 +253,24 @@ int wql_server_destroy(wql_server *srv);
 /* copies the descriptors and arrays; *id identifies the scenario */
 int wql_server_submit(wql_server *srv, const wql_problem *prob, const wql_block_desc blocks[2], int nsteps,
                       int *id);
-/* non-blocking progress: retires finished scenarios (ids into done[], at
- * most max_done; the return value is their number), launches the next chunk
- * of every active scenario whose chunk ended, admits queued ones */
+/* non-blocking progress: retires finished scenarios, launches the next
+ * chunk of every active scenario whose chunk ended, admits queued ones, and
+ * reports retired (finished or failed) ids in order into done[], at most
+ * max_done per call (the return value is their number; the rest are
+ * reported by later calls) */
 int wql_server_poll(wql_server *srv, int *done, int max_done);
 /* block 1's fault copy of a retired scenario: S[4][nr*ns], trup, peak slip
  * rate [nr*ns], wql_fault_stats; frees the record (a failed scenario returns
  * its error here) */
 int wql_server_result(wql_server *srv, int id, double *S, double *trup, double *psr, double stats[6]);
-/* {queued, active, device bytes in use, budget} */
+/* {queued, active, device bytes in use (active and pooled contexts), budget} */
 int wql_server_status(const wql_server *srv, long out[4]);
+/* the context pool: {contexts created, contexts reused, contexts freed,
+ * contexts pooled now}.  A retired scenario's context is kept and cleared
+ * for the next scenario with the same blocks (shapes, descriptors,
+ * coordinates and material), precision and plasticity model; the least
+ * recently used pooled contexts are freed when the budget needs the room. */
+int wql_server_pool_stats(const wql_server *srv, long out[4]);
 /* admission order of queued scenarios: 0 arrival order (the default),
  * 1 shortest job first with aging (key: nodes x steps / (1 + seconds
  * waited)).  No counterpart in the reference. */

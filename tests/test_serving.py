@@ -42,9 +42,13 @@ def test_alone_equals_in_the_mix(precision, policy):
     print(f'[gpu] serving mix {precision} {policy}', file=sys.stderr, flush=True)
     scen = [_scenario(k) for k in range(6)]
     arrivals = np.array([0.0, 0.0, 0.01, 0.02, 0.02, 0.05])
+    pool = {}
     lat, wall, res = run_stream(scen, arrivals, max_active=3, steps_per_launch=7, precision=precision,
-                                keep_results=True, policy=policy)
+                                keep_results=True, policy=policy, stats=pool)
     assert len(res) == len(scen)
+    # scenarios k and k + 4 share their blocks: the later one runs on the
+    # earlier one's pooled context whenever that one retired first
+    assert pool['created'] + pool['reused'] == len(scen) and pool['reused'] >= 1, pool
     assert (lat > 0).all()
     for k, sc in enumerate(scen):
         a = _alone(sc, precision)
@@ -94,3 +98,48 @@ def test_shortest_job_first_order():
         assert done == ids[::-1]
         for sid in ids:
             assert srv.result(sid)['stats']['max_slip'] > 0.0
+
+
+def test_poll_reports_every_retirement_with_a_small_buffer():
+    """poll with room for one id per call: every retired scenario is still
+    reported, once, in a later call"""
+    import ctypes as C
+    from wql import native
+    from wql.serving import Server
+    print('[gpu] serving small poll buffer', file=sys.stderr, flush=True)
+    scen = [_scenario(k) for k in (0, 0, 0, 1)]
+    with Server(max_active=4, steps_per_launch=50) as srv:
+        ids = [srv.submit(s['problem'], s['coords'], s['steps'], 'fp32') for s in scen]
+        done = []
+        one = (C.c_int * 1)()
+        for _ in range(200000):
+            n = native.lib().wql_server_poll(srv.srv, one, 1)
+            assert n in (0, 1)
+            if n:
+                done.append(one[0])
+            if len(done) == len(ids):
+                break
+        assert sorted(done) == sorted(ids)
+        for sid in ids:
+            assert srv.result(sid)['stats']['max_slip'] > 0.0
+
+
+def test_pool_frees_least_recently_used_for_new_blocks():
+    """a budget that holds one context at a time: a scenario with other
+    blocks frees the pooled context, and the results are unchanged"""
+    from wql.serving import Server
+    from wql.solver import estimate_bytes
+    print('[gpu] serving pool eviction', file=sys.stderr, flush=True)
+    a, b = _scenario(0), _scenario(2)
+    need = max(estimate_bytes(s['problem'], 'fp32') for s in (a, b))
+    with Server(max_active=1, hbm_budget=int(need * 1.2), steps_per_launch=10) as srv:
+        ids = [srv.submit(s['problem'], s['coords'], s['steps'], 'fp32') for s in (a, b, a, a)]
+        done = []
+        while len(done) < len(ids):
+            done += srv.poll()
+        st = srv.pool_stats()
+        assert (st['created'], st['reused'], st['freed']) == (3, 1, 2), st
+        res = [srv.result(sid) for sid in ids]
+    ref = _alone(a, 'fp32')
+    for r in (res[0], res[2], res[3]):
+        assert np.array_equal(r['S'], ref['S'])

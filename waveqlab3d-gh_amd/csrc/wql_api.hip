@@ -548,6 +548,70 @@ void drop_graphs(wql_ctx *c)
         }
 }
 
+int check_launch();
+
+// Put a context back to the state wql_create leaves, for a new problem on
+// the same blocks (geometry, material, precision and plasticity model):
+// fields, rates, PML auxiliary fields, plastic state and interface state
+// are cleared on the context's stream (no allocation, no host wait), the
+// fault description, dt and t0 are taken from prob, and captured graphs
+// (which hold the old fault description) are dropped.  Metrics, face
+// frames and material are kept.
+template <typename real>
+void reset_typed(wql_ctx *c)
+{
+    auto fill = [&](void *p, long n, real v) {
+        if (p && n > 0) k_fill<real><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, (real *)p, v);
+    };
+    auto fill64 = [&](double *p, long n, double v) {
+        if (p && n > 0) k_fill<double><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, p, v);
+    };
+    for (int ib = 0; ib < 2; ib++) {
+        DevBlock &d = c->b[ib];
+        fill(d.F[0], 9 * d.NP, (real)0);
+        fill(d.F[1], 9 * d.NP, (real)0);
+        fill(d.DF, 9 * d.NP, (real)0);
+        for (int s = 0; s < 6; s++) {
+            fill(d.Q[s], 9 * d.SP[s], (real)0);
+            fill(d.DQ[s], 9 * d.SP[s], (real)0);
+        }
+        fill64(d.P, 2 * d.NP, 0.0);
+        const long FP = (long)d.nr * d.ns;
+        Iface &I = c->I[ib];
+        fill64(I.S, 4 * FP, 0.0);
+        fill64(I.DS, 4 * FP, 0.0);
+        fill64(I.W, FP, 0.0);
+        fill64(I.DW, FP, 0.0);
+        fill64(I.trup, FP, 1.0e9);  // iface.f90:41
+        fill64(I.Svel, 3 * FP, 0.0);
+        fill64(I.Uhat, 6 * FP, 0.0);
+        fill64(I.Vhat, 6 * FP, 0.0);
+        fill64(I.psr, FP, 0.0);
+    }
+    k_fill<int><<<1, 64, 0, c->stream>>>(1, c->d_flag, 0);
+}
+
+int valid_problem(const wql_problem *prob);
+
+int ctx_reset(wql_ctx *c, const wql_problem *prob)
+{
+    if (int rc = valid_problem(prob)) return rc;
+    if (prob->precision != c->prec || prob->plastic != c->prob.plastic || prob->s0_kind != c->prob.s0_kind)
+        return fail(WQL_E_ARG, "a context is reused only for the same precision and plasticity model");
+    drop_graphs(c);
+    c->prob = *prob;
+    memcpy(&c->fault, &prob->fault, sizeof(FaultParams));
+    c->dt = prob->dt;
+    c->t = prob->t0;
+    c->cur = 0;
+    c->stage_next = 1;
+    c->vol_ms = 0.0;
+    c->vol_launches = 0;
+    if (c->prec == WQL_FP64) reset_typed<double>(c);
+    else reset_typed<float>(c);
+    return check_launch();
+}
+
 int check_launch()
 {
     hipError_t e = hipGetLastError();
@@ -881,6 +945,23 @@ double *scratch_ptr(int dev)
     return g_scratch[dev];
 }
 
+int valid_problem(const wql_problem *prob)
+{
+    if (prob->precision != WQL_FP64 && prob->precision != WQL_FP32) return fail(WQL_E_ARG, "bad precision");
+    if (!(prob->dt > 0)) return fail(WQL_E_ARG, "dt must be positive");
+    const wql_fault_params &f = prob->fault;
+    if (f.law < 0 || f.law > 2 || f.npatch < 0 || f.npatch > 4 || f.stress_mode < 0 || f.stress_mode > 2 ||
+        f.nuc < 0 || f.nuc > 3 || f.rs_law < 0 || f.rs_law > 2 || f.rs_init < 0 || f.rs_init > 1 ||
+        f.sw_law < 0 || f.sw_law > 1 || f.rs_solver < 0 || f.rs_solver > 1)
+        return fail(WQL_E_ARG, "bad fault description");
+    if (f.law == WQL_SLIP_WEAKENING && !(f.Dc > 0)) return fail(WQL_E_ARG, "slip weakening needs Dc > 0");
+    if (f.law == WQL_RATE_STATE && !(f.rs_b > 0 && f.rs_L > 0 && f.rs_V0 > 0 && f.rs_a > 0))
+        return fail(WQL_E_ARG, "rate and state needs a, b, L, V0 > 0");
+    if (prob->plastic < 0 || prob->plastic > 2 || prob->s0_kind < 0 || prob->s0_kind > 1)
+        return fail(WQL_E_ARG, "bad plasticity description");
+    return 0;
+}
+
 int valid_block(const wql_block_desc &b)
 {
     if (b.nq < 13 || b.nr < 13 || b.ns < 13) return fail(WQL_E_ARG, "block needs at least 13 nodes per direction");
@@ -942,18 +1023,7 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
         return fail(WQL_E_ARG, "blocks must share the fault face size (nr, ns)");
     if (blocks[0].bc[1] != 0 || blocks[1].bc[0] != 0)
         return fail(WQL_E_ARG, "the fault joins block 0 side Rx and block 1 side Lx (bc code 0)");
-    {
-        const wql_fault_params &f = prob->fault;
-        if (f.law < 0 || f.law > 2 || f.npatch < 0 || f.npatch > 4 || f.stress_mode < 0 || f.stress_mode > 2 ||
-            f.nuc < 0 || f.nuc > 3 || f.rs_law < 0 || f.rs_law > 2 || f.rs_init < 0 || f.rs_init > 1 ||
-            f.sw_law < 0 || f.sw_law > 1 || f.rs_solver < 0 || f.rs_solver > 1)
-            return fail(WQL_E_ARG, "bad fault description");
-        if (f.law == WQL_SLIP_WEAKENING && !(f.Dc > 0)) return fail(WQL_E_ARG, "slip weakening needs Dc > 0");
-        if (f.law == WQL_RATE_STATE && !(f.rs_b > 0 && f.rs_L > 0 && f.rs_V0 > 0 && f.rs_a > 0))
-            return fail(WQL_E_ARG, "rate and state needs a, b, L, V0 > 0");
-    }
-    if (prob->plastic < 0 || prob->plastic > 2 || prob->s0_kind < 0 || prob->s0_kind > 1)
-        return fail(WQL_E_ARG, "bad plasticity description");
+    if (int rc = valid_problem(prob)) return rc;
     for (int ib = 0; ib < 2; ib++) {
         const size_t elem = prob->precision == WQL_FP32 ? 4 : 8;
         const long lane = 128 / (long)elem;

@@ -2,19 +2,27 @@
 // scenarios (part of libwql; included by wql_api.hip after the C ABI).
 //
 // A scenario is a full two-block problem with its own context: its state is
-// created on the device when it is admitted and stays there until it is
-// retired.  Admission is first-fit by device footprint (wql_estimate_bytes)
-// against the server's HBM budget and a cap on concurrently active
-// scenarios.  Every active scenario runs on its own stream with its LSRK
-// step captured once into a HIP graph; the server advances each one in
-// chunks of steps and tracks chunk completion with an event, so scenarios
-// that arrive while others run are admitted as soon as memory and a slot
-// free up, and their work overlaps on the device.  The server never blocks
-// on the device in wql_server_poll except to download a retired scenario's
-// fault arrays.  Queued scenarios are taken in arrival order (policy 0) or
-// shortest job first with aging (policy 1: the key is nodes x steps /
-// (1 + seconds waited), so a large scenario overtakes fresh small ones after
-// waiting as many seconds as its work is a multiple of theirs).
+// on the device from admission until it is retired.  Admission is first-fit
+// by device footprint (wql_estimate_bytes) against the server's HBM budget
+// and a cap on concurrently active scenarios.  Every active scenario runs on
+// its own stream with its LSRK step captured once into a HIP graph; the
+// server advances each one in chunks of steps and tracks chunk completion
+// with an event, so scenarios that arrive while others run are admitted as
+// soon as memory and a slot free up, and their work overlaps on the device.
+//
+// Retired contexts are not freed: they wait in a pool keyed by everything
+// their allocation and set-up depend on (block shapes, boundary and PML
+// descriptors, a hash of the coordinates and material, precision,
+// plasticity).  A scenario with the same key takes a pooled context, which
+// is cleared on its stream and keeps its metrics, face frames and material,
+// so neither admission nor retirement allocates, frees or recomputes
+// geometry, and the poll loop never waits for the device except to download
+// a retired scenario's fault arrays.  Pooled contexts count against the
+// budget; the least recently used ones are freed when a scenario with a new
+// key needs the room.  Queued scenarios are taken in arrival order (policy
+// 0) or shortest job first with aging (policy 1: the key is nodes x steps /
+// (1 + seconds waited), so a large scenario overtakes fresh small ones
+// after waiting as many seconds as its work is a multiple of theirs).
 
 #include <algorithm>
 #include <chrono>
@@ -30,6 +38,7 @@ struct wql_scenario {
     std::vector<double> X[2], M[2];
     int nsteps = 0, steps_done = 0, steps_inflight = 0;
     size_t need = 0;
+    uint64_t key = 0;       // pool key: what the context's allocation and set-up depend on
     double work = 0.0;      // nodes x steps
     double t_submit = 0.0;  // host clock at submission (s)
     wql_ctx *ctx = nullptr;
@@ -40,15 +49,26 @@ struct wql_scenario {
     std::string msg;
 };
 
+struct wql_pooled {
+    wql_ctx *ctx;
+    uint64_t key;
+    size_t bytes;
+    long last_use;
+};
+
 struct wql_server {
     int max_active = 8;
     size_t budget = 0;
     int chunk = 50;
     int next_id = 1;
     int policy = 0;  // 0 arrival order, 1 shortest job first with aging
-    size_t used = 0;
+    size_t used = 0;  // active and pooled contexts
+    long clock = 0;
+    long created = 0, reused = 0, freed = 0;
     std::deque<int> queue;
     std::vector<int> active;
+    std::deque<int> unreported;  // retired (finished or failed) ids not yet returned by poll
+    std::vector<wql_pooled> pool;
     std::unordered_map<int, std::unique_ptr<wql_scenario>> all;
 };
 
@@ -59,16 +79,55 @@ double host_seconds()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-void scenario_release(wql_server *s, wql_scenario *sc)
+// FNV-1a over 64-bit words
+uint64_t fnv(uint64_t h, const void *p, size_t bytes)
+{
+    const unsigned char *b = (const unsigned char *)p;
+    size_t n = bytes / 8;
+    for (size_t k = 0; k < n; k++) {
+        uint64_t w;
+        memcpy(&w, b + 8 * k, 8);
+        h = (h ^ w) * 1099511628211ull;
+    }
+    for (size_t k = 8 * n; k < bytes; k++) h = (h ^ b[k]) * 1099511628211ull;
+    return h;
+}
+
+uint64_t scenario_key(const wql_scenario *sc)
+{
+    uint64_t h = 1469598103934665603ull;
+    const int flags[3] = {sc->prob.precision, sc->prob.plastic, sc->prob.s0_kind};
+    h = fnv(h, flags, sizeof flags);
+    for (int ib = 0; ib < 2; ib++) {
+        const wql_block_desc &b = sc->blocks[ib];
+        const int shape[17] = {b.nq, b.nr, b.ns, b.bc[0], b.bc[1], b.bc[2], b.bc[3], b.bc[4], b.bc[5],
+                               b.pml[0], b.pml[1], b.pml[2], b.pml[3], b.pml[4], b.pml[5], b.npml, b.cartesian};
+        h = fnv(h, shape, sizeof shape);
+        h = fnv(h, b.rho_s_p, sizeof b.rho_s_p);
+        h = fnv(h, sc->X[ib].data(), sc->X[ib].size() * sizeof(double));
+        const int het = sc->M[ib].empty() ? 0 : 1;
+        h = fnv(h, &het, sizeof het);
+        if (het) h = fnv(h, sc->M[ib].data(), sc->M[ib].size() * sizeof(double));
+    }
+    return h;
+}
+
+// a retired scenario's context goes to the pool (it stays counted in used)
+void scenario_release(wql_server *s, wql_scenario *sc, bool keep)
 {
     if (sc->ev) {
         hipEventDestroy(sc->ev);
         sc->ev = nullptr;
     }
     if (sc->ctx) {
-        wql_destroy(sc->ctx);
+        if (keep) {
+            s->pool.push_back({sc->ctx, sc->key, sc->need, s->clock++});
+        } else {
+            wql_destroy(sc->ctx);
+            s->used -= sc->need;
+            s->freed++;
+        }
         sc->ctx = nullptr;
-        s->used -= sc->need;
     }
     for (int ib = 0; ib < 2; ib++) {
         std::vector<double>().swap(sc->X[ib]);
@@ -76,12 +135,27 @@ void scenario_release(wql_server *s, wql_scenario *sc)
     }
 }
 
+// free pooled contexts, least recently used first, until `need` more
+// bytes fit the budget; false if they cannot
+bool pool_make_room(wql_server *s, size_t need)
+{
+    while (s->used + need > s->budget && !s->pool.empty()) {
+        auto it = std::min_element(s->pool.begin(), s->pool.end(),
+                                   [](const wql_pooled &a, const wql_pooled &b) { return a.last_use < b.last_use; });
+        wql_destroy(it->ctx);
+        s->used -= it->bytes;
+        s->pool.erase(it);
+        s->freed++;
+    }
+    return s->used + need <= s->budget;
+}
+
 int scenario_fail(wql_server *s, wql_scenario *sc, int rc)
 {
     sc->err = rc;
     sc->msg = wql_last_error();
     sc->state = 3;
-    scenario_release(s, sc);
+    scenario_release(s, sc, false);  // a failed context is not reused
     return rc;
 }
 
@@ -98,9 +172,20 @@ int scenario_launch(wql_scenario *sc, int chunk)
 int scenario_admit(wql_server *s, wql_scenario *sc)
 {
     wql_ctx *ctx = nullptr;
-    if (int rc = wql_create(&sc->prob, sc->blocks, &ctx)) return rc;
-    sc->ctx = ctx;
-    s->used += sc->need;
+    auto hit = std::find_if(s->pool.begin(), s->pool.end(), [&](const wql_pooled &p) { return p.key == sc->key; });
+    if (hit != s->pool.end()) {
+        ctx = hit->ctx;
+        sc->need = hit->bytes;
+        s->pool.erase(hit);
+        s->reused++;
+        sc->ctx = ctx;
+        if (int rc = ctx_reset(ctx, &sc->prob)) return rc;
+    } else {
+        if (int rc = wql_create(&sc->prob, sc->blocks, &ctx)) return rc;
+        sc->ctx = ctx;
+        s->used += sc->need;
+        s->created++;
+    }
     for (int ib = 0; ib < 2; ib++) {  // host copies are no longer needed
         std::vector<double>().swap(sc->X[ib]);
         std::vector<double>().swap(sc->M[ib]);
@@ -124,7 +209,7 @@ int scenario_finish(wql_server *s, wql_scenario *sc)
     if (int rc = wql_get_fault_peak(sc->ctx, 0, sc->psr.data())) return rc;
     if (int rc = wql_fault_stats(sc->ctx, 0, sc->stats)) return rc;
     sc->state = 2;
-    scenario_release(s, sc);
+    scenario_release(s, sc, true);
     return 0;
 }
 
@@ -156,7 +241,8 @@ int wql_server_set_policy(wql_server *s, int policy)
 int wql_server_destroy(wql_server *s)
 {
     if (!s) return 0;
-    for (auto &kv : s->all) scenario_release(s, kv.second.get());
+    for (auto &kv : s->all) scenario_release(s, kv.second.get(), false);
+    for (auto &p : s->pool) wql_destroy(p.ctx);
     delete s;
     return 0;
 }
@@ -181,6 +267,7 @@ int wql_server_submit(wql_server *s, const wql_problem *prob, const wql_block_de
         }
     }
     sc->need = wql_estimate_bytes(prob, sc->blocks);
+    sc->key = scenario_key(sc.get());
     sc->work = (double)nsteps * ((double)blocks[0].nq * blocks[0].nr * blocks[0].ns +
                                  (double)blocks[1].nq * blocks[1].nr * blocks[1].ns);
     sc->t_submit = host_seconds();
@@ -213,11 +300,8 @@ int wql_server_poll(wql_server *s, int *done, int max_done)
             int rc = sc->steps_done >= sc->nsteps ? scenario_finish(s, sc) : scenario_launch(sc, s->chunk);
             if (rc) scenario_fail(s, sc, rc);
         }
-        if (sc->state == 1) {
-            still.push_back(id);
-        } else if (ndone < max_done) {
-            done[ndone++] = id;
-        }
+        if (sc->state == 1) still.push_back(id);
+        else s->unreported.push_back(id);
     }
     s->active.swap(still);
     // admission: first fit by footprint, in arrival order or by the aged
@@ -237,18 +321,26 @@ int wql_server_poll(wql_server *s, int *done, int max_done)
     }
     for (auto it = s->queue.begin(); it != s->queue.end() && (int)s->active.size() < s->max_active;) {
         wql_scenario *sc = s->all[*it].get();
-        if (s->used + sc->need > s->budget) {
+        const bool pooled = std::any_of(s->pool.begin(), s->pool.end(),
+                                        [&](const wql_pooled &p) { return p.key == sc->key; });
+        if (!pooled && !pool_make_room(s, sc->need)) {
             ++it;
             continue;
         }
         int rc = scenario_admit(s, sc);
         if (rc) {
             scenario_fail(s, sc, rc);
-            if (ndone < max_done) done[ndone++] = sc->id;
+            s->unreported.push_back(sc->id);
         } else {
             s->active.push_back(sc->id);
         }
         it = s->queue.erase(it);
+    }
+    // retirements are reported in order, at most max_done per call; the
+    // rest wait for the next call
+    while (ndone < max_done && !s->unreported.empty()) {
+        done[ndone++] = s->unreported.front();
+        s->unreported.pop_front();
     }
     return ndone;
 }
@@ -271,6 +363,16 @@ int wql_server_result(wql_server *s, int id, double *S, double *trup, double *ps
     if (psr) memcpy(psr, sc->psr.data(), sc->psr.size() * sizeof(double));
     if (stats) memcpy(stats, sc->stats, sizeof sc->stats);
     s->all.erase(f);
+    return 0;
+}
+
+int wql_server_pool_stats(const wql_server *s, long out[4])
+{
+    if (!s || !out) return fail(WQL_E_ARG, "bad argument");
+    out[0] = s->created;
+    out[1] = s->reused;
+    out[2] = s->freed;
+    out[3] = (long)s->pool.size();
     return 0;
 }
 

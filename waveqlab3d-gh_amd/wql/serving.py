@@ -68,6 +68,11 @@ class Server:
                 'first_rupture_time')
         return dict(S=S, trup=trup, psr=psr, stats=dict(zip(keys, stats.tolist())))
 
+    def pool_stats(self):
+        out = (C.c_long * 4)()
+        check(native.lib().wql_server_pool_stats(self.srv, out))
+        return dict(created=out[0], reused=out[1], freed=out[2], pooled=out[3])
+
     def status(self):
         out = (C.c_long * 4)()
         check(native.lib().wql_server_status(self.srv, out))
@@ -75,10 +80,12 @@ class Server:
 
 
 def run_stream(scenarios, arrivals, max_active=8, steps_per_launch=50, precision='fp32', keep_results=False,
-               idle_sleep=2e-4, policy='fifo'):
+               idle_sleep=2e-4, policy='fifo', stats=None, progress=None):
     """Serve `scenarios` (list of dicts with problem, coords, steps) arriving
     at `arrivals` seconds after the start.  Returns per-scenario latency (s),
-    the wall time from first arrival to last result, and optionally results."""
+    the wall time from first arrival to last result, and optionally results;
+    `stats` (a dict) receives the context pool's counters; `progress(n)`
+    is called after each retired scenario with the number retired."""
     order = np.argsort(arrivals)
     lat = np.zeros(len(scenarios))
     results = {}
@@ -101,9 +108,13 @@ def run_stream(scenarios, arrivals, max_active=8, steps_per_launch=50, precision
                 r = srv.result(sid)
                 lat[k] = time.perf_counter() - t_start - ta
                 ndone += 1
+                if progress is not None:
+                    progress(ndone)
                 if keep_results:
                     results[k] = r
             if not finished and (nxt >= len(order) or arrivals[order[nxt]] > now):
                 time.sleep(idle_sleep)
         wall = time.perf_counter() - t_start - float(np.min(arrivals))
+        if stats is not None:
+            stats.update(srv.pool_stats())
     return lat, wall, results
