@@ -323,6 +323,16 @@ BoxList box_list(const wql_ctx *c)
     return L;
 }
 
+// the PML slabs of one stage (after the interior kernels): per node, one
+// thread each
+template <typename real>
+void launch_pml(wql_ctx *c, Pair<real> &P, real A, real bdt)
+{
+    BoxList BL = box_list(c);
+    if (BL.total > 0)
+        k_volume_generic<real><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(P, BL, A, A, bdt);
+}
+
 template <typename real, int TX, int TY>
 void launch_main_v(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0, real A1, real bdt)
 {
@@ -481,11 +491,7 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
         }
     }
     launch_band<real>(c, P, (real)A, (real)bdt);
-    BoxList BL = box_list(c);
-    if (BL.total > 0) {
-        unsigned g = (unsigned)((BL.total + 255) / 256);
-        k_volume_generic<real><<<g, 256, 0, c->stream>>>(P, BL, (real)A, (real)A, (real)bdt);
-    }
+    launch_pml<real>(c, P, (real)A, (real)bdt);
     if (!fused) {
         a.mode = 0;
         k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
@@ -1504,17 +1510,14 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
             Pf.b[1] = blk_view<float>(c, 1, c->cur);
         }
         TileList TL = tile_list(c);
-        BoxList BL = box_list(c);
         if (c->prec == WQL_FP64) {
             if (TL.total > 0) launch_main<double>(c, Pd, TL, 0.0, 0.0, 0.0);
             launch_band<double>(c, Pd, 0.0, 0.0);
-            if (BL.total > 0)
-                k_volume_generic<double><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(Pd, BL, 0.0, 0.0, 0.0);
+            launch_pml<double>(c, Pd, 0.0, 0.0);
         } else {
             if (TL.total > 0) launch_main<float>(c, Pf, TL, 0.f, 0.f, 0.f);
             launch_band<float>(c, Pf, 0.f, 0.f);
-            if (BL.total > 0)
-                k_volume_generic<float><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(Pf, BL, 0.f, 0.f, 0.f);
+            launch_pml<float>(c, Pf, 0.f, 0.f);
         }
     } else {
         if (c->prec == WQL_FP64) {

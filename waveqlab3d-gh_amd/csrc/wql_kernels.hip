@@ -167,6 +167,10 @@ __device__ __forceinline__ double gmet(const Blk<real> &B, int c, long p)
 }
 
 template <typename real>
+__device__ void node_tail(const Blk<real> &B, int i, int j, int k, long p, real (&U)[3][NF], const real (&fin)[NF],
+                          real Jc, real A, real bdt);
+
+template <typename real>
 __device__ void volume_node_generic(const Blk<real> &B, int i, int j, int k, real A, real bdt)
 {
     const long NP = B.NP;
@@ -217,11 +221,30 @@ __device__ void volume_node_generic(const Blk<real> &B, int i, int j, int k, rea
             }
         }
     }
+    real fin[NF];
+    for (int f = 0; f < NF; f++) fin[f] = B.Fin[f * NP + p];
+    node_tail<real>(B, i, j, k, p, U, fin, (real)gmet(B, 9, p), A, bdt);
+}
+
+// Rates and stage output of one node from its derivative terms U[fam][f]:
+// U[fam][0..2] = d(v_f)/d(x_fam) and U[fam][3+f'] = the flux derivative of
+// stress f' in family fam.  Only the nine stress terms of the momentum
+// equations are read (Ux: sxx sxy sxz, Uy: syy sxy syz, Uz: szz sxz syz).
+// Hooke's law and the divergence (RHS_Interior.f90:221-268), the PML
+// damping and auxiliary update of every PML slab holding the node
+// (1782-2201), the boundary penalty, DF = A DF + rate, F_out = F + B dt DF.
+// fin: the node's stage input, Jc: its Jacobian.
+template <typename real>
+__device__ void node_tail(const Blk<real> &B, int i, int j, int k, long p, real (&U)[3][NF], const real (&fin)[NF],
+                          real Jc, real A, real bdt)
+{
+    const long NP = B.NP;
+    const int nn[3] = {B.nq, B.nr, B.ns};
+    const int pos[3] = {i, j, k};
     const real *Ux = U[0], *Uy = U[1], *Uz = U[2];
     real lam, mu, rho;
     if (B.M) { lam = B.M[p]; mu = B.M[NP + p]; rho = B.M[2 * NP + p]; }
     else { lam = (real)B.lam; mu = (real)B.mu; rho = (real)B.rho; }
-    real Jc = (real)gmet(B, 9, p);
     real rhoJ_inv = (real)1 / (rho * Jc);
     real l2m = lam + (real)2 * mu;
     real DU[NF];
@@ -307,7 +330,7 @@ __device__ void volume_node_generic(const Blk<real> &B, int i, int j, int k, rea
         df = df + DU[f];
         if (bi >= 0) df = df + B.satb[f * B.nbnd + bi];
         B.DF[f * NP + p] = df;
-        const real fo = B.Fin[f * NP + p] + bdt * df;
+        const real fo = fin[f] + bdt * df;
         B.Fout[f * NP + p] = fo;
         if (fq) fq[(long)f * B.nr * B.ns] = fo;
     }
