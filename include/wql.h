@@ -193,6 +193,18 @@ int wql_get_fault_rates(wql_ctx *ctx, int block, double *DS, double *DW);
  * library keeps lambda = (1 - r) tau / (G dt) and gammap += dt lambda.
  * WQL_E_STATE if the problem is elastic. */
 int wql_get_plastic(wql_ctx *ctx, int block, double *P);
+/* station seismograms (seismogram.f90:12-527).  wql_find_stations is
+ * Find_Coordinates (seismogram.f90:531-650) on the host, no device call:
+ * for each station (x, y, z) of xyz[n][3] the node of X[3][ns][nr][nq]
+ * nearest to it among those within half the local grid diagonal, the last
+ * such node in (s, r, q) order on ties, as 0-based (i, j, k) in ijk[n][3],
+ * or -1 where the station lies off the block.  wql_set_stations records the
+ * stations' nodes of a block (n = 0 clears them); wql_get_stations returns
+ * v[n][3] = vx vy vz at them from the current fields, the values
+ * write_seismogram prints (seismogram.f90:460-478). */
+int wql_find_stations(const double *X, int nq, int nr, int ns, int n, const double *xyz, int *ijk);
+int wql_set_stations(wql_ctx *ctx, int block, int n, const int *ijk);
+int wql_get_stations(wql_ctx *ctx, int block, double *v);
 /* peak tangential slip rate per fault node over the run, sampled where the
  * reference samples Svel (stage 1, CouplingForcing.f90:213-228) [nr*ns] */
 int wql_get_fault_peak(wql_ctx *ctx, int block, double *psr);
@@ -275,6 +287,13 @@ int wql_server_pool_stats(const wql_server *srv, long out[4]);
  * 1 shortest job first with aging (key: nodes x steps / (1 + seconds
  * waited)).  No counterpart in the reference. */
 int wql_server_set_policy(wql_server *srv, int policy);
+/* launch shape of the main volume kernel (as wql_set_tile / _order) for
+ * scenarios whose blocks are n[0] x n[1] x n[2] in `precision` on a
+ * Cartesian (cartesian = 1) or curvilinear grid, applied when such a
+ * scenario is admitted; the autotuner's per-shape choices are registered
+ * here.  No counterpart in the reference. */
+int wql_server_set_shape(wql_server *srv, const int n[3], int precision, int cartesian, int variant, int zchunk,
+                         int order, int group);
 
 #ifdef __cplusplus
 }

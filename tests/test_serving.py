@@ -143,3 +143,34 @@ def test_pool_frees_least_recently_used_for_new_blocks():
     ref = _alone(a, 'fp32')
     for r in (res[0], res[2], res[3]):
         assert np.array_equal(r['S'], ref['S'])
+
+
+def test_tuned_shape_registered(tmp_path):
+    """the autotuner's per-size launch shape reaches the server: it is
+    registered at the first submission of that size, bad shapes are refused,
+    and the scenario's result equals the same scenario run alone with that
+    launch shape"""
+    import ctypes as C
+    import json
+
+    from wql import autotune, native
+    from wql.serving import Server, run_stream
+    from wql.solver import Simulation
+    print('[gpu] serving tuned shape', file=sys.stderr, flush=True)
+    sc = _scenario(1)
+    cache = tmp_path / 'tune.json'
+    entry = {'variant': 6, 'zchunk': 4, 'order': 0, 'group': 1}
+    cache.write_text(json.dumps({autotune.key(sc['problem'], 'fp32'): entry}))
+    with Server(tune_cache=str(cache)) as srv:
+        assert srv.register_shape(sc['problem'], 'fp32') == entry
+        assert srv.register_shape(_scenario(0)['problem'], 'fp32') is None  # size not in the cache
+        n = (C.c_int * 3)(*sc['problem'].blocks[0].n)
+        assert native.lib().wql_server_set_shape(srv.srv, n, native.WQL_FP32, 1, 99, 4, 0, 1) == native.WQL_E_ARG
+    _, _, res = run_stream([sc], np.zeros(1), max_active=1, steps_per_launch=7, precision='fp32',
+                           keep_results=True, tune_cache=str(cache))
+    with Simulation(sc['problem'], precision='fp32', coords=sc['coords'], tile=6, zchunk=4) as sim:
+        sim.step(sc['steps'])
+        sim.sync()
+        f = sim.fault(0)
+        assert np.array_equal(res[0]['S'], f['S'])
+        assert np.array_equal(res[0]['trup'], f['trup'][0])

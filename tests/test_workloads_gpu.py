@@ -77,15 +77,34 @@ def _run_oracle(name, scale):
     return out
 
 
+def _tuned(name, prec):
+    """the launch shape the autotuner's cache holds for the full-size workload"""
+    from wql import autotune
+    from wql import workloads as Wk
+    if name.startswith('scenario'):
+        return None
+    full = Wk.WORKLOADS[name](scale=1.0)['problem']
+    return autotune.lookup(full, prec)
+
+
 @pytest.mark.parametrize('prec', ['fp64', 'fp32'])
+@pytest.mark.parametrize('shape', ['default', 'tuned'])
 @pytest.mark.parametrize('name,scale', CASES)
-def test_workload_vs_oracle(name, scale, prec):
+def test_workload_vs_oracle(name, scale, shape, prec):
+    """shape 'tuned': the reduced run on the launch shape the autotuner
+    chose for the full-size workload"""
     from wql.solver import Simulation
-    _say(f'[gpu] workload {name} scale {scale} {prec}')
+    kw = {}
+    if shape == 'tuned':
+        e = _tuned(name, prec)
+        if e is None:
+            pytest.skip('no tuned shape for this workload')
+        kw = dict(tile=e['variant'], zchunk=e['zchunk'], order=e.get('order', 0))
+    _say(f'[gpu] workload {name} scale {scale} {prec} {shape} {kw}')
     ref = _run_oracle(name, scale)
     W = _workload(name, scale)
     prob = W['problem']
-    with Simulation(prob, precision=prec, coords=W['coords']) as sim:
+    with Simulation(prob, precision=prec, coords=W['coords'], **kw) as sim:
         _start(sim, W, False)
         sim.step(STEPS)
         sim.sync()

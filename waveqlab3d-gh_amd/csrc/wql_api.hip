@@ -123,6 +123,10 @@ struct DevBlock {
     double *frame[6] = {}, *fgm[6] = {}, *fmat[6] = {};
     double *fx = nullptr;
     double *P = nullptr, *Y = nullptr;
+    // stations (wql_set_stations): node offsets [nst] and velocity staging [nst][3]
+    int nst = 0, st_cap = 0;
+    long *st_off = nullptr;
+    double *st_v = nullptr;
 };
 
 }  // namespace
@@ -1272,6 +1276,81 @@ int wql_get_fault_faces(wql_ctx *c, double *Uface, double *Vface)
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return fail(WQL_E_HIP, "fault face download: %s", hipGetErrorString(e));
         if (int rc = check_launch()) return rc;
+    }
+    return 0;
+}
+
+int wql_set_stations(wql_ctx *c, int ib, int n, const int *ijk)
+{
+    if (!c || ib < 0 || ib > 1 || n < 0 || (n > 0 && !ijk)) return fail(WQL_E_ARG, "bad argument");
+    DevBlock &d = c->b[ib];
+    std::vector<long> off(n);
+    for (int s = 0; s < n; s++) {
+        const int i = ijk[3 * s], j = ijk[3 * s + 1], k = ijk[3 * s + 2];
+        if (i < 0 || i >= d.nq || j < 0 || j >= d.nr || k < 0 || k >= d.ns)
+            return fail(WQL_E_ARG, "station %d: node (%d, %d, %d) outside block %d", s, i, j, k, ib + 1);
+        off[s] = (long)i + (long)d.px * ((long)j + (long)d.nr * k);
+    }
+    if (n > d.st_cap) {
+        if (int rc = dalloc(c, (void **)&d.st_off, n * sizeof(long))) return rc;
+        if (int rc = dalloc(c, (void **)&d.st_v, 3L * n * sizeof(double))) return rc;
+        d.st_cap = n;
+    }
+    d.nst = n;
+    if (n > 0) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(d.st_off, off.data(), n * sizeof(long), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int wql_get_stations(wql_ctx *c, int ib, double *v)
+{
+    if (!c || ib < 0 || ib > 1 || (c->b[ib].nst > 0 && !v)) return fail(WQL_E_ARG, "bad argument");
+    const DevBlock &d = c->b[ib];
+    if (d.nst == 0) return 0;
+    const unsigned g = (unsigned)((3 * d.nst + 255) / 256);
+    if (c->prec == WQL_FP64)
+        k_station_gather<double><<<g, 256, 0, c->stream>>>(d.nst, d.st_off, d.NP, (const double *)d.F[c->cur], d.st_v);
+    else
+        k_station_gather<float><<<g, 256, 0, c->stream>>>(d.nst, d.st_off, d.NP, (const float *)d.F[c->cur], d.st_v);
+    hipError_t e = hipMemcpyAsync(v, d.st_v, 3L * d.nst * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(WQL_E_HIP, "station download: %s", hipGetErrorString(e));
+    return check_launch();
+}
+
+// Find_Coordinates (seismogram.f90:531-650): per station the node nearest
+// to (x, y, z) among those within half the local grid diagonal (spacings to
+// the next node in q, r and s, or to the previous one at the last node),
+// the last such node in (s, r, q) loop order on ties; -1 where none is
+int wql_find_stations(const double *X, int nq, int nr, int ns, int n, const double *xyz, int *ijk)
+{
+#pragma clang fp contract(off)
+    if (!X || nq < 2 || nr < 2 || ns < 2 || n < 0 || (n > 0 && (!xyz || !ijk))) return fail(WQL_E_ARG, "bad argument");
+    const long N = (long)nq * nr * ns;
+    auto at = [&](int c, int i, int j, int k) { return X[c * N + i + (long)nq * ((long)j + (long)nr * k)]; };
+    for (int st = 0; st < n; st++) {
+        const double x1 = xyz[3 * st], y1 = xyz[3 * st + 1], z1 = xyz[3 * st + 2];
+        double mindist = 1.0e8;
+        int fi = -1, fj = -1, fk = -1;
+        for (int k = 0; k < ns; k++)
+            for (int j = 0; j < nr; j++)
+                for (int i = 0; i < nq; i++) {
+                    const double hx = i < nq - 1 ? at(0, i + 1, j, k) - at(0, i, j, k) : at(0, nq - 1, j, k) - at(0, nq - 2, j, k);
+                    const double hy = j < nr - 1 ? at(1, i, j + 1, k) - at(1, i, j, k) : at(1, i, nr - 1, k) - at(1, i, nr - 2, k);
+                    const double hz = k < ns - 1 ? at(2, i, j, k + 1) - at(2, i, j, k) : at(2, i, j, ns - 1) - at(2, i, j, ns - 2);
+                    const double v0 = at(0, i, j, k) - x1, v1 = at(1, i, j, k) - y1, v2 = at(2, i, j, k) - z1;
+                    const double dist = std::sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+                    const double dist0 = std::sqrt(hx * hx + hy * hy + hz * hz);
+                    if (dist <= mindist && dist <= 0.5 * dist0) {
+                        fi = i; fj = j; fk = k;
+                        mindist = dist;
+                    }
+                }
+        ijk[3 * st] = fi;
+        ijk[3 * st + 1] = fj;
+        ijk[3 * st + 2] = fk;
     }
     return 0;
 }

@@ -2269,6 +2269,17 @@ __global__ void k_qface(int i, int nr, int ns, int px, long NP, const real *__re
     out[t] = (double)F[f * NP + i + (long)px * ((long)j + (long)nr * k)];
 }
 
+// velocities at the station nodes (write_seismogram, seismogram.f90:460-478):
+// out[n][3] = vx vy vz of node off[n]
+template <typename real>
+__global__ void k_station_gather(int n, const long *__restrict__ off, long NP, const real *__restrict__ F,
+                                 double *__restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * n) return;
+    out[t] = (double)F[(long)(t % 3) * NP + off[t / 3]];
+}
+
 // step time advance of a captured step graph: t = t + inc, the same FP64
 // operation as the host-side clock
 __global__ void k_advance_time(double *t, double inc)

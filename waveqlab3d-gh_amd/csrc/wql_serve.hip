@@ -70,6 +70,10 @@ struct wql_server {
     std::deque<int> unreported;  // retired (finished or failed) ids not yet returned by poll
     std::vector<wql_pooled> pool;
     std::unordered_map<int, std::unique_ptr<wql_scenario>> all;
+    // tuned launch shapes of the main volume kernel by block size, precision
+    // and grid kind (wql_server_set_shape), applied at admission
+    struct shape { int n[3], prec, cart, variant, zchunk, order, group; };
+    std::vector<shape> shapes;
 };
 
 namespace {
@@ -190,6 +194,15 @@ int scenario_admit(wql_server *s, wql_scenario *sc)
         std::vector<double>().swap(sc->X[ib]);
         std::vector<double>().swap(sc->M[ib]);
     }
+    for (const auto &e : s->shapes) {
+        const wql_block_desc &b = sc->blocks[0];
+        if (e.n[0] == b.nq && e.n[1] == b.nr && e.n[2] == b.ns && e.prec == sc->prob.precision &&
+            e.cart == (b.cartesian ? 1 : 0)) {
+            if (int rc = wql_set_tile(ctx, e.variant, e.zchunk)) return rc;
+            if (int rc = wql_set_tile_order(ctx, e.order, e.group)) return rc;
+            break;
+        }
+    }
     if (int rc = wql_init_state(ctx)) return rc;
     if (int rc = wql_use_graph(ctx, 1)) return rc;
     if (hipEventCreateWithFlags(&sc->ev, hipEventDisableTiming) != hipSuccess) return fail(WQL_E_HIP, "hipEventCreate");
@@ -235,6 +248,22 @@ int wql_server_set_policy(wql_server *s, int policy)
 {
     if (!s || policy < 0 || policy > 1) return fail(WQL_E_ARG, "bad server policy");
     s->policy = policy;
+    return 0;
+}
+
+int wql_server_set_shape(wql_server *s, const int n[3], int precision, int cartesian, int variant, int zchunk,
+                         int order, int group)
+{
+    if (!s || !n) return fail(WQL_E_ARG, "null argument");
+    if (variant < 0 || variant >= wql_tile_variants() || zchunk < 1 || order < 0 || order > 3 || group < 1)
+        return fail(WQL_E_ARG, "bad launch shape");
+    wql_server::shape e = {{n[0], n[1], n[2]}, precision, cartesian ? 1 : 0, variant, zchunk, order, group};
+    for (auto &o : s->shapes)
+        if (o.n[0] == n[0] && o.n[1] == n[1] && o.n[2] == n[2] && o.prec == precision && o.cart == e.cart) {
+            o = e;
+            return 0;
+        }
+    s->shapes.push_back(e);
     return 0;
 }
 

@@ -11,7 +11,7 @@ module wql_domain
   use wql_input
   implicit none
   private
-  public :: domain_type, block_type, RK_type, init_RK, init_domain, time_step_RK, close_domain, &
+  public :: domain_type, block_type, station_set, RK_type, init_RK, init_domain, time_step_RK, close_domain, &
             get_fields, get_fault, wp
 
   integer, parameter :: wp = c_double
@@ -28,6 +28,13 @@ module wql_domain
      real(kind = wp), allocatable :: X(:,:,:,:)   ! X(q, r, s, 1:3)
   end type block_type
 
+  !> one block's stations (seismogram_type, datatypes.f90): the located
+  !> stations' output units
+  type :: station_set
+     integer :: n = 0
+     integer, allocatable :: units(:)
+  end type station_set
+
   type :: domain_type
      type(problem_params) :: p                   ! the input namelists
      character(len=256) :: name, problem, coupling, response
@@ -39,6 +46,8 @@ module wql_domain
      type(wql_block_desc) :: desc(2)
      type(c_ptr) :: ctx = c_null_ptr
      integer :: fault_units(8) = -1              ! the eight fault output files
+     type(output_params) :: out                  ! &output_list
+     type(station_set) :: st(2)                  ! station seismograms per block
   end type domain_type
 
   character(len=5), parameter :: EXTS(8) = ['Uface', 'Vface', 'S    ', 'Uhat ', 'Vhat ', 'Svel ', 'trup ', 'state']
@@ -137,6 +146,12 @@ contains
     ! Init_Vel_State (domain.f90:526-529)
     call wql_check(wql_init_state(D%ctx), 'wql_init_state')
     if (D%w_fault) call open_fault_output(D)
+    call read_output_list(ifname, D%out)
+    if (D%out%output_seismograms) then
+       do ib = 1, 2
+          call init_stations(D, ib, ifname)
+       end do
+    end if
   end subroutine init_domain
 
   !> time_step_RK (time_step.f90:68-150): one LSRK(5,4) step on the device;
@@ -150,6 +165,8 @@ contains
     integer, intent(in) :: n
     logical :: out
     if (abs(dt - D%dt) > 1e-15_wp * D%dt .or. RK%nstage /= 5) error stop 'time_step_RK: dt and RK are fixed at init_domain'
+    ! write_output (time_step.f90:137): stations at the start of the step
+    if (mod(n - 1, D%w_stride) == 0) call write_stations(D)
     out = D%w_fault .and. mod(n - 1, D%w_stride) == 0
     if (out) call write_fault_start(D)
     call wql_check(wql_step(D%ctx, 1_c_int), 'wql_step')
@@ -160,7 +177,7 @@ contains
   !> close_domain (domain.f90:567-584)
   subroutine close_domain(D)
     type(domain_type), intent(inout) :: D
-    integer :: k
+    integer :: k, ib
     if (c_associated(D%ctx)) then
        call wql_check(wql_sync(D%ctx), 'wql_sync')
        call wql_check(wql_destroy(D%ctx), 'wql_destroy')
@@ -169,6 +186,13 @@ contains
     do k = 1, 8
        if (D%fault_units(k) >= 0) close(D%fault_units(k))
        D%fault_units(k) = -1
+    end do
+    ! destroy_seismogram (seismogram.f90:491-527)
+    do ib = 1, 2
+       do k = 1, D%st(ib)%n
+          close(D%st(ib)%units(k))
+       end do
+       D%st(ib)%n = 0
     end do
   end subroutine close_domain
 
@@ -188,6 +212,74 @@ contains
     call wql_check(wql_get_fault(D%ctx, int(ib - 1, c_int), c_loc(S), c_loc(W), c_loc(trup), c_loc(Svel), &
                                  c_null_ptr, c_null_ptr), 'wql_get_fault')
   end subroutine get_fault
+
+  ! ---- station seismograms (seismogram.f90:12-489) -------------------------
+  !> init_seismogram for block ib: the stations' nodes (Find_Coordinates,
+  !> seismogram.f90:531-650, in the library), one file per located station
+  !> named <name>_<i>_<j>_<k>_block<b>.dat or, with station_xyz_index,
+  !> <name>_<x>_<y>_<z>.dat, under <station_file_directory>/block<b>
+  subroutine init_stations(D, ib, ifname)
+    type(domain_type), intent(inout) :: D
+    integer, intent(in) :: ib
+    character(*), intent(in) :: ifname
+    real(kind = wp), allocatable :: xyz(:,:)
+    integer(c_int), allocatable :: ijk(:,:), nodes(:,:)
+    integer :: n, s, m, stat
+    character(len=32) :: xs, ys, zs, bs
+    character(len=512) :: dir, fn
+    call read_station_list(ifname, D%out, ib, xyz, n)
+    allocate(ijk(3, max(n, 1)), nodes(3, max(n, 1)))
+    call wql_check(wql_find_stations(D%B(ib)%X, int(D%B(ib)%nq, c_int), int(D%B(ib)%nr, c_int), &
+                                     int(D%B(ib)%ns, c_int), int(n, c_int), xyz, ijk), 'wql_find_stations')
+    write(bs, '(i0)') ib
+    if (len_trim(D%out%station_file_directory) > 0 .and. trim(adjustl(D%out%station_file_directory)) /= '.') then
+       dir = trim(adjustl(D%out%station_file_directory)) // '/block' // trim(bs)
+    else
+       dir = 'block' // trim(bs)
+    end if
+    call execute_command_line('mkdir -p "' // trim(dir) // '"')
+    allocate(D%st(ib)%units(max(n, 1)))
+    m = 0
+    do s = 1, n
+       if (any(ijk(:, s) < 0)) cycle
+       m = m + 1
+       nodes(:, m) = ijk(:, s)
+       if (D%out%station_xyz_index) then
+          write(xs, '(f20.3)') xyz(1, s)
+          write(ys, '(f20.3)') xyz(2, s)
+          write(zs, '(f20.3)') xyz(3, s)
+          fn = trim(adjustl(D%name)) // '_' // trim(adjustl(xs)) // '_' // trim(adjustl(ys)) // '_' // &
+               trim(adjustl(zs)) // '.dat'
+       else
+          write(fn, '(a,i0,a,i0,a,i0,a,i0,a)') trim(adjustl(D%name)) // '_', ijk(1, s) + 1, '_', ijk(2, s) + 1, '_', &
+               ijk(3, s) + 1, '_block', ib, '.dat'
+       end if
+       ! the problem name may carry a directory of its own (name = 'data/...')
+       if (index(fn, '/', back = .true.) > 0) &
+            call execute_command_line('mkdir -p "' // trim(dir) // '/' // fn(1:index(fn, '/', back = .true.) - 1) // '"')
+       open(newunit = D%st(ib)%units(m), file = trim(dir) // '/' // trim(fn), status = 'replace', action = 'write', &
+            iostat = stat)
+       if (stat /= 0) error stop 'init_seismogram: cannot open a station file'
+    end do
+    D%st(ib)%n = m
+    call wql_check(wql_set_stations(D%ctx, int(ib - 1, c_int), int(m, c_int), nodes), 'wql_set_stations')
+  end subroutine init_stations
+
+  !> write_seismogram (seismogram.f90:460-478): t, vx, vy, vz per station
+  subroutine write_stations(D)
+    type(domain_type), intent(inout) :: D
+    real(kind = wp), allocatable :: v(:,:)
+    integer :: ib, s
+    do ib = 1, 2
+       if (D%st(ib)%n == 0) cycle
+       allocate(v(3, D%st(ib)%n))
+       call wql_check(wql_get_stations(D%ctx, int(ib - 1, c_int), v), 'wql_get_stations')
+       do s = 1, D%st(ib)%n
+          write(D%st(ib)%units(s), '(4f15.10)') D%t, v(:, s)
+       end do
+       deallocate(v)
+    end do
+  end subroutine write_stations
 
   ! ---- fault output (fault_output.f90:13-135): raw FP32 records ------------
   subroutine open_fault_output(D)

@@ -168,6 +168,25 @@ module wql_gpu
        type(c_ptr), value :: ctx
        type(c_ptr), value :: Uface, Vface
      end function wql_get_fault_faces
+     integer(c_int) function wql_find_stations(X, nq, nr, ns, n, xyz, ijk) bind(C, name='wql_find_stations')
+       import :: c_int, c_double
+       real(c_double), intent(in) :: X(*)
+       integer(c_int), value :: nq, nr, ns, n
+       real(c_double), intent(in) :: xyz(*)
+       integer(c_int), intent(out) :: ijk(*)
+     end function wql_find_stations
+     integer(c_int) function wql_set_stations(ctx, block, n, ijk) bind(C, name='wql_set_stations')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: block, n
+       integer(c_int), intent(in) :: ijk(*)
+     end function wql_set_stations
+     integer(c_int) function wql_get_stations(ctx, block, v) bind(C, name='wql_get_stations')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: block
+       real(c_double), intent(out) :: v(*)
+     end function wql_get_stations
      integer(c_int) function wql_set_fault(ctx, block, S, W) bind(C, name='wql_set_fault')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -8,7 +8,7 @@ module wql_input
   use iso_c_binding, only : c_double
   implicit none
   private
-  public :: problem_params, block_temp_parameters, read_input
+  public :: problem_params, block_temp_parameters, read_input, output_params, read_output_list, read_station_list
 
   integer, parameter :: wp = c_double
 
@@ -38,7 +38,122 @@ module wql_input
      type(block_temp_parameters) :: btp(2)
   end type problem_params
 
+  !> the station-output variables of &output_list with init_seismogram's
+  !> defaults (seismogram.f90:54-95)
+  type :: output_params
+     logical :: output_seismograms = .false., station_xyz_index = .false.
+     character(len=256) :: station_list = 'infile', station_list_file = '', station_file_directory = 'seismogram'
+  end type output_params
+
 contains
+
+  !> &output_list of the file `path` (seismogram.f90:54-104); the group's
+  !> other variables are read and not used by this host
+  subroutine read_output_list(path, o)
+    character(*), intent(in) :: path
+    type(output_params), intent(out) :: o
+    logical :: output_exact_moment, output_seismograms, output_station_info, output_fault_topo, &
+               output_fields_block1, output_fields_block2, station_xyz_index, debug_station_info, save_debug_station
+    integer :: stride_fields, u, stat
+    character(len=256) :: station_list, station_list_file, station_file_directory
+    character(len=512) :: msg
+    namelist /output_list/ output_exact_moment, output_seismograms, output_station_info, output_fault_topo, &
+                           output_fields_block1, output_fields_block2, stride_fields, station_xyz_index, &
+                           debug_station_info, save_debug_station, station_list, station_list_file, &
+                           station_file_directory
+    output_exact_moment = .false.; output_seismograms = o%output_seismograms; output_station_info = .true.
+    output_fault_topo = .false.; output_fields_block1 = .false.; output_fields_block2 = .false.
+    stride_fields = 1; station_xyz_index = o%station_xyz_index; debug_station_info = .false.
+    save_debug_station = .false.; station_list = o%station_list; station_list_file = o%station_list_file
+    station_file_directory = o%station_file_directory
+    open(newunit = u, file = trim(path), status = 'old', action = 'read', iostat = stat)
+    if (stat /= 0) error stop ':: Cannot open input file'
+    read(u, nml = output_list, iostat = stat, iomsg = msg)
+    close(u)
+    if (stat > 0) then
+       write(*, '(a)') trim(msg)
+       error stop 'error reading namelist output_list'
+    end if
+    o%output_seismograms = output_seismograms
+    o%station_xyz_index = station_xyz_index
+    o%station_list = station_list
+    o%station_list_file = station_list_file
+    o%station_file_directory = station_file_directory
+  end subroutine read_output_list
+
+  !> the stations block `block` looks for (seismogram.f90:128-326):
+  !> xyz(3, n) from the lines between '!---begin:station_list---' and
+  !> '!---end:station_list---' of the input file or of station_list_file
+  !> (station_list = 'extfile'), or the block's own list
+  !> ('..._listU---' block 1, '..._listV---' block 2) in the input file
+  subroutine read_station_list(path, o, block, xyz, n)
+    character(*), intent(in) :: path
+    type(output_params), intent(in) :: o
+    integer, intent(in) :: block
+    real(kind = wp), allocatable, intent(out) :: xyz(:,:)
+    integer, intent(out) :: n
+    character(len=1024) :: src
+    character(len=1) :: tag
+    logical :: ext, found
+    ext = trim(o%station_list) == 'extfile' .or. trim(o%station_list) == 'exfile'
+    src = path
+    if (ext) then
+       if (len_trim(o%station_list_file) == 0) error stop 'error: station_list_file must be specified when station_list=extfile'
+       src = o%station_list_file
+    end if
+    call list_between(trim(src), '!---begin:station_list---', '!---end:station_list---', xyz, n, found)
+    if (found) return
+    if (ext) error stop 'error: !---begin:station_list--- not found in external station file'
+    tag = merge('U', 'V', block == 1)
+    call list_between(trim(src), '!---begin:station_list' // tag // '---', '!---end:station_list' // tag // '---', &
+                      xyz, n, found)
+    if (.not. found) error stop 'error reading seismogram station list'
+  end subroutine read_station_list
+
+  subroutine list_between(path, begin, end, xyz, n, found)
+    character(*), intent(in) :: path, begin, end
+    real(kind = wp), allocatable, intent(out) :: xyz(:,:)
+    integer, intent(out) :: n
+    logical, intent(out) :: found
+    character(len=1024) :: line
+    integer :: u, stat, k
+    n = 0
+    found = .false.
+    open(newunit = u, file = path, status = 'old', action = 'read', iostat = stat)
+    if (stat /= 0) then
+       write(*, '(a,a)') 'error: cannot open station list file: ', trim(path)
+       error stop
+    end if
+    do
+       read(u, '(a)', iostat = stat) line
+       if (stat /= 0) exit
+       if (trim(line) == begin) then
+          found = .true.
+          exit
+       end if
+    end do
+    if (.not. found) then
+       close(u)
+       allocate(xyz(3, 0))
+       return
+    end if
+    do
+       read(u, '(a)', iostat = stat) line
+       if (stat /= 0) error stop 'error reading seismogram station list'
+       if (trim(line) == end) exit
+       n = n + 1
+    end do
+    allocate(xyz(3, n))
+    rewind(u)
+    do
+       read(u, '(a)') line
+       if (trim(line) == begin) exit
+    end do
+    do k = 1, n
+       read(u, *) xyz(:, k)
+    end do
+    close(u)
+  end subroutine list_between
 
   !> read &problem_list and &block_list from the file `path`
   subroutine read_input(path, p)

@@ -71,15 +71,27 @@ class FaultOutput:
             sim.step(nsteps - 1)
 
 
-def run(sim, nt, w_stride, name):
-    """the reference's time loop (main.f90:83-96) with fault output every
-    `w_stride` steps: nt steps, records at steps 1, 1 + w_stride, ..."""
-    with FaultOutput(name) as out:
+def run(sim, nt, w_stride, name, stations=None, fault=True):
+    """the reference's time loop (main.f90:83-96) with its output every
+    `w_stride` steps (write_output, time_step.f90:137): nt steps, records at
+    steps 1, 1 + w_stride, ...; fault output when `fault` (w_fault), and the
+    station lines of `stations` (a seismogram.Seismograms) at the start of
+    each output step"""
+    out = FaultOutput(name) if fault else None
+    try:
         n = 1
         while n <= nt:
             k = min(w_stride, nt - n + 1)
-            out.write_step(sim, k)
+            if stations is not None:
+                stations.write()
+            if out is not None:
+                out.write_step(sim, k)
+            else:
+                sim.step(k)
             n += k
+    finally:
+        if out is not None:
+            out.close()
     sim.sync()
 
 

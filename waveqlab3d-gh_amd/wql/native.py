@@ -24,7 +24,8 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_server_submit', 'wql_server_poll', 'wql_server_result', 'wql_server_status', 'wql_server_set_policy', 'wql_set_tile', 'wql_get_tile', 'wql_set_tile_order', 'wql_get_tile_order', 'wql_tile_variants', 'wql_use_graph', 'wql_profile',
            'wql_volume_kernel_ms', 'wql_get_fault_faces', 'wql_get_plastic',
            'wql_get_fields_noalloc', 'wql_problem_keys', 'wql_grid_block',
-           'wql_stage', 'wql_server_pool_stats']
+           'wql_stage', 'wql_server_pool_stats', 'wql_server_set_shape', 'wql_find_stations',
+           'wql_set_stations', 'wql_get_stations']
 
 
 class FaultParams(C.Structure):
@@ -107,6 +108,11 @@ def lib():
             'wql_server_status': ([vp, C.POINTER(C.c_long)], C.c_int),
             'wql_server_pool_stats': ([vp, C.POINTER(C.c_long)], C.c_int),
             'wql_server_set_policy': ([vp, C.c_int], C.c_int),
+            'wql_find_stations': ([dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.POINTER(C.c_int)], C.c_int),
+            'wql_set_stations': ([vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
+            'wql_get_stations': ([vp, C.c_int, dp], C.c_int),
+            'wql_server_set_shape': ([vp, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+                                     C.c_int),
             'wql_fault_stats': ([vp, C.c_int, dp], C.c_int),
             'wql_set_tile': ([vp, C.c_int, C.c_int], C.c_int),
             'wql_get_tile': ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),

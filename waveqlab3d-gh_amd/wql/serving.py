@@ -20,14 +20,36 @@ from .solver import block_descs, problem_desc
 class Server:
     POLICIES = {'fifo': 0, 'sjf': 1}
 
-    def __init__(self, max_active=8, hbm_budget=0, steps_per_launch=50, policy='fifo'):
+    def __init__(self, max_active=8, hbm_budget=0, steps_per_launch=50, policy='fifo', tuned=True, tune_cache=None):
         """policy: 'fifo' admits queued scenarios in arrival order, 'sjf'
-        shortest job first with aging (nodes x steps / (1 + seconds waited))"""
+        shortest job first with aging (nodes x steps / (1 + seconds waited));
+        tuned: scenarios whose block size the autotuner's cache (the file
+        tune_cache, default the package's) holds run that launch shape"""
         self.srv = C.c_void_p()
         check(native.lib().wql_server_create(max_active, hbm_budget, steps_per_launch, C.byref(self.srv)))
         check(native.lib().wql_server_set_policy(self.srv, self.POLICIES[policy]))
         self.shape = {}
+        self.tuned = tuned
+        self.tune_cache = tune_cache
+        self.registered = {}
         self._done = (C.c_int * 4096)()
+
+    def register_shape(self, prob, precision):
+        """hand the autotuner's launch shape for this block size to the
+        server (once per size, precision and grid kind); returns it or None"""
+        from . import autotune
+        b = prob.blocks[0]
+        cart = 1 if b.mesh == 'cartesian' else 0
+        k = (tuple(b.n), precision, cart)
+        if k not in self.registered:
+            best = autotune.lookup(prob, precision, self.tune_cache)
+            if best is not None:
+                n = (C.c_int * 3)(*b.n)
+                prec = native.WQL_FP32 if precision == 'fp32' else native.WQL_FP64
+                check(native.lib().wql_server_set_shape(self.srv, n, prec, cart, best['variant'], best['zchunk'],
+                                                        best.get('order', 0), best.get('group', 2)))
+            self.registered[k] = best
+        return self.registered[k]
 
     def close(self):
         if self.srv:
@@ -41,6 +63,8 @@ class Server:
         self.close()
 
     def submit(self, prob, coords, nsteps, precision='fp32'):
+        if self.tuned:
+            self.register_shape(prob, precision)
         pd = problem_desc(prob, precision)
         arr, keep = block_descs(prob, coords)
         sid = C.c_int()
@@ -80,12 +104,13 @@ class Server:
 
 
 def run_stream(scenarios, arrivals, max_active=8, steps_per_launch=50, precision='fp32', keep_results=False,
-               idle_sleep=2e-4, policy='fifo', stats=None, progress=None):
+               idle_sleep=2e-4, policy='fifo', stats=None, progress=None, tuned=True, tune_cache=None):
     """Serve `scenarios` (list of dicts with problem, coords, steps) arriving
     at `arrivals` seconds after the start.  Returns per-scenario latency (s),
     the wall time from first arrival to last result, and optionally results;
     `stats` (a dict) receives the context pool's counters; `progress(n)`
-    is called after each retired scenario with the number retired."""
+    is called after each retired scenario with the number retired; tuned /
+    tune_cache: as for Server."""
     order = np.argsort(arrivals)
     lat = np.zeros(len(scenarios))
     results = {}
@@ -93,7 +118,8 @@ def run_stream(scenarios, arrivals, max_active=8, steps_per_launch=50, precision
     sub_t = {}
     nxt = 0
     ndone = 0
-    with Server(max_active=max_active, steps_per_launch=steps_per_launch, policy=policy) as srv:
+    with Server(max_active=max_active, steps_per_launch=steps_per_launch, policy=policy, tuned=tuned,
+                tune_cache=tune_cache) as srv:
         while ndone < len(scenarios):
             now = time.perf_counter() - t_start
             while nxt < len(order) and arrivals[order[nxt]] <= now:
