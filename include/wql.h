@@ -193,6 +193,45 @@ int wql_get_fault_rates(wql_ctx *ctx, int block, double *DS, double *DW);
  * library keeps lambda = (1 - r) tau / (G dt) and gammap += dt lambda.
  * WQL_E_STATE if the problem is elastic. */
 int wql_get_plastic(wql_ctx *ctx, int block, double *P);
+/* moment-tensor point sources (moment_tensor.f90:82-1270): each source adds
+ * -A(t) M delta to the stress rates of the nodes around its nearest node,
+ * delta the product of three 1-D discrete deltas (singular_source,
+ * moment_tensor.f90:1083-1270: five nodes with the source's offset for
+ * order 2, nine nodes from a table chosen by `alpha` for order 4), A the
+ * source time function at the stage time (source_time, 958-1017).  Applied
+ * in every RK stage after the volume rates (set_rates, domain.f90:908-951). */
+enum {
+    WQL_STF_GAUSSIAN = 0,          /* 'gaussian' */
+    WQL_STF_LOH_DISCONTINUITY = 1, /* 'LOH_discontinuity' */
+    WQL_STF_BRUNE = 2,             /* 'Brune' */
+    WQL_STF_COS_BELL = 3,          /* 'Cos-Bell' */
+    WQL_STF_COSINE = 4,            /* 'cosine' */
+    WQL_STF_LOH_MOMENT = 5,        /* 'LOH_moment_time_history' */
+    WQL_STF_SINE_MOMENT = 6        /* 'sine_moment_time_history' */
+};
+typedef struct {
+    int type;              /* WQL_STF_* */
+    double duration;       /* w of the source time function */
+    double t_init;         /* t0 */
+    double m[6];           /* mXX mYY mZZ mXY mXZ mYZ */
+    double x[3];           /* location */
+    int node[3];           /* 0-based nearest node; -2 in each component where the block does not
+                              hold the source (the reference's index -1, kept: its 1-D deltas
+                              still reach the block's first nodes) */
+    double node_x[3];      /* coordinates of that node (-1 where none, as the reference) */
+} wql_source;
+/* init_moment_tensor's location step (moment_tensor.f90:280-372 with
+ * Find_Coordinates_moment, seismogram.f90:652-732) on the host, no device
+ * call: the nearest node of X[3][ns][nr][nq] to each of xyz[n][3] if the
+ * location lies within the block's coordinate ranges and between its first
+ * and last nodes; src[s].node and .node_x are filled, the other fields are
+ * left as given */
+int wql_locate_sources(const double *X, int nq, int nr, int ns, int n, const double *xyz, wql_source *src);
+/* the sources of one block (n = 0: none); order 2 or 4; alpha (order 4:
+ * 0, 25, 50 or 100) is the first source's, as the reference uses
+ * alpha(1) for all of a block's sources */
+int wql_set_sources(wql_ctx *ctx, int block, int n, const wql_source *src, int order, int alpha);
+
 /* station seismograms (seismogram.f90:12-527).  wql_find_stations is
  * Find_Coordinates (seismogram.f90:531-650) on the host, no device call:
  * for each station (x, y, z) of xyz[n][3] the node of X[3][ns][nr][nq]

@@ -26,11 +26,17 @@ class OraFault(C.Structure):
                 ('frc', C.c_double * 5), ('rs_solver', C.c_int)]
 
 
+class OraSource(C.Structure):
+    _fields_ = [('type', C.c_int), ('duration', C.c_double), ('t_init', C.c_double), ('m', C.c_double * 6),
+                ('x', C.c_double * 3), ('node', C.c_int * 3), ('node_x', C.c_double * 3)]
+
+
 class OraBlock(C.Structure):
     _fields_ = [('nq', C.c_int), ('nr', C.c_int), ('ns', C.c_int), ('cartesian', C.c_int),
                 ('bc', C.c_int * 6), ('pml', C.c_int * 6), ('npml', C.c_int),
                 ('X', dp), ('metric', dp), ('M', dp), ('nml', dp * 6), ('F', dp), ('DF', dp),
-                ('Q', dp * 6), ('DQ', dp * 6), ('P', dp)]
+                ('Q', dp * 6), ('DQ', dp * 6), ('P', dp),
+                ('nsrc', C.c_int), ('src_order', C.c_int), ('src_alpha', C.c_int), ('src', C.POINTER(OraSource))]
 
 
 class OraIface(C.Structure):
@@ -58,6 +64,8 @@ def lib():
         _lib.ora_interface.argtypes = [C.POINTER(OraDomain), C.c_int, C.c_double, C.c_int]
         _lib.ora_step.argtypes = [C.POINTER(OraDomain), C.c_int]
         _lib.ora_init_state.argtypes = [C.POINTER(OraDomain)]
+        _lib.ora_sources.argtypes = [C.POINTER(OraBlock), C.c_double]
+        _lib.ora_locate_sources.argtypes = [C.POINTER(OraBlock), C.c_int, dp, C.POINTER(OraSource)]
         _lib.ora_d1.argtypes = [dp, C.c_int, C.c_int, C.c_double]
         _lib.ora_d1.restype = C.c_double
         _lib.ora_rk_coeffs.argtypes = [dp, dp, dp]
@@ -145,9 +153,38 @@ class OracleDomain:
             setattr(self.d, k, getattr(prob, k))
         self.d.t = 0.0
         self.d.dt = prob.dt()
+        if getattr(prob, 'moment', None):
+            from wql.problem import block_sources
+            for ib in range(2):
+                src, held, order, alpha = block_sources(prob, ib)
+                self.set_sources(ib, [dict(s_, unlocated=not h) for s_, h in zip(src, held)], order, alpha)
 
     def init_state(self):
         lib().ora_init_state(C.byref(self.d))
+
+    def set_sources(self, ib, sources, order=2, alpha=0):
+        """moment-tensor sources of block ib: dicts with type, duration,
+        t_init, m (6), x (3); located on the block's grid by the oracle"""
+        n = len(sources)
+        arr = (OraSource * max(n, 1))()
+        xyz = np.ascontiguousarray([s_['x'] for s_ in sources] or [[0.0, 0.0, 0.0]], dtype=np.float64)
+        lib().ora_locate_sources(C.byref(self.d.b[ib]), n, _p(xyz), arr)
+        for k, s_ in enumerate(sources):
+            arr[k].type = s_['type']
+            arr[k].duration = s_['duration']
+            arr[k].t_init = s_['t_init']
+            arr[k].m[:] = list(s_['m'])
+            if s_.get('unlocated'):
+                arr[k].node[:] = [-2, -2, -2]
+                arr[k].node_x[:] = [-1.0, -1.0, -1.0]
+        self.keep.append(arr)
+        b = self.d.b[ib]
+        b.nsrc, b.src_order, b.src_alpha = n, order, alpha
+        b.src = C.cast(arr, C.POINTER(OraSource))
+        return [(tuple(arr[k].node), tuple(arr[k].node_x)) for k in range(n)]
+
+    def sources(self, ib, t):
+        lib().ora_sources(C.byref(self.d.b[ib]), t)
 
     def set_rates(self, ib):
         lib().ora_set_rates(C.byref(self.d.b[ib]))

@@ -1193,6 +1193,118 @@ void ora_init_state(ora_domain *d)
 
 /* time_step_RK time_step.f90:68-150 with the 2-rank domain semantics of
  * domain.f90:586-994 */
+/* ---- moment-tensor point sources (moment_tensor.f90:958-1270) ---------- */
+
+static double stf(int type, double t, double t0, double w) /* source_time, 958-1017 */
+{
+    const double pi = 3.141592653589793;
+    switch (type) {
+    case 0: return (w / sqrt(2.0 * pi)) * exp((-(w * w) * ((t - t0) * (t - t0))) / 2.0);
+    case 1: return (t / (w * w)) * exp(-t / w);
+    case 2: return (w * w) * t * exp(-w * t);
+    case 3: return t < t0 ? (1.0 - cos(2.0 * pi * t / t0)) / t0 : 0.0;
+    case 4: return (t >= 0.0 && t <= w) ? 1.0 - cos(2.0 * pi * t / w) : 0.0;
+    case 5: return 1.0 - (1.0 + t / w) * exp(-t / w);
+    default: return t <= w ? t - w / (2.0 * pi) * sin(2.0 * pi * t / w) : w;
+    }
+}
+
+/* singular_source (1083-1270): 1-D discrete delta at node i (0-based) */
+static double singular(int i, int i0, double h, double a, int order, int alpha)
+{
+    int k = i - i0;
+    if (order == 2) {
+        double beta = (1.0 - 2.0 * a) / 4.0 + 0.25, gam = (2.0 * a - 1.0) / 16.0, gap = (2.0 * a + 1.0) / 16.0;
+        if (k == -2) return (-gam) / h;
+        if (k == -1) return (beta + 3.0 * gam - gap) / h;
+        if (k == 0) return (1.0 - a - 2.0 * beta - 3.0 * gam + 3.0 * gap) / h;
+        if (k == 1) return (a + beta + gam - 3.0 * gap) / h;
+        if (k == 2) return gap / h;
+        return 0.0;
+    }
+    static const double tab[4][9] = {
+        {-0.01171875, -0.03125, 0.046875, 0.28125, 0.4296875, 0.28125, 0.046875, -0.03125, -0.01171875},
+        {-0.00246875, -0.02275, -0.014125, 0.16575, 0.4171875, 0.38675, 0.117875, -0.02975, -0.01846875},
+        {-0.00728125, -0.028, 0.014875, 0.224, 0.4265625, 0.336, 0.081375, -0.032, -0.01553125},
+        {-0.0095625, -0.02990625, 0.03053125, 0.25284375, 0.42890625, 0.30903125, 0.06384375, -0.03196875,
+         -0.01371875}};
+    if (k < -4 || k > 4) return 0.0;
+    int row = alpha == 0 ? 0 : alpha == 25 ? 1 : alpha == 50 ? 2 : 3;
+    return tab[row][k + 4] / h;
+}
+
+void ora_sources(ora_block *b, double t) /* set_moment_tensor, 1020-1080 */
+{
+    const int nq = b->nq, nr = b->nr, ns = b->ns;
+    const size_t NP = NPTS(b);
+    const int w = b->src_order == 2 ? 2 : 4;
+    for (int s = 0; s < b->nsrc; s++) {
+        const ora_source *S = &b->src[s];
+        const double A = stf(S->type, t, S->t_init, S->duration);
+        /* nodes outside the support get delta = 0 (an exact no-op there) */
+        for (int k = S->node[2] - w; k <= S->node[2] + w; k++)
+            for (int j = S->node[1] - w; j <= S->node[1] + w; j++)
+                for (int i = S->node[0] - w; i <= S->node[0] + w; i++) {
+                    if (i < 0 || i >= nq || j < 0 || j >= nr || k < 0 || k >= ns) continue;
+                    size_t p = IDX(b, i, j, k);
+                    double hx = (1.0 / ((double)nq - 1.0)) / b->metric[0 * NP + p];
+                    double hy = (1.0 / ((double)nr - 1.0)) / b->metric[4 * NP + p];
+                    double hz = (1.0 / ((double)ns - 1.0)) / b->metric[8 * NP + p];
+                    double ax = (S->x[0] - S->node_x[0]) / hx;
+                    double ay = (S->x[1] - S->node_x[1]) / hy;
+                    double az = (S->x[2] - S->node_x[2]) / hz;
+                    double dx = singular(i, S->node[0], hx, ax, b->src_order, b->src_alpha);
+                    double dy = singular(j, S->node[1], hy, ay, b->src_order, b->src_alpha);
+                    double dz = singular(k, S->node[2], hz, az, b->src_order, b->src_alpha);
+                    double delta = dx * dy * dz;
+                    for (int c = 0; c < 6; c++) b->DF[(3 + c) * NP + p] = b->DF[(3 + c) * NP + p] - A * S->m[c] * delta;
+                }
+    }
+}
+
+void ora_locate_sources(const ora_block *b, int n, const double *xyz, ora_source *src)
+{
+    const int nq = b->nq, nr = b->nr, ns = b->ns;
+    const size_t NP = NPTS(b);
+    const double *X = b->X;
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; c++) {
+        lo[c] = hi[c] = X[c * NP];
+        for (size_t p = 1; p < NP; p++) {
+            if (X[c * NP + p] < lo[c]) lo[c] = X[c * NP + p];
+            if (X[c * NP + p] > hi[c]) hi[c] = X[c * NP + p];
+        }
+    }
+    const size_t last = IDX(b, nq - 1, nr - 1, ns - 1);
+    for (int s = 0; s < n; s++) {
+        const double *x = xyz + 3 * s;
+        int f[3] = {-1, -1, -1};
+        if (lo[0] <= x[0] && x[0] <= hi[0] && lo[1] <= x[1] && x[1] <= hi[1] && lo[2] <= x[2] && x[2] <= hi[2]) {
+            double mindist = 1.0e8;
+            for (int k = 0; k < ns; k++)
+                for (int j = 0; j < nr; j++)
+                    for (int i = 0; i < nq; i++) {
+                        size_t p = IDX(b, i, j, k);
+                        double v0 = X[p] - x[0], v1 = X[NP + p] - x[1], v2 = X[2 * NP + p] - x[2];
+                        double dist = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+                        if (dist <= mindist) { f[0] = i; f[1] = j; f[2] = k; mindist = dist; }
+                    }
+        }
+        int inb = X[0] <= x[0] && x[0] <= X[last] && X[NP] <= x[1] && x[1] <= X[NP + last] &&
+                  X[2 * NP] <= x[2] && x[2] <= X[2 * NP + last];
+        for (int c = 0; c < 3; c++) {
+            src[s].x[c] = x[c];
+            if (inb && f[0] >= 0) {
+                src[s].node[c] = f[c];
+                src[s].node_x[c] = X[c * NP + IDX(b, f[0], f[1], f[2])];
+            } else {
+                src[s].node[c] = -2;
+                src[s].node_x[c] = -1.0;
+            }
+        }
+    }
+}
+
 int ora_step(ora_domain *d, int nsteps)
 {
     double A[5], Bc[5], C[6];
@@ -1218,6 +1330,8 @@ int ora_step(ora_domain *d, int nsteps)
             }
             ora_set_rates(&d->b[0]);
             ora_set_rates(&d->b[1]);
+            ora_sources(&d->b[0], d->t);
+            ora_sources(&d->b[1], d->t);
             for (int ib = 1; ib <= 2; ib++) {
                 ora_boundary(&d->b[ib - 1]);
                 int rc = ora_interface(d, ib, d->t, stage);

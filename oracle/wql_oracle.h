@@ -49,6 +49,17 @@ typedef struct {
     int rs_solver;      /* 0 the reference's regula falsi, 1 the library's bracketed Newton */
 } ora_fault;
 
+/* one moment-tensor point source (same meaning as wql_source) */
+typedef struct {
+    int type;           /* 0 gaussian, 1 LOH_discontinuity, 2 Brune, 3 Cos-Bell, 4 cosine,
+                           5 LOH_moment_time_history, 6 sine_moment_time_history */
+    double duration, t_init;
+    double m[6];        /* mXX mYY mZZ mXY mXZ mYZ */
+    double x[3];
+    int node[3];        /* 0-based nearest node, -2 where the block does not hold the source */
+    double node_x[3];
+} ora_source;
+
 typedef struct {
     int nq, nr, ns;
     int cartesian;      /* 1: reference 'cartesian' interior kernel (JU..:16357-16445) */
@@ -62,6 +73,8 @@ typedef struct {
     double *F, *DF;     /* [9][NP] */
     double *Q[6], *DQ[6]; /* PML slab arrays [9][slab pts] (slab = owned range restricted to the layer) */
     double *P;          /* [2][NP] plastic strain / strain rate (may be NULL) */
+    int nsrc, src_order, src_alpha; /* moment-tensor sources (order 2 or 4; alpha of order 4) */
+    const ora_source *src;
 } ora_block;
 
 typedef struct {        /* one block's copy of the interface state, q-face arrays */
@@ -97,6 +110,13 @@ int ora_step(ora_domain *d, int nsteps);
 void ora_init_state(ora_domain *d);
 /* one SBP first derivative row (for tests): D applied to u at index i of n points */
 double ora_d1(const double *u, int n, int i, double h);
+/* DF -= A(t) M delta of the block's moment-tensor sources (set_moment_tensor,
+ * moment_tensor.f90:1020-1080) */
+void ora_sources(ora_block *b, double t);
+/* nearest nodes of source locations (Find_Coordinates_moment,
+ * seismogram.f90:652-732, and the first/last-node test of
+ * init_moment_tensor, moment_tensor.f90:305-322) */
+void ora_locate_sources(const ora_block *b, int n, const double *xyz, ora_source *src);
 /* LSRK coefficients A[5], B[5], C[6] (time_step.f90:53-62) */
 void ora_rk_coeffs(double *A, double *B, double *C);
 /* the viscoplastic return map of the last RK stage on both blocks

@@ -32,7 +32,8 @@ BASE_BLOCKS = dict(
 def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
              plastic_model='default', pml_l=((False,) * 3,) * 2, pml_r=((False,) * 3,) * 2,
              npml=0, lc=(1.0, 0.0), rc=(0.0, 1.0), profile='analytical_test_problem',
-             rho_s_p=(2.67, 3.464, 6.0), t_final=1.0, aqrs=None, bqrs=None, mu_beta_eta=(0.0, 0.0, 0.0)):
+             rho_s_p=(2.67, 3.464, 6.0), t_final=1.0, aqrs=None, bqrs=None, mu_beta_eta=(0.0, 0.0, 0.0),
+             moment=None):
     T = lambda b: 'T' if b else 'F'
     s = ["&problem_list",
          f" name = 'data/{name}',", f" problem = '{problem}',", f" response = '{response}',",
@@ -57,8 +58,19 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
               f"btp({i})%pml_lqrs = {', '.join(T(x) for x in pml_l[b])},",
               f"btp({i})%pml_rqrs = {', '.join(T(x) for x in pml_r[b])},",
               f"btp({i})%npml = {npml},", f"btp({i})%faultsize = {n[1]}, {n[2]}" + (',' if b == 0 else ' /')]
-    s += ["", "&moment_list", " use_moment_tensor = F /", "", "&mms_list", " use_mms = F /", "",
+    if moment is None:
+        s += ["", "&moment_list", " use_moment_tensor = F /"]
+    else:
+        s += ["", "&moment_list", " use_moment_tensor = T,", f" order = {moment['order']},",
+              f" tensor_block_preference = {moment.get('preference', 1)} /"]
+    s += ["", "&mms_list", " use_mms = F /", "",
           "&output_list", " output_seismograms = F,", " output_fault_topo = F /", ""]
+    if moment is not None:
+        # the source lists (moment_tensor.f90:82-270): one line per source,
+        # type duration t_init mXX mYY mZZ mXY mXZ mYZ x y z alpha
+        for tag in ('', 'U', 'V'):
+            if moment.get(tag or 'union'):
+                s += [f"!---begin:tensor_list{tag}---"] + moment[tag or 'union'] + [f"!---end:tensor_list{tag}---", ""]
     return "\n".join(s) + "\n"
 
 
@@ -109,6 +121,22 @@ CASES = {
     # TPV26 with the 'scec' viscoplastic return map: a 2 m/s velocity pulse
     # drives the stresses past the Drucker-Prager yield over the depth range,
     # so the return map runs on the TPV26 background tensor
+    # moment-tensor point sources in a locked two-block domain: the unified
+    # list (the preferred block 1 keeps the sources; one lies in block 2, so
+    # neither block holds it), order 2; and the per-block lists on the
+    # curvilinear grid, order 4 with the alpha = 25 table
+    'mt_cart': (dict(problem='TPV5', coupling='locked', mesh='cartesian', n=N_STD, lq=LQ_STD, rq=RQ_STD,
+                     moment=dict(order=2, preference=1,
+                                 union=['gaussian 20.0 0.05 1.0 0.5 -0.3 0.2 0.1 -0.4 -1.73 7.31 0.22 0',
+                                        'Brune 30.0 0.0 -0.5 0.8 0.4 0.0 0.3 0.1 1.41 7.62 -0.37 0',
+                                        "'Cos-Bell' 0.0 0.08 0.2 0.2 0.2 0.0 0.0 0.0 -2.2 8.1 -0.9 0"])),
+                [('rhs', ['0.04']), ('traj', ['12'])]),
+    'mt_curv': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=N_STD, lq=LQ_STD, rq=RQ_STD,
+                     moment=dict(order=4, preference=1,
+                                 U=['Brune 15.0 0.0 0.7 -0.2 0.4 0.3 0.0 -0.1 -1.2 7.7 0.4 25'],
+                                 V=['cosine 0.1 0.0 0.3 0.3 -0.6 0.2 0.5 0.1 1.1 7.1 -0.3 25',
+                                    'sine_moment_time_history 0.06 0.0 0.1 -0.4 0.2 0.0 0.2 0.3 1.6 7.9 0.5 25'])),
+                [('rhs', ['0.03']), ('traj', ['12'])]),
     'tpv26_plastic': (dict(problem='TPV26', coupling='slip-weakening_friction', mesh='cartesian',
                            n=N_STD, lq=LQ_STD, rq=RQ_STD, response='plastic', plastic_model='scec',
                            aqrs=[(-6.0, 2.0, -9.0), (0.0, 2.0, -9.0)], bqrs=[(0.0, 24.0, 3.0), (6.0, 24.0, 3.0)]),

@@ -30,6 +30,10 @@ def oracle_rhs(prob, t):
     load_pattern_state(_NullSim(), prob, od)
     for ib in range(2):
         od.set_rates(ib)
+    # set_rates (domain.f90:908-951): the moment-tensor sources follow the
+    # volume rates of both blocks
+    for ib in range(2):
+        od.sources(ib, t)
     vol = [od.blocks[ib]['DF'].copy() for ib in range(2)]
     dqv = [[None if od.blocks[ib]['DQ'][s] is None else od.blocks[ib]['DQ'][s].copy() for s in range(6)]
            for ib in range(2)]

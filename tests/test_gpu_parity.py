@@ -292,7 +292,7 @@ def test_graph_replays_match_eager(case):
 
 
 @pytest.mark.parametrize('case', ['curv_sw', 'curv_rs_long', 'pml_locked', 'plastic_sw', 'rw_plastic', 'tpv26_sw',
-                                  'tpv26_plastic'])
+                                  'tpv26_plastic', 'mt_curv'])
 def test_fp32_volume_tracks_fp64(case):
     """FP32 volume fields with the FP64 fault stay close to the FP64 path:
     relative L2 of the fields and of the slip <= 1e-5 at the trajectory's end

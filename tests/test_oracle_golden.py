@@ -70,3 +70,33 @@ def test_trajectory(case):
         for s in range(6):
             if f'b{ib + 1}_Q{s + 1}' in z:
                 assert G.rel_l2(a['Q'][s].ravel(), z[f'b{ib + 1}_Q{s + 1}']) <= 1e-12
+
+
+@pytest.mark.parametrize('case', ['mt_cart', 'mt_curv'])
+def test_source_location_library_matches_oracle(case):
+    """the library's host-side source location (wql_locate_sources) and the
+    oracle's give the same nodes, for the golden sources and for points on
+    nodes, between nodes, on block faces and outside (CPU: no device call)"""
+    import ctypes as C
+    from wql import native
+    prob = G.problem(case)
+    rng = np.random.default_rng(5)
+    od = O.OracleDomain(prob)
+    for ib, b in enumerate(prob.blocks):
+        X = np.ascontiguousarray(b.coords())
+        pts = [s_['x'] for lst in prob.moment['lists'].values() if lst for s_ in lst]
+        pts += [X[:, 3, 4, 5], X[:, 0, 0, 0], X[:, -1, -1, -1], 0.5 * (X[:, 2, 2, 2] + X[:, 2, 2, 3])]
+        lo, hi = X.reshape(3, -1).min(axis=1), X.reshape(3, -1).max(axis=1)
+        pts += list(rng.uniform(lo - 0.5, hi + 0.5, (12, 3)))
+        xyz = np.ascontiguousarray(pts, dtype=np.float64)
+        n = len(xyz)
+        arr = (native.Source * n)()
+        nq, nr, ns = b.n
+        assert native.lib().wql_locate_sources(X.ctypes.data_as(native.dp), nq, nr, ns, n,
+                                               xyz.ctypes.data_as(native.dp), arr) == 0
+        ref = od.set_sources(ib, [dict(type=0, duration=1.0, t_init=0.0, m=(0,) * 6, x=tuple(p)) for p in xyz])
+        for k in range(n):
+            assert tuple(arr[k].node) == ref[k][0], (ib, k)
+            assert tuple(arr[k].node_x) == ref[k][1], (ib, k)
+        assert any(tuple(arr[k].node) == (-2, -2, -2) for k in range(n))
+        assert any(min(arr[k].node) >= 0 for k in range(n))

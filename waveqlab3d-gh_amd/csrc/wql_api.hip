@@ -127,6 +127,10 @@ struct DevBlock {
     int nst = 0, st_cap = 0;
     long *st_off = nullptr;
     double *st_v = nullptr;
+    // moment-tensor sources (wql_set_sources) and the node box of their supports
+    int nsrc = 0, src_cap = 0, src_order = 2, src_alpha = 0;
+    int src_lo[3] = {0, 0, 0}, src_hi[3] = {0, 0, 0};
+    SourceDev *src = nullptr;
 };
 
 }  // namespace
@@ -337,6 +341,35 @@ void launch_pml(wql_ctx *c, Pair<real> &P, real A, real bdt)
         k_volume_generic<real><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(P, BL, A, A, bdt);
 }
 
+// the moment-tensor sources of both blocks at the stage time (after the
+// volume kernels, before the plastic return map)
+template <typename real>
+void launch_sources(wql_ctx *c, Pair<real> &P, double t, const double *tdev, double tinc, double bdt)
+{
+    for (int ib = 0; ib < 2; ib++) {
+        const DevBlock &d = c->b[ib];
+        if (d.nsrc == 0) continue;
+        SourceArgs a;
+        memset(&a, 0, sizeof a);
+        a.src = d.src;
+        a.n = d.nsrc;
+        a.order = d.src_order;
+        a.alpha = d.src_alpha;
+        long nbox = 1;
+        for (int k = 0; k < 3; k++) {
+            a.lo[k] = d.src_lo[k];
+            a.hi[k] = d.src_hi[k];
+            nbox *= (long)(a.hi[k] - a.lo[k]);
+        }
+        if (nbox <= 0) continue;
+        a.t = t;
+        a.tdev = tdev;
+        a.tinc = tinc;
+        a.bdt = bdt;
+        k_sources<real><<<(unsigned)((nbox + 255) / 256), 256, 0, c->stream>>>(P.b[ib], a);
+    }
+}
+
 template <typename real, int TX, int TY>
 void launch_main_v(const wql_ctx *c, Pair<real> &P, const TileList &TL, real A0, real A1, real bdt)
 {
@@ -496,6 +529,7 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
     }
     launch_band<real>(c, P, (real)A, (real)bdt);
     launch_pml<real>(c, P, (real)A, (real)bdt);
+    launch_sources<real>(c, P, t_stage, tdev, tinc, bdt);
     if (!fused) {
         a.mode = 0;
         k_boundary<real><<<gb, 128, 0, c->stream>>>(P, a);
@@ -1280,6 +1314,100 @@ int wql_get_fault_faces(wql_ctx *c, double *Uface, double *Vface)
     return 0;
 }
 
+int wql_locate_sources(const double *X, int nq, int nr, int ns, int n, const double *xyz, wql_source *src)
+{
+    if (!X || nq < 2 || nr < 2 || ns < 2 || n < 0 || (n > 0 && (!xyz || !src))) return fail(WQL_E_ARG, "bad argument");
+    const long N = (long)nq * nr * ns;
+    auto at = [&](int c, int i, int j, int k) { return X[c * N + i + (long)nq * ((long)j + (long)nr * k)]; };
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; c++) {
+        lo[c] = hi[c] = X[c * N];
+        for (long p = 1; p < N; p++) {
+            lo[c] = std::min(lo[c], X[c * N + p]);
+            hi[c] = std::max(hi[c], X[c * N + p]);
+        }
+    }
+    for (int s = 0; s < n; s++) {
+        const double x1 = xyz[3 * s], y1 = xyz[3 * s + 1], z1 = xyz[3 * s + 2];
+        int fi = -1, fj = -1, fk = -1;
+        // Find_Coordinates_moment (seismogram.f90:652-732): the nearest node,
+        // the last one in (s, r, q) order on ties, for a location inside the
+        // block's coordinate ranges
+        if (lo[0] <= x1 && x1 <= hi[0] && lo[1] <= y1 && y1 <= hi[1] && lo[2] <= z1 && z1 <= hi[2]) {
+            double mindist = 1.0e8;
+            for (int k = 0; k < ns; k++)
+                for (int j = 0; j < nr; j++)
+                    for (int i = 0; i < nq; i++) {
+                        const double v0 = at(0, i, j, k) - x1, v1 = at(1, i, j, k) - y1, v2 = at(2, i, j, k) - z1;
+                        const double dist = std::sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+                        if (dist <= mindist) {
+                            fi = i; fj = j; fk = k;
+                            mindist = dist;
+                        }
+                    }
+        }
+        // init_moment_tensor (moment_tensor.f90:305-322): kept only when the
+        // location lies between the block's first and last nodes
+        const bool inb = at(0, 0, 0, 0) <= x1 && x1 <= at(0, nq - 1, nr - 1, ns - 1) && at(1, 0, 0, 0) <= y1 &&
+                         y1 <= at(1, nq - 1, nr - 1, ns - 1) && at(2, 0, 0, 0) <= z1 && z1 <= at(2, nq - 1, nr - 1, ns - 1);
+        if (inb && fi >= 0) {
+            src[s].node[0] = fi; src[s].node[1] = fj; src[s].node[2] = fk;
+            src[s].node_x[0] = at(0, fi, fj, fk);
+            src[s].node_x[1] = at(1, fi, fj, fk);
+            src[s].node_x[2] = at(2, fi, fj, fk);
+        } else {
+            for (int c = 0; c < 3; c++) { src[s].node[c] = -2; src[s].node_x[c] = -1.0; }
+        }
+    }
+    return 0;
+}
+
+int wql_set_sources(wql_ctx *c, int ib, int n, const wql_source *src, int order, int alpha)
+{
+    if (!c || ib < 0 || ib > 1 || n < 0 || (n > 0 && !src)) return fail(WQL_E_ARG, "bad argument");
+    if (order != 2 && order != 4) return fail(WQL_E_ARG, "moment tensor order %d: 2 or 4", order);
+    if (order == 4 && alpha != 0 && alpha != 25 && alpha != 50 && alpha != 100)
+        return fail(WQL_E_ARG, "moment tensor alpha %d: 0, 25, 50 or 100 with order 4", alpha);
+    DevBlock &d = c->b[ib];
+    std::vector<SourceDev> h(n);
+    const int w = order == 2 ? 2 : 4;
+    const int nn[3] = {d.nq, d.nr, d.ns};
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int s = 0; s < n; s++) {
+        if (src[s].type < WQL_STF_GAUSSIAN || src[s].type > WQL_STF_SINE_MOMENT)
+            return fail(WQL_E_ARG, "source %d: unknown source time function %d", s, src[s].type);
+        SourceDev &o = h[s];
+        o.type = src[s].type;
+        o.duration = src[s].duration;
+        o.t_init = src[s].t_init;
+        for (int k = 0; k < 6; k++) o.m[k] = src[s].m[k];
+        for (int k = 0; k < 3; k++) {
+            o.x[k] = src[s].x[k];
+            o.node[k] = src[s].node[k];
+            o.node_x[k] = src[s].node_x[k];
+            if (o.node[k] < -2 || o.node[k] >= nn[k])
+                return fail(WQL_E_ARG, "source %d: node index %d outside block %d", s, o.node[k], ib + 1);
+            const int a0 = std::max(0, o.node[k] - w), a1 = std::min(nn[k], o.node[k] + w + 1);
+            if (s == 0 || a0 < lo[k]) lo[k] = a0;
+            if (s == 0 || a1 > hi[k]) hi[k] = a1;
+        }
+    }
+    if (n > d.src_cap) {
+        if (int rc = dalloc(c, (void **)&d.src, n * sizeof(SourceDev))) return rc;
+        d.src_cap = n;
+    }
+    drop_graphs(c);
+    d.nsrc = n;
+    d.src_order = order;
+    d.src_alpha = alpha;
+    for (int k = 0; k < 3; k++) { d.src_lo[k] = lo[k]; d.src_hi[k] = hi[k]; }
+    if (n > 0) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(d.src, h.data(), n * sizeof(SourceDev), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
 int wql_set_stations(wql_ctx *c, int ib, int n, const int *ijk)
 {
     if (!c || ib < 0 || ib > 1 || n < 0 || (n > 0 && !ijk)) return fail(WQL_E_ARG, "bad argument");
@@ -1593,10 +1721,12 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
             if (TL.total > 0) launch_main<double>(c, Pd, TL, 0.0, 0.0, 0.0);
             launch_band<double>(c, Pd, 0.0, 0.0);
             launch_pml<double>(c, Pd, 0.0, 0.0);
+            launch_sources<double>(c, Pd, t, nullptr, 0.0, 0.0);
         } else {
             if (TL.total > 0) launch_main<float>(c, Pf, TL, 0.f, 0.f, 0.f);
             launch_band<float>(c, Pf, 0.f, 0.f);
             launch_pml<float>(c, Pf, 0.f, 0.f);
+            launch_sources<float>(c, Pf, t, nullptr, 0.0, 0.0);
         }
     } else {
         if (c->prec == WQL_FP64) {

@@ -46,6 +46,16 @@ struct FaultParams {
     int rs_solver;           // 0 regula falsi (reference), 1 bracketed Newton
 };
 
+// One moment-tensor point source, identical in meaning to wql_source.
+struct SourceDev {
+    int type;
+    double duration, t_init;
+    double m[6];
+    double x[3];
+    int node[3];
+    double node_x[3];
+};
+
 // One block as the kernels see it.
 template <typename real>
 struct Blk {

@@ -2269,6 +2269,114 @@ __global__ void k_qface(int i, int nr, int ns, int px, long NP, const real *__re
     out[t] = (double)F[f * NP + i + (long)px * ((long)j + (long)nr * k)];
 }
 
+// ----------------------------------------------------------------------------
+// moment-tensor point sources (moment_tensor.f90:958-1270)
+// (FP64, no FMA contraction: the reference's operation order)
+
+// source time function at t (source_time, 958-1017)
+__device__ double source_time_fn(int type, double t, double t0, double w)
+{
+#pragma clang fp contract(off)
+    const double pi = 3.141592653589793;
+    switch (type) {
+    case 0: return (w / sqrt(2.0 * pi)) * exp((-(w * w) * ((t - t0) * (t - t0))) / 2.0);
+    case 1: return (t / (w * w)) * exp(-t / w);
+    case 2: return (w * w) * t * exp(-w * t);
+    case 3: return t < t0 ? (1.0 - cos(2.0 * pi * t / t0)) / t0 : 0.0;
+    case 4: return (t >= 0.0 && t <= w) ? 1.0 - cos(2.0 * pi * t / w) : 0.0;
+    case 5: return 1.0 - (1.0 + t / w) * exp(-t / w);
+    default: return t <= w ? t - w / (2.0 * pi) * sin(2.0 * pi * t / w) : w;
+    }
+}
+
+// one 1-D discrete delta weight at node i for a source at node i0 with
+// offset a (in units of h) (singular_source, 1083-1270): five nodes for
+// order 2, the nine-node table of alpha for order 4
+__device__ double singular_weight(int i, int i0, double h, double a, int order, int alpha)
+{
+#pragma clang fp contract(off)
+    const int k = i - i0;
+    if (order == 2) {
+        if (k < -2 || k > 2) return 0.0;
+        const double beta = (1.0 - 2.0 * a) / 4.0 + 0.25;
+        const double gam = (2.0 * a - 1.0) / 16.0;
+        const double gap = (2.0 * a + 1.0) / 16.0;
+        switch (k) {
+        case -2: return (-gam) / h;
+        case -1: return (beta + 3.0 * gam - gap) / h;
+        case 0: return (1.0 - a - 2.0 * beta - 3.0 * gam + 3.0 * gap) / h;
+        case 1: return (a + beta + gam - 3.0 * gap) / h;
+        default: return gap / h;
+        }
+    }
+    if (k < -4 || k > 4) return 0.0;
+    static constexpr double W[4][9] = {
+        {-0.01171875, -0.03125, 0.046875, 0.28125, 0.4296875, 0.28125, 0.046875, -0.03125, -0.01171875},
+        {-0.00246875, -0.02275, -0.014125, 0.16575, 0.4171875, 0.38675, 0.117875, -0.02975, -0.01846875},
+        {-0.00728125, -0.028, 0.014875, 0.224, 0.4265625, 0.336, 0.081375, -0.032, -0.01553125},
+        {-0.0095625, -0.02990625, 0.03053125, 0.25284375, 0.42890625, 0.30903125, 0.06384375, -0.03196875,
+         -0.01371875}};
+    const int row = alpha == 0 ? 0 : alpha == 25 ? 1 : alpha == 50 ? 2 : 3;
+    return W[row][k + 4] / h;
+}
+
+struct SourceArgs {
+    const SourceDev *src;
+    int n, order, alpha;
+    int lo[3], hi[3];      // node box holding every source's support (hi exclusive)
+    double t;
+    const double *tdev;    // non-null: stage time = *tdev + tinc
+    double tinc;
+    double bdt;
+};
+
+// DF -= A M delta at the nodes of the box, sources in list order, and the
+// stage output and q-face buffer follow (F_out = F_in + B dt DF)
+template <typename real>
+__global__ void k_sources(Blk<real> B, SourceArgs a)
+{
+#pragma clang fp contract(off)
+    const int bx = a.hi[0] - a.lo[0], by = a.hi[1] - a.lo[1];
+    const long nbox = (long)bx * by * (a.hi[2] - a.lo[2]);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbox) return;
+    const int i = a.lo[0] + (int)(t % bx), j = a.lo[1] + (int)((t / bx) % by), k = a.lo[2] + (int)(t / ((long)bx * by));
+    const long p = pidx(B.px, B.nr, i, j, k);
+    const double ts = a.tdev ? *a.tdev + a.tinc : a.t;
+    // local spacing (moment_tensor.f90:1044-1047): h / (q_x, r_y, s_z)
+    const double hx = (1.0 / (double)(B.nq - 1)) / gmet(B, 0, p);
+    const double hy = (1.0 / (double)(B.nr - 1)) / gmet(B, 4, p);
+    const double hz = (1.0 / (double)(B.ns - 1)) / gmet(B, 8, p);
+    double df[6];
+    for (int c = 0; c < 6; c++) df[c] = (double)B.DF[(3 + c) * B.NP + p];
+    const double df0[6] = {df[0], df[1], df[2], df[3], df[4], df[5]};
+    bool hit = false;
+    for (int s = 0; s < a.n; s++) {
+        const SourceDev &S = a.src[s];
+        const double ax = (S.x[0] - S.node_x[0]) / hx;
+        const double ay = (S.x[1] - S.node_x[1]) / hy;
+        const double az = (S.x[2] - S.node_x[2]) / hz;
+        const double A = source_time_fn(S.type, ts, S.t_init, S.duration);
+        const double dx = singular_weight(i, S.node[0], hx, ax, a.order, a.alpha);
+        const double dy = singular_weight(j, S.node[1], hy, ay, a.order, a.alpha);
+        const double dz = singular_weight(k, S.node[2], hz, az, a.order, a.alpha);
+        const double delta = dx * dy * dz;
+        if (delta == 0.0) continue;
+        hit = true;
+        for (int c = 0; c < 6; c++) df[c] = df[c] - A * S.m[c] * delta;
+    }
+    if (!hit) return;
+    real *fq = nullptr;
+    if (B.FQ && (i == 0 || i == B.nq - 1))
+        fq = B.FQ + (i == 0 ? 0L : 9L * B.nr * B.ns) + j + (long)B.nr * k;
+    for (int c = 0; c < 6; c++) {
+        const long q = (3 + c) * B.NP + p;
+        B.DF[q] = (real)df[c];
+        const real fo = B.Fout[q] + (real)(a.bdt * (df[c] - df0[c]));
+        B.Fout[q] = fo;
+        if (fq) fq[(long)(3 + c) * B.nr * B.ns] = fo;
+    }
+}
 // velocities at the station nodes (write_seismogram, seismogram.f90:460-478):
 // out[n][3] = vx vy vz of node off[n]
 template <typename real>

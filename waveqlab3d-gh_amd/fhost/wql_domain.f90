@@ -146,6 +146,7 @@ contains
     ! Init_Vel_State (domain.f90:526-529)
     call wql_check(wql_init_state(D%ctx), 'wql_init_state')
     if (D%w_fault) call open_fault_output(D)
+    call init_sources(D, ifname)
     call read_output_list(ifname, D%out)
     if (D%out%output_seismograms) then
        do ib = 1, 2
@@ -212,6 +213,51 @@ contains
     call wql_check(wql_get_fault(D%ctx, int(ib - 1, c_int), c_loc(S), c_loc(W), c_loc(trup), c_loc(Svel), &
                                  c_null_ptr, c_null_ptr), 'wql_get_fault')
   end subroutine get_fault
+
+  ! ---- moment-tensor sources (moment_tensor.f90:82-1080) --------------------
+  !> init_moment_tensor for both blocks: each block's source list, located
+  !> on its grid; with the unified list only the preferred block holds the
+  !> sources (moment_tensor.f90:346-358)
+  subroutine init_sources(D, ifname)
+    type(domain_type), intent(inout) :: D
+    character(*), intent(in) :: ifname
+    type(moment_params) :: mp
+    character(len=64), allocatable :: kind(:)
+    real(kind = wp), allocatable :: dur(:), t0(:), mt(:,:), xyz(:,:)
+    integer, allocatable :: alpha(:)
+    type(wql_source), allocatable :: src(:)
+    integer :: ib, n, s, k
+    logical :: union
+    character(len=24), parameter :: names(0:6) = [character(len=24) :: 'gaussian', 'LOH_discontinuity', 'Brune', &
+         'Cos-Bell', 'cosine', 'LOH_moment_time_history', 'sine_moment_time_history']
+    call read_moment_list(ifname, mp)
+    if (.not. mp%use_moment_tensor) return
+    do ib = 1, 2
+       call read_tensor_list(ifname, ib, kind, dur, t0, mt, xyz, alpha, n, union)
+       allocate(src(max(n, 1)))
+       do s = 1, n
+          src(s)%type = -1
+          do k = 0, 6
+             if (trim(kind(s)) == trim(names(k))) src(s)%type = k
+          end do
+          if (src(s)%type < 0) error stop 'invalid source type'
+          src(s)%duration = dur(s); src(s)%t_init = t0(s); src(s)%m = mt(:, s); src(s)%x = xyz(:, s)
+       end do
+       call wql_check(wql_locate_sources(D%B(ib)%X, int(D%B(ib)%nq, c_int), int(D%B(ib)%nr, c_int), &
+                                         int(D%B(ib)%ns, c_int), int(n, c_int), xyz, src), 'wql_locate_sources')
+       if (union .and. ib /= mp%tensor_block_preference) then
+          do s = 1, n
+             src(s)%node = -2
+             src(s)%node_x = -1.0_wp
+          end do
+       end if
+       k = 0
+       if (n > 0) k = alpha(1)
+       call wql_check(wql_set_sources(D%ctx, int(ib - 1, c_int), int(n, c_int), src, int(mp%order, c_int), &
+                                      int(k, c_int)), 'wql_set_sources')
+       deallocate(src)
+    end do
+  end subroutine init_sources
 
   ! ---- station seismograms (seismogram.f90:12-489) -------------------------
   !> init_seismogram for block ib: the stations' nodes (Find_Coordinates,

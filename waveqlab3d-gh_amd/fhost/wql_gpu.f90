@@ -65,6 +65,16 @@ module wql_gpu
      type(c_ptr) :: top_y = c_null_ptr       ! optional top_y(nq, ns)
   end type wql_grid_desc
 
+  !> wql_source: one moment-tensor point source (moment_tensor.f90:82-270)
+  type, bind(C) :: wql_source
+     integer(c_int) :: type = 0               ! WQL_STF_*: 0 gaussian ... 6 sine_moment_time_history
+     real(c_double) :: duration = 0, t_init = 0
+     real(c_double) :: m(6) = 0               ! mXX mYY mZZ mXY mXZ mYZ
+     real(c_double) :: x(3) = 0
+     integer(c_int) :: node(3) = -2           ! 0-based nearest node, -2 where the block does not hold it
+     real(c_double) :: node_x(3) = -1
+  end type wql_source
+
   interface
      ! ---- set-up ----------------------------------------------------------
      integer(c_int) function wql_problem_keys(problem, coupling, response, plastic_model, mu_beta_eta, prob) &
@@ -175,6 +185,20 @@ module wql_gpu
        real(c_double), intent(in) :: xyz(*)
        integer(c_int), intent(out) :: ijk(*)
      end function wql_find_stations
+     integer(c_int) function wql_locate_sources(X, nq, nr, ns, n, xyz, src) bind(C, name='wql_locate_sources')
+       import :: c_int, c_double, wql_source
+       real(c_double), intent(in) :: X(*)
+       integer(c_int), value :: nq, nr, ns, n
+       real(c_double), intent(in) :: xyz(*)
+       type(wql_source), intent(inout) :: src(*)
+     end function wql_locate_sources
+     integer(c_int) function wql_set_sources(ctx, block, n, src, order, alpha) bind(C, name='wql_set_sources')
+       import :: c_int, c_ptr, wql_source
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: block, n
+       type(wql_source), intent(in) :: src(*)
+       integer(c_int), value :: order, alpha
+     end function wql_set_sources
      integer(c_int) function wql_set_stations(ctx, block, n, ijk) bind(C, name='wql_set_stations')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

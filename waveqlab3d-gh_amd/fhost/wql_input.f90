@@ -8,7 +8,8 @@ module wql_input
   use iso_c_binding, only : c_double
   implicit none
   private
-  public :: problem_params, block_temp_parameters, read_input, output_params, read_output_list, read_station_list
+  public :: problem_params, block_temp_parameters, read_input, output_params, read_output_list, read_station_list, &
+            moment_params, read_moment_list, read_tensor_list
 
   integer, parameter :: wp = c_double
 
@@ -45,7 +46,118 @@ module wql_input
      character(len=256) :: station_list = 'infile', station_list_file = '', station_file_directory = 'seismogram'
   end type output_params
 
+  !> &moment_list with init_block's defaults (block.f90:63-85)
+  type :: moment_params
+     logical :: use_moment_tensor = .false.
+     integer :: order = 2, tensor_block_preference = 1
+  end type moment_params
+
 contains
+
+  !> &moment_list of the file `path` (block.f90:63-85)
+  subroutine read_moment_list(path, m)
+    character(*), intent(in) :: path
+    type(moment_params), intent(out) :: m
+    logical :: use_moment_tensor, show_moment_tensor_info, show_moment_tesnsor_info
+    integer :: order, tensor_block_preference, u, stat
+    character(len=512) :: msg
+    namelist /moment_list/ use_moment_tensor, order, tensor_block_preference, show_moment_tensor_info, &
+                           show_moment_tesnsor_info
+    use_moment_tensor = .false.; order = 2; tensor_block_preference = 1
+    show_moment_tensor_info = .false.; show_moment_tesnsor_info = .false.
+    open(newunit = u, file = trim(path), status = 'old', action = 'read', iostat = stat)
+    if (stat /= 0) error stop ':: Cannot open input file'
+    read(u, nml = moment_list, iostat = stat, iomsg = msg)
+    close(u)
+    if (stat > 0) then
+       write(*, '(a)') trim(msg)
+       error stop 'error reading namelist moment_list'
+    end if
+    m%use_moment_tensor = use_moment_tensor
+    m%order = order
+    m%tensor_block_preference = merge(2, 1, tensor_block_preference == 2)
+  end subroutine read_moment_list
+
+  !> the moment-tensor source lines block `block` reads (init_moment_tensor,
+  !> moment_tensor.f90:82-270): the unified '!---begin:tensor_list---' list
+  !> (union = .true.) or the block's own '..._listU---' / '..._listV---'
+  !> list; per source type, duration, t_init, mXX mYY mZZ mXY mXZ mYZ, x y z
+  !> and alpha, list-directed
+  subroutine read_tensor_list(path, block, kind, dur, t0, mt, xyz, alpha, n, union)
+    character(*), intent(in) :: path
+    integer, intent(in) :: block
+    character(len=64), allocatable, intent(out) :: kind(:)
+    real(kind = wp), allocatable, intent(out) :: dur(:), t0(:), mt(:,:), xyz(:,:)
+    integer, allocatable, intent(out) :: alpha(:)
+    integer, intent(out) :: n
+    logical, intent(out) :: union
+    character(len=256) :: begin, end, line, t
+    integer :: u, stat, k
+    logical :: found
+    call marker_present(path, '!---begin:tensor_list---', union)
+    if (union) then
+       begin = '!---begin:tensor_list---'; end = '!---end:tensor_list---'
+    else if (block == 1) then
+       begin = '!---begin:tensor_listU---'; end = '!---end:tensor_listU---'
+    else
+       begin = '!---begin:tensor_listV---'; end = '!---end:tensor_listV---'
+    end if
+    open(newunit = u, file = trim(path), status = 'old', action = 'read', iostat = stat)
+    if (stat /= 0) error stop ':: Cannot open input file'
+    found = .false.
+    do
+       read(u, '(a)', iostat = stat) line
+       if (stat /= 0) exit
+       if (trim(line) == trim(begin)) then
+          found = .true.
+          exit
+       end if
+    end do
+    if (.not. found) error stop 'error reading the moment tensor list'
+    n = 0
+    do
+       read(u, '(a)', iostat = stat) line
+       if (stat /= 0) error stop 'error reading the moment tensor list'
+       if (trim(line) == trim(end)) exit
+       t = adjustl(line)
+       if (len_trim(t) == 0 .or. t(1:1) == '!') cycle
+       n = n + 1
+    end do
+    allocate(kind(n), dur(n), t0(n), mt(6, n), xyz(3, n), alpha(n))
+    rewind(u)
+    do
+       read(u, '(a)') line
+       if (trim(line) == trim(begin)) exit
+    end do
+    k = 0
+    do while (k < n)
+       read(u, '(a)') line
+       t = adjustl(line)
+       if (len_trim(t) == 0 .or. t(1:1) == '!') cycle
+       k = k + 1
+       read(line, *) kind(k), dur(k), t0(k), mt(:, k), xyz(:, k), alpha(k)
+    end do
+    close(u)
+  end subroutine read_tensor_list
+
+  subroutine marker_present(path, marker, found)
+    character(*), intent(in) :: path, marker
+    logical, intent(out) :: found
+    character(len=1024) :: line
+    integer :: u, stat
+    found = .false.
+    open(newunit = u, file = trim(path), status = 'old', action = 'read', iostat = stat)
+    if (stat /= 0) return
+    do
+       read(u, '(a)', iostat = stat) line
+       if (stat /= 0) exit
+       if (trim(line) == marker) then
+          found = .true.
+          exit
+       end if
+    end do
+    close(u)
+  end subroutine marker_present
 
   !> &output_list of the file `path` (seismogram.f90:54-104); the group's
   !> other variables are read and not used by this host

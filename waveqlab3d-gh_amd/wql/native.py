@@ -25,7 +25,12 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_volume_kernel_ms', 'wql_get_fault_faces', 'wql_get_plastic',
            'wql_get_fields_noalloc', 'wql_problem_keys', 'wql_grid_block',
            'wql_stage', 'wql_server_pool_stats', 'wql_server_set_shape', 'wql_find_stations',
-           'wql_set_stations', 'wql_get_stations']
+           'wql_set_stations', 'wql_get_stations', 'wql_locate_sources', 'wql_set_sources']
+
+
+class Source(C.Structure):
+    _fields_ = [('type', C.c_int), ('duration', C.c_double), ('t_init', C.c_double), ('m', C.c_double * 6),
+                ('x', C.c_double * 3), ('node', C.c_int * 3), ('node_x', C.c_double * 3)]
 
 
 class FaultParams(C.Structure):
@@ -110,6 +115,8 @@ def lib():
             'wql_server_set_policy': ([vp, C.c_int], C.c_int),
             'wql_find_stations': ([dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.POINTER(C.c_int)], C.c_int),
             'wql_set_stations': ([vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
+            'wql_locate_sources': ([dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.POINTER(Source)], C.c_int),
+            'wql_set_sources': ([vp, C.c_int, C.c_int, C.POINTER(Source), C.c_int, C.c_int], C.c_int),
             'wql_get_stations': ([vp, C.c_int, dp], C.c_int),
             'wql_server_set_shape': ([vp, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
                                      C.c_int),

@@ -165,6 +165,11 @@ class Problem:
     name: str = 'default'
     w_stride: int = 1          # fault output every w_stride steps (time_step.f90:137)
     w_fault: bool = False      # write the fault output files (fault_output.f90)
+    # moment-tensor point sources (&moment_list, moment_tensor.f90:82-270):
+    # None, or dict(order, preference, lists) with lists {'union': [...]} or
+    # {'U': [...], 'V': [...]} of source dicts (type, duration, t_init, m, x,
+    # alpha); see block_sources
+    moment: Optional[dict] = None
 
     def dt(self):
         """Global time step (domain.f90:222-235, block.f90:437-452)."""
@@ -176,6 +181,82 @@ class Problem:
 
     def nt(self):
         return int(math.floor(self.t_final / self.dt()))
+
+
+SOURCE_TYPES = {'gaussian': 0, 'LOH_discontinuity': 1, 'Brune': 2, 'Cos-Bell': 3, 'cosine': 4,
+                'LOH_moment_time_history': 5, 'sine_moment_time_history': 6}
+
+
+def _tensor_lines(lines, begin, end):
+    """source lines between two markers, blank and '!' lines skipped
+    (moment_tensor.f90:24-30), or None without the begin marker"""
+    if begin not in lines:
+        return None
+    out = []
+    for ln in lines[lines.index(begin) + 1:]:
+        if ln.strip() == end:
+            return out
+        t = ln.strip()
+        if not t or t.startswith('!'):
+            continue
+        out.append(t)
+    raise ValueError(f'moment tensor list: {end} missing')
+
+
+def _source(line):
+    """one source line: type duration t_init mXX mYY mZZ mXY mXZ mYZ x y z alpha
+    (read list-directed, moment_tensor.f90:218-223)"""
+    tok = re.findall(r"'[^']*'|\"[^\"]*\"|[^\s,]+", line)
+    if len(tok) < 13:
+        raise ValueError(f'moment tensor line needs 13 values: {line!r}')
+    kind = tok[0].strip("'\"")
+    if kind not in SOURCE_TYPES:
+        raise ValueError(f'invalid source type {kind!r} (moment_tensor.f90:970)')
+    v = [float(t.replace('d', 'e').replace('D', 'e')) for t in tok[1:12]]
+    return dict(type=SOURCE_TYPES[kind], duration=v[0], t_init=v[1], m=tuple(v[2:8]), x=tuple(v[8:11]),
+                alpha=int(float(tok[12].replace('d', 'e').replace('D', 'e'))))
+
+
+def read_moment(text):
+    """&moment_list and the source lists of an input file (block.f90:63-85,
+    moment_tensor.f90:82-270), or None when use_moment_tensor is off"""
+    ml = read_namelist(text, 'moment_list')
+    if not ml.get('use_moment_tensor', False):
+        return None
+    lines = [ln.rstrip('\n').rstrip('\r') for ln in text.splitlines()]
+    lines = [ln.strip() if ln.strip().startswith('!---') else ln for ln in lines]
+    pref = int(ml.get('tensor_block_preference', 1))
+    out = dict(order=int(ml.get('order', 2)), preference=2 if pref == 2 else 1)
+    union = _tensor_lines(lines, '!---begin:tensor_list---', '!---end:tensor_list---')
+    if union is not None:
+        out['lists'] = {'union': [_source(ln) for ln in union]}
+    else:
+        out['lists'] = {}
+        for tag in ('U', 'V'):
+            body = _tensor_lines(lines, f'!---begin:tensor_list{tag}---', f'!---end:tensor_list{tag}---')
+            out['lists'][tag] = [_source(ln) for ln in body] if body is not None else None
+    return out
+
+
+def block_sources(prob, ib):
+    """(sources, held, order, alpha) of block ib (0 or 1): the block's list
+    (the unified one, or 'U' for block 1 and 'V' for block 2), whether the
+    block may hold each source (with the unified list only the preferred
+    block does, moment_tensor.f90:346-358), the operator order, and the
+    first source's alpha, which the reference uses for all of a block's
+    sources (singular_source's alpha(1))"""
+    m = prob.moment
+    if not m:
+        return [], [], 2, 0
+    if 'union' in m['lists']:
+        src = m['lists']['union']
+        held = [ib + 1 == m['preference']] * len(src)
+    else:
+        src = m['lists'].get('U' if ib == 0 else 'V')
+        if src is None:
+            raise ValueError('error reading the moment tensor list of block %d' % (ib + 1))
+        held = [True] * len(src)
+    return src, held, m['order'], (src[0]['alpha'] if src else 0)
 
 
 def _nml_value(tok):
@@ -237,7 +318,7 @@ def from_namelist(path):
     prob = Problem(blocks=blocks, fault=fault_params(problem, coupling),
                    cfl=float(pl.get('cfl', 0.5)), t_final=float(pl.get('t_final', 0.0)),
                    plastic=plastic, name=pl.get('name', 'default'), w_stride=int(pl.get('w_stride', 1)),
-                   w_fault=bool(pl.get('w_fault', False)))
+                   w_fault=bool(pl.get('w_fault', False)), moment=read_moment(text))
     if plastic == 2:
         # per-block mu_beta_eta (plastic_material.f90:38-44); one set for both blocks here
         mbe = [tuple(float(v) for v in bl.get(f'btp({i})%mu_beta_eta', (0.0, 0.0, 0.0))) for i in (1, 2)]

@@ -106,8 +106,36 @@ class Simulation:
             check(L.wql_set_tile(self.ctx, v0.value if tile is None else tile, z0.value if zchunk is None else zchunk))
         if order is not None:
             check(L.wql_set_tile_order(self.ctx, *order))
+        if prob.moment:
+            for ib in range(2):
+                X = coords[ib] if coords is not None else prob.blocks[ib].coords()
+                self.set_sources(ib, X)
         if graph:
             check(L.wql_use_graph(self.ctx, 1))
+
+    def set_sources(self, ib, X):
+        """the problem's moment-tensor sources of block ib, located on the
+        block's node coordinates X[3][ns][nr][nq] (init_moment_tensor)"""
+        from .problem import block_sources
+        src, held, order, alpha = block_sources(self.prob, ib)
+        n = len(src)
+        arr = (native.Source * max(n, 1))()
+        for k, s_ in enumerate(src):
+            arr[k].type = s_['type']
+            arr[k].duration = s_['duration']
+            arr[k].t_init = s_['t_init']
+            arr[k].m[:] = list(s_['m'])
+            arr[k].x[:] = list(s_['x'])
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        nq, nr, ns = self.prob.blocks[ib].n
+        xyz = np.ascontiguousarray([s_['x'] for s_ in src] or [[0.0, 0.0, 0.0]], dtype=np.float64)
+        check(native.lib().wql_locate_sources(_p(X), nq, nr, ns, n, _p(xyz), arr))
+        for k in range(n):
+            if not held[k]:
+                arr[k].node[:] = [-2, -2, -2]
+                arr[k].node_x[:] = [-1.0, -1.0, -1.0]
+        check(native.lib().wql_set_sources(self.ctx, ib, n, arr, order, alpha))
+        return [tuple(arr[k].node) for k in range(n)]
 
     @property
     def tile(self):
