@@ -256,7 +256,7 @@ def test_small_grid_default_chunks():
     for wl, fp32, fp64 in (('plain', 2, 4), ('rupture', 6, 12)):
         W = workloads.WORKLOADS[wl](scale=1.0)
         for prec, want in (('fp32', fp32), ('fp64', fp64)):
-            with _sim(W['problem'], precision=prec, coords=W['coords']) as sim:
+            with _sim(W['problem'], precision=prec, coords=W['coords'], tuned=False) as sim:
                 v, zc, _, _ = sim.tile
                 assert zc == want, (wl, prec, v, zc)
 

@@ -83,7 +83,7 @@ def estimate_bytes(prob, precision='fp64'):
 
 class Simulation:
     def __init__(self, prob, precision='fp64', tile=None, zchunk=None, coords=None, graph=False, dt=None,
-                 order=None):
+                 order=None, tuned=True):
         L = native.lib()
         self.prob = prob
         self.precision = precision
@@ -94,7 +94,7 @@ class Simulation:
         self.dt = pd.dt
         if order is not None:
             order = (order, 2)
-        if tile is None and zchunk is None and order is None:
+        if tuned and tile is None and zchunk is None and order is None:
             from . import autotune
             best = autotune.lookup(prob, precision)
             if best is not None:
