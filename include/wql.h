@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define WQL_ABI_VERSION 2
+#define WQL_ABI_VERSION 3
 
 enum {
     WQL_OK = 0,
@@ -87,6 +87,8 @@ typedef struct {
                               (1170-1239); 1 bracketed Newton, |step| <= 1e-12 |V| + 1e-300 */
 } wql_fault_params;
 
+enum { WQL_FD_TRADITIONAL = 0, WQL_FD_UPWIND = 1, WQL_FD_UPWIND_DRP = 2 };
+
 typedef struct {
     int precision;         /* WQL_FP64, or WQL_FP32: volume fields/metrics FP32, fault state FP64 */
     double dt;             /* time step (domain.f90:222-235) */
@@ -97,6 +99,14 @@ typedef struct {
     double pl_mu, pl_beta, pl_eta;
     int s0_kind;           /* plastic background stress: 0 constant fault.s0, 1 TPV26 depth profile */
     wql_fault_params fault;
+    /* ---- ABI 3 ---- */
+    int fd_type;           /* &problem_list fd_type (domain.f90:106, 135-136): WQL_FD_TRADITIONAL (the
+                              sixth-order central operator, JU_xJU_yJU_z6.f90:12-812, 16141-16449),
+                              WQL_FD_UPWIND or WQL_FD_UPWIND_DRP: forward operator Dp on the velocities,
+                              backward operator Dm on the stress fluxes (815-16122, 16451-20922) */
+    int fd_order;          /* &problem_list order: 2..9 (upwind), 3..7, 66, 679 (upwind_drp); it also sets
+                              the SAT penalty tau0 and the band width (block.f90:103-255,
+                              RHS_Interior.f90:133-1514).  Ignored for WQL_FD_TRADITIONAL */
 } wql_problem;
 
 /* The fault law, prestress and plasticity settings the reference keys by

@@ -36,7 +36,32 @@ class OraBlock(C.Structure):
                 ('bc', C.c_int * 6), ('pml', C.c_int * 6), ('npml', C.c_int),
                 ('X', dp), ('metric', dp), ('M', dp), ('nml', dp * 6), ('F', dp), ('DF', dp),
                 ('Q', dp * 6), ('DQ', dp * 6), ('P', dp),
-                ('nsrc', C.c_int), ('src_order', C.c_int), ('src_alpha', C.c_int), ('src', C.POINTER(OraSource))]
+                ('nsrc', C.c_int), ('src_order', C.c_int), ('src_alpha', C.c_int), ('src', C.POINTER(OraSource)),
+                ('sbp', C.c_int)]
+
+
+class SbpHalf(C.Structure):
+    """wql_sbp_half (waveqlab3d-gh_amd/csrc/wql_sbp_ops.h)"""
+    _fields_ = [('nl', C.c_int), ('nr', C.c_int), ('lo', C.c_int), ('hi', C.c_int), ('band', C.c_double * 16),
+                ('exact', C.c_double * 16), ('left', (C.c_double * 16) * 8), ('right', (C.c_double * 16) * 8)]
+
+
+class SbpOp(C.Structure):
+    """wql_sbp_op: one upwind / upwind-DRP operator, tables per direction"""
+    _fields_ = [('family', C.c_char_p), ('order', C.c_int), ('band', C.c_int), ('tau0', C.c_double),
+                ('v', SbpHalf * 3), ('s', SbpHalf * 3)]
+
+
+def sbp_ops():
+    """every operator table the library and the oracle share"""
+    L = lib()
+    out, k = [], 0
+    while True:
+        p = L.ora_sbp_table(k)
+        if not p:
+            return out
+        out.append(SbpOp.from_address(p))
+        k += 1
 
 
 class OraIface(C.Structure):
@@ -68,6 +93,9 @@ def lib():
         _lib.ora_locate_sources.argtypes = [C.POINTER(OraBlock), C.c_int, dp, C.POINTER(OraSource)]
         _lib.ora_d1.argtypes = [dp, C.c_int, C.c_int, C.c_double]
         _lib.ora_d1.restype = C.c_double
+        _lib.ora_sbp_op.argtypes = [C.c_int, C.c_int]
+        _lib.ora_sbp_table.argtypes = [C.c_int]
+        _lib.ora_sbp_table.restype = C.c_void_p
         _lib.ora_rk_coeffs.argtypes = [dp, dp, dp]
         _lib.ora_plastic.argtypes = [C.POINTER(OraDomain)]
         _lib.ora_initial_stress.argtypes = [C.POINTER(OraDomain), dp, dp]
@@ -124,6 +152,9 @@ class OracleDomain:
             b.bc[:] = bs.bc
             b.pml[:] = [int(v) for v in bs.pml]
             b.npml = bs.npml
+            b.sbp = L.ora_sbp_op({'traditional': 0, 'upwind': 1, 'upwind_drp': 2}[prob.fd_type], prob.order)
+            if b.sbp < -1:
+                raise ValueError(f'no {prob.fd_type} operator of order {prob.order}')
             for k in ('X', 'metric', 'M', 'F', 'DF', 'P'):
                 setattr(b, k, _p(arr[k]))
             for s in range(6):

@@ -11,7 +11,11 @@
  *   ora_metrics      golden metrics/J/normals of the reference build (curv_sw.npz)
  *                    + free-stream identity + exactness on linear maps
  *   ora_set_rates    golden DF after set_rates (cart_sw, curv_sw, curv_rs, pml_locked)
- *                    + SBP identity / polynomial exactness of the closure table
+ *                    + SBP identity / polynomial exactness of the closure table;
+ *                    upwind / upwind-DRP operators: golden DF of seven operators
+ *                    (up2_cart, up4_curv, up6_pml, up9_curv, drp3_curv,
+ *                    drp66_cart, drp679_curv) + the upwind SBP identity and
+ *                    consistency of every table row (all fifteen operators)
  *   ora_boundary,
  *   ora_interface    golden DF after the SAT terms, golden DS/DW (same cases)
  *                    + closed-form welded/locked checks
@@ -19,6 +23,7 @@
  *   ora_hat          slip-weakening closed form, R&S residual |f(V)| ~ 0
  */
 #include "wql_oracle.h"
+#include "wql_sbp_ops.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -343,8 +348,183 @@ static void pml_point(ora_block *b, double DU[9], const double Ux[9], const doub
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* upwind and upwind-DRP operators (tables in wql_sbp_ops.h) */
+
+const void *ora_sbp_table(int k) { return k >= 0 && k < WQL_SBP_NOPS ? (const void *)&WQL_SBP_OPS[k] : NULL; }
+
+int ora_sbp_op(int fd_type, int order)
+{
+    if (fd_type == 0) return -1;
+    const char *fam = fd_type == 1 ? "upwind" : fd_type == 2 ? "upwind_drp" : "";
+    for (int k = 0; k < WQL_SBP_NOPS; k++)
+        if (strcmp(WQL_SBP_OPS[k].family, fam) == 0 && WQL_SBP_OPS[k].order == order) return k;
+    return -2;
+}
+
+/* taps of node pos on a line of n nodes (band routine rows, JJU_x<o>_upwind
+ * JU..:815-16122): closure row at either face, else the printed interior row */
+static int op_taps(const wql_sbp_half *h, int pos, int n, int *idx, double *c)
+{
+    int m = 0;
+    if (pos < h->nl) {
+        for (int t = 0; t < WQL_SBP_MAXW; t++)
+            if (h->left[pos][t] != 0.0) { idx[m] = t; c[m] = h->left[pos][t]; m++; }
+    } else if (pos >= n - h->nr) {
+        int r = n - 1 - pos;
+        for (int t = 0; t < WQL_SBP_MAXW; t++)
+            if (h->right[r][t] != 0.0) { idx[m] = n - 1 - t; c[m] = h->right[r][t]; m++; }
+    } else {
+        for (int o = h->lo; o <= h->hi; o++) { idx[m] = pos + o; c[m] = h->band[o - h->lo]; m++; }
+    }
+    return m;
+}
+
+/* one direction of the band routine for metric component mc: velocities
+ * X_x(centre) * sum c F (forward operator), stresses sum c X_x J F at the
+ * taps (backward operator) */
+static void op_axis(const ora_block *b, const wql_sbp_op *op, int dir, int mc, int i, int j, int k, double out[9])
+{
+    int n = dir == 0 ? b->nq : dir == 1 ? b->nr : b->ns;
+    int pos = dir == 0 ? i : dir == 1 ? j : k;
+    size_t NP = NPTS(b);
+    const double *F = b->F, *G = b->metric;
+    for (int grp = 0; grp < 2; grp++) {
+        int idx[WQL_SBP_MAXW]; double c[WQL_SBP_MAXW];
+        int m = op_taps(grp == 0 ? &op->v[dir] : &op->s[dir], pos, n, idx, c);
+        for (int f = grp == 0 ? 0 : 3; f < (grp == 0 ? 3 : 9); f++) {
+            double s = 0.0;
+            for (int t = 0; t < m; t++) {
+                int ii = i, jj = j, kk = k;
+                if (dir == 0) ii = idx[t]; else if (dir == 1) jj = idx[t]; else kk = idx[t];
+                size_t p = IDX(b, ii, jj, kk);
+                double v = grp == 0 ? c[t] * F[f * NP + p] : c[t] * G[mc * NP + p] * G[9 * NP + p] * F[f * NP + p];
+                s = (t == 0) ? v : s + v;
+            }
+            out[f] = grp == 0 ? G[mc * NP + IDX(b, i, j, k)] * s : s;
+        }
+    }
+}
+
+/* RHS_near_boundaries with the operator's band routine
+ * (RHS_Interior.f90:144-1506): Ju = (1/hq) Jq + (1/hr) Jr + (1/hs) Js */
+static void op_band(const ora_block *b, const wql_sbp_op *op, int fam, int i, int j, int k, double U[9])
+{
+    double hq = 1.0 / (double)(b->nq - 1), hr = 1.0 / (double)(b->nr - 1), hs = 1.0 / (double)(b->ns - 1);
+    double a[9], bb[9], cc[9];
+    op_axis(b, op, 0, 3 * fam + 0, i, j, k, a);
+    op_axis(b, op, 1, 3 * fam + 1, i, j, k, bb);
+    op_axis(b, op, 2, 3 * fam + 2, i, j, k, cc);
+    for (int f = 0; f < 9; f++) U[f] = (1.0 / hq) * a[f] + (1.0 / hr) * bb[f] + (1.0 / hs) * cc[f];
+}
+
+/* interior routine JJU_x<o>_interior_upwind[_drp] (JU..:16451-20922): one
+ * loop over the stencil taps, the three directions added per tap, with the
+ * exact rational coefficients over h; the 'cartesian' form takes the
+ * centre's diagonal metric and no J */
+static void op_interior(const ora_block *b, const wql_sbp_op *op, int i, int j, int k, double DFx[9], double DFy[9],
+                        double DFz[9])
+{
+    size_t NP = NPTS(b);
+    const double *F = b->F, *G = b->metric, *J = G + 9 * NP;
+    double h[3] = {1.0 / (double)(b->nq - 1), 1.0 / (double)(b->nr - 1), 1.0 / (double)(b->ns - 1)};
+    size_t p = IDX(b, i, j, k);
+    double *D[3] = {DFx, DFy, DFz};
+    for (int f = 0; f < 9; f++) DFx[f] = DFy[f] = DFz[f] = 0.0;
+    int nt = op->v[0].hi - op->v[0].lo + 1;
+    for (int t = 0; t < nt; t++) {
+        size_t pv[3], ps[3];
+        double cv[3], cs[3];
+        for (int d = 0; d < 3; d++) {
+            int ov = op->v[d].lo + t, os = op->s[d].lo + t;
+            pv[d] = IDX(b, i + (d == 0) * ov, j + (d == 1) * ov, k + (d == 2) * ov);
+            ps[d] = IDX(b, i + (d == 0) * os, j + (d == 1) * os, k + (d == 2) * os);
+            cv[d] = op->v[d].exact[t] / h[d];
+            cs[d] = op->s[d].exact[t] / h[d];
+        }
+        for (int fam = 0; fam < 3; fam++) {
+            double *Df = D[fam];
+            const double *g0 = G + (3 * fam + 0) * NP, *g1 = G + (3 * fam + 1) * NP, *g2 = G + (3 * fam + 2) * NP;
+            if (b->cartesian) {
+                const double *gd = G + (3 * fam + fam) * NP;
+                for (int f = 0; f < 3; f++) Df[f] = Df[f] + gd[p] * cv[fam] * F[f * NP + pv[fam]];
+                for (int f = 3; f < 9; f++) Df[f] = Df[f] + gd[p] * cs[fam] * F[f * NP + ps[fam]];
+            } else {
+                for (int f = 0; f < 3; f++)
+                    Df[f] = Df[f] + g0[p] * cv[0] * F[f * NP + pv[0]] + g1[p] * cv[1] * F[f * NP + pv[1]]
+                                  + g2[p] * cv[2] * F[f * NP + pv[2]];
+                for (int f = 3; f < 9; f++)
+                    Df[f] = Df[f] + cs[0] * F[f * NP + ps[0]] * g0[ps[0]] * J[ps[0]]
+                                  + cs[1] * F[f * NP + ps[1]] * g1[ps[1]] * J[ps[1]]
+                                  + cs[2] * F[f * NP + ps[2]] * g2[ps[2]] * J[ps[2]];
+            }
+        }
+    }
+}
+
+static void set_rates_op(ora_block *b)
+{
+    const wql_sbp_op *op = &WQL_SBP_OPS[b->sbp];
+    size_t NP = NPTS(b);
+    int nx = b->nq, ny = b->nr, nz = b->ns;
+    /* band width per order (RHS_Interior.f90:144-1506), widened to npml on a PML side */
+    int w = op->band;
+    int ix = b->pml[0] && b->npml > w ? b->npml : w, fx = b->pml[1] && b->npml > w ? b->npml : w;
+    int iy = b->pml[2] && b->npml > w ? b->npml : w, fy = b->pml[3] && b->npml > w ? b->npml : w;
+    int iz = b->pml[4] && b->npml > w ? b->npml : w, fz = b->pml[5] && b->npml > w ? b->npml : w;
+    const double *G = b->metric, *M = b->M;
+    double *DF = b->DF;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                int x = i + 1, y = j + 1, z = k + 1;
+                size_t p = IDX(b, i, j, k);
+                int interior = (iz + 1 <= z && z <= nz - fz) && (iy + 1 <= y && y <= ny - fy) &&
+                               (ix + 1 <= x && x <= nx - fx);
+                if (interior) {
+                    double DFx[9], DFy[9], DFz[9];
+                    op_interior(b, op, i, j, k, DFx, DFy, DFz);
+                    double a1 = b->cartesian ? 1.0 / M[2 * NP + p] : 1.0 / (M[2 * NP + p] * G[9 * NP + p]);
+                    double a2 = 2.0 * M[NP + p] + M[p], a3 = M[p], a4 = M[NP + p];
+                    DF[0 * NP + p] = DF[0 * NP + p] + a1 * (DFx[3] + DFy[6] + DFz[7]);
+                    DF[1 * NP + p] = DF[1 * NP + p] + a1 * (DFx[6] + DFy[4] + DFz[8]);
+                    DF[2 * NP + p] = DF[2 * NP + p] + a1 * (DFx[7] + DFy[8] + DFz[5]);
+                    DF[3 * NP + p] = DF[3 * NP + p] + a2 * DFx[0] + a3 * (DFy[1] + DFz[2]);
+                    DF[4 * NP + p] = DF[4 * NP + p] + a2 * DFy[1] + a3 * (DFx[0] + DFz[2]);
+                    DF[5 * NP + p] = DF[5 * NP + p] + a2 * DFz[2] + a3 * (DFx[0] + DFy[1]);
+                    DF[6 * NP + p] = DF[6 * NP + p] + a4 * (DFy[0] + DFx[1]);
+                    DF[7 * NP + p] = DF[7 * NP + p] + a4 * (DFz[0] + DFx[2]);
+                    DF[8 * NP + p] = DF[8 * NP + p] + a4 * (DFz[1] + DFy[2]);
+                } else {
+                    double Ux[9], Uy[9], Uz[9], DU[9];
+                    op_band(b, op, 0, i, j, k, Ux);
+                    op_band(b, op, 1, i, j, k, Uy);
+                    op_band(b, op, 2, i, j, k, Uz);
+                    double rhoJ_inv = 1.0 / (M[2 * NP + p] * G[9 * NP + p]);
+                    double l2m = M[p] + 2.0 * M[NP + p];
+                    double lam = M[p], mu = M[NP + p];
+                    DU[0] = (Ux[3] + Uy[6] + Uz[7]) * rhoJ_inv;
+                    DU[1] = (Ux[6] + Uy[4] + Uz[8]) * rhoJ_inv;
+                    DU[2] = (Ux[7] + Uy[8] + Uz[5]) * rhoJ_inv;
+                    DU[3] = l2m * Ux[0] + lam * (Uy[1] + Uz[2]);
+                    DU[4] = l2m * Uy[1] + lam * (Ux[0] + Uz[2]);
+                    DU[5] = l2m * Uz[2] + lam * (Ux[0] + Uy[1]);
+                    DU[6] = mu * (Uy[0] + Ux[1]);
+                    DU[7] = mu * (Uz[0] + Ux[2]);
+                    DU[8] = mu * (Uz[1] + Uy[2]);
+                    pml_point(b, DU, Ux, Uy, Uz, i, j, k, rhoJ_inv, l2m);
+                    for (int f = 0; f < 9; f++) DF[f * NP + p] = DF[f * NP + p] + DU[f];
+                }
+            }
+}
+
 void ora_set_rates(ora_block *b)
 {
+    if (b->sbp >= 0) {
+        set_rates_op(b);
+        return;
+    }
     size_t NP = NPTS(b);
     int nx = b->nq, ny = b->nr, nz = b->ns;
     double hx = 1.0 / (double)(nx - 1), hy = 1.0 / (double)(ny - 1), hz = 1.0 / (double)(nz - 1);
@@ -510,6 +690,8 @@ static void structure(double Ux[9], const double Uq[9], double rho, double mu, d
 }
 
 #define TAU0 (1.0 / (0.315949074074074)) /* block.f90:191 */
+/* SAT penalty of the block's operator (block.f90:191-255) */
+static double tau0_of(const ora_block *b) { return b->sbp < 0 ? TAU0 : WQL_SBP_OPS[b->sbp].tau0; }
 
 void ora_boundary(ora_block *b)
 {
@@ -545,12 +727,12 @@ void ora_boundary(ora_block *b)
                 for (int f = 0; f < 9; f++) u[f] = b->F[f * NP + p] - 0.0 * 0.0;
                 bc_point(ubc, u, rho, mu, lambda, ll, mm, nn, norm, Zs, Zp, r, plus);
                 structure(Ux, ubc, rho, mu, lambda);
-                for (int f = 0; f < 9; f++) b->DF[f * NP + p] = b->DF[f * NP + p] - TAU0 / h * Ux[f];
+                for (int f = 0; f < 9; f++) b->DF[f * NP + p] = b->DF[f * NP + p] - tau0_of(b) / h * Ux[f];
                 if (b->pml[side]) {
                     long s = slab_index(b, side, i, j, k);
                     size_t SP = ora_slab_points(b, side);
                     for (int f = 0; f < 9; f++)
-                        b->DQ[side][f * SP + s] = b->DQ[side][f * SP + s] - TAU0 / h * ubc[f];
+                        b->DQ[side][f * SP + s] = b->DQ[side][f * SP + s] - tau0_of(b) / h * ubc[f];
                 }
             }
     }
@@ -1043,7 +1225,7 @@ int ora_interface(ora_domain *d, int ib, double t, int stage)
                 bc[7] = -mu * (q_z * e1 + q_x * e3);
                 bc[8] = -mu * (q_z * e2 + q_y * e3);
             }
-            for (int f = 0; f < 9; f++) B->DF[f * NP + pe] = B->DF[f * NP + pe] - TAU0 / hx * bc[f];
+            for (int f = 0; f < 9; f++) B->DF[f * NP + pe] = B->DF[f * NP + pe] - tau0_of(B) / hx * bc[f];
         }
     return err;
 }

@@ -75,6 +75,8 @@ typedef struct {
     double *P;          /* [2][NP] plastic strain / strain rate (may be NULL) */
     int nsrc, src_order, src_alpha; /* moment-tensor sources (order 2 or 4; alpha of order 4) */
     const ora_source *src;
+    int sbp;            /* spatial operator: -1 the sixth-order central one ('traditional'); else the
+                           upwind / upwind-DRP operator WQL_SBP_OPS[sbp] (ora_sbp_op) */
 } ora_block;
 
 typedef struct {        /* one block's copy of the interface state, q-face arrays */
@@ -108,6 +110,12 @@ int ora_interface(ora_domain *d, int ib, double t, int stage);
 int ora_step(ora_domain *d, int nsteps);
 /* initial state variable and velocities for rate and state (Interface_Condition.f90:2722-2850) */
 void ora_init_state(ora_domain *d);
+/* index of the upwind (fd_type 1) or upwind-DRP (fd_type 2) operator of the
+ * given order in WQL_SBP_OPS, -1 for fd_type 0 (block.f90:101-118), -2 for a
+ * combination the reference does not define */
+int ora_sbp_op(int fd_type, int order);
+/* the k-th operator table (wql_sbp_ops.h), NULL past the last (for tests) */
+const void *ora_sbp_table(int k);
 /* one SBP first derivative row (for tests): D applied to u at index i of n points */
 double ora_d1(const double *u, int n, int i, double h);
 /* DF -= A(t) M delta of the block's moment-tensor sources (set_moment_tensor,

@@ -33,11 +33,14 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
              plastic_model='default', pml_l=((False,) * 3,) * 2, pml_r=((False,) * 3,) * 2,
              npml=0, lc=(1.0, 0.0), rc=(0.0, 1.0), profile='analytical_test_problem',
              rho_s_p=(2.67, 3.464, 6.0), t_final=1.0, aqrs=None, bqrs=None, mu_beta_eta=(0.0, 0.0, 0.0),
-             moment=None):
+             moment=None, fd_type=None, order=None):
     T = lambda b: 'T' if b else 'F'
     s = ["&problem_list",
          f" name = 'data/{name}',", f" problem = '{problem}',", f" response = '{response}',",
-         f" plastic_model = '{plastic_model}',", " nblocks = 2,", f" t_final = {t_final}d0,",
+         f" plastic_model = '{plastic_model}',", " nblocks = 2,", f" t_final = {t_final}d0,"]
+    if fd_type is not None:
+        s += [f" fd_type = '{fd_type}',", f" order = {order},"]
+    s += [
          " CFL = 0.5d0,", f" coupling = '{coupling}',", f" type_of_mesh = '{mesh}',",
          " w_stride = 1,", " w_fault = F /", "", "&block_list"]
     for b in range(2):
@@ -141,6 +144,36 @@ CASES = {
                            n=N_STD, lq=LQ_STD, rq=RQ_STD, response='plastic', plastic_model='scec',
                            aqrs=[(-6.0, 2.0, -9.0), (0.0, 2.0, -9.0)], bqrs=[(0.0, 24.0, 3.0), (6.0, 24.0, 3.0)]),
                       [('traj', ['20', '-1.0', '13.0', '-3.0', '4.0', '2.0'])]),
+    # upwind and upwind-DRP SBP operators (fd_type / order, block.f90:101-255):
+    # each grid holds every closure row of its operator and a few interior
+    # rows, and at least 13 nodes per direction (the metric derivatives use
+    # the sixth-order closure, metrics.f90:16-490); the cases cover the q-only stress tap of the order-9 row n-7
+    # (curvilinear and rate and state), the DRP operators whose band and
+    # interior routines print different interior rows (66, 679), PML slabs
+    # with the upwind band, and both mesh kinds
+    'up2_cart': (dict(problem='TPV5', coupling='slip-weakening_friction', mesh='cartesian', n=(13, 13, 13),
+                      lq=LQ_STD, rq=RQ_STD, fd_type='upwind', order=2),
+                 [('rhs', ['0.3'])]),
+    'up4_curv': (dict(problem='TPV28', coupling='slip-weakening_friction', mesh='curvilinear', n=(14, 13, 13),
+                      lq=LQ_STD, rq=RQ_STD, fd_type='upwind', order=4),
+                 [('rhs', ['0.3']), ('traj', ['10'])]),
+    'up6_pml': (dict(problem='TPV5', coupling='locked', mesh='cartesian', n=(16, 15, 14),
+                     lq=[(1, 1, 1), (0, 1, 1)], rq=[(0, 1, 1), (1, 2, 1)],
+                     pml_l=((True, True, False), (False, False, True)),
+                     pml_r=((False, False, True), (True, False, False)), npml=7, fd_type='upwind', order=6),
+                [('rhs', ['0.1']), ('traj', ['8', '-1.2', '7.4', '0.2', '0.5', '1.0'])]),
+    'up9_curv': (dict(problem='TPV101', coupling='rate-and-state_friction', mesh='curvilinear', n=(19, 18, 17),
+                      lq=LQ_STD, rq=RQ_STD, fd_type='upwind', order=9),
+                 [('rhs', ['0.55'])]),
+    'drp3_curv': (dict(problem='TPV5', coupling='slip-weakening_friction', mesh='curvilinear', n=(13, 14, 13),
+                       lq=LQ_STD, rq=RQ_STD, fd_type='upwind_drp', order=3),
+                  [('rhs', ['0.3'])]),
+    'drp66_cart': (dict(problem='TPV28', coupling='slip-weakening_friction', mesh='cartesian', n=(15, 14, 13),
+                        lq=LQ_STD, rq=RQ_STD, fd_type='upwind_drp', order=66),
+                   [('rhs', ['0.3']), ('traj', ['8'])]),
+    'drp679_curv': (dict(problem='TPV5', coupling='slip-weakening_friction', mesh='curvilinear', n=(19, 18, 17),
+                         lq=LQ_STD, rq=RQ_STD, fd_type='upwind_drp', order=679),
+                    [('rhs', ['0.3'])]),
 }
 
 

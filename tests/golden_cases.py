@@ -10,9 +10,13 @@ sys.path.insert(0, os.path.join(ROOT, 'waveqlab3d-gh_amd'))
 
 from wql import problem as P  # noqa: E402
 
-RHS_CASES = ['cart_sw', 'curv_sw', 'curv_rs', 'pml_locked', 'rw_plastic', 'tpv26_sw', 'mt_cart', 'mt_curv']
+# the upwind and upwind-DRP operators (fd_type / order)
+SBP_RHS_CASES = ['up2_cart', 'up4_curv', 'up6_pml', 'up9_curv', 'drp3_curv', 'drp66_cart', 'drp679_curv']
+SBP_TRAJ_CASES = ['up4_curv', 'up6_pml', 'drp66_cart']
+RHS_CASES = ['cart_sw', 'curv_sw', 'curv_rs', 'pml_locked', 'rw_plastic', 'tpv26_sw', 'mt_cart',
+             'mt_curv'] + SBP_RHS_CASES
 TRAJ_CASES = ['cart_sw', 'curv_sw', 'curv_rs', 'curv_rs_long', 'pml_locked', 'plastic_sw', 'locked_pulse',
-              'rw_plastic', 'tpv26_sw', 'tpv26_plastic', 'mt_cart', 'mt_curv']
+              'rw_plastic', 'tpv26_sw', 'tpv26_plastic', 'mt_cart', 'mt_curv'] + SBP_TRAJ_CASES
 
 # Relative-L2 bar for FP64 trajectories.  'curv_rs' stops before rupture:
 # slip rates there are ~1e-12 m/s, equal to the absolute step tolerance of

@@ -70,6 +70,7 @@ def test_fortran_reader_matches_python_reader(case, tmp_path):
         got = _check(path)
         assert got['problem'] == [pl['problem'], pl['coupling'], pl['response'], pl['type_of_mesh']]
         assert float(got['time'][0]) == pl['t_final'] and float(got['time'][1]) == pl['cfl']
+        assert got['operator'] == [pl.get('fd_type', 'traditional'), str(pl.get('order', 5))]
         for i in (1, 2):
             v = got[f'block{i}']
             g = lambda k: bl[f'btp({i})%{k}']  # noqa: E731
@@ -111,7 +112,7 @@ def _run_main(inp, steps, cwd, extra=()):
 @pytest.mark.gpu
 @pytest.mark.skipif(not _have(MAIN), reason='Fortran host not built (no flang)')
 @pytest.mark.parametrize('case', ['cart_sw', 'curv_sw', 'curv_rs_long', 'plastic_sw', 'rw_plastic', 'tpv26_sw',
-                                  'mt_cart', 'mt_curv'])
+                                  'mt_cart', 'mt_curv', 'up4_curv', 'drp66_cart'])
 def test_fortran_main_matches_reference(case, tmp_path):
     _say(f'[gpu] fortran main {case}')
     z = G.load(case)

@@ -167,6 +167,11 @@ struct wql_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double vol_ms = 0.0;
     long vol_launches = 0;
+    // spatial operator: -1 the sixth-order central one, else WQL_SBP_OPS[sbp]
+    // (its v and s tables copied to d_sbp); tau0 the SAT penalty of its norm
+    int sbp = -1;
+    double tau0 = 0.0;
+    wql_sbp_half *d_sbp = nullptr;
 };
 
 namespace {
@@ -207,6 +212,8 @@ Blk<real> blk_view(const wql_ctx *c, int ib, int cur)
     B.Y = d.Y;
     B.satb = nullptr;
     B.nbnd = d.nbnd;
+    B.sbp = c->sbp >= 0 ? c->d_sbp : nullptr;
+    B.sbw = c->sbp >= 0 ? WQL_SBP_OPS[c->sbp].band : 0;
     return B;
 }
 
@@ -339,6 +346,25 @@ void launch_pml(wql_ctx *c, Pair<real> &P, real A, real bdt)
     BoxList BL = box_list(c);
     if (BL.total > 0)
         k_volume_generic<real><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(P, BL, A, A, bdt);
+}
+
+// the upwind / upwind-DRP operators: every node of both blocks, one thread
+// each (replaces the interior, s-closure and PML-slab kernels)
+template <typename real>
+void launch_op(wql_ctx *c, Pair<real> &P, real A, real bdt)
+{
+    BoxList L;
+    memset(&L, 0, sizeof L);
+    long start = 0;
+    for (int ib = 0; ib < 2; ib++) {
+        const DevBlock &d = c->b[ib];
+        Box &b = L.box[L.nbox++];
+        b.blk = ib; b.x0 = 0; b.x1 = d.nq; b.y0 = 0; b.y1 = d.nr; b.z0 = 0; b.z1 = d.ns;
+        b.start = start;
+        start += (long)d.nq * d.nr * d.ns;
+    }
+    L.total = start;
+    k_volume_op<real><<<(unsigned)((L.total + 255) / 256), 256, 0, c->stream>>>(P, L, A, A, bdt);
 }
 
 // the moment-tensor sources of both blocks at the stage time (after the
@@ -478,7 +504,7 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
     a.tinc = tinc;
     a.A[0] = a.A[1] = A;
     a.bdt = bdt;
-    a.tau0 = TAU0;
+    a.tau0 = c->tau0;
     a.stage = stage;
     a.rates_only = rates_only;
     a.flag = c->d_flag;
@@ -515,7 +541,9 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
     }
     // mixed grids: a Cartesian block inside a curvilinear pair still has no G
     TileList TL = tile_list(c);
-    if (TL.total > 0) {
+    if (c->sbp >= 0) {
+        launch_op<real>(c, P, (real)A, (real)bdt);
+    } else if (TL.total > 0) {
         if (c->profile && !c->use_graph) hipEventRecord(c->ev0, c->stream);
         launch_main<real>(c, P, TL, (real)A, (real)A, (real)bdt);
         if (c->profile && !c->use_graph) {
@@ -527,8 +555,10 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
             c->vol_launches++;
         }
     }
-    launch_band<real>(c, P, (real)A, (real)bdt);
-    launch_pml<real>(c, P, (real)A, (real)bdt);
+    if (c->sbp < 0) {
+        launch_band<real>(c, P, (real)A, (real)bdt);
+        launch_pml<real>(c, P, (real)A, (real)bdt);
+    }
     launch_sources<real>(c, P, t_stage, tdev, tinc, bdt);
     if (!fused) {
         a.mode = 0;
@@ -636,12 +666,14 @@ void reset_typed(wql_ctx *c)
 }
 
 int valid_problem(const wql_problem *prob);
+int sbp_index(const wql_problem *prob);
 
 int ctx_reset(wql_ctx *c, const wql_problem *prob)
 {
     if (int rc = valid_problem(prob)) return rc;
     if (prob->precision != c->prec || prob->plastic != c->prob.plastic || prob->s0_kind != c->prob.s0_kind)
         return fail(WQL_E_ARG, "a context is reused only for the same precision and plasticity model");
+    if (sbp_index(prob) != c->sbp) return fail(WQL_E_ARG, "a context is reused only for the same spatial operator");
     drop_graphs(c);
     c->prob = *prob;
     memcpy(&c->fault, &prob->fault, sizeof(FaultParams));
@@ -989,9 +1021,23 @@ double *scratch_ptr(int dev)
     return g_scratch[dev];
 }
 
+// the operator of &problem_list fd_type / order (block.f90:101-255): -1 the
+// sixth-order central one, else the index in WQL_SBP_OPS, -2 if undefined
+int sbp_index(const wql_problem *prob)
+{
+    if (prob->fd_type == WQL_FD_TRADITIONAL) return -1;
+    const char *fam = prob->fd_type == WQL_FD_UPWIND ? "upwind" : prob->fd_type == WQL_FD_UPWIND_DRP ? "upwind_drp" : "";
+    for (int k = 0; k < WQL_SBP_NOPS; k++)
+        if (strcmp(WQL_SBP_OPS[k].family, fam) == 0 && WQL_SBP_OPS[k].order == prob->fd_order) return k;
+    return -2;
+}
+
 int valid_problem(const wql_problem *prob)
 {
     if (prob->precision != WQL_FP64 && prob->precision != WQL_FP32) return fail(WQL_E_ARG, "bad precision");
+    if (sbp_index(prob) < -1)
+        return fail(WQL_E_ARG, "no operator for fd_type %d order %d (upwind: 2-9; upwind_drp: 3-7, 66, 679)",
+                    prob->fd_type, prob->fd_order);
     if (!(prob->dt > 0)) return fail(WQL_E_ARG, "dt must be positive");
     const wql_fault_params &f = prob->fault;
     if (f.law < 0 || f.law > 2 || f.npatch < 0 || f.npatch > 4 || f.stress_mode < 0 || f.stress_mode > 2 ||
@@ -1068,6 +1114,15 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
     if (blocks[0].bc[1] != 0 || blocks[1].bc[0] != 0)
         return fail(WQL_E_ARG, "the fault joins block 0 side Rx and block 1 side Lx (bc code 0)");
     if (int rc = valid_problem(prob)) return rc;
+    const int sbp = sbp_index(prob);
+    if (sbp >= 0)
+        for (int ib = 0; ib < 2; ib++) {
+            // every closure row of the operator and an interior row in each direction
+            const int need = 2 * WQL_SBP_OPS[sbp].band + 1;
+            if (blocks[ib].nq < need || blocks[ib].nr < need || blocks[ib].ns < need)
+                return fail(WQL_E_ARG, "block %d: the %s operator of order %d needs %d nodes per direction", ib,
+                            WQL_SBP_OPS[sbp].family, WQL_SBP_OPS[sbp].order, need);
+        }
     for (int ib = 0; ib < 2; ib++) {
         const size_t elem = prob->precision == WQL_FP32 ? 4 : 8;
         const long lane = 128 / (long)elem;
@@ -1096,6 +1151,8 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
     c->prob = *prob;
     c->dt = prob->dt;
     c->t = prob->t0;
+    c->sbp = sbp;
+    c->tau0 = sbp >= 0 ? WQL_SBP_OPS[sbp].tau0 : TAU0;
     static_assert(sizeof(FaultParams) == sizeof(wql_fault_params), "fault layout");
     memcpy(&c->fault, &prob->fault, sizeof(FaultParams));
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1128,6 +1185,19 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
         return fail(WQL_E_NOMEM, "hipMalloc(time)");
     }
     c->allocs.push_back(c->d_t);
+    if (c->sbp >= 0) {
+        // the operator's velocity (Dp) and stress (Dm) tables, per direction
+        wql_sbp_half h[6];
+        for (int a = 0; a < 3; a++) { h[a] = WQL_SBP_OPS[c->sbp].v[a]; h[3 + a] = WQL_SBP_OPS[c->sbp].s[a]; }
+        if (int rc2 = dalloc(c, (void **)&c->d_sbp, sizeof h)) {
+            wql_destroy(c);
+            return rc2;
+        }
+        if (hipMemcpy(c->d_sbp, h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) {
+            wql_destroy(c);
+            return fail(WQL_E_HIP, "hipMemcpy(operator table)");
+        }
+    }
     {
         const size_t nf = 9 * ((size_t)c->b[0].nr * c->b[0].ns + (size_t)c->b[1].nr * c->b[1].ns);
         if (hipMalloc(&c->d_faces, nf * sizeof(double)) != hipSuccess ||
@@ -1718,14 +1788,22 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
         }
         TileList TL = tile_list(c);
         if (c->prec == WQL_FP64) {
-            if (TL.total > 0) launch_main<double>(c, Pd, TL, 0.0, 0.0, 0.0);
-            launch_band<double>(c, Pd, 0.0, 0.0);
-            launch_pml<double>(c, Pd, 0.0, 0.0);
+            if (c->sbp >= 0) {
+                launch_op<double>(c, Pd, 0.0, 0.0);
+            } else {
+                if (TL.total > 0) launch_main<double>(c, Pd, TL, 0.0, 0.0, 0.0);
+                launch_band<double>(c, Pd, 0.0, 0.0);
+                launch_pml<double>(c, Pd, 0.0, 0.0);
+            }
             launch_sources<double>(c, Pd, t, nullptr, 0.0, 0.0);
         } else {
-            if (TL.total > 0) launch_main<float>(c, Pf, TL, 0.f, 0.f, 0.f);
-            launch_band<float>(c, Pf, 0.f, 0.f);
-            launch_pml<float>(c, Pf, 0.f, 0.f);
+            if (c->sbp >= 0) {
+                launch_op<float>(c, Pf, 0.f, 0.f);
+            } else {
+                if (TL.total > 0) launch_main<float>(c, Pf, TL, 0.f, 0.f, 0.f);
+                launch_band<float>(c, Pf, 0.f, 0.f);
+                launch_pml<float>(c, Pf, 0.f, 0.f);
+            }
             launch_sources<float>(c, Pf, t, nullptr, 0.0, 0.0);
         }
     } else {

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "wql_sbp_ops.h"
+
 namespace wql {
 
 constexpr int NF = 9;  // vx vy vz sxx syy szz sxy sxz syz (RHS_Interior.f90:241-251)
@@ -93,6 +95,10 @@ struct Blk {
     // node j + nr k): written with the stage output by the volume kernels,
     // read as the next stage's input by the boundary kernel (nullptr: unused)
     real *FQ;
+    // upwind / upwind-DRP operator (device copy of WQL_SBP_OPS[k].v[0..2]
+    // then .s[0..2]) and its band width; nullptr: the sixth-order central one
+    const wql_sbp_half *sbp;
+    int sbw;
 };
 
 // index of node (i, j, k) in the boundary-node enumeration (q faces, then r

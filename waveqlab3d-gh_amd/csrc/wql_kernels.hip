@@ -355,6 +355,172 @@ __global__ void __launch_bounds__(256) k_volume_generic(Pair<real> P, BoxList L,
 }
 
 // ----------------------------------------------------------------------------
+// volume: the upwind and upwind-DRP operators ('upwind' / 'upwind_drp')
+//
+// One thread per node of the whole block.  The operator is a table
+// (wql_sbp_ops.h): per direction, the closure rows at both faces and the
+// interior row, for the velocities (forward operator Dp) and for the stress
+// fluxes J a_x sigma (backward operator Dm).  A node whose three directions
+// are all interior (band width per order, widened to npml on PML sides)
+// takes the form of the interior routine JJU_x<o>_interior_upwind[_drp]
+// (JU_xJU_yJU_z6.f90:16451-20922): the exact rational coefficients, one
+// loop over the taps adding the three directions per tap; with Cartesian
+// metrics the diagonal metric of the node and no J.  Every other node
+// takes the band routine JJU_x<o>_upwind[_drp] (815-16122, through
+// RHS_near_boundaries, RHS_Interior.f90:144-1506): per direction the row's
+// taps, velocities X_x(node) sum c v, stresses sum c X_x J sigma, and
+// Ju = (1/hq) Jq + (1/hr) Jr + (1/hs) Js.  Hooke's law, the PML terms, the
+// boundary penalty and the stage update follow in node_tail.
+
+// taps of node pos on a line of n nodes: a closure row or the printed
+// interior row
+template <typename real>
+__device__ __forceinline__ int op_taps(const wql_sbp_half &h, int pos, int n, int *idx, real *c)
+{
+    int m = 0;
+    if (pos < h.nl) {
+        for (int t = 0; t < WQL_SBP_MAXW; t++) {
+            const double v = h.left[pos][t];
+            if (v != 0.0) { idx[m] = t; c[m] = (real)v; m++; }
+        }
+    } else if (pos >= n - h.nr) {
+        const int r = n - 1 - pos;
+        for (int t = 0; t < WQL_SBP_MAXW; t++) {
+            const double v = h.right[r][t];
+            if (v != 0.0) { idx[m] = n - 1 - t; c[m] = (real)v; m++; }
+        }
+    } else {
+        for (int o = h.lo; o <= h.hi; o++) { idx[m] = pos + o; c[m] = (real)h.band[o - h.lo]; m++; }
+    }
+    return m;
+}
+
+template <typename real>
+__device__ void volume_node_op(const Blk<real> &B, int i, int j, int k, real A, real bdt)
+{
+    const long NP = B.NP;
+    const long p = pidx(B.px, B.nr, i, j, k);
+    const int nn[3] = {B.nq, B.nr, B.ns};
+    const int pos[3] = {i, j, k};
+    const wql_sbp_half *V = B.sbp, *S = B.sbp + 3;
+    real U[3][NF];
+    const int w = B.sbw;
+    bool interior = true;
+    for (int a = 0; a < 3; a++) {
+        const int lo = B.pml[2 * a] && B.npml > w ? B.npml : w;
+        const int hi = B.pml[2 * a + 1] && B.npml > w ? B.npml : w;
+        interior = interior && pos[a] >= lo && pos[a] <= nn[a] - 1 - hi;
+    }
+    real Jc;
+    if (interior) {
+        for (int fam = 0; fam < 3; fam++)
+            for (int f = 0; f < NF; f++) U[fam][f] = 0;
+        real gp[9];
+        for (int c = 0; c < 9; c++) gp[c] = (real)gmet(B, c, p);
+        real ih[3];
+        for (int a = 0; a < 3; a++) ih[a] = (real)(1.0 / (1.0 / (double)(nn[a] - 1)));
+        const int nt = V[0].hi - V[0].lo + 1;
+        for (int t = 0; t < nt; t++) {
+            long pv[3], ps[3];
+            real cv[3], cs[3];
+            for (int a = 0; a < 3; a++) {
+                const int ov = V[a].lo + t, os = S[a].lo + t;
+                int q[3] = {i, j, k};
+                q[a] += ov;
+                pv[a] = pidx(B.px, B.nr, q[0], q[1], q[2]);
+                q[a] += os - ov;
+                ps[a] = pidx(B.px, B.nr, q[0], q[1], q[2]);
+                cv[a] = (real)V[a].exact[t] * ih[a];
+                cs[a] = (real)S[a].exact[t] * ih[a];
+            }
+            for (int fam = 0; fam < 3; fam++) {
+                if (B.cart) {
+                    const real g = gp[3 * fam + fam];
+                    for (int f = 0; f < 3; f++) U[fam][f] = U[fam][f] + g * cv[fam] * B.Fin[f * NP + pv[fam]];
+                    for (int f = 3; f < NF; f++) U[fam][f] = U[fam][f] + g * cs[fam] * B.Fin[f * NP + ps[fam]];
+                } else {
+                    for (int f = 0; f < 3; f++) {
+                        real v = U[fam][f];
+                        for (int a = 0; a < 3; a++) v = v + gp[3 * fam + a] * cv[a] * B.Fin[f * NP + pv[a]];
+                        U[fam][f] = v;
+                    }
+                    real gJ[3];
+                    for (int a = 0; a < 3; a++) gJ[a] = (real)gmet(B, 3 * fam + a, ps[a]) * (real)gmet(B, 9, ps[a]);
+                    for (int f = 3; f < NF; f++) {
+                        real v = U[fam][f];
+                        for (int a = 0; a < 3; a++) v = v + cs[a] * B.Fin[f * NP + ps[a]] * gJ[a];
+                        U[fam][f] = v;
+                    }
+                }
+            }
+        }
+        Jc = B.cart ? (real)1 : (real)gmet(B, 9, p);
+    } else {
+        for (int a = 0; a < 3; a++) {
+            int idx[WQL_SBP_MAXW];
+            real c[WQL_SBP_MAXW];
+            real sv[3] = {0, 0, 0};
+            int m = op_taps<real>(V[a], pos[a], nn[a], idx, c);
+            for (int t = 0; t < m; t++) {
+                int q[3] = {i, j, k};
+                q[a] = idx[t];
+                const long qq = pidx(B.px, B.nr, q[0], q[1], q[2]);
+                for (int f = 0; f < 3; f++) sv[f] = sv[f] + c[t] * B.Fin[f * NP + qq];
+            }
+            real ss[3][6];
+            for (int fam = 0; fam < 3; fam++)
+                for (int f = 0; f < 6; f++) ss[fam][f] = 0;
+            m = op_taps<real>(S[a], pos[a], nn[a], idx, c);
+            for (int t = 0; t < m; t++) {
+                int q[3] = {i, j, k};
+                q[a] = idx[t];
+                const long qq = pidx(B.px, B.nr, q[0], q[1], q[2]);
+                const real J = (real)gmet(B, 9, qq);
+                real cm[3];
+                for (int fam = 0; fam < 3; fam++) cm[fam] = c[t] * (real)gmet(B, 3 * fam + a, qq) * J;
+                for (int f = 0; f < 6; f++) {
+                    const real sg = B.Fin[(3 + f) * NP + qq];
+                    for (int fam = 0; fam < 3; fam++) ss[fam][f] = ss[fam][f] + cm[fam] * sg;
+                }
+            }
+            const real ih = (real)(1.0 / (1.0 / (double)(nn[a] - 1)));
+            for (int fam = 0; fam < 3; fam++) {
+                const real mc = (real)gmet(B, 3 * fam + a, p);
+                for (int f = 0; f < 3; f++) {
+                    const real v = ih * (mc * sv[f]);
+                    U[fam][f] = a == 0 ? v : U[fam][f] + v;
+                }
+                for (int f = 0; f < 6; f++) {
+                    const real v = ih * ss[fam][f];
+                    U[fam][3 + f] = a == 0 ? v : U[fam][3 + f] + v;
+                }
+            }
+        }
+        Jc = (real)gmet(B, 9, p);
+    }
+    real fin[NF];
+    for (int f = 0; f < NF; f++) fin[f] = B.Fin[f * NP + p];
+    node_tail<real>(B, i, j, k, p, U, fin, Jc, A, bdt);
+}
+
+template <typename real>
+__global__ void __launch_bounds__(256) k_volume_op(Pair<real> P, BoxList L, real A0, real A1, real bdt)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.total) return;
+    int bi = 0;
+    for (int b = 1; b < L.nbox; b++)
+        if (t >= L.box[b].start) bi = b;
+    const Box &bx = L.box[bi];
+    long r = t - bx.start;
+    int w = bx.x1 - bx.x0, h = bx.y1 - bx.y0;
+    int i = bx.x0 + (int)(r % w);
+    int j = bx.y0 + (int)((r / w) % h);
+    int k = bx.z0 + (int)(r / ((long)w * h));
+    volume_node_op<real>(P.b[bx.blk], i, j, k, bx.blk == 0 ? A0 : A1, bdt);
+}
+
+// ----------------------------------------------------------------------------
 // volume: main tiled kernel
 //
 // A thread block owns a TX x TY tile of (q, r) columns and marches a chunk of

@@ -13,7 +13,7 @@
 // Retired contexts are not freed: they wait in a pool keyed by everything
 // their allocation and set-up depend on (block shapes, boundary and PML
 // descriptors, a hash of the coordinates and material, precision,
-// plasticity).  A scenario with the same key takes a pooled context, which
+// plasticity, spatial operator).  A scenario with the same key takes a pooled context, which
 // is cleared on its stream and keeps its metrics, face frames and material,
 // so neither admission nor retirement allocates, frees or recomputes
 // geometry, and the poll loop never waits for the device except to download
@@ -100,7 +100,8 @@ uint64_t fnv(uint64_t h, const void *p, size_t bytes)
 uint64_t scenario_key(const wql_scenario *sc)
 {
     uint64_t h = 1469598103934665603ull;
-    const int flags[3] = {sc->prob.precision, sc->prob.plastic, sc->prob.s0_kind};
+    const int flags[5] = {sc->prob.precision, sc->prob.plastic, sc->prob.s0_kind, sc->prob.fd_type,
+                          sc->prob.fd_type ? sc->prob.fd_order : 0};
     h = fnv(h, flags, sizeof flags);
     for (int ib = 0; ib < 2; ib++) {
         const wql_block_desc &b = sc->blocks[ib];

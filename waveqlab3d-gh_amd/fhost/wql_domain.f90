@@ -142,6 +142,16 @@ contains
     call wql_check(wql_problem_keys(c_string(D%problem), c_string(D%coupling), c_string(D%p%response), &
                                     c_string(D%p%plastic_model), D%p%btp(1)%mu_beta_eta, D%prob), &
                    'wql_problem_keys')
+    ! spatial operator (&problem_list fd_type / order, block.f90:101-255)
+    select case (trim(D%p%fd_type))
+    case ('upwind')
+       D%prob%fd_type = WQL_FD_UPWIND
+    case ('upwind_drp')
+       D%prob%fd_type = WQL_FD_UPWIND_DRP
+    case default
+       D%prob%fd_type = WQL_FD_TRADITIONAL
+    end select
+    D%prob%fd_order = D%p%order
     call wql_check(wql_create(D%prob, D%desc, D%ctx), 'wql_create')
     ! Init_Vel_State (domain.f90:526-529)
     call wql_check(wql_init_state(D%ctx), 'wql_init_state')

@@ -10,6 +10,7 @@ module wql_gpu
 
   integer(c_int), parameter :: WQL_FP64 = 0, WQL_FP32 = 1
   integer(c_int), parameter :: WQL_LOCKED = 0, WQL_SLIP_WEAKENING = 1, WQL_RATE_STATE = 2
+  integer(c_int), parameter :: WQL_FD_TRADITIONAL = 0, WQL_FD_UPWIND = 1, WQL_FD_UPWIND_DRP = 2
   integer(c_int), parameter :: WQL_OK = 0, WQL_E_ARG = -1, WQL_E_NOMEM = -2, WQL_E_HIP = -3, &
                                WQL_E_NONFINITE = -4, WQL_E_SOLVER = -5, WQL_E_STATE = -6
 
@@ -51,6 +52,8 @@ module wql_gpu
      real(c_double) :: pl_c = 1.36d0, pl_nu = 0.1934d0, pl_Tv = 0.03d0, pl_mu = 0, pl_beta = 0, pl_eta = 0
      integer(c_int) :: s0_kind = 0
      type(wql_fault_params) :: fault
+     integer(c_int) :: fd_type = WQL_FD_TRADITIONAL  ! &problem_list fd_type
+     integer(c_int) :: fd_order = 5                  ! &problem_list order
   end type wql_problem
 
   !> wql_grid_desc (init_grid_cartesian / init_grid_curve, grid.f90:13-144)

@@ -36,6 +36,7 @@ program wql_main
         write(*, '(a,1x,a,1x,a,1x,a,1x,a)') 'problem', trim(p%problem), trim(p%coupling), trim(p%response), &
              trim(p%type_of_mesh)
         write(*, '(a,es25.17,1x,es25.17,1x,i0)') 'time', p%t_final, p%CFL, p%w_stride
+        write(*, '(a,1x,a,1x,i0)') 'operator', trim(p%fd_type), p%order
         do ib = 1, 2
            write(*, '(a,i0,3(1x,i0),6(1x,es25.17),2(1x,es25.17),6(1x,i0),6(1x,l1),1x,i0,1x,a)') 'block', ib, &
                 p%btp(ib)%nqrs, p%btp(ib)%aqrs, p%btp(ib)%bqrs, p%btp(ib)%lc, p%btp(ib)%rc, &

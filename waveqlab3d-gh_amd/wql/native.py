@@ -61,7 +61,10 @@ class ProblemDesc(C.Structure):
     _fields_ = [('precision', C.c_int), ('dt', C.c_double), ('t0', C.c_double), ('plastic', C.c_int),
                 ('pl_c', C.c_double), ('pl_nu', C.c_double), ('pl_Tv', C.c_double),
                 ('pl_mu', C.c_double), ('pl_beta', C.c_double), ('pl_eta', C.c_double),
-                ('s0_kind', C.c_int), ('fault', FaultParams)]
+                ('s0_kind', C.c_int), ('fault', FaultParams), ('fd_type', C.c_int), ('fd_order', C.c_int)]
+
+
+FD_TYPES = {'traditional': 0, 'upwind': 1, 'upwind_drp': 2}
 
 
 _lib = None

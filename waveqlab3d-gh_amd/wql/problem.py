@@ -170,6 +170,11 @@ class Problem:
     # {'U': [...], 'V': [...]} of source dicts (type, duration, t_init, m, x,
     # alpha); see block_sources
     moment: Optional[dict] = None
+    # spatial operator (&problem_list fd_type, order; domain.f90:106, 135-136):
+    # 'traditional' (sixth-order central), 'upwind' (order 2-9) or
+    # 'upwind_drp' (order 3-7, 66, 679); the order is unused for traditional
+    fd_type: str = 'traditional'
+    order: int = 5
 
     def dt(self):
         """Global time step (domain.f90:222-235, block.f90:437-452)."""
@@ -318,7 +323,8 @@ def from_namelist(path):
     prob = Problem(blocks=blocks, fault=fault_params(problem, coupling),
                    cfl=float(pl.get('cfl', 0.5)), t_final=float(pl.get('t_final', 0.0)),
                    plastic=plastic, name=pl.get('name', 'default'), w_stride=int(pl.get('w_stride', 1)),
-                   w_fault=bool(pl.get('w_fault', False)), moment=read_moment(text))
+                   w_fault=bool(pl.get('w_fault', False)), moment=read_moment(text),
+                   fd_type=pl.get('fd_type', 'traditional'), order=int(pl.get('order', 5)))
     if plastic == 2:
         # per-block mu_beta_eta (plastic_material.f90:38-44); one set for both blocks here
         mbe = [tuple(float(v) for v in bl.get(f'btp({i})%mu_beta_eta', (0.0, 0.0, 0.0))) for i in (1, 2)]

@@ -44,6 +44,8 @@ def problem_desc(prob, precision='fp64', dt=None):
     for k in ('pl_c', 'pl_nu', 'pl_Tv', 'pl_mu', 'pl_beta', 'pl_eta', 's0_kind'):
         setattr(d, k, getattr(prob, k))
     d.fault = fault_struct(prob.fault)
+    d.fd_type = native.FD_TYPES[prob.fd_type]
+    d.fd_order = prob.order
     return d
 
 
