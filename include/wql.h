@@ -107,6 +107,13 @@ typedef struct {
     int fd_order;          /* &problem_list order: 2..9 (upwind), 3..7, 66, 679 (upwind_drp); it also sets
                               the SAT penalty tau0 and the band width (block.f90:103-255,
                               RHS_Interior.f90:133-1514).  Ignored for WQL_FD_TRADITIONAL */
+    int anelastic;         /* 1: response = 'anelastic' (material.f90:11-145): four GSLS mechanisms per
+                              stress component, memory variables in every volume kernel
+                              (RHS_Interior.f90:2390-2532); not together with plastic */
+    double an_c;           /* &anelastic_list c: Q_s = c v_s (v_s in km/s), Q_p = Q_s / 2 */
+    double an_weight_exp;  /* &anelastic_list weight_exp: < 0.01 or in (0.59, 0.61), the two weight sets
+                              the reference defines (material.f90:80-106) */
+    double an_fref;        /* &anelastic_list fref: reference frequency of the unrelaxed moduli */
 } wql_problem;
 
 /* The fault law, prestress and plasticity settings the reference keys by
@@ -115,9 +122,13 @@ typedef struct {
  * initial_stress_condition.f90; plastic.f90:174-182; plastic_material.f90:
  * 38-44) for TPV5, TPV28, TPV26/27 (slip weakening), TPV101/102 and
  * 'rate-weakening' (rate and state), 'locked' and the 'scec' / default
- * plasticity models.  Fills out->fault and the plastic fields; keeps
- * out->precision, dt and t0.  WQL_E_ARG for keys outside that set (among
- * them 'KDN', which the reference cannot run: it has no prestress). */
+ * plasticity models; response 'anelastic' sets anelastic with the
+ * &anelastic_list defaults (c 1, weight_exp 0, fref 1), which the caller
+ * replaces by the list's values.  Fills out->fault and the plastic and
+ * anelastic fields; keeps out->precision, dt and t0 (the operator fields
+ * are zeroed: the caller sets fd_type and fd_order).  WQL_E_ARG for keys
+ * outside that set (among them 'KDN', which the reference cannot run: it
+ * has no prestress). */
 int wql_problem_keys(const char *problem, const char *coupling, const char *response, const char *plastic_model,
                      const double mu_beta_eta[3], wql_problem *out);
 
@@ -165,6 +176,12 @@ int wql_init_state(wql_ctx *ctx);
 int wql_set_fields(wql_ctx *ctx, int block, const double *F);
 int wql_get_fields(wql_ctx *ctx, int block, double *F);
 int wql_get_rates(wql_ctx *ctx, int block, double *DF);
+/* anelastic memory variables eta (what 0) or their rates (what 1) of a
+ * block, [24][ns][nr][nq]: stress component sxx syy szz sxy sxz syz
+ * (M%eta4..eta9, datatypes.f90:50-51) times mechanism 1..4.  WQL_E_ARG
+ * for a problem without anelastic response. */
+int wql_set_anelastic(wql_ctx *ctx, int block, const double *eta);
+int wql_get_anelastic(wql_ctx *ctx, int block, int what, double *out);
 /* wql_get_fields without any device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops).  It stages through a fixed

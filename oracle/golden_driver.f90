@@ -53,6 +53,11 @@ program golden_driver
   if (in_block_comm(2) .and. .not. in_block_comm(1)) ib = 2
 
   call dump_geometry(D, ib, prefix)
+  ! the unrelaxed moduli and Q^-1 of init_anelastic_properties (material.f90:11-145)
+  if (D%B(ib)%M%anelastic) then
+     call dump3(D%B(ib)%M%M, D, ib, prefix, 'Mat')
+     call dump3(reshape(D%B(ib)%M%Qs_inv, [shape(D%B(ib)%M%Qs_inv), 1]), D, ib, prefix, 'Qsinv')
+  end if
 
   select case (trim(mode))
   case ('rhs')
@@ -61,6 +66,7 @@ program golden_driver
      call fill_pattern(D, ib)
      call set_rates(D)
      call dump3(D%B(ib)%F%DF, D, ib, prefix, 'DFvol')
+     call dump_anelastic(D, ib, prefix, 'Deta')
      call dump_pml(D, ib, prefix, 'DQvol')
      call exchange_fields_interface(D)
      call enforce_bound_iface_conditions(D, 2)
@@ -91,6 +97,7 @@ program golden_driver
      call dump_face(D%I(1)%Svel, D, ib, prefix, 'Svel')
      call dump_pml(D, ib, prefix, 'Q')
      if (allocated(D%B(ib)%P%P)) call dump3(D%B(ib)%P%P, D, ib, prefix, 'P')
+     call dump_anelastic(D, ib, prefix, 'eta')
      if (rank == 0) then
         open(newunit = unit_in, file = trim(prefix)//'_meta.txt', status = 'replace')
         write(unit_in, '(a,es25.17)') 'dt ', D%dt
@@ -158,8 +165,51 @@ contains
       end do
       D%I(1)%DS = 0.0_wp
       D%I(1)%DW = 0.0_wp
+      ! anelastic memory variables: component c (4..9), mechanism l (1..4)
+      if (D%B(ib)%M%anelastic) then
+         do k = C%ms, C%ps
+            do j = C%mr, C%pr
+               do i = C%mq, C%pq
+                  do f = 1, 4
+                     D%B(ib)%M%eta4(i, j, k, f) = 0.05_wp*pat(i, j, k, 30 + f, ib + 4)
+                     D%B(ib)%M%eta5(i, j, k, f) = 0.05_wp*pat(i, j, k, 34 + f, ib + 4)
+                     D%B(ib)%M%eta6(i, j, k, f) = 0.05_wp*pat(i, j, k, 38 + f, ib + 4)
+                     D%B(ib)%M%eta7(i, j, k, f) = 0.05_wp*pat(i, j, k, 42 + f, ib + 4)
+                     D%B(ib)%M%eta8(i, j, k, f) = 0.05_wp*pat(i, j, k, 46 + f, ib + 4)
+                     D%B(ib)%M%eta9(i, j, k, f) = 0.05_wp*pat(i, j, k, 50 + f, ib + 4)
+                  end do
+               end do
+            end do
+         end do
+         D%B(ib)%M%Deta4 = 0.0_wp; D%B(ib)%M%Deta5 = 0.0_wp; D%B(ib)%M%Deta6 = 0.0_wp
+         D%B(ib)%M%Deta7 = 0.0_wp; D%B(ib)%M%Deta8 = 0.0_wp; D%B(ib)%M%Deta9 = 0.0_wp
+      end if
     end associate
   end subroutine fill_pattern
+
+  !> the anelastic memory variables (name 'eta') or their rates ('Deta') of
+  !> block ib as six dumps of 4 mechanisms each: <name>4 .. <name>9
+  subroutine dump_anelastic(D, ib, prefix, name)
+    type(domain_type), intent(in) :: D
+    integer, intent(in) :: ib
+    character(*), intent(in) :: prefix, name
+    if (.not. D%B(ib)%M%anelastic) return
+    if (name == 'eta') then
+       call dump3(D%B(ib)%M%eta4, D, ib, prefix, 'eta4')
+       call dump3(D%B(ib)%M%eta5, D, ib, prefix, 'eta5')
+       call dump3(D%B(ib)%M%eta6, D, ib, prefix, 'eta6')
+       call dump3(D%B(ib)%M%eta7, D, ib, prefix, 'eta7')
+       call dump3(D%B(ib)%M%eta8, D, ib, prefix, 'eta8')
+       call dump3(D%B(ib)%M%eta9, D, ib, prefix, 'eta9')
+    else
+       call dump3(D%B(ib)%M%Deta4, D, ib, prefix, 'Deta4')
+       call dump3(D%B(ib)%M%Deta5, D, ib, prefix, 'Deta5')
+       call dump3(D%B(ib)%M%Deta6, D, ib, prefix, 'Deta6')
+       call dump3(D%B(ib)%M%Deta7, D, ib, prefix, 'Deta7')
+       call dump3(D%B(ib)%M%Deta8, D, ib, prefix, 'Deta8')
+       call dump3(D%B(ib)%M%Deta9, D, ib, prefix, 'Deta9')
+    end if
+  end subroutine dump_anelastic
 
   subroutine add_pulse(D, ib, x0, y0, z0, sig, amp)
     type(domain_type), intent(inout) :: D

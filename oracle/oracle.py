@@ -37,7 +37,8 @@ class OraBlock(C.Structure):
                 ('X', dp), ('metric', dp), ('M', dp), ('nml', dp * 6), ('F', dp), ('DF', dp),
                 ('Q', dp * 6), ('DQ', dp * 6), ('P', dp),
                 ('nsrc', C.c_int), ('src_order', C.c_int), ('src_alpha', C.c_int), ('src', C.POINTER(OraSource)),
-                ('sbp', C.c_int)]
+                ('sbp', C.c_int), ('anelastic', C.c_int), ('an_tau', C.c_double * 4), ('an_w', C.c_double * 4),
+                ('eta', dp), ('deta', dp), ('qinv', dp)]
 
 
 class SbpHalf(C.Structure):
@@ -94,6 +95,7 @@ def lib():
         _lib.ora_d1.argtypes = [dp, C.c_int, C.c_int, C.c_double]
         _lib.ora_d1.restype = C.c_double
         _lib.ora_sbp_op.argtypes = [C.c_int, C.c_int]
+        _lib.ora_anelastic_init.argtypes = [C.POINTER(OraBlock), C.c_double, C.c_double, C.c_double]
         _lib.ora_sbp_table.argtypes = [C.c_int]
         _lib.ora_sbp_table.restype = C.c_void_p
         _lib.ora_rk_coeffs.argtypes = [dp, dp, dp]
@@ -169,6 +171,13 @@ class OracleDomain:
                     b.DQ[s] = _p(arr['DQ'][s])
             if L.ora_metrics(C.byref(b)) != 0:
                 raise RuntimeError('non-unit normal in face frame')
+            if getattr(prob, 'anelastic', None):
+                # memory variables and Q^-1; the unrelaxed moduli replace M
+                for k, nf in (('eta', 24), ('deta', 24), ('qinv', 2)):
+                    arr[k] = np.zeros((nf, ns, nr, nq))
+                    setattr(b, k, _p(arr[k]))
+                if L.ora_anelastic_init(C.byref(b), *[float(v) for v in prob.anelastic]) != 0:
+                    raise ValueError(f'anelastic weight_exp {prob.anelastic[1]} has no weight set')
             self.blocks.append(arr)
             FPn = nr * ns
             ia = dict(S=np.zeros((4, ns, nr)), DS=np.zeros((4, ns, nr)), W=np.zeros((1, ns, nr)),

@@ -349,6 +349,118 @@ static void pml_point(ora_block *b, double DU[9], const double Ux[9], const doub
 }
 
 /* ------------------------------------------------------------------------ */
+/* anelastic response: GSLS memory variables */
+
+int ora_anelastic_init(ora_block *b, double c, double weight_exp, double fref)
+{
+    static const double w1[4] = {1.6126, 0.6255, 0.6382, 1.5969}; /* weight_exp ~ 0 (material.f90:87-90) */
+    static const double w2[4] = {0.0336, 0.6873, 0.8767, 1.5202}; /* weight_exp ~ 0.6 (101-104) */
+    const double *w;
+    if (weight_exp < 0.01) w = w1;
+    else if (weight_exp > 0.59 && weight_exp < 0.61) w = w2;
+    else return -1;
+    size_t NP = NPTS(b);
+    double *M = b->M, *Qs = b->qinv, *Qp = b->qinv + NP;
+    for (size_t p = 0; p < NP; p++) { /* 64-71 */
+        Qs[p] = 1.0 / (c * sqrt(M[NP + p] / M[2 * NP + p]));
+        Qp[p] = 0.5 * Qs[p];
+    }
+    const double FAC = 1.0;
+    double taumin = 1.0 / (2.0 * PI * 15.0) * 1.0 * FAC, taumax = 1.0 / (2.0 * PI * 0.08) * 200.0 * FAC;
+    for (int k = 1; k <= 4; k++) { /* 82-85 */
+        b->an_tau[k - 1] = exp(log(taumin) + (2.0 * k - 1.0) / 16.0 * (log(taumax) - log(taumin)));
+        b->an_w[k - 1] = w[k - 1];
+    }
+    double wref = 2.0 * PI * fref, facex = 1.0;
+    for (size_t p = 0; p < NP; p++) { /* 108-140: unrelaxed moduli */
+        double val_S = 0.0, val_P = 0.0;
+        for (int l = 0; l < 4; l++) {
+            double den_S = (pow(wref, 2.0) * pow(b->an_tau[l], 2.0) + 1.0) * (1.0 / Qs[p]) * facex;
+            double den_P = (pow(wref, 2.0) * pow(b->an_tau[l], 2.0) + 1.0) * (1.0 / Qp[p]) * facex;
+            val_S = val_S + b->an_w[l] / den_S;
+            val_P = val_P + b->an_w[l] / den_P;
+        }
+        double lam = M[p], mu = M[NP + p], rho = M[2 * NP + p];
+        double vs = sqrt(mu / rho), vp = sqrt((lam + 2.0 * mu) / rho);
+        double mu_S = rho * pow(vs, 2.0) / (1.0 - val_S), mu_P = rho * pow(vp, 2.0) / (1.0 - val_P);
+        vs = sqrt(mu_S / rho);
+        vp = sqrt(mu_P / rho);
+        M[NP + p] = pow(vs, 2.0) * rho;
+        M[p] = pow(vp, 2.0) * rho - 2.0 * M[NP + p];
+    }
+    b->anelastic = 1;
+    return 0;
+}
+
+/* PML damping and auxiliary fields of the slabs holding node (i,j,k), per
+ * direction (pml_damping_and_q, RHS_Interior.f90:2287-2387); returns 0 when
+ * no slab holds it (point_in_pml, 2225-2285) */
+static int pml_dq(const ora_block *b, int i, int j, int k, double d[3], double Q[3][9])
+{
+    const double d0 = 6.0 / (2.0 * 1.0) * log(1.0 / 1.0e-3);
+    int in = 0, np = b->npml, pos[3] = {i + 1, j + 1, k + 1}, n[3] = {b->nq, b->nr, b->ns};
+    for (int a = 0; a < 3; a++) {
+        d[a] = 0.0;
+        for (int f = 0; f < 9; f++) Q[a][f] = 0.0;
+        for (int side = 2 * a; side <= 2 * a + 1; side++) {
+            if (!b->pml[side]) continue;
+            long s = slab_index(b, side, i, j, k);
+            if (s < 0) continue;
+            in = 1;
+            if (side == 4) d[a] = d0 * fabs((double)(pos[a] - np) / (double)np);
+            else if (side % 2 == 0) d[a] = d0 * fabs((double)(pos[a] - (1 + np)) / (double)np);
+            else d[a] = d0 * fabs((double)(pos[a] - (n[a] - np)) / (double)np);
+            size_t SP = ora_slab_points(b, side);
+            for (int f = 0; f < 9; f++) Q[a][f] = b->Q[side][f * SP + s];
+        }
+    }
+    return in;
+}
+
+/* memory-variable terms of one node: out[c] -= sum_l eta(c,l) for the six
+ * stress rates (out[c] at stride `st`), Deta += rates (apply_anelastic_point,
+ * RHS_Interior.f90:2390-2441; in PML slabs the strains net of the auxiliary
+ * fields, apply_anelastic_point_pml 2444-2509; the interior routines' form
+ * JU_xJU_yJU_z6.f90:16317-16353 is the same arithmetic) */
+static void anelastic_point(ora_block *b, int i, int j, int k, const double Ux[9], const double Uy[9],
+                            const double Uz[9], double *out, size_t st, int pml_form)
+{
+    size_t NP = NPTS(b), p = IDX(b, i, j, k);
+    double lam = b->M[p], mu = b->M[NP + p], Qs = b->qinv[p], Qp = b->qinv[NP + p];
+    double e[6];
+    double d[3], Q[3][9];
+    if (pml_form && pml_dq(b, i, j, k, d, Q)) {
+        e[0] = Ux[0] - d[0] * Q[0][3];
+        e[1] = Uy[1] - d[1] * Q[1][4];
+        e[2] = Uz[2] - d[2] * Q[2][5];
+        e[3] = (Uy[0] + Ux[1]) - d[0] * Q[0][6] - d[1] * Q[1][6];
+        e[4] = (Uz[0] + Ux[2]) - d[0] * Q[0][7] - d[2] * Q[2][7];
+        e[5] = (Uz[1] + Uy[2]) - d[1] * Q[1][8] - d[2] * Q[2][8];
+    } else {
+        e[0] = Ux[0]; e[1] = Uy[1]; e[2] = Uz[2];
+        e[3] = Uy[0] + Ux[1]; e[4] = Uz[0] + Ux[2]; e[5] = Uz[1] + Uy[2];
+    }
+    double *eta = b->eta, *deta = b->deta;
+    for (int c = 0; c < 6; c++) {
+        size_t q = (size_t)(4 * c) * NP + p;
+        out[c * st] = out[c * st] - (eta[q] + eta[q + NP] + eta[q + 2 * NP] + eta[q + 3 * NP]);
+    }
+    double tr = e[0] + e[1] + e[2];
+    for (int l = 0; l < 4; l++) {
+        double w = b->an_w[l], tau = b->an_tau[l];
+        for (int c = 0; c < 3; c++) {
+            size_t q = (size_t)(4 * c + l) * NP + p;
+            deta[q] = deta[q] + (((w * 2.0 * mu * Qs) * e[c] + (w * ((lam + 2.0 * mu) * Qp - 2.0 * mu * Qs)) * tr)
+                                 - eta[q]) / tau;
+        }
+        for (int c = 3; c < 6; c++) {
+            size_t q = (size_t)(4 * c + l) * NP + p;
+            deta[q] = deta[q] + (w * mu * Qs * e[c] - eta[q]) / tau;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* upwind and upwind-DRP operators (tables in wql_sbp_ops.h) */
 
 const void *ora_sbp_table(int k) { return k >= 0 && k < WQL_SBP_NOPS ? (const void *)&WQL_SBP_OPS[k] : NULL; }
@@ -496,6 +608,7 @@ static void set_rates_op(ora_block *b)
                     DF[6 * NP + p] = DF[6 * NP + p] + a4 * (DFy[0] + DFx[1]);
                     DF[7 * NP + p] = DF[7 * NP + p] + a4 * (DFz[0] + DFx[2]);
                     DF[8 * NP + p] = DF[8 * NP + p] + a4 * (DFz[1] + DFy[2]);
+                    if (b->anelastic) anelastic_point(b, i, j, k, DFx, DFy, DFz, DF + 3 * NP + p, NP, 0);
                 } else {
                     double Ux[9], Uy[9], Uz[9], DU[9];
                     op_band(b, op, 0, i, j, k, Ux);
@@ -513,6 +626,7 @@ static void set_rates_op(ora_block *b)
                     DU[6] = mu * (Uy[0] + Ux[1]);
                     DU[7] = mu * (Uz[0] + Ux[2]);
                     DU[8] = mu * (Uz[1] + Uy[2]);
+                    if (b->anelastic) anelastic_point(b, i, j, k, Ux, Uy, Uz, DU + 3, 1, 1);
                     pml_point(b, DU, Ux, Uy, Uz, i, j, k, rhoJ_inv, l2m);
                     for (int f = 0; f < 9; f++) DF[f * NP + p] = DF[f * NP + p] + DU[f];
                 }
@@ -591,6 +705,7 @@ void ora_set_rates(ora_block *b)
                     DF[6 * NP + p] = DF[6 * NP + p] + a4 * (DFy[0] + DFx[1]);
                     DF[7 * NP + p] = DF[7 * NP + p] + a4 * (DFz[0] + DFx[2]);
                     DF[8 * NP + p] = DF[8 * NP + p] + a4 * (DFz[1] + DFy[2]);
+                    if (b->anelastic) anelastic_point(b, i, j, k, DFx, DFy, DFz, DF + 3 * NP + p, NP, 0);
                 } else {
                     /* RHS_near_boundaries RHS_Interior.f90:221-268 */
                     double Ux[9], Uy[9], Uz[9], DU[9];
@@ -609,6 +724,7 @@ void ora_set_rates(ora_block *b)
                     DU[6] = mu * (Uy[0] + Ux[1]);
                     DU[7] = mu * (Uz[0] + Ux[2]);
                     DU[8] = mu * (Uz[1] + Uy[2]);
+                    if (b->anelastic) anelastic_point(b, i, j, k, Ux, Uy, Uz, DU + 3, 1, 1);
                     pml_point(b, DU, Ux, Uy, Uz, i, j, k, rhoJ_inv, l2m);
                     for (int f = 0; f < 9; f++) DF[f * NP + p] = DF[f * NP + p] + DU[f];
                 }
@@ -1500,6 +1616,8 @@ int ora_step(ora_domain *d, int nsteps)
                 size_t NP = NPTS(b);
 #pragma omp parallel for schedule(static)
                 for (size_t p = 0; p < 9 * NP; p++) b->DF[p] = a * b->DF[p];
+                if (b->anelastic) /* scale_rates_interior, fields.f90:61-68 */
+                    for (size_t p = 0; p < 24 * NP; p++) b->deta[p] = a * b->deta[p];
                 for (int s = 0; s < 6; s++)
                     if (b->pml[s]) {
                         size_t SP = ora_slab_points(b, s);
@@ -1526,6 +1644,8 @@ int ora_step(ora_domain *d, int nsteps)
                 size_t NP = NPTS(b);
 #pragma omp parallel for schedule(static)
                 for (size_t p = 0; p < 9 * NP; p++) b->F[p] = b->F[p] + bdt * b->DF[p];
+                if (b->anelastic) /* update_fields_interior, fields.f90:122-129 */
+                    for (size_t p = 0; p < 24 * NP; p++) b->eta[p] = b->eta[p] + bdt * b->deta[p];
                 for (int s = 0; s < 6; s++)
                     if (b->pml[s]) {
                         size_t SP = ora_slab_points(b, s);

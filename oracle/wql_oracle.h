@@ -77,6 +77,10 @@ typedef struct {
     const ora_source *src;
     int sbp;            /* spatial operator: -1 the sixth-order central one ('traditional'); else the
                            upwind / upwind-DRP operator WQL_SBP_OPS[sbp] (ora_sbp_op) */
+    int anelastic;      /* 1: GSLS memory variables (response = 'anelastic') */
+    double an_tau[4], an_w[4]; /* relaxation times and weights (set by ora_anelastic_init) */
+    double *eta, *deta; /* [24][NP]: components sxx syy szz sxy sxz syz x mechanisms 1..4 */
+    double *qinv;       /* [2][NP]: Q_s^-1, Q_p^-1 */
 } ora_block;
 
 typedef struct {        /* one block's copy of the interface state, q-face arrays */
@@ -110,6 +114,11 @@ int ora_interface(ora_domain *d, int ib, double t, int stage);
 int ora_step(ora_domain *d, int nsteps);
 /* initial state variable and velocities for rate and state (Interface_Condition.f90:2722-2850) */
 void ora_init_state(ora_domain *d);
+/* init_anelastic_properties (material.f90:11-145): Q^-1 from the block's
+ * material, the relaxation times and weights of the GSLS, and the unrelaxed
+ * lambda and mu written back into b->M.  -1 for a weight_exp outside the two
+ * weight sets the reference defines */
+int ora_anelastic_init(ora_block *b, double c, double weight_exp, double fref);
 /* index of the upwind (fd_type 1) or upwind-DRP (fd_type 2) operator of the
  * given order in WQL_SBP_OPS, -1 for fd_type 0 (block.f90:101-118), -2 for a
  * combination the reference does not define */

@@ -33,7 +33,7 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
              plastic_model='default', pml_l=((False,) * 3,) * 2, pml_r=((False,) * 3,) * 2,
              npml=0, lc=(1.0, 0.0), rc=(0.0, 1.0), profile='analytical_test_problem',
              rho_s_p=(2.67, 3.464, 6.0), t_final=1.0, aqrs=None, bqrs=None, mu_beta_eta=(0.0, 0.0, 0.0),
-             moment=None, fd_type=None, order=None):
+             moment=None, fd_type=None, order=None, anelastic=None):
     T = lambda b: 'T' if b else 'F'
     s = ["&problem_list",
          f" name = 'data/{name}',", f" problem = '{problem}',", f" response = '{response}',",
@@ -66,6 +66,10 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
     else:
         s += ["", "&moment_list", " use_moment_tensor = T,", f" order = {moment['order']},",
               f" tensor_block_preference = {moment.get('preference', 1)} /"]
+    if anelastic is not None:
+        # &anelastic_list (material.f90:30-39): c (Q_s = c v_s), weight_exp, fref
+        s += ["", "&anelastic_list", f" c = {anelastic[0]}d0,", f" weight_exp = {anelastic[1]}d0,",
+              f" fref = {anelastic[2]}d0 /"]
     s += ["", "&mms_list", " use_mms = F /", "",
           "&output_list", " output_seismograms = F,", " output_fault_topo = F /", ""]
     if moment is not None:
@@ -174,6 +178,27 @@ CASES = {
     'drp679_curv': (dict(problem='TPV5', coupling='slip-weakening_friction', mesh='curvilinear', n=(19, 18, 17),
                          lq=LQ_STD, rq=RQ_STD, fd_type='upwind_drp', order=679),
                     [('rhs', ['0.3'])]),
+    # anelastic response (init_anelastic_properties, material.f90:11-145; memory
+    # variables in the interior and band routines, RHS_Interior.f90:2390-2532):
+    # the sixth-order operator on the curvilinear slip-weakening pair (weight
+    # set of weight_exp = 0.6), with PML slabs (the slab form of the strains,
+    # weight set of weight_exp = 0), and the DRP-6 operator with PML as in the
+    # reference's own PML-12 runs
+    'an_curv': (dict(problem='TPV5', coupling='slip-weakening_friction', mesh='curvilinear', n=N_STD,
+                     lq=LQ_STD, rq=RQ_STD, response='anelastic', anelastic=(2.0, 0.6, 1.0)),
+                [('rhs', ['0.3']), ('traj', ['10'])]),
+    'an_pml': (dict(problem='TPV5', coupling='locked', mesh='cartesian', n=(16, 15, 14),
+                    lq=[(1, 1, 1), (0, 1, 1)], rq=[(0, 1, 1), (1, 2, 1)],
+                    pml_l=((True, True, False), (False, False, True)),
+                    pml_r=((False, False, True), (True, False, False)), npml=7,
+                    response='anelastic', anelastic=(1.5, 0.0, 1.0)),
+               [('rhs', ['0.1']), ('traj', ['8', '-1.2', '7.4', '0.2', '0.5', '1.0'])]),
+    'an_drp6_pml': (dict(problem='TPV5', coupling='locked', mesh='cartesian', n=(18, 17, 17),
+                         lq=[(1, 1, 1), (0, 1, 1)], rq=[(0, 1, 1), (1, 2, 1)],
+                         pml_l=((True, True, False), (False, False, True)),
+                         pml_r=((False, False, True), (True, False, False)), npml=7,
+                         fd_type='upwind_drp', order=6, response='anelastic', anelastic=(2.0, 0.6, 1.0)),
+                    [('rhs', ['0.1']), ('traj', ['8', '-1.2', '7.4', '0.2', '0.5', '1.0'])]),
 }
 
 
@@ -200,6 +225,13 @@ def run_case(name, kw, runs, work):
         vol = lambda b, nm, nf: read_block(prefix, b, nm, (nq, nr, ns, nf)).transpose(3, 2, 1, 0)
         face = lambda b, nm, nf: read_block(prefix, b, nm, (nr, ns, nf)).transpose(2, 1, 0)
         for b in (1, 2):
+            if os.path.exists(f'{prefix}_b{b}_Mat.bin'):
+                out[f'b{b}_Mat'] = vol(b, 'Mat', 3)
+                out[f'b{b}_Qsinv'] = vol(b, 'Qsinv', 1)
+            for tag in ('Deta', 'eta'):
+                if os.path.exists(f'{prefix}_b{b}_{tag}4.bin'):
+                    # [component 4..9][mechanism 1..4] -> 24 fields
+                    out[f'b{b}_{tag}'] = np.concatenate([vol(b, f'{tag}{c}', 4) for c in range(4, 10)])
             if mode == 'rhs':
                 out[f'rhs_t'] = np.array(float(args[0]))
                 out[f'b{b}_DFvol'] = vol(b, 'DFvol', 9)
@@ -240,6 +272,19 @@ def run_case(name, kw, runs, work):
                         a = np.fromfile(f'{prefix}_b{b}_nml{side}.bin', '<f8')
                         nml.append(a)
                     out[f'b{b}_nml'] = np.concatenate(nml)
+    if kw.get('response') == 'anelastic':
+        # the physics under test is pointwise: keep the rates with all terms
+        # (DFfull), the memory variables and the state; drop what the other
+        # cases already pin (volume-only rates, the PML auxiliary arrays), and
+        # store a uniform material or Q^-1 as one value
+        for k in list(out):
+            if k.endswith('_DFvol') or '_DQvol' in k or '_DQfull' in k or '_Q' in k and not k.endswith('Qsinv'):
+                del out[k]
+            elif k.endswith('_Mat') or k.endswith('_Qsinv'):
+                a = out[k]
+                flat = a.reshape(a.shape[0], -1)
+                if (flat == flat[:, :1]).all():
+                    out[k] = flat[:, 0].copy()
     path = os.path.join(HERE, f'{name}.npz')
     np.savez_compressed(path, **out)
     with open(os.path.join(HERE, 'inputs', f'{name}.in'), 'w') as f:

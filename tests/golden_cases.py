@@ -13,10 +13,13 @@ from wql import problem as P  # noqa: E402
 # the upwind and upwind-DRP operators (fd_type / order)
 SBP_RHS_CASES = ['up2_cart', 'up4_curv', 'up6_pml', 'up9_curv', 'drp3_curv', 'drp66_cart', 'drp679_curv']
 SBP_TRAJ_CASES = ['up4_curv', 'up6_pml', 'drp66_cart']
+# the anelastic response (GSLS memory variables); their fixtures keep the
+# rates with every term (DFfull) and the memory variables, not DFvol
+AN_CASES = ['an_curv', 'an_pml', 'an_drp6_pml']
 RHS_CASES = ['cart_sw', 'curv_sw', 'curv_rs', 'pml_locked', 'rw_plastic', 'tpv26_sw', 'mt_cart',
-             'mt_curv'] + SBP_RHS_CASES
+             'mt_curv'] + SBP_RHS_CASES + AN_CASES
 TRAJ_CASES = ['cart_sw', 'curv_sw', 'curv_rs', 'curv_rs_long', 'pml_locked', 'plastic_sw', 'locked_pulse',
-              'rw_plastic', 'tpv26_sw', 'tpv26_plastic', 'mt_cart', 'mt_curv'] + SBP_TRAJ_CASES
+              'rw_plastic', 'tpv26_sw', 'tpv26_plastic', 'mt_cart', 'mt_curv'] + SBP_TRAJ_CASES + AN_CASES
 
 # Relative-L2 bar for FP64 trajectories.  'curv_rs' stops before rupture:
 # slip rates there are ~1e-12 m/s, equal to the absolute step tolerance of
@@ -80,7 +83,11 @@ def pattern_state(prob):
         idx = 1000 * jj.astype(np.int64) + 1000000 * kk
         h = (idx * 7919 + 104729 * 8) % 1000003
         W = (0.6 + 0.05 * (2.0 * (h / 1000003.0) - 1.0))[None]
-        out.append(dict(F=F, Q=Q, S=np.stack(S), W=W))
+        eta = None
+        if getattr(prob, 'anelastic', None):
+            # memory variables: component c (sxx..syz), mechanism l (golden_driver fill_pattern)
+            eta = np.stack([pattern(b.n, 31 + f, ib + 5, 0.05) for f in range(24)])
+        out.append(dict(F=F, Q=Q, S=np.stack(S), W=W, eta=eta))
     return out
 
 

@@ -15,7 +15,11 @@ def load_pattern_state(sim, prob, od=None):
             if st[ib]['Q'][s] is not None:
                 sim.set_pml(ib, s, st[ib]['Q'][s])
         sim.set_fault(ib, st[ib]['S'], st[ib]['W'])
+        if st[ib]['eta'] is not None:
+            sim.set_anelastic(ib, st[ib]['eta'])
         if od is not None:
+            if st[ib]['eta'] is not None:
+                od.blocks[ib]['eta'][...] = st[ib]['eta']
             od.blocks[ib]['F'][...] = st[ib]['F']
             for s in range(6):
                 if st[ib]['Q'][s] is not None:
@@ -44,6 +48,7 @@ def oracle_rhs(prob, t):
 
 
 class _NullSim:
+    def set_anelastic(self, *a): pass
     def set_fields(self, *a): pass
     def set_pml(self, *a): pass
     def set_fault(self, *a, **k): pass

@@ -30,7 +30,7 @@ def _nvariants():
     return native.lib().wql_tile_variants()
 
 
-@pytest.mark.parametrize('case', [c for c in G.RHS_CASES if c not in G.SBP_RHS_CASES])
+@pytest.mark.parametrize('case', [c for c in G.RHS_CASES if c not in G.SBP_RHS_CASES + G.AN_CASES])
 @pytest.mark.parametrize('order', [0, 1])
 @pytest.mark.parametrize('tile', list(range(7)))
 def test_rhs_vs_golden(case, tile, order):
@@ -123,6 +123,9 @@ def test_trajectory_vs_golden(case, tile):
                     # reference's Q(i,j,k,f) in column-major order
                     got = sim.pml(ib, s).reshape(9, -1)
                     assert G.rel_l2(got, z[key].reshape(9, -1)) <= TOL, key
+            if f'b{ib + 1}_eta' in z:
+                # anelastic memory variables after the run
+                assert G.rel_l2(sim.anelastic(ib), z[f'b{ib + 1}_eta']) <= tol, f'eta block {ib + 1}'
 
 
 def test_metrics_and_frames_vs_golden():

@@ -33,8 +33,17 @@ def test_rhs(case):
     prob = G.problem(case)
     od, vol, dqv = H.oracle_rhs(prob, float(z['rhs_t']))
     for ib in range(2):
-        assert G.rel_l2(vol[ib], z[f'b{ib + 1}_DFvol']) <= 1e-15
+        if f'b{ib + 1}_DFvol' in z:
+            assert G.rel_l2(vol[ib], z[f'b{ib + 1}_DFvol']) <= 1e-15
         assert G.rel_l2(od.blocks[ib]['DF'], z[f'b{ib + 1}_DFfull']) <= 1e-15
+        if f'b{ib + 1}_Deta' in z:
+            # anelastic: memory-variable rates, unrelaxed moduli and Q_s^-1
+            # of init_anelastic_properties (uniform here: stored as one value)
+            assert G.rel_l2(od.blocks[ib]['deta'], z[f'b{ib + 1}_Deta']) <= 1e-15
+            M = od.blocks[ib]['M'].reshape(3, -1)
+            assert np.abs(M - z[f'b{ib + 1}_Mat'][:, None]).max() <= 1e-14 * np.abs(M).max()
+            Qs = od.blocks[ib]['qinv'][0].ravel()
+            assert np.abs(Qs - z[f'b{ib + 1}_Qsinv'][0]).max() <= 1e-16
         ia = od.blocks[ib]['iface']
         assert np.abs(ia['DS'] - z[f'b{ib + 1}_DS']).max() <= 1e-13
         assert np.abs(ia['DW'] - z[f'b{ib + 1}_DW']).max() <= 1e-13
@@ -70,6 +79,8 @@ def test_trajectory(case):
         for s in range(6):
             if f'b{ib + 1}_Q{s + 1}' in z:
                 assert G.rel_l2(a['Q'][s].ravel(), z[f'b{ib + 1}_Q{s + 1}']) <= 1e-12
+        if f'b{ib + 1}_eta' in z:
+            assert G.rel_l2(a['eta'], z[f'b{ib + 1}_eta']) <= 1e-12
 
 
 @pytest.mark.parametrize('case', ['mt_cart', 'mt_curv'])

@@ -119,6 +119,8 @@ struct DevBlock {
     void *M = nullptr;      // [3][NP] real (heterogeneous)
     void *Q[6] = {}, *DQ[6] = {};
     void *pq[6] = {};       // [9][face nodes] real: PML face penalties of the stage
+    void *eta = nullptr, *deta = nullptr;  // [24][NP] real: anelastic memory variables and rates
+    void *qinv = nullptr;   // [2][NP] real: Q_s^-1, Q_p^-1
     long SP[6] = {};
     double *frame[6] = {}, *fgm[6] = {}, *fmat[6] = {};
     double *fx = nullptr;
@@ -172,6 +174,8 @@ struct wql_ctx {
     int sbp = -1;
     double tau0 = 0.0;
     wql_sbp_half *d_sbp = nullptr;
+    // anelastic response: relaxation times and weights of the four mechanisms
+    double an_tau[4] = {0, 0, 0, 0}, an_w[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -214,6 +218,10 @@ Blk<real> blk_view(const wql_ctx *c, int ib, int cur)
     B.nbnd = d.nbnd;
     B.sbp = c->sbp >= 0 ? c->d_sbp : nullptr;
     B.sbw = c->sbp >= 0 ? WQL_SBP_OPS[c->sbp].band : 0;
+    B.eta = (real *)d.eta;
+    B.deta = (real *)d.deta;
+    B.qinv = (const real *)d.qinv;
+    for (int l = 0; l < 4; l++) { B.an_tau[l] = c->an_tau[l]; B.an_w[l] = c->an_w[l]; }
     return B;
 }
 
@@ -348,8 +356,12 @@ void launch_pml(wql_ctx *c, Pair<real> &P, real A, real bdt)
         k_volume_generic<real><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(P, BL, A, A, bdt);
 }
 
-// the upwind / upwind-DRP operators: every node of both blocks, one thread
-// each (replaces the interior, s-closure and PML-slab kernels)
+// problems on the per-node path: the upwind / upwind-DRP operators and the
+// anelastic response (neither is in the tiled kernels)
+bool per_node(const wql_ctx *c) { return c->sbp >= 0 || c->prob.anelastic; }
+
+// the per-node path: every node of both blocks, one thread each (replaces
+// the interior, s-closure and PML-slab kernels)
 template <typename real>
 void launch_op(wql_ctx *c, Pair<real> &P, real A, real bdt)
 {
@@ -541,7 +553,7 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
     }
     // mixed grids: a Cartesian block inside a curvilinear pair still has no G
     TileList TL = tile_list(c);
-    if (c->sbp >= 0) {
+    if (per_node(c)) {
         launch_op<real>(c, P, (real)A, (real)bdt);
     } else if (TL.total > 0) {
         if (c->profile && !c->use_graph) hipEventRecord(c->ev0, c->stream);
@@ -555,7 +567,7 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
             c->vol_launches++;
         }
     }
-    if (c->sbp < 0) {
+    if (!per_node(c)) {
         launch_band<real>(c, P, (real)A, (real)bdt);
         launch_pml<real>(c, P, (real)A, (real)bdt);
     }
@@ -645,6 +657,8 @@ void reset_typed(wql_ctx *c)
         fill(d.F[0], 9 * d.NP, (real)0);
         fill(d.F[1], 9 * d.NP, (real)0);
         fill(d.DF, 9 * d.NP, (real)0);
+        fill(d.eta, 24 * d.NP, (real)0);
+        fill(d.deta, 24 * d.NP, (real)0);
         for (int s = 0; s < 6; s++) {
             fill(d.Q[s], 9 * d.SP[s], (real)0);
             fill(d.DQ[s], 9 * d.SP[s], (real)0);
@@ -674,6 +688,10 @@ int ctx_reset(wql_ctx *c, const wql_problem *prob)
     if (prob->precision != c->prec || prob->plastic != c->prob.plastic || prob->s0_kind != c->prob.s0_kind)
         return fail(WQL_E_ARG, "a context is reused only for the same precision and plasticity model");
     if (sbp_index(prob) != c->sbp) return fail(WQL_E_ARG, "a context is reused only for the same spatial operator");
+    if (prob->anelastic != c->prob.anelastic ||
+        (prob->anelastic && (prob->an_c != c->prob.an_c || prob->an_weight_exp != c->prob.an_weight_exp ||
+                             prob->an_fref != c->prob.an_fref)))
+        return fail(WQL_E_ARG, "a context is reused only for the same anelastic response (its moduli are set at create)");
     drop_graphs(c);
     c->prob = *prob;
     memcpy(&c->fault, &prob->fault, sizeof(FaultParams));
@@ -807,6 +825,29 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
         }
         k_metrics<<<g, 256, 0, c->stream>>>(d.nq, d.nr, d.ns, X, G64);
         if (int rc = check_launch()) return rc;
+        if (c->prob.anelastic) {
+            // init_anelastic_properties (material.f90:11-145): Q^-1 from the
+            // material, then the unrelaxed lambda and mu replace it before the
+            // face frames, the faces' material and the volume kernels read it
+            double *q64 = nullptr;
+            HIPCHK(hipMalloc(&q64, 2 * ND * sizeof(double)));
+            const double *t = c->an_tau, *w = c->an_w;
+            k_anelastic_init<<<g, 256, 0, c->stream>>>(ND, M64, q64, c->prob.an_c, 2.0 * 3.141592653589793 * c->prob.an_fref,
+                                                       t[0], t[1], t[2], t[3], w[0], w[1], w[2], w[3]);
+            if (int rc = dalloc(c, &d.qinv, 2 * NP * sizeof(real))) return rc;
+            k_pad<real><<<(unsigned)((2 * NP + 255) / 256), 256, 0, c->stream>>>(2, d.nq, d.nr, d.ns, d.px, q64, (real *)d.qinv);
+            for (void **a : {&d.eta, &d.deta}) {
+                if (int rc = dalloc(c, a, 24 * NP * sizeof(real))) return rc;
+                k_fill<real><<<(unsigned)((24 * NP + 255) / 256), 256, 0, c->stream>>>(24 * NP, (real *)*a, (real)0);
+            }
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (!pair_het) {  // uniform material stays uniform: its unrelaxed moduli
+                HIPCHK(hipMemcpy(&d.lam, M64, sizeof(double), hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(&d.mu, M64 + ND, sizeof(double), hipMemcpyDeviceToHost));
+            }
+            hipFree(q64);
+            if (int rc = check_launch()) return rc;
+        }
         for (int side = 0; side < 6; side++) {
             int dir = side / 2;
             long FP = dir == 0 ? (long)d.nr * d.ns : dir == 1 ? (long)d.nq * d.ns : (long)d.nq * d.nr;
@@ -1038,6 +1079,14 @@ int valid_problem(const wql_problem *prob)
     if (sbp_index(prob) < -1)
         return fail(WQL_E_ARG, "no operator for fd_type %d order %d (upwind: 2-9; upwind_drp: 3-7, 66, 679)",
                     prob->fd_type, prob->fd_order);
+    if (prob->anelastic) {
+        if (prob->plastic) return fail(WQL_E_ARG, "a response is either plastic or anelastic");
+        if (!(prob->an_c > 0) || !(prob->an_fref > 0))
+            return fail(WQL_E_ARG, "anelastic c and fref must be positive");
+        if (!(prob->an_weight_exp < 0.01 || (prob->an_weight_exp > 0.59 && prob->an_weight_exp < 0.61)))
+            return fail(WQL_E_ARG, "anelastic weight_exp %g has no weight set (the reference defines < 0.01 "
+                        "and 0.6, material.f90:80-106)", prob->an_weight_exp);
+    }
     if (!(prob->dt > 0)) return fail(WQL_E_ARG, "dt must be positive");
     const wql_fault_params &f = prob->fault;
     if (f.law < 0 || f.law > 2 || f.npatch < 0 || f.npatch > 4 || f.stress_mode < 0 || f.stress_mode > 2 ||
@@ -1087,6 +1136,7 @@ size_t wql_estimate_bytes(const wql_problem *prob, const wql_block_desc blocks[2
         if (!blocks[0].cartesian || !blocks[1].cartesian) v += 10 * NP * r;  // metrics + J (pair curvilinear)
         if (blocks[0].M || blocks[1].M) v += 3 * NP * r;       // lambda mu rho (pair heterogeneous)
         if (prob && prob->plastic) v += 3 * NP * 8;            // strain accumulators, y
+        if (prob && prob->anelastic) v += 50 * NP * r;         // memory variables, rates, Q^-1
         size_t init = 16 * ND * 8;                             // dense X, M64, G64 during wql_create
         for (int s = 0; s < 6; s++)
             if (b.pml[s]) {
@@ -1153,6 +1203,16 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
     c->t = prob->t0;
     c->sbp = sbp;
     c->tau0 = sbp >= 0 ? WQL_SBP_OPS[sbp].tau0 : TAU0;
+    if (prob->anelastic) {
+        // relaxation times and weights of the four mechanisms (material.f90:80-106)
+        static const double w1[4] = {1.6126, 0.6255, 0.6382, 1.5969}, w2[4] = {0.0336, 0.6873, 0.8767, 1.5202};
+        const double pi = 3.141592653589793, FAC = 1.0;
+        const double taumin = 1.0 / (2.0 * pi * 15.0) * 1.0 * FAC, taumax = 1.0 / (2.0 * pi * 0.08) * 200.0 * FAC;
+        for (int k = 1; k <= 4; k++) {
+            c->an_tau[k - 1] = std::exp(std::log(taumin) + (2.0 * k - 1.0) / 16.0 * (std::log(taumax) - std::log(taumin)));
+            c->an_w[k - 1] = (prob->an_weight_exp < 0.01 ? w1 : w2)[k - 1];
+        }
+    }
     static_assert(sizeof(FaultParams) == sizeof(wql_fault_params), "fault layout");
     memcpy(&c->fault, &prob->fault, sizeof(FaultParams));
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1292,6 +1352,20 @@ int wql_get_rates(wql_ctx *c, int ib, double *DF)
 {
     if (!c || ib < 0 || ib > 1 || !DF) return fail(WQL_E_ARG, "bad argument");
     return copy_vol(c, c->b[ib], c->b[ib].DF, DF, 9);
+}
+
+int wql_set_anelastic(wql_ctx *c, int ib, const double *eta)
+{
+    if (!c || ib < 0 || ib > 1 || !eta) return fail(WQL_E_ARG, "bad argument");
+    if (!c->b[ib].eta) return fail(WQL_E_ARG, "the problem has no anelastic response");
+    return put_vol(c, c->b[ib], c->b[ib].eta, eta, 24);
+}
+
+int wql_get_anelastic(wql_ctx *c, int ib, int what, double *out)
+{
+    if (!c || ib < 0 || ib > 1 || !out || what < 0 || what > 1) return fail(WQL_E_ARG, "bad argument");
+    if (!c->b[ib].eta) return fail(WQL_E_ARG, "the problem has no anelastic response");
+    return copy_vol(c, c->b[ib], what == 0 ? c->b[ib].eta : c->b[ib].deta, out, 24);
 }
 
 int wql_set_pml(wql_ctx *c, int ib, int side, const double *Q)
@@ -1689,6 +1763,10 @@ int wql_problem_keys(const char *problem, const char *coupling, const char *resp
         if (p.plastic == 2 && mu_beta_eta) {  // plastic_material.f90:38-44
             p.pl_mu = mu_beta_eta[0]; p.pl_beta = mu_beta_eta[1]; p.pl_eta = mu_beta_eta[2];
         }
+    } else if (rk == "anelastic") {
+        // the &anelastic_list defaults (material.f90:32-34); the host sets
+        // the list's values
+        p.anelastic = 1; p.an_c = 1.0; p.an_weight_exp = 0.0; p.an_fref = 1.0;
     } else if (rk != "elastic") {
         return fail(WQL_E_ARG, "unsupported response '%s'", rk.c_str());
     }
@@ -1788,7 +1866,7 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
         }
         TileList TL = tile_list(c);
         if (c->prec == WQL_FP64) {
-            if (c->sbp >= 0) {
+            if (per_node(c)) {
                 launch_op<double>(c, Pd, 0.0, 0.0);
             } else {
                 if (TL.total > 0) launch_main<double>(c, Pd, TL, 0.0, 0.0, 0.0);
@@ -1797,7 +1875,7 @@ int wql_eval_rates(wql_ctx *c, double t, int what, int stage)
             }
             launch_sources<double>(c, Pd, t, nullptr, 0.0, 0.0);
         } else {
-            if (c->sbp >= 0) {
+            if (per_node(c)) {
                 launch_op<float>(c, Pf, 0.f, 0.f);
             } else {
                 if (TL.total > 0) launch_main<float>(c, Pf, TL, 0.f, 0.f, 0.f);

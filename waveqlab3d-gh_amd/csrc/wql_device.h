@@ -99,6 +99,12 @@ struct Blk {
     // then .s[0..2]) and its band width; nullptr: the sixth-order central one
     const wql_sbp_half *sbp;
     int sbw;
+    // anelastic response: memory variables and their rates [24][NP]
+    // (component c of sxx..syz, mechanism l at field 4c + l), Q_s^-1 and
+    // Q_p^-1 [2][NP], relaxation times and weights; eta == nullptr: elastic
+    real *eta, *deta;
+    const real *qinv;
+    double an_tau[4], an_w[4];
 };
 
 // index of node (i, j, k) in the boundary-node enumeration (q faces, then r

@@ -257,9 +257,66 @@ __device__ void node_tail(const Blk<real> &B, int i, int j, int k, long p, real 
     DU[6] = mu * (Uy[0] + Ux[1]);
     DU[7] = mu * (Uz[0] + Ux[2]);
     DU[8] = mu * (Uz[1] + Uy[2]);
+    const double d0 = 6.0 / (2.0 * 1.0) * log(1.0 / 1.0e-3);
+    if (B.eta) {
+        // anelastic memory variables (apply_anelastic_point, RHS_Interior.f90:
+        // 2390-2441): DU -= sum over mechanisms, Deta = A Deta + rate, and the
+        // stage update eta += B dt Deta in place (a node's memory variables are
+        // read by that node only).  In PML slabs the strains are taken net of
+        // the auxiliary fields (apply_anelastic_point_pml, 2444-2509,
+        // pml_damping_and_q 2287-2387); off the slabs d = 0 and Q = 0 leave
+        // them as they are.
+        real dd[3] = {0, 0, 0}, qx[3] = {0, 0, 0}, qy[3] = {0, 0, 0}, qz[3] = {0, 0, 0};
+        for (int side = 0; side < 6; side++) {
+            if (!B.pml[side]) continue;
+            const int dir = side / 2, np = B.npml, n = nn[dir];
+            const int p1 = pos[dir] + 1;
+            if (side % 2 == 0 ? p1 > np : p1 < n - np + 1) continue;
+            int lo[3] = {0, 0, 0}, hi[3] = {B.nq - 1, B.nr - 1, B.ns - 1};
+            if (side % 2 == 0) hi[dir] = np - 1; else lo[dir] = n - np;
+            const long n0 = hi[0] - lo[0] + 1, n1 = hi[1] - lo[1] + 1, n2 = hi[2] - lo[2] + 1;
+            const long s = (long)(pos[0] - lo[0]) + n0 * ((long)(pos[1] - lo[1]) + n1 * (long)(pos[2] - lo[2]));
+            const long SP = n0 * n1 * n2;
+            double dv;
+            if (side == 4) dv = d0 * fabs((double)(p1 - np) / (double)np);
+            else if (side % 2 == 0) dv = d0 * fabs((double)(p1 - (1 + np)) / (double)np);
+            else dv = d0 * fabs((double)(p1 - (n - np)) / (double)np);
+            dd[dir] = (real)dv;
+            const real *Qs = B.Q[side];
+            if (dir == 0) { qx[0] = Qs[3 * SP + s]; qx[1] = Qs[6 * SP + s]; qx[2] = Qs[7 * SP + s]; }
+            else if (dir == 1) { qy[0] = Qs[4 * SP + s]; qy[1] = Qs[6 * SP + s]; qy[2] = Qs[8 * SP + s]; }
+            else { qz[0] = Qs[5 * SP + s]; qz[1] = Qs[7 * SP + s]; qz[2] = Qs[8 * SP + s]; }
+        }
+        real e[6];
+        e[0] = Ux[0] - dd[0] * qx[0];
+        e[1] = Uy[1] - dd[1] * qy[0];
+        e[2] = Uz[2] - dd[2] * qz[0];
+        e[3] = (Uy[0] + Ux[1]) - dd[0] * qx[1] - dd[1] * qy[1];
+        e[4] = (Uz[0] + Ux[2]) - dd[0] * qx[2] - dd[2] * qz[1];
+        e[5] = (Uz[1] + Uy[2]) - dd[1] * qy[2] - dd[2] * qz[2];
+        const real Qsi = B.qinv[p], Qpi = B.qinv[NP + p];
+        const real tr = e[0] + e[1] + e[2];
+        real et[24];
+        for (int q = 0; q < 24; q++) et[q] = B.eta[q * NP + p];
+        for (int c = 0; c < 6; c++) DU[3 + c] = DU[3 + c] - (et[4 * c] + et[4 * c + 1] + et[4 * c + 2] + et[4 * c + 3]);
+        for (int l = 0; l < 4; l++) {
+            const real w = (real)B.an_w[l], tau = (real)B.an_tau[l];
+            for (int c = 0; c < 6; c++) {
+                const int q = 4 * c + l;
+                real rate;
+                if (c < 3)
+                    rate = (((w * (real)2 * mu * Qsi) * e[c] + (w * ((lam + (real)2 * mu) * Qpi - (real)2 * mu * Qsi)) * tr)
+                            - et[q]) / tau;
+                else
+                    rate = (w * mu * Qsi * e[c] - et[q]) / tau;
+                const real de = A * B.deta[q * NP + p] + rate;
+                B.deta[q * NP + p] = de;
+                B.eta[q * NP + p] = et[q] + bdt * de;
+            }
+        }
+    }
     // PML (RHS_Interior.f90:1782-2201), order z-low, z-high, y-low, y-high, x-low, x-high
     const int order[6] = {4, 5, 2, 3, 0, 1};
-    const double d0 = 6.0 / (2.0 * 1.0) * log(1.0 / 1.0e-3);
     const real cfs = (real)1;
     for (int o = 0; o < 6; o++) {
         int side = order[o];
@@ -398,6 +455,12 @@ __device__ __forceinline__ int op_taps(const wql_sbp_half &h, int pos, int n, in
 template <typename real>
 __device__ void volume_node_op(const Blk<real> &B, int i, int j, int k, real A, real bdt)
 {
+    if (!B.sbp) {
+        // the sixth-order central operator at every node (an anelastic block
+        // on the per-node path)
+        volume_node_generic<real>(B, i, j, k, A, bdt);
+        return;
+    }
     const long NP = B.NP;
     const long p = pidx(B.px, B.nr, i, j, k);
     const int nn[3] = {B.nq, B.nr, B.ns};
@@ -518,6 +581,35 @@ __global__ void __launch_bounds__(256) k_volume_op(Pair<real> P, BoxList L, real
     int j = bx.y0 + (int)((r / w) % h);
     int k = bx.z0 + (int)(r / ((long)w * h));
     volume_node_op<real>(P.b[bx.blk], i, j, k, bx.blk == 0 ? A0 : A1, bdt);
+}
+
+// init_anelastic_properties (material.f90:11-145) on the FP64 material M
+// [3][nd] (lambda, mu, rho): Q_s^-1 = 1/(c v_s), Q_p^-1 = Q_s^-1 / 2 into
+// qinv [2][nd], then the unrelaxed lambda and mu written back into M.  tau,
+// w: relaxation times and weights of the four mechanisms; wref = 2 pi fref.
+__global__ void k_anelastic_init(long nd, double *M, double *qinv, double c, double wref, double t0, double t1,
+                                 double t2, double t3, double w0, double w1, double w2, double w3)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nd) return;
+    const double tau[4] = {t0, t1, t2, t3}, w[4] = {w0, w1, w2, w3};
+    const double lam = M[p], mu = M[nd + p], rho = M[2 * nd + p];
+    const double Qs = 1.0 / (c * sqrt(mu / rho)), Qp = 0.5 * Qs;
+    qinv[p] = Qs;
+    qinv[nd + p] = Qp;
+    double val_S = 0.0, val_P = 0.0;
+    for (int l = 0; l < 4; l++) {
+        const double den_S = ((wref * wref) * (tau[l] * tau[l]) + 1.0) * (1.0 / Qs) * 1.0;
+        const double den_P = ((wref * wref) * (tau[l] * tau[l]) + 1.0) * (1.0 / Qp) * 1.0;
+        val_S = val_S + w[l] / den_S;
+        val_P = val_P + w[l] / den_P;
+    }
+    double vs = sqrt(mu / rho), vp = sqrt((lam + 2.0 * mu) / rho);
+    const double mu_S = rho * (vs * vs) / (1.0 - val_S), mu_P = rho * (vp * vp) / (1.0 - val_P);
+    vs = sqrt(mu_S / rho);
+    vp = sqrt(mu_P / rho);
+    M[nd + p] = (vs * vs) * rho;
+    M[p] = (vp * vp) * rho - 2.0 * M[nd + p];
 }
 
 // ----------------------------------------------------------------------------

@@ -103,6 +103,8 @@ uint64_t scenario_key(const wql_scenario *sc)
     const int flags[5] = {sc->prob.precision, sc->prob.plastic, sc->prob.s0_kind, sc->prob.fd_type,
                           sc->prob.fd_type ? sc->prob.fd_order : 0};
     h = fnv(h, flags, sizeof flags);
+    const double an[4] = {(double)sc->prob.anelastic, sc->prob.an_c, sc->prob.an_weight_exp, sc->prob.an_fref};
+    if (sc->prob.anelastic) h = fnv(h, an, sizeof an);
     for (int ib = 0; ib < 2; ib++) {
         const wql_block_desc &b = sc->blocks[ib];
         const int shape[17] = {b.nq, b.nr, b.ns, b.bc[0], b.bc[1], b.bc[2], b.bc[3], b.bc[4], b.bc[5],

@@ -152,6 +152,9 @@ contains
        D%prob%fd_type = WQL_FD_TRADITIONAL
     end select
     D%prob%fd_order = D%p%order
+    ! anelastic response: &anelastic_list (init_anelastic_properties, material.f90:11-145)
+    if (trim(D%p%response) == 'anelastic') &
+         call read_anelastic_list(ifname, D%prob%an_c, D%prob%an_weight_exp, D%prob%an_fref)
     call wql_check(wql_create(D%prob, D%desc, D%ctx), 'wql_create')
     ! Init_Vel_State (domain.f90:526-529)
     call wql_check(wql_init_state(D%ctx), 'wql_init_state')

@@ -54,6 +54,8 @@ module wql_gpu
      type(wql_fault_params) :: fault
      integer(c_int) :: fd_type = WQL_FD_TRADITIONAL  ! &problem_list fd_type
      integer(c_int) :: fd_order = 5                  ! &problem_list order
+     integer(c_int) :: anelastic = 0                 ! response = 'anelastic'
+     real(c_double) :: an_c = 1, an_weight_exp = 0, an_fref = 1  ! &anelastic_list
   end type wql_problem
 
   !> wql_grid_desc (init_grid_cartesian / init_grid_curve, grid.f90:13-144)
@@ -238,6 +240,18 @@ module wql_gpu
        integer(c_int), value :: block
        real(c_double), intent(out) :: P(*)
      end function wql_get_plastic
+     integer(c_int) function wql_set_anelastic(ctx, block, eta) bind(C, name='wql_set_anelastic')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: block
+       real(c_double), intent(in) :: eta(*)
+     end function wql_set_anelastic
+     integer(c_int) function wql_get_anelastic(ctx, block, what, out) bind(C, name='wql_get_anelastic')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: block, what
+       real(c_double), intent(out) :: out(*)
+     end function wql_get_anelastic
      integer(c_int) function wql_get_fault_peak(ctx, block, psr) bind(C, name='wql_get_fault_peak')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -9,7 +9,7 @@ module wql_input
   implicit none
   private
   public :: problem_params, block_temp_parameters, read_input, output_params, read_output_list, read_station_list, &
-            moment_params, read_moment_list, read_tensor_list
+            moment_params, read_moment_list, read_tensor_list, read_anelastic_list
 
   integer, parameter :: wp = c_double
 
@@ -266,6 +266,23 @@ contains
     end do
     close(u)
   end subroutine list_between
+
+  !> &anelastic_list of the file `path` with init_anelastic_properties'
+  !> defaults (material.f90:30-39): Q_s = c v_s, the weight set, fref
+  subroutine read_anelastic_list(path, c, weight_exp, fref)
+    character(*), intent(in) :: path
+    real(kind = wp), intent(out) :: c, weight_exp, fref
+    namelist /anelastic_list/ c, weight_exp, fref
+    integer :: u, stat
+    c = 1.0_wp
+    weight_exp = 0.0_wp
+    fref = 1.0_wp
+    open(newunit = u, file = trim(path), status = 'old', action = 'read', iostat = stat)
+    if (stat /= 0) error stop ':: Cannot open input file'
+    read(u, nml = anelastic_list, iostat = stat)
+    close(u)
+    if (stat > 0) error stop 'error reading namelist anelastic_list'
+  end subroutine read_anelastic_list
 
   !> read &problem_list and &block_list from the file `path`
   subroutine read_input(path, p)

@@ -23,6 +23,7 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_sync', 'wql_time', 'wql_get_fault_peak', 'wql_fault_stats', 'wql_server_create', 'wql_server_destroy',
            'wql_server_submit', 'wql_server_poll', 'wql_server_result', 'wql_server_status', 'wql_server_set_policy', 'wql_set_tile', 'wql_get_tile', 'wql_set_tile_order', 'wql_get_tile_order', 'wql_tile_variants', 'wql_use_graph', 'wql_profile',
            'wql_volume_kernel_ms', 'wql_get_fault_faces', 'wql_get_plastic',
+           'wql_set_anelastic', 'wql_get_anelastic',
            'wql_get_fields_noalloc', 'wql_problem_keys', 'wql_grid_block',
            'wql_stage', 'wql_server_pool_stats', 'wql_server_set_shape', 'wql_find_stations',
            'wql_set_stations', 'wql_get_stations', 'wql_locate_sources', 'wql_set_sources']
@@ -61,7 +62,9 @@ class ProblemDesc(C.Structure):
     _fields_ = [('precision', C.c_int), ('dt', C.c_double), ('t0', C.c_double), ('plastic', C.c_int),
                 ('pl_c', C.c_double), ('pl_nu', C.c_double), ('pl_Tv', C.c_double),
                 ('pl_mu', C.c_double), ('pl_beta', C.c_double), ('pl_eta', C.c_double),
-                ('s0_kind', C.c_int), ('fault', FaultParams), ('fd_type', C.c_int), ('fd_order', C.c_int)]
+                ('s0_kind', C.c_int), ('fault', FaultParams), ('fd_type', C.c_int), ('fd_order', C.c_int),
+                ('anelastic', C.c_int), ('an_c', C.c_double), ('an_weight_exp', C.c_double),
+                ('an_fref', C.c_double)]
 
 
 FD_TYPES = {'traditional': 0, 'upwind': 1, 'upwind_drp': 2}
@@ -97,6 +100,8 @@ def lib():
             'wql_get_fault': ([vp, C.c_int, dp, dp, dp, dp, dp, dp], C.c_int),
             'wql_get_fault_rates': ([vp, C.c_int, dp, dp], C.c_int),
             'wql_get_plastic': ([vp, C.c_int, dp], C.c_int),
+            'wql_set_anelastic': ([vp, C.c_int, dp], C.c_int),
+            'wql_get_anelastic': ([vp, C.c_int, C.c_int, dp], C.c_int),
             'wql_get_fields_noalloc': ([vp, C.c_int, dp], C.c_int),
             'wql_problem_keys': ([C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, dp, C.POINTER(ProblemDesc)],
                                  C.c_int),

@@ -175,6 +175,9 @@ class Problem:
     # 'upwind_drp' (order 3-7, 66, 679); the order is unused for traditional
     fd_type: str = 'traditional'
     order: int = 5
+    # anelastic response (response = 'anelastic', &anelastic_list; material.f90:
+    # 11-145): None, or (c, weight_exp, fref) with Q_s = c v_s, Q_p = Q_s / 2
+    anelastic: Optional[tuple] = None
 
     def dt(self):
         """Global time step (domain.f90:222-235, block.f90:437-452)."""
@@ -325,6 +328,9 @@ def from_namelist(path):
                    plastic=plastic, name=pl.get('name', 'default'), w_stride=int(pl.get('w_stride', 1)),
                    w_fault=bool(pl.get('w_fault', False)), moment=read_moment(text),
                    fd_type=pl.get('fd_type', 'traditional'), order=int(pl.get('order', 5)))
+    if response == 'anelastic':
+        al = read_namelist(text, 'anelastic_list')
+        prob.anelastic = (float(al.get('c', 1.0)), float(al.get('weight_exp', 0.0)), float(al.get('fref', 1.0)))
     if plastic == 2:
         # per-block mu_beta_eta (plastic_material.f90:38-44); one set for both blocks here
         mbe = [tuple(float(v) for v in bl.get(f'btp({i})%mu_beta_eta', (0.0, 0.0, 0.0))) for i in (1, 2)]

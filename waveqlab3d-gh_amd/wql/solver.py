@@ -46,6 +46,9 @@ def problem_desc(prob, precision='fp64', dt=None):
     d.fault = fault_struct(prob.fault)
     d.fd_type = native.FD_TYPES[prob.fd_type]
     d.fd_order = prob.order
+    if prob.anelastic:
+        d.anelastic = 1
+        d.an_c, d.an_weight_exp, d.an_fref = (float(v) for v in prob.anelastic)
     return d
 
 
@@ -260,6 +263,19 @@ class Simulation:
         P = np.empty(self._shape(ib, 2))
         check(native.lib().wql_get_plastic(self.ctx, ib, _p(P)))
         return P
+
+    def set_anelastic(self, ib, eta):
+        """anelastic memory variables [24][ns][nr][nq] (component sxx..syz
+        times mechanism 1..4)"""
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        assert eta.shape == self._shape(ib, 24)
+        check(native.lib().wql_set_anelastic(self.ctx, ib, _p(eta)))
+
+    def anelastic(self, ib, what=0):
+        """memory variables (what 0) or their rates (what 1), [24][ns][nr][nq]"""
+        a = np.empty(self._shape(ib, 24))
+        check(native.lib().wql_get_anelastic(self.ctx, ib, what, _p(a)))
+        return a
 
     def fault_rates(self, ib):
         nq, nr, ns = self.prob.blocks[ib].n
