@@ -25,8 +25,16 @@ VARIANTS = [('fused', 32, 8), ('hold', 32, 8), ('fused', 64, 8), ('hold', 64, 8)
             ('hold', 32, 16), ('split', 32, 8)]
 
 
+def per_node(prob):
+    """the upwind / upwind-DRP operators and the anelastic response run on the
+    per-node volume kernel"""
+    return prob.fd_type != 'traditional' or prob.anelastic is not None
+
+
 def kernel_name(prob, precision, variant):
     """template name of the main volume kernel a launch variant selects"""
+    if per_node(prob):
+        return f'k_volume_op<{"float" if precision == "fp32" else "double"}>'
     fam, tx, ty = VARIANTS[variant]
     curv = 'true' if any(b.mesh != 'cartesian' for b in prob.blocks) else 'false'
     het = 'true' if any(b.material is not None for b in prob.blocks) else 'false'
@@ -58,6 +66,8 @@ def measured_traffic(workload, shape, kname):
 
 def main_kernel_nodes(prob):
     """nodes handled by the main volume kernel per launch (both blocks)"""
+    if per_node(prob):
+        return sum(b.n[0] * b.n[1] * b.n[2] for b in prob.blocks)
     tot = 0
     for b in prob.blocks:
         nq, nr, ns = b.n
@@ -74,8 +84,12 @@ def bytes_per_node(prob, precision):
     """algorithmic HBM bytes per node and RK stage of the volume update,
     averaged over the five stages of a step: F in 9 R, DF 9 R + 9 W, F out
     9 W, except that the first stage (A = 0) reads no DF; + 10 (metrics + J)
-    if curvilinear; + 3 (lambda, mu, rho) if heterogeneous"""
+    if curvilinear; + 3 (lambda, mu, rho) if heterogeneous; with the
+    anelastic response + 98 (memory variables 24 R + 24 W, their rates 24 R
+    + 24 W, Q^-1 2 R), less the rates' read at the first stage"""
     vals = 36 - 9 / 5
+    if prob.anelastic is not None:
+        vals += 98 - 24 / 5
     if any(b.mesh != 'cartesian' for b in prob.blocks):
         vals += 10
     if any(b.material is not None for b in prob.blocks):
@@ -244,6 +258,10 @@ def main():
     ap.add_argument('--zchunk', type=int, default=None)
     ap.add_argument('--order', default=None, help='tile dispatch order:group, e.g. 2:2')
     ap.add_argument('--graph', type=int, default=0)
+    ap.add_argument('--operator', default=None,
+                    help="spatial operator 'upwind:<order>' or 'upwind_drp:<order>' (default: the sixth-order "
+                         "central one)")
+    ap.add_argument('--anelastic', default=None, help="anelastic response 'c,weight_exp,fref', e.g. 2,0.6,1")
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--err-steps', type=int, default=None,
@@ -266,6 +284,11 @@ def main():
     from wql import workloads
     W = workloads.WORKLOADS[a.workload](scale=a.scale)
     prob = W['problem']
+    if a.operator:
+        prob.fd_type, order = a.operator.split(':')
+        prob.order = int(order)
+    if a.anelastic:
+        prob.anelastic = tuple(float(v) for v in a.anelastic.split(','))
     nodes = workloads.points(prob)
     res = {}
     for prec in (['fp32'] if a.no_fp64 else ['fp64', 'fp32']):
@@ -274,6 +297,10 @@ def main():
         res[prec] = dict(sec=sec, kms=kms, dev=dev, shape=shape, mpts=nodes * 5 * a.steps / sec / 1e6)
     r32 = res['fp32']
     workload = f'{a.workload}: 2 blocks x {"x".join(str(v) for v in prob.blocks[0].n)}'
+    if a.operator:
+        workload += f', {prob.fd_type} order {prob.order}'
+    if a.anelastic:
+        workload += f', anelastic {a.anelastic}'
     kname = kernel_name(prob, 'fp32', r32['shape'][0])
     traffic = measured_traffic(workload, r32['shape'], kname)
     bpn = bytes_per_node(prob, 'fp32')

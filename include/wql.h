@@ -312,8 +312,10 @@ int wql_tile_variants(void);
  * directly (0) */
 int wql_use_graph(wql_ctx *ctx, int on);
 
-/* average device time (ms) of the main volume kernel over the last wql_step
- * call when profiling is on (HIP events on the context's stream) */
+/* average device time (ms) of the main volume kernel (the tiled interior
+ * kernel, or the per-node kernel of the upwind / DRP operators and the
+ * anelastic response) over the last wql_step call when profiling is on (HIP
+ * events on the context's stream) */
 int wql_profile(wql_ctx *ctx, int on);
 double wql_volume_kernel_ms(wql_ctx *ctx);
 

@@ -553,11 +553,10 @@ void launch_stage(wql_ctx *c, int cur, double A, double bdt, double t_stage, int
     }
     // mixed grids: a Cartesian block inside a curvilinear pair still has no G
     TileList TL = tile_list(c);
-    if (per_node(c)) {
-        launch_op<real>(c, P, (real)A, (real)bdt);
-    } else if (TL.total > 0) {
+    if (per_node(c) || TL.total > 0) {
         if (c->profile && !c->use_graph) hipEventRecord(c->ev0, c->stream);
-        launch_main<real>(c, P, TL, (real)A, (real)A, (real)bdt);
+        if (per_node(c)) launch_op<real>(c, P, (real)A, (real)bdt);
+        else launch_main<real>(c, P, TL, (real)A, (real)A, (real)bdt);
         if (c->profile && !c->use_graph) {
             hipEventRecord(c->ev1, c->stream);
             hipEventSynchronize(c->ev1);
