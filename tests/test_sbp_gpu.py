@@ -120,3 +120,17 @@ def test_fp32_tracks_fp64(op):
         assert G.rel_l2(g32[ib][0], g64[ib][0]) <= 1e-5
         s64, s32 = g64[ib][1]['S'], g32[ib][1]['S']
         assert np.abs(s32 - s64).max() <= 1e-5 * max(1e-12, np.abs(s64).max())
+
+
+@pytest.mark.parametrize('op', [('upwind', 4), ('upwind_drp', 6), ('upwind_drp', 66)],
+                         ids=['upwind4', 'upwind_drp6', 'upwind_drp66'])
+def test_tiled_operator_kernel_larger_grid(op):
+    """2 x 70x45x40 curvilinear slip-weakening (TPV28) blocks: the tiled
+    operator kernel over several q/r tiles (partial last tiles) and s chunks,
+    the per-node kernel on the s-band planes; 4 LSRK steps against the
+    oracle"""
+    _say(f'[gpu] sbp tiled {op}')
+    prob = _problem('up4_curv', *op, n=(70, 45, 40))
+    got, od = _run_both(prob, 4, np.array([4.0, -0.9, 7.4, 0.2, 0.5, 1.0]))
+    for ib in range(2):
+        assert G.rel_l2(got[ib][0], od.blocks[ib]['F']) <= TOL, ib

@@ -176,6 +176,9 @@ struct wql_ctx {
     wql_sbp_half *d_sbp = nullptr;
     // anelastic response: relaxation times and weights of the four mechanisms
     double an_tau[4] = {0, 0, 0, 0}, an_w[4] = {0, 0, 0, 0};
+    // tiled operator kernel: halo (the operator's largest reach: 3, 5 or 7)
+    // and s-chunk length
+    int opt_h = 7, opt_zc = 12;
 };
 
 namespace {
@@ -356,27 +359,94 @@ void launch_pml(wql_ctx *c, Pair<real> &P, real A, real bdt)
         k_volume_generic<real><<<(unsigned)((BL.total + 255) / 256), 256, 0, c->stream>>>(P, BL, A, A, bdt);
 }
 
-// problems on the per-node path: the upwind / upwind-DRP operators and the
-// anelastic response (neither is in the tiled kernels)
+// problems on the operator path: the upwind / upwind-DRP operators and the
+// anelastic response (neither is in the sixth-order tiled kernels)
 bool per_node(const wql_ctx *c) { return c->sbp >= 0 || c->prob.anelastic; }
 
-// the per-node path: every node of both blocks, one thread each (replaces
-// the interior, s-closure and PML-slab kernels)
+// node ranges of the tiled operator kernel k_volume_opt in one block: q and
+// r outside the PML slabs, s the planes whose s row is the interior row and
+// at least HS planes from each s face (the s queue of the march); r[6] =
+// x0 x1 y0 y1 z0 z1, empty when the block has none
+void opt_range(const wql_ctx *c, const DevBlock &d, int r[6])
+{
+    const int w = WQL_SBP_OPS[c->sbp].band, hs = std::min(c->opt_h, 5);
+    r[0] = d.pml[0] ? d.npml : 0; r[1] = d.nq - (d.pml[1] ? d.npml : 0);
+    r[2] = d.pml[2] ? d.npml : 0; r[3] = d.nr - (d.pml[3] ? d.npml : 0);
+    r[4] = std::max(hs, d.pml[4] ? std::max(d.npml, w) : w);
+    r[5] = d.ns - std::max(hs, d.pml[5] ? std::max(d.npml, w) : w);
+    if (r[1] <= r[0] || r[3] <= r[2] || r[5] <= r[4]) r[1] = r[0] = r[3] = r[2] = r[5] = r[4] = 0;
+}
+
+// the tiled operator kernel's tiles: 32 x 8 columns, s chunks of opt_zc planes
+TileList opt_tiles(const wql_ctx *c)
+{
+    TileList L;
+    memset(&L, 0, sizeof L);
+    long first = 0;
+    for (int ib = 0; ib < 2; ib++) {
+        int r[6];
+        opt_range(c, c->b[ib], r);
+        if (r[1] <= r[0]) continue;
+        TileGeom &g = L.g[L.n++];
+        g.blk = ib;
+        g.xa0 = r[0]; g.xa1 = r[1]; g.ya0 = r[2]; g.ya1 = r[3]; g.za0 = r[4]; g.za1 = r[5];
+        g.ntx = (r[1] - r[0] + 31) / 32;
+        g.nty = (r[3] - r[2] + 7) / 8;
+        g.zc = std::max(1, c->opt_zc);
+        g.nzc = (r[5] - r[4] + g.zc - 1) / g.zc;
+        g.first = first;
+        first += (long)g.ntx * g.nty * g.nzc;
+    }
+    L.total = first;
+    return L;
+}
+
+// the operator path: k_volume_opt on the tiles of opt_tiles when the
+// operator is tabled and the tiles are not empty, k_volume_op (one thread per
+// node) on every other node (s-band planes, PML slabs; the whole block for
+// the sixth-order operator with the anelastic response)
 template <typename real>
 void launch_op(wql_ctx *c, Pair<real> &P, real A, real bdt)
 {
     BoxList L;
     memset(&L, 0, sizeof L);
     long start = 0;
+    auto add = [&](int ib, int x0, int x1, int y0, int y1, int z0, int z1) {
+        if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
+        Box &b = L.box[L.nbox++];
+        b.blk = ib; b.x0 = x0; b.x1 = x1; b.y0 = y0; b.y1 = y1; b.z0 = z0; b.z1 = z1;
+        b.start = start;
+        start += (long)(x1 - x0) * (y1 - y0) * (z1 - z0);
+    };
+    TileList TL;
+    memset(&TL, 0, sizeof TL);
+    if (c->sbp >= 0) TL = opt_tiles(c);
     for (int ib = 0; ib < 2; ib++) {
         const DevBlock &d = c->b[ib];
-        Box &b = L.box[L.nbox++];
-        b.blk = ib; b.x0 = 0; b.x1 = d.nq; b.y0 = 0; b.y1 = d.nr; b.z0 = 0; b.z1 = d.ns;
-        b.start = start;
-        start += (long)d.nq * d.nr * d.ns;
+        int r[6] = {0, 0, 0, 0, 0, 0};
+        if (c->sbp >= 0) opt_range(c, d, r);
+        if (r[1] <= r[0]) {
+            add(ib, 0, d.nq, 0, d.nr, 0, d.ns);
+            continue;
+        }
+        add(ib, 0, d.nq, 0, d.nr, 0, r[4]);           // s-low planes
+        add(ib, 0, d.nq, 0, d.nr, r[5], d.ns);        // s-high planes
+        add(ib, 0, r[0], 0, d.nr, r[4], r[5]);        // q-low PML slab
+        add(ib, r[1], d.nq, 0, d.nr, r[4], r[5]);     // q-high PML slab
+        add(ib, r[0], r[1], 0, r[2], r[4], r[5]);     // r-low PML slab
+        add(ib, r[0], r[1], r[3], d.nr, r[4], r[5]);  // r-high PML slab
     }
     L.total = start;
-    k_volume_op<real><<<(unsigned)((L.total + 255) / 256), 256, 0, c->stream>>>(P, L, A, A, bdt);
+    if (TL.total > 0) {
+        // a multiple of 8 workgroups: the kernel's XCD remap of block ids is a
+        // bijection onto [0, grid) only then
+        const dim3 blk(32, 8), grd((unsigned)(((TL.total + 7) / 8) * 8));
+        if (c->opt_h == 3) k_volume_opt<real, 32, 8, 3><<<grd, blk, 0, c->stream>>>(P, TL, A, A, bdt);
+        else if (c->opt_h == 5) k_volume_opt<real, 32, 8, 5><<<grd, blk, 0, c->stream>>>(P, TL, A, A, bdt);
+        else k_volume_opt<real, 32, 8, 7><<<grd, blk, 0, c->stream>>>(P, TL, A, A, bdt);
+    }
+    if (L.total > 0)
+        k_volume_op<real><<<(unsigned)((L.total + 255) / 256), 256, 0, c->stream>>>(P, L, A, A, bdt);
 }
 
 // the moment-tensor sources of both blocks at the stage time (after the
@@ -1202,6 +1272,22 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
     c->t = prob->t0;
     c->sbp = sbp;
     c->tau0 = sbp >= 0 ? WQL_SBP_OPS[sbp].tau0 : TAU0;
+    if (sbp >= 0) {
+        // the largest distance between a row and its taps, over every row
+        const wql_sbp_op &op = WQL_SBP_OPS[sbp];
+        int reach = 0;
+        for (int a = 0; a < 3; a++)
+            for (const wql_sbp_half *h : {&op.v[a], &op.s[a]}) {
+                reach = std::max(reach, std::max(-h->lo, h->hi));
+                for (int r = 0; r < h->nl; r++)
+                    for (int t = 0; t < WQL_SBP_MAXW; t++)
+                        if (h->left[r][t] != 0.0) reach = std::max(reach, std::abs(t - r));
+                for (int r = 0; r < h->nr; r++)
+                    for (int t = 0; t < WQL_SBP_MAXW; t++)
+                        if (h->right[r][t] != 0.0) reach = std::max(reach, std::abs(t - r));
+            }
+        c->opt_h = reach <= 3 ? 3 : reach <= 5 ? 5 : 7;
+    }
     if (prob->anelastic) {
         // relaxation times and weights of the four mechanisms (material.f90:80-106)
         static const double w1[4] = {1.6126, 0.6255, 0.6382, 1.5969}, w2[4] = {0.0336, 0.6873, 0.8767, 1.5202};

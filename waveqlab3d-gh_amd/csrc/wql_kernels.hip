@@ -1623,6 +1623,252 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
 }
 
 // ----------------------------------------------------------------------------
+// volume: the upwind / upwind-DRP operators (and the anelastic response),
+// tiled s-march
+//
+// A thread block owns a TX x TY tile of (q, r) columns and marches a chunk
+// of the planes whose s row is the operator's interior row (band..ns-1-band,
+// widened to npml on PML sides), over the q/r nodes outside the PML slabs.
+// Per plane the velocities and the q- and r-fluxes
+//     f_a,i = J (a_x s_xi + a_y s_yi + a_z s_zi),  a = q, r
+// of the tile and an H-node halo sit in LDS, H the operator's largest reach
+// (every closure row's taps lie within H nodes of the row, so no tile needs
+// an alignment with the faces); the velocities and s-fluxes of planes
+// c-HS..c+HS ride in a register queue.  A node applies its q and r rows (a
+// closure row at a face, the interior row elsewhere) to LDS and the interior
+// s row to the queue: the forward operator Dp to the velocities, the backward
+// operator Dm to the fluxes, with the interior routine's rationals at nodes
+// interior in all three directions and the band routine's printed row
+// elsewhere (the forms of JJU_x<o>_interior_upwind and JJU_x<o>_upwind,
+// JU_xJU_yJU_z6.f90:16451-20922, 815-16122; the linearity of the operator
+// lets the three flux families of a direction be summed before the
+// derivative).  The velocity gradient (chain rule with the node's metrics)
+// and the three flux divergences go to node_tail (Hooke's law, anelastic
+// terms, boundary penalty, stage update).  The s-band planes and the PML
+// slabs run on k_volume_op.
+
+// one row of an operator in LDS: 0..15 closure-left taps (node t), 16..31
+// closure-right taps (node n-1-t), 32..47 the printed interior row, 48..63
+// the interior routine's rationals (offsets lo..hi)
+struct OptRows {
+    int nl, nr, lo, hi;
+};
+
+template <typename real, int TX, int TY, int H>
+__global__ void __launch_bounds__(TX * TY, (sizeof(real) == 4 ? 2 : 1))
+    k_volume_opt(Pair<real> P, TileList TL, real A0, real A1, real bdt)
+{
+    constexpr int HS = H < 5 ? H : 5;  // interior rows reach at most 5 planes
+    constexpr int QL = 2 * HS + 1;
+    constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    constexpr int NT = TX * TY;
+    __shared__ real sv[3][WY][WX];    // velocities (corners unused)
+    __shared__ real sfq[3][TY][WX];   // q-fluxes
+    __shared__ real sfr[3][WY][TX];   // r-fluxes
+    __shared__ real tab[6][8][16];    // v q/r/s then s q/r/s: [0..7] left rows
+    __shared__ real tabr[6][8][16];   // right rows
+    __shared__ real tint[6][2][16];   // interior row: printed (0), rationals (1)
+    __shared__ OptRows trows[6];
+
+    const long nwg = (long)gridDim.x;
+    const long per = (nwg + 7) / 8;
+    const long logical = (long)(blockIdx.x % 8) * per + (long)(blockIdx.x / 8);
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    if (logical >= TL.total) return;
+    int gi = 0;
+    for (int k = 1; k < TL.n; k++)
+        if (logical >= TL.g[k].first) gi = k;
+    const TileGeom &T = TL.g[gi];
+    const Blk<real> &B = P.b[T.blk];
+    const real A = T.blk == 0 ? A0 : A1;
+    const long lid = logical - T.first;
+    const int tx_i = (int)(lid % T.ntx), ty_i = (int)((lid / T.ntx) % T.nty), zc_i = (int)(lid / ((long)T.ntx * T.nty));
+
+    // the operator's tables into LDS
+    for (int e = tid; e < 6 * 8 * 16; e += NT) {
+        const int h = e / 128, r = (e / 16) % 8, t = e % 16;
+        tab[h][r][t] = (real)B.sbp[h].left[r][t];
+        tabr[h][r][t] = (real)B.sbp[h].right[r][t];
+    }
+    for (int e = tid; e < 6 * 16; e += NT) {
+        const int h = e / 16, t = e % 16;
+        tint[h][0][t] = (real)B.sbp[h].band[t];
+        tint[h][1][t] = (real)B.sbp[h].exact[t];
+    }
+    if (tid < 6) {
+        trows[tid].nl = B.sbp[tid].nl; trows[tid].nr = B.sbp[tid].nr;
+        trows[tid].lo = B.sbp[tid].lo; trows[tid].hi = B.sbp[tid].hi;
+    }
+
+    const int xs = T.xa0 + tx_i * TX, ys = T.ya0 + ty_i * TY;
+    const int span = T.za1 - T.za0;
+    const int z0 = T.za0 + (int)((long)zc_i * span / T.nzc);
+    const int z1 = T.za0 + (int)((long)(zc_i + 1) * span / T.nzc);
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int gx = xs + lx, gy = ys + ly;
+    const bool own = gx < T.xa1 && gy < T.ya1;
+    const long NP = B.NP;
+    const int nq = B.nq, nr = B.nr, ns = B.ns;
+    // interior in q and r (the tile's planes are interior in s)
+    const int w = B.sbw;
+    const int ix0 = B.pml[0] && B.npml > w ? B.npml : w, ix1 = B.pml[1] && B.npml > w ? B.npml : w;
+    const int iy0 = B.pml[2] && B.npml > w ? B.npml : w, iy1 = B.pml[3] && B.npml > w ? B.npml : w;
+    const bool interior = gx >= ix0 && gx <= nq - 1 - ix1 && gy >= iy0 && gy <= nr - 1 - iy1;
+    const real ihq = (real)(nq - 1), ihr = (real)(nr - 1), ihs = (real)(ns - 1);
+
+    auto met = [&](int c, long p) -> real { return (real)gmet(B, c, p); };
+    auto flux = [&](long p, int a, real fl[3]) {  // J (a_x s_x. + a_y s_y. + a_z s_z.), a = 0 q, 1 r, 2 s
+        const real J = met(9, p);
+        const real jx = J * met(a, p), jy = J * met(3 + a, p), jz = J * met(6 + a, p);
+        const real sxx = B.Fin[3 * NP + p], syy = B.Fin[4 * NP + p], szz = B.Fin[5 * NP + p];
+        const real sxy = B.Fin[6 * NP + p], sxz = B.Fin[7 * NP + p], syz = B.Fin[8 * NP + p];
+        fl[0] = jx * sxx + jy * sxy + jz * sxz;
+        fl[1] = jx * sxy + jy * syy + jz * syz;
+        fl[2] = jx * sxz + jy * syz + jz * szz;
+    };
+    // the queue: planes c-HS..c+HS of the own column
+    real vq[QL][3], fs[QL][3];
+#pragma unroll
+    for (int a = 0; a < QL; a++)
+#pragma unroll
+        for (int f = 0; f < 3; f++) { vq[a][f] = 0; fs[a][f] = 0; }
+    auto lead = [&](int plane, real v[3], real fl[3]) {
+        const long p = pidx(B.px, nr, gx, gy, plane);
+#pragma unroll
+        for (int f = 0; f < 3; f++) v[f] = B.Fin[f * NP + p];
+        flux(p, 2, fl);
+    };
+    if (own)
+        for (int a = 0; a < QL - 1; a++) lead(z0 - HS + a, vq[a + 1], fs[a + 1]);
+    __syncthreads();
+
+    const int nwin = TX * TY + 2 * H * TY + 2 * H * TX;
+    for (int c = z0; c < z1; c++) {
+        // plane c+HS enters the queue
+#pragma unroll
+        for (int a = 0; a < QL - 1; a++)
+#pragma unroll
+            for (int f = 0; f < 3; f++) { vq[a][f] = vq[a + 1][f]; fs[a][f] = fs[a + 1][f]; }
+        if (own) lead(c + HS, vq[QL - 1], fs[QL - 1]);
+        // the plane's window: the tile, its q halo (TY rows), its r halo (TX columns)
+        for (int e = tid; e < nwin; e += NT) {
+            int wx, wy;
+            if (e < TX * TY) { wx = H + e % TX; wy = H + e / TX; }
+            else if (e < TX * TY + 2 * H * TY) {
+                const int k = e - TX * TY;
+                wx = k % (2 * H); wx = wx < H ? wx : TX + wx; wy = H + k / (2 * H);
+            } else {
+                const int k = e - TX * TY - 2 * H * TY;
+                wx = H + k % TX; wy = k / TX; wy = wy < H ? wy : TY + wy;
+            }
+            const int qx = xs - H + wx, ry = ys - H + wy;
+            if (qx < 0 || qx >= nq || ry < 0 || ry >= nr) continue;
+            const long p = pidx(B.px, nr, qx, ry, c);
+#pragma unroll
+            for (int f = 0; f < 3; f++) sv[f][wy][wx] = B.Fin[f * NP + p];
+            real fl[3];
+            if (wy >= H && wy < H + TY) {
+                flux(p, 0, fl);
+#pragma unroll
+                for (int f = 0; f < 3; f++) sfq[f][wy - H][wx] = fl[f];
+            }
+            if (wx >= H && wx < H + TX) {
+                flux(p, 1, fl);
+#pragma unroll
+                for (int f = 0; f < 3; f++) sfr[f][wy][wx - H] = fl[f];
+            }
+        }
+        __syncthreads();
+        if (own) {
+            // q and r: rows at the node's position from LDS
+            real dv[3][3], df[3][3];  // [direction][field]
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const int pos = a == 0 ? gx : gy, n = a == 0 ? nq : nr;
+                const int base = a == 0 ? xs - H : ys - H;   // window coordinate of node 0
+#pragma unroll
+                for (int grp = 0; grp < 2; grp++) {
+                    const int h = grp * 3 + a;
+                    const OptRows R = trows[h];
+                    real s3[3] = {0, 0, 0};
+                    auto tap = [&](int node, real cf) {
+                        const int wpos = node - base;
+#pragma unroll
+                        for (int f = 0; f < 3; f++) {
+                            real v;
+                            if (grp == 0) v = a == 0 ? sv[f][H + ly][wpos] : sv[f][wpos][H + lx];
+                            else v = a == 0 ? sfq[f][ly][wpos] : sfr[f][wpos][lx];
+                            s3[f] = s3[f] + cf * v;
+                        }
+                    };
+                    if (pos < R.nl) {
+                        for (int t = 0; t < 16; t++) {
+                            const real cf = tab[h][pos][t];
+                            if (cf != (real)0) tap(t, cf);
+                        }
+                    } else if (pos >= n - R.nr) {
+                        const int r = n - 1 - pos;
+                        for (int t = 0; t < 16; t++) {
+                            const real cf = tabr[h][r][t];
+                            if (cf != (real)0) tap(n - 1 - t, cf);
+                        }
+                    } else {
+                        const int set = interior ? 1 : 0;
+                        for (int o = R.lo; o <= R.hi; o++) tap(pos + o, tint[h][set][o - R.lo]);
+                    }
+#pragma unroll
+                    for (int f = 0; f < 3; f++) {
+                        if (grp == 0) dv[a][f] = s3[f];
+                        else df[a][f] = s3[f];
+                    }
+                }
+            }
+            // s: the interior row from the queue (plane c + o is queue slot HS + o)
+#pragma unroll
+            for (int grp = 0; grp < 2; grp++) {
+                const int h = grp * 3 + 2;
+                const OptRows R = trows[h];
+                const int set = interior ? 1 : 0;
+                real s3[3] = {0, 0, 0};
+#pragma unroll
+                for (int o = -HS; o <= HS; o++) {
+                    if (o < R.lo || o > R.hi) continue;
+                    const real cf = tint[h][set][o - R.lo];
+#pragma unroll
+                    for (int f = 0; f < 3; f++) s3[f] = s3[f] + cf * (grp == 0 ? vq[HS + o][f] : fs[HS + o][f]);
+                }
+#pragma unroll
+                for (int f = 0; f < 3; f++) {
+                    if (grp == 0) dv[2][f] = s3[f];
+                    else df[2][f] = s3[f];
+                }
+            }
+            const long p = pidx(B.px, nr, gx, gy, c);
+            const real ih[3] = {ihq, ihr, ihs};
+            real U[3][NF];
+#pragma unroll
+            for (int fam = 0; fam < 3; fam++) {
+#pragma unroll
+                for (int f = 0; f < 3; f++)
+                    U[fam][f] = met(3 * fam, p) * (ih[0] * dv[0][f]) + met(3 * fam + 1, p) * (ih[1] * dv[1][f]) +
+                                met(3 * fam + 2, p) * (ih[2] * dv[2][f]);
+#pragma unroll
+                for (int f = 3; f < NF; f++) U[fam][f] = 0;
+            }
+            // the flux divergences in the slots node_tail sums for the velocity rates
+            U[0][3] = ih[0] * df[0][0] + ih[1] * df[1][0] + ih[2] * df[2][0];
+            U[0][6] = ih[0] * df[0][1] + ih[1] * df[1][1] + ih[2] * df[2][1];
+            U[0][7] = ih[0] * df[0][2] + ih[1] * df[1][2] + ih[2] * df[2][2];
+            real fin[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) fin[f] = B.Fin[f * NP + p];
+            node_tail<real>(B, gx, gy, c, p, U, fin, met(9, p), A, bdt);
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------------
 // boundary: outer-face SAT terms and the fault interface
 //
 // FP64 throughout and without FMA contraction, so that the friction solve
