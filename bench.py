@@ -34,7 +34,11 @@ def per_node(prob):
 def kernel_name(prob, precision, variant):
     """template name of the main volume kernel a launch variant selects"""
     if per_node(prob):
-        return f'k_volume_op<{"float" if precision == "fp32" else "double"}>'
+        # the operator path: the tiled k_volume_opt (upwind / DRP operators)
+        # and k_volume_op on the remaining nodes, timed together
+        real = "float" if precision == "fp32" else "double"
+        return f'k_volume_opt<{real}> + k_volume_op<{real}>' if prob.fd_type != 'traditional' \
+            else f'k_volume_op<{real}>'
     fam, tx, ty = VARIANTS[variant]
     curv = 'true' if any(b.mesh != 'cartesian' for b in prob.blocks) else 'false'
     het = 'true' if any(b.material is not None for b in prob.blocks) else 'false'

@@ -179,6 +179,8 @@ struct wql_ctx {
     // tiled operator kernel: halo (the operator's largest reach: 3, 5 or 7)
     // and s-chunk length
     int opt_h = 7, opt_zc = 12;
+    // band width of the tabled operator (d_sbp != nullptr)
+    int band = 6;
 };
 
 namespace {
@@ -219,8 +221,8 @@ Blk<real> blk_view(const wql_ctx *c, int ib, int cur)
     B.Y = d.Y;
     B.satb = nullptr;
     B.nbnd = d.nbnd;
-    B.sbp = c->sbp >= 0 ? c->d_sbp : nullptr;
-    B.sbw = c->sbp >= 0 ? WQL_SBP_OPS[c->sbp].band : 0;
+    B.sbp = c->d_sbp;
+    B.sbw = c->d_sbp ? c->band : 0;
     B.eta = (real *)d.eta;
     B.deta = (real *)d.deta;
     B.qinv = (const real *)d.qinv;
@@ -369,7 +371,7 @@ bool per_node(const wql_ctx *c) { return c->sbp >= 0 || c->prob.anelastic; }
 // x0 x1 y0 y1 z0 z1, empty when the block has none
 void opt_range(const wql_ctx *c, const DevBlock &d, int r[6])
 {
-    const int w = WQL_SBP_OPS[c->sbp].band, hs = std::min(c->opt_h, 5);
+    const int w = c->band, hs = std::min(c->opt_h, 5);
     r[0] = d.pml[0] ? d.npml : 0; r[1] = d.nq - (d.pml[1] ? d.npml : 0);
     r[2] = d.pml[2] ? d.npml : 0; r[3] = d.nr - (d.pml[3] ? d.npml : 0);
     r[4] = std::max(hs, d.pml[4] ? std::max(d.npml, w) : w);
@@ -401,10 +403,8 @@ TileList opt_tiles(const wql_ctx *c)
     return L;
 }
 
-// the operator path: k_volume_opt on the tiles of opt_tiles when the
-// operator is tabled and the tiles are not empty, k_volume_op (one thread per
-// node) on every other node (s-band planes, PML slabs; the whole block for
-// the sixth-order operator with the anelastic response)
+// the operator path: k_volume_opt on the tiles of opt_tiles, k_volume_op
+// (one thread per node) on every other node (s-band planes, PML slabs)
 template <typename real>
 void launch_op(wql_ctx *c, Pair<real> &P, real A, real bdt)
 {
@@ -420,11 +420,11 @@ void launch_op(wql_ctx *c, Pair<real> &P, real A, real bdt)
     };
     TileList TL;
     memset(&TL, 0, sizeof TL);
-    if (c->sbp >= 0) TL = opt_tiles(c);
+    if (c->d_sbp) TL = opt_tiles(c);
     for (int ib = 0; ib < 2; ib++) {
         const DevBlock &d = c->b[ib];
         int r[6] = {0, 0, 0, 0, 0, 0};
-        if (c->sbp >= 0) opt_range(c, d, r);
+        if (c->d_sbp) opt_range(c, d, r);
         if (r[1] <= r[0]) {
             add(ib, 0, d.nq, 0, d.nr, 0, d.ns);
             continue;
@@ -1272,12 +1272,22 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
     c->t = prob->t0;
     c->sbp = sbp;
     c->tau0 = sbp >= 0 ? WQL_SBP_OPS[sbp].tau0 : TAU0;
-    if (sbp >= 0) {
+    // the operator kernels' tables of an upwind / DRP operator (the
+    // sixth-order operator with the anelastic response runs on the per-node
+    // kernel's own sixth-order rows: as a table on the tiled kernel it was no
+    // faster in FP32, 8.24 against 8.33 ms per step on 2 x 201x101x101, and
+    // slower in FP64, 15.6 against 11.0)
+    wql_sbp_half halves[6];
+    const bool tabled = sbp >= 0;
+    if (tabled) {
+        for (int a = 0; a < 3; a++) { halves[a] = WQL_SBP_OPS[sbp].v[a]; halves[3 + a] = WQL_SBP_OPS[sbp].s[a]; }
+        c->band = WQL_SBP_OPS[sbp].band;
+    }
+    if (tabled) {
         // the largest distance between a row and its taps, over every row
-        const wql_sbp_op &op = WQL_SBP_OPS[sbp];
         int reach = 0;
-        for (int a = 0; a < 3; a++)
-            for (const wql_sbp_half *h : {&op.v[a], &op.s[a]}) {
+        for (int a = 0; a < 6; a++)
+            for (const wql_sbp_half *h : {&halves[a]}) {
                 reach = std::max(reach, std::max(-h->lo, h->hi));
                 for (int r = 0; r < h->nl; r++)
                     for (int t = 0; t < WQL_SBP_MAXW; t++)
@@ -1330,15 +1340,13 @@ int wql_create(const wql_problem *prob, const wql_block_desc blocks[2], wql_ctx 
         return fail(WQL_E_NOMEM, "hipMalloc(time)");
     }
     c->allocs.push_back(c->d_t);
-    if (c->sbp >= 0) {
+    if (tabled) {
         // the operator's velocity (Dp) and stress (Dm) tables, per direction
-        wql_sbp_half h[6];
-        for (int a = 0; a < 3; a++) { h[a] = WQL_SBP_OPS[c->sbp].v[a]; h[3 + a] = WQL_SBP_OPS[c->sbp].s[a]; }
-        if (int rc2 = dalloc(c, (void **)&c->d_sbp, sizeof h)) {
+        if (int rc2 = dalloc(c, (void **)&c->d_sbp, sizeof halves)) {
             wql_destroy(c);
             return rc2;
         }
-        if (hipMemcpy(c->d_sbp, h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMemcpy(c->d_sbp, halves, sizeof halves, hipMemcpyHostToDevice) != hipSuccess) {
             wql_destroy(c);
             return fail(WQL_E_HIP, "hipMemcpy(operator table)");
         }

@@ -429,27 +429,26 @@ __global__ void __launch_bounds__(256) k_volume_generic(Pair<real> P, BoxList L,
 // Ju = (1/hq) Jq + (1/hr) Jr + (1/hs) Js.  Hooke's law, the PML terms, the
 // boundary penalty and the stage update follow in node_tail.
 
-// taps of node pos on a line of n nodes: a closure row or the printed
-// interior row
-template <typename real>
-__device__ __forceinline__ int op_taps(const wql_sbp_half &h, int pos, int n, int *idx, real *c)
+// calls fn(node, coefficient) for each tap of node pos on a line of n
+// nodes: a closure row or the printed interior row (no tap arrays, which
+// would sit in scratch memory)
+template <typename real, typename Fn>
+__device__ __forceinline__ void op_taps(const wql_sbp_half &h, int pos, int n, Fn fn)
 {
-    int m = 0;
     if (pos < h.nl) {
         for (int t = 0; t < WQL_SBP_MAXW; t++) {
             const double v = h.left[pos][t];
-            if (v != 0.0) { idx[m] = t; c[m] = (real)v; m++; }
+            if (v != 0.0) fn(t, (real)v);
         }
     } else if (pos >= n - h.nr) {
         const int r = n - 1 - pos;
         for (int t = 0; t < WQL_SBP_MAXW; t++) {
             const double v = h.right[r][t];
-            if (v != 0.0) { idx[m] = n - 1 - t; c[m] = (real)v; m++; }
+            if (v != 0.0) fn(n - 1 - t, (real)v);
         }
     } else {
-        for (int o = h.lo; o <= h.hi; o++) { idx[m] = pos + o; c[m] = (real)h.band[o - h.lo]; m++; }
+        for (int o = h.lo; o <= h.hi; o++) fn(pos + o, (real)h.band[o - h.lo]);
     }
-    return m;
 }
 
 template <typename real>
@@ -519,33 +518,35 @@ __device__ void volume_node_op(const Blk<real> &B, int i, int j, int k, real A, 
         }
         Jc = B.cart ? (real)1 : (real)gmet(B, 9, p);
     } else {
+#pragma unroll 1
         for (int a = 0; a < 3; a++) {
-            int idx[WQL_SBP_MAXW];
-            real c[WQL_SBP_MAXW];
+            // node index along direction a -> its linear index
+            const long base = p - (long)pos[a] * (a == 0 ? 1L : a == 1 ? (long)B.px : (long)B.px * B.nr);
+            const long step = a == 0 ? 1L : a == 1 ? (long)B.px : (long)B.px * B.nr;
             real sv[3] = {0, 0, 0};
-            int m = op_taps<real>(V[a], pos[a], nn[a], idx, c);
-            for (int t = 0; t < m; t++) {
-                int q[3] = {i, j, k};
-                q[a] = idx[t];
-                const long qq = pidx(B.px, B.nr, q[0], q[1], q[2]);
-                for (int f = 0; f < 3; f++) sv[f] = sv[f] + c[t] * B.Fin[f * NP + qq];
-            }
+            op_taps<real>(V[a], pos[a], nn[a], [&](int node, real cf) {
+                const long qq = base + (long)node * step;
+#pragma unroll
+                for (int f = 0; f < 3; f++) sv[f] = sv[f] + cf * B.Fin[f * NP + qq];
+            });
             real ss[3][6];
+#pragma unroll
             for (int fam = 0; fam < 3; fam++)
+#pragma unroll
                 for (int f = 0; f < 6; f++) ss[fam][f] = 0;
-            m = op_taps<real>(S[a], pos[a], nn[a], idx, c);
-            for (int t = 0; t < m; t++) {
-                int q[3] = {i, j, k};
-                q[a] = idx[t];
-                const long qq = pidx(B.px, B.nr, q[0], q[1], q[2]);
+            op_taps<real>(S[a], pos[a], nn[a], [&](int node, real cf) {
+                const long qq = base + (long)node * step;
                 const real J = (real)gmet(B, 9, qq);
                 real cm[3];
-                for (int fam = 0; fam < 3; fam++) cm[fam] = c[t] * (real)gmet(B, 3 * fam + a, qq) * J;
+#pragma unroll
+                for (int fam = 0; fam < 3; fam++) cm[fam] = cf * (real)gmet(B, 3 * fam + a, qq) * J;
+#pragma unroll
                 for (int f = 0; f < 6; f++) {
                     const real sg = B.Fin[(3 + f) * NP + qq];
+#pragma unroll
                     for (int fam = 0; fam < 3; fam++) ss[fam][f] = ss[fam][f] + cm[fam] * sg;
                 }
-            }
+            });
             const real ih = (real)(1.0 / (1.0 / (double)(nn[a] - 1)));
             for (int fam = 0; fam < 3; fam++) {
                 const real mc = (real)gmet(B, 3 * fam + a, p);
