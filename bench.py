@@ -26,9 +26,9 @@ VARIANTS = [('fused', 32, 8), ('hold', 32, 8), ('fused', 64, 8), ('hold', 64, 8)
 
 
 def per_node(prob):
-    """the upwind / upwind-DRP operators and the anelastic response run on the
-    per-node volume kernel"""
-    return prob.fd_type != 'traditional' or prob.anelastic is not None
+    """the upwind / upwind-DRP operators run on the operator kernels (the
+    tiled k_volume_opt and the per-node k_volume_op)"""
+    return prob.fd_type != 'traditional'
 
 
 def kernel_name(prob, precision, variant):
@@ -37,16 +37,19 @@ def kernel_name(prob, precision, variant):
         # the operator path: the tiled k_volume_opt (upwind / DRP operators)
         # and k_volume_op on the remaining nodes, timed together
         real = "float" if precision == "fp32" else "double"
-        return f'k_volume_opt<{real}> + k_volume_op<{real}>' if prob.fd_type != 'traditional' \
-            else f'k_volume_op<{real}>'
+        return f'k_volume_opt<{real}> + k_volume_op<{real}>'
     fam, tx, ty = VARIANTS[variant]
+    an = prob.anelastic is not None
+    if an:
+        # the anelastic response runs on 32 x 8 tiles, fused or held
+        fam, tx, ty = ('hold', 32, 8) if fam == 'hold' else ('fused', 32, 8)
     curv = 'true' if any(b.mesh != 'cartesian' for b in prob.blocks) else 'false'
     het = 'true' if any(b.material is not None for b in prob.blocks) else 'false'
     real = 'float' if precision == 'fp32' else 'double'
     if fam == 'split':
         return f'k_volume_split<{real}, {tx}, {ty}, {curv}, {het}, 0>'  # interior (not s-closure) instantiation
     hold = 'true' if fam == 'hold' else 'false'  # lead plane held in registers
-    return f'k_volume_fused<{real}, {tx}, {ty}, {curv}, {het}, {hold}>'
+    return f'k_volume_fused<{real}, {tx}, {ty}, {curv}, {het}, {hold}, {"true" if an else "false"}>'
 
 
 def measured_traffic(workload, shape, kname):
@@ -221,14 +224,33 @@ def serving(n_scen, steps, rate, max_active, n_seq, seed=5, fifo=True):
                                                 'scenarios_per_s': round(n_scen / wall_f, 4)}}
 
 
+def configure(a, scale):
+    """the workload named on the command line at `scale`, with the operator,
+    the anelastic response and the PML switched as asked"""
+    from wql import workloads
+    W = workloads.WORKLOADS[a.workload](scale=scale)
+    prob = W['problem']
+    if a.operator:
+        prob.fd_type, order = a.operator.split(':')
+        prob.order = int(order)
+    if a.anelastic == 'off':
+        prob.anelastic = None
+    elif a.anelastic:
+        prob.anelastic = tuple(float(v) for v in a.anelastic.split(','))
+    if a.pml == 'off':
+        for b in prob.blocks:
+            b.pml, b.npml = (False,) * 6, 0
+    return W
+
+
 def cpu_baseline(W, min_seconds=12.0, max_steps=40):
     """the oracle (C, OpenMP over the host's cores) on full LSRK steps of the
-    same workload: steps until at least min_seconds of wall time"""
+    same workload at full size: steps until at least min_seconds of wall time"""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle as O
     from wql import workloads
     name = W['problem'].name
-    sub = workloads.WORKLOADS.get(name, workloads.rupture)(scale=1.0)
+    sub = W
     prob = sub['problem']
     od = O.OracleDomain(prob, coords=sub['coords'])
     if sub.get('F0') is not None:
@@ -265,7 +287,9 @@ def main():
     ap.add_argument('--operator', default=None,
                     help="spatial operator 'upwind:<order>' or 'upwind_drp:<order>' (default: the sixth-order "
                          "central one)")
-    ap.add_argument('--anelastic', default=None, help="anelastic response 'c,weight_exp,fref', e.g. 2,0.6,1")
+    ap.add_argument('--anelastic', default=None,
+                    help="anelastic response 'c,weight_exp,fref', e.g. 2,0.6,1, or 'off'")
+    ap.add_argument('--pml', default=None, help="'off': the workload without its PML slabs")
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--err-steps', type=int, default=None,
@@ -286,13 +310,8 @@ def main():
               'once per RK stage)', file=sys.stderr)
         sys.exit(2)
     from wql import workloads
-    W = workloads.WORKLOADS[a.workload](scale=a.scale)
+    W = configure(a, a.scale)
     prob = W['problem']
-    if a.operator:
-        prob.fd_type, order = a.operator.split(':')
-        prob.order = int(order)
-    if a.anelastic:
-        prob.anelastic = tuple(float(v) for v in a.anelastic.split(','))
     nodes = workloads.points(prob)
     res = {}
     for prec in (['fp32'] if a.no_fp64 else ['fp64', 'fp32']):
@@ -301,10 +320,12 @@ def main():
         res[prec] = dict(sec=sec, kms=kms, dev=dev, shape=shape, mpts=nodes * 5 * a.steps / sec / 1e6)
     r32 = res['fp32']
     workload = f'{a.workload}: 2 blocks x {"x".join(str(v) for v in prob.blocks[0].n)}'
-    if a.operator:
+    if a.operator or prob.fd_type != 'traditional':
         workload += f', {prob.fd_type} order {prob.order}'
-    if a.anelastic:
-        workload += f', anelastic {a.anelastic}'
+    if prob.anelastic is not None:
+        workload += ', anelastic ' + ','.join(f'{v:g}' for v in prob.anelastic)
+    if a.pml == 'off':
+        workload += ', PML off'
     kname = kernel_name(prob, 'fp32', r32['shape'][0])
     traffic = measured_traffic(workload, r32['shape'], kname)
     bpn = bytes_per_node(prob, 'fp32')
@@ -345,7 +366,7 @@ def main():
     elif a.serve > 0:
         out['serving'] = serving(a.serve, a.serve_steps, a.serve_rate, a.serve_active, min(a.serve_seq, a.serve))
     if not a.no_cpu:
-        out['cpu_baseline'] = cpu_baseline(W)
+        out['cpu_baseline'] = cpu_baseline(W if a.scale == 1.0 else configure(a, 1.0))
     print(json.dumps(out), flush=True)
 
 

@@ -1256,11 +1256,13 @@ __global__ void __launch_bounds__(TX * TY, split_min_waves<real>()) k_volume_spl
 // Occupancy asked of the register allocator: FP32 re-reading the centre
 // plane fits three waves per SIMD (<= 168 VGPRs), holding it needs two; FP64
 // runs one wave per SIMD with AGPRs (two would spill heavily).
-template <typename real, bool HOLD>
-constexpr int fused_min_waves() { return sizeof(real) == 4 ? (HOLD ? 2 : 3) : 1; }
+// With the anelastic response (AN) FP32 asks for two waves per SIMD: the
+// memory-variable update adds its loads to the live prefetch buffers.
+template <typename real, bool HOLD, bool AN>
+constexpr int fused_min_waves() { return sizeof(real) == 4 ? (HOLD || AN ? 2 : 3) : 1; }
 
-template <typename real, int TX, int TY, bool CURV, bool HET, bool HOLD>
-__global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
+template <typename real, int TX, int TY, bool CURV, bool HET, bool HOLD, bool AN>
+__global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD, AN>()))
     k_volume_fused(Pair<real> P, TileList TL, real A0, real A1, real bdt)
 {
     constexpr int WX = TX + 6, WY = TY + 6;
@@ -1333,6 +1335,10 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
     const __amdgpu_buffer_rsrc_t rO = brsrc(B.Fout, 9u * NPb);
     const __amdgpu_buffer_rsrc_t rG = brsrc(CURV ? (const void *)B.G : (const void *)B.Fin, CURV ? 10u * NPb : 0u);
     const __amdgpu_buffer_rsrc_t rM = brsrc(HET ? (const void *)B.M : (const void *)B.Fin, HET ? 3u * NPb : 0u);
+    // anelastic memory variables, their rates and Q_s^-1, Q_p^-1 (AN only)
+    const __amdgpu_buffer_rsrc_t rE = brsrc(AN ? (const void *)B.eta : (const void *)B.Fin, AN ? 24u * NPb : 0u);
+    const __amdgpu_buffer_rsrc_t rDE = brsrc(AN ? (const void *)B.deta : (const void *)B.Fin, AN ? 24u * NPb : 0u);
+    const __amdgpu_buffer_rsrc_t rQi = brsrc(AN ? (const void *)B.qinv : (const void *)B.Fin, AN ? 2u * NPb : 0u);
     // metric c of the node at byte offset o.  A field that is constant over
     // the block (bit gconst) is loaded through a zero-length buffer (0, and no
     // memory access); K(v, c) then selects its constant.  K is applied where
@@ -1599,6 +1605,37 @@ __global__ void __launch_bounds__(TX * TY, (fused_min_waves<real, HOLD>()))
             DU[3] = mu * (Uy[0] + Ux[1]);
             DU[4] = mu * (Uz[0] + Ux[2]);
             DU[5] = mu * (Uz[1] + Uy[2]);
+            if constexpr (AN) {
+                // the anelastic memory variables of the node (apply_anelastic_point,
+                // RHS_Interior.f90:2390-2441; the march covers no PML slab, so the
+                // strains are the velocity gradient's): DU -= sum over the four
+                // mechanisms, Deta = A Deta + rate, eta += B dt Deta in place,
+                // one stress component at a time
+                const real e[6] = {Ux[0], Uy[1], Uz[2], Uy[0] + Ux[1], Uz[0] + Ux[2], Uz[1] + Uy[2]};
+                const real tr = e[0] + e[1] + e[2];
+                const real Qsi = bload<real>(rQi, o, 0), Qpi = bload<real>(rQi, o, NPb);
+                const real wa = (real)2 * mu * Qsi, wb = (lam + (real)2 * mu) * Qpi - (real)2 * mu * Qsi;
+                const real wc = mu * Qsi;
+#pragma unroll
+                for (int f = 0; f < 6; f++) {
+                    real et[4], de0[4];
+#pragma unroll
+                    for (int l = 0; l < 4; l++) {
+                        et[l] = bload_nt<real>(rE, o, (unsigned)(4 * f + l) * NPb);
+                        de0[l] = rdA ? bload_nt<real>(rDE, o, (unsigned)(4 * f + l) * NPb) : (real)0;
+                    }
+                    DU[f] = DU[f] - (et[0] + et[1] + et[2] + et[3]);
+#pragma unroll
+                    for (int l = 0; l < 4; l++) {
+                        const real w = (real)B.an_w[l], tau = (real)B.an_tau[l];
+                        const real rate = f < 3 ? (((w * wa) * e[f] + (w * wb) * tr) - et[l]) / tau
+                                                : (w * wc * e[f] - et[l]) / tau;
+                        const real de = A * de0[l] + rate;
+                        bstore_nt<real>(rDE, o, (unsigned)(4 * f + l) * NPb, de);
+                        bstore_nt<real>(rE, o, (unsigned)(4 * f + l) * NPb, et[l] + bdt * de);
+                    }
+                }
+            }
 #pragma unroll
             for (int f = 0; f < 6; f++) {
                 real df = A * cdf[3 + f] + DU[f];

@@ -189,6 +189,8 @@ int scenario_admit(wql_server *s, wql_scenario *sc)
         if (int rc = ctx_reset(ctx, &sc->prob)) return rc;
     } else {
         if (int rc = wql_create(&sc->prob, sc->blocks, &ctx)) return rc;
+        // one stream per scenario: the scenarios in flight already overlap
+        ctx->overlap = 0;
         sc->ctx = ctx;
         s->used += sc->need;
         s->created++;

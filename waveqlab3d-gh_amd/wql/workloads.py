@@ -16,6 +16,11 @@ physics and seeds; scale=1 gives the full workload.
            Drucker-Prager viscoplasticity, TPV26-style slip weakening
   serving  per-scenario two-block problems drawn from {61^3, 121x61x61,
            201x101x101, 241x121x121} with seed-drawn friction
+  pml12    the configuration of the reference's PML and anelasticity study
+           (simulation/PML-12/input/*.in): two Cartesian 76x101x151 blocks,
+           locked interface, 12-node PML, a moment-tensor point source, the
+           anelastic response with Q_s = 2 v_s; the operator, the PML and the
+           response are switched per deck
 """
 import math
 
@@ -223,7 +228,33 @@ def scenario(k, seed=5, scale=1.0):
     return dict(problem=prob, coords=coords, F0=None, steps=1000, precision='fp32')
 
 
-WORKLOADS = {'plain': plain, 'rupture': rupture, 'rough': rough, 'plastic': plastic}
+def pml12(scale=1.0, fd_type='traditional', order=6, pml=True, anelastic=True):
+    """The decks of the reference's PML study (simulation/PML-12/input/
+    <fd>_6_pml-<on|off>_<elastic|anelastic-c2>.in): two Cartesian blocks of
+    76x101x151 nodes over 15 x 20 x 30 km each (h = 0.2 km in q and r), a
+    locked interface at x = 15 km, the free surface at y = 0 and absorbing
+    SAT on the other outer faces, with `pml` a 12-node PML on each of them;
+    one order-4 Cos-Bell moment-tensor source (M_xz = -10, t_init 0.2 s) at
+    (5, 1.8, 5) km in block 1; with `anelastic` Q_s = 2 v_s, weight_exp 0.6,
+    fref 1 Hz; 8 s of simulated time at CFL 0.5."""
+    n = _scaled((76, 101, 151), scale)
+    h = 0.2
+    L = tuple((v - 1) * h for v in n)
+    blocks = []
+    for ib in range(2):
+        bc = (1, 0, 2, 1, 1, 1) if ib == 0 else (0, 1, 2, 1, 1, 1)
+        pm = (True, False, False, True, True, True) if ib == 0 else (False, True, False, True, True, True)
+        blocks.append(Block(n=n, a=(ib * L[0], 0.0, 0.0), b=((ib + 1) * L[0], L[1], L[2]),
+                            rho_s_p=(2.7, 3.464, 6.0), bc=bc, pml=pm if pml else (False,) * 6,
+                            npml=12 if pml else 0, mesh='cartesian'))
+    src = dict(type=3, duration=0.0, t_init=0.2, m=(0.0, 0.0, 0.0, 0.0, -10.0, 0.0), x=(5.0, 1.8, 5.0), alpha=0)
+    prob = Problem(blocks=blocks, fault=Fault(law=LOCKED), cfl=0.5, t_final=8.0, name='pml12',
+                   moment=dict(order=4, preference=1, lists={'U': [src], 'V': []}),
+                   fd_type=fd_type, order=order, anelastic=(2.0, 0.6, 1.0) if anelastic else None)
+    return dict(problem=prob, coords=[b.coords() for b in blocks], F0=None, steps=prob.nt(), precision='fp32')
+
+
+WORKLOADS = {'plain': plain, 'rupture': rupture, 'rough': rough, 'plastic': plastic, 'pml12': pml12}
 
 
 def points(prob):
