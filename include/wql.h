@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define WQL_ABI_VERSION 3
+#define WQL_ABI_VERSION 4
 
 enum {
     WQL_OK = 0,
@@ -85,7 +85,24 @@ typedef struct {
     double frc[5];         /* forced rupture (yc, zc, r_crit, Vs, t0) (2625-2660) */
     int rs_solver;         /* slip-rate solve: 0 regula falsi, the reference's iteration step for step
                               (1170-1239); 1 bracketed Newton, |step| <= 1e-12 |V| + 1e-300 */
+    /* ---- ABI 4 ---- */
+    int key;               /* WQL_KEY_*: a slip-weakening problem the reference keys by name whose
+                              prestress, strength, cohesion and forced-rupture time the library
+                              evaluates at every fault node at create (prestress 1802-2435,
+                              Tau_strength 1268-1797, Frictional_Cohesion 2491-2580,
+                              Time_of_Forced_Rupture 2583-2660, the fault-node material of
+                              Interface_RHS 100-260, initial_stress_tensor in
+                              initial_stress_condition.f90:87-240); needs law WQL_SLIP_WEAKENING,
+                              and then no other prestress or strength field is read.  0: none */
 } wql_fault_params;
+
+/* wql_fault_params.key: TPV30 is TPV29's fault, TPV32 is TPV31's with its own
+ * fault-node material */
+enum {
+    WQL_KEY_NONE = 0, WQL_KEY_TPV10 = 1, WQL_KEY_TPV29 = 2, WQL_KEY_TPV31 = 3, WQL_KEY_TPV32 = 4,
+    WQL_KEY_TPV33 = 5, WQL_KEY_TPV34 = 6, WQL_KEY_SCITS2016 = 7, WQL_KEY_TPV36 = 8, WQL_KEY_TPV37 = 9,
+    WQL_KEY_WASATCH100 = 10
+};
 
 enum { WQL_FD_TRADITIONAL = 0, WQL_FD_UPWIND = 1, WQL_FD_UPWIND_DRP = 2 };
 
@@ -97,7 +114,8 @@ typedef struct {
                               2 default model (plastic.f90:221-290) */
     double pl_c, pl_nu, pl_Tv;
     double pl_mu, pl_beta, pl_eta;
-    int s0_kind;           /* plastic background stress: 0 constant fault.s0, 1 TPV26 depth profile */
+    int s0_kind;           /* plastic background stress: 0 constant fault.s0, 1 TPV26/27 depth profile,
+                              2 TPV29/30 depth profile (initial_stress_condition.f90:26-112) */
     wql_fault_params fault;
     /* ---- ABI 3 ---- */
     int fd_type;           /* &problem_list fd_type (domain.f90:106, 135-136): WQL_FD_TRADITIONAL (the
@@ -137,15 +155,19 @@ typedef struct wql_ctx wql_ctx;
 /* Node coordinates of one block, generated on the device: the uniform grid
  * of cartesian_grid_3d (grid.f90:213-240) or the transfinite interpolation
  * of Curve_Grid_3D3 (grid.f90:471-1907; faces Gen_Surf_Mesh 2418-2455,
- * volume Gen3_Curve_Mesh 2290-2416) with planar q faces or the
- * 'analytical_test_problem' bump (775-831), plus optional offset fields of
- * one q face (a fault surface) and of the r = 0 face (topography), the
- * surfaces the reference reads from files through the same interpolation. */
+ * volume Gen3_Curve_Mesh 2290-2416) with planar q faces, the
+ * 'analytical_test_problem' bump (775-831) or the 'analytical_tpv28',
+ * 'analytical_tpv10', 'analytical_tpv36' fault profiles (625-773), plus
+ * optional offset fields of one q face (a fault surface, such as the x column
+ * of a 'read_from_memomry_fractal' file times lc or rc, 832-925) and of the
+ * r = 0 face (topography), the surfaces the reference reads from files
+ * through the same interpolation. */
 typedef struct {
     int nq, nr, ns;
     double a[3], b[3];     /* btp%aqrs, btp%bqrs: corners of the block */
     int mesh;              /* 0 uniform ('cartesian'), 1 transfinite ('curvilinear') */
-    int profile;           /* q faces: 0 planar, 1 'analytical_test_problem' weighted by lc / rc */
+    int profile;           /* q faces: 0 planar, 1 'analytical_test_problem', 2 'analytical_tpv28',
+                              3 'analytical_tpv10', 4 'analytical_tpv36', each weighted by lc / rc */
     double lc, rc;         /* btp%lc, btp%rc */
     const double *fault_x; /* optional [ns][nr] x offsets of the q face fault_side (mesh 1, profile 0) */
     int fault_side;        /* 0: q = 0 face, 1: q = nq-1 face */
@@ -153,6 +175,14 @@ typedef struct {
 } wql_grid_desc;
 /* X[3][ns][nr][nq] (the reference's X(q,r,s,1:3)) */
 int wql_grid_block(const wql_grid_desc *grid, double *X);
+
+/* init_material's media keyed by problem name (material.f90:179-690): for
+ * 'LOH1', 'LOH1_Harmonic', 'TPV31', 'TPV32', 'TPV33' and 'OKLAHOMA' fills
+ * M[3][ns][nr][nq] (lambda, mu, rho, as wql_block_desc.M) from the node
+ * coordinates X[3][ns][nr][nq] and returns 1; returns 0 for any other name
+ * (the homogeneous rho_s_p medium) without touching M.  Host only: no
+ * device call. */
+int wql_problem_material(const char *problem, int nq, int nr, int ns, const double *X, double *M);
 
 /* replaces init_domain's device-relevant part (domain.f90:31-564): uploads
  * geometry/material, computes metrics and face frames on the device, zeroes

@@ -53,9 +53,11 @@ program golden_driver
   if (in_block_comm(2) .and. .not. in_block_comm(1)) ib = 2
 
   call dump_geometry(D, ib, prefix)
-  ! the unrelaxed moduli and Q^-1 of init_anelastic_properties (material.f90:11-145)
+  ! the material of init_material (material.f90:179-690; with the anelastic
+  ! response the unrelaxed moduli of init_anelastic_properties, 11-145, and
+  ! Q^-1)
+  call dump3(D%B(ib)%M%M, D, ib, prefix, 'Mat')
   if (D%B(ib)%M%anelastic) then
-     call dump3(D%B(ib)%M%M, D, ib, prefix, 'Mat')
      call dump3(reshape(D%B(ib)%M%Qs_inv, [shape(D%B(ib)%M%Qs_inv), 1]), D, ib, prefix, 'Qsinv')
   end if
 

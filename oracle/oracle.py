@@ -112,6 +112,9 @@ def _p(a):
 
 
 def fault_struct(f):
+    if getattr(f, 'key', 0):
+        raise NotImplementedError('the oracle does not restate the keyed fault problems (their parity is '
+                                  'against the reference build)')
     s = OraFault()
     s.law, s.stress_mode = f.law, f.stress_mode
     s.T0[:] = f.T0
@@ -188,6 +191,8 @@ class OracleDomain:
                 setattr(I, k, _p(v))
             arr['iface'] = ia
         self.d.fault = fault_struct(prob.fault)
+        if prob.plastic and prob.s0_kind > 1:
+            raise NotImplementedError('the oracle restates the constant and TPV26/27 plastic background stresses')
         self.d.plastic = prob.plastic
         for k in ('pl_c', 'pl_nu', 'pl_Tv', 'pl_mu', 'pl_beta', 'pl_eta', 's0_kind'):
             setattr(self.d, k, getattr(prob, k))

@@ -33,7 +33,8 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
              plastic_model='default', pml_l=((False,) * 3,) * 2, pml_r=((False,) * 3,) * 2,
              npml=0, lc=(1.0, 0.0), rc=(0.0, 1.0), profile='analytical_test_problem',
              rho_s_p=(2.67, 3.464, 6.0), t_final=1.0, aqrs=None, bqrs=None, mu_beta_eta=(0.0, 0.0, 0.0),
-             moment=None, fd_type=None, order=None, anelastic=None):
+             moment=None, fd_type=None, order=None, anelastic=None, profile_path='No-path', fractal=None,
+             keep_mat=False):
     T = lambda b: 'T' if b else 'F'
     s = ["&problem_list",
          f" name = 'data/{name}',", f" problem = '{problem}',", f" response = '{response}',",
@@ -55,7 +56,7 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
               f"btp({i})%lqrs = {lq[b][0]}, {lq[b][1]}, {lq[b][2]},",
               f"btp({i})%rqrs = {rq[b][0]}, {rq[b][1]}, {rq[b][2]},",
               f"btp({i})%lc = {lc[b]}d0,", f"btp({i})%rc = {rc[b]}d0,",
-              f"btp({i})%profile_type = '{profile}',", f"btp({i})%profile_path = 'No-path',",
+              f"btp({i})%profile_type = '{profile}',", f"btp({i})%profile_path = '{profile_path}',",
               f"btp({i})%topography_type = 'none',", f"btp({i})%topography_path = 'No-path',",
               "btp(%d)%%material_path = 'none', 'none', 'none'," % i,
               f"btp({i})%pml_lqrs = {', '.join(T(x) for x in pml_l[b])},",
@@ -199,7 +200,107 @@ CASES = {
                          pml_r=((False, False, True), (True, False, False)), npml=7,
                          fd_type='upwind_drp', order=6, response='anelastic', anelastic=(2.0, 0.6, 1.0)),
                     [('rhs', ['0.1']), ('traj', ['8', '-1.2', '7.4', '0.2', '0.5', '1.0'])]),
+    # fault profiles of the curvilinear grid generator (Curve_Grid_3D3,
+    # grid.f90:625-925): only the node coordinates are kept.  The TPV28 hills
+    # (centres y 7.5, z -10.5 / 10.5, radius 3), the TPV10 60-degree dip with
+    # its rounding below 15 km, the TPV36 15-degree dip with its bend at 8 km,
+    # and a fault surface read from a text file (x column used)
+    # the slip-weakening problems the reference keys by name, whose fault
+    # fields the library tables per node: each grid spans the problem's
+    # barriers, nucleation patch, cohesion depth range and (TPV29/30,
+    # TPV36/37, wasatch100) the forced-rupture disk, with rates at a time
+    # inside the forced weakening of the nodes nearest the centre; TPV31,
+    # TPV32 and TPV33 with init_material's layered media (kept)
+    'tpv10_sw': (dict(problem='TPV10', coupling='slip-weakening_friction', mesh='curvilinear', n=N_STD,
+                      lq=LQ_STD, rq=RQ_STD, profile='analytical_tpv10',
+                      aqrs=[(-4.0, 8.0, -2.0), (0.0, 8.0, -2.0)], bqrs=[(0.0, 14.0, 2.0), (4.0, 14.0, 2.0)]),
+                 [('rhs', ['0.1']), ('traj', ['20'])]),
+    'tpv29_sw': (dict(problem='TPV29', coupling='slip-weakening_friction', mesh='cartesian', n=N_STD,
+                      lq=LQ_STD, rq=RQ_STD,
+                      aqrs=[(-6.0, 2.0, -9.0), (0.0, 2.0, -9.0)], bqrs=[(0.0, 24.0, 3.0), (6.0, 24.0, 3.0)]),
+                 [('rhs', ['0.3']), ('traj', ['30'])]),
+    'tpv31_sw': (dict(problem='TPV31', coupling='slip-weakening_friction', mesh='cartesian', n=N_STD,
+                      lq=LQ_STD, rq=RQ_STD, keep_mat=True,
+                      aqrs=[(-4.0, 0.0, -2.0), (0.0, 0.0, -2.0)], bqrs=[(0.0, 16.0, 17.0), (4.0, 16.0, 17.0)]),
+                 [('rhs', ['0.2']), ('traj', ['20'])]),
+    'tpv32_sw': (dict(problem='TPV32', coupling='slip-weakening_friction', mesh='cartesian', n=N_STD,
+                      lq=LQ_STD, rq=RQ_STD, keep_mat=True,
+                      aqrs=[(-4.0, 0.0, -2.0), (0.0, 0.0, -2.0)], bqrs=[(0.0, 16.0, 17.0), (4.0, 16.0, 17.0)]),
+                 [('rhs', ['0.2']), ('traj', ['20'])]),
+    'tpv33_sw': (dict(problem='TPV33', coupling='slip-weakening_friction', mesh='cartesian', n=N_STD,
+                      lq=LQ_STD, rq=RQ_STD, keep_mat=True,
+                      aqrs=[(-3.0, 0.0, -12.0), (0.0, 0.0, -12.0)], bqrs=[(0.0, 12.0, 3.0), (3.0, 12.0, 3.0)]),
+                 [('rhs', ['0.2']), ('traj', ['20'])]),
+    'tpv34_sw': (dict(problem='TPV34', coupling='slip-weakening_friction', mesh='cartesian', n=N_STD,
+                      lq=LQ_STD, rq=RQ_STD,
+                      aqrs=[(-4.0, 0.0, -2.0), (0.0, 0.0, -2.0)], bqrs=[(0.0, 16.0, 17.0), (4.0, 16.0, 17.0)]),
+                 [('rhs', ['0.2']), ('traj', ['20'])]),
+    'scits_sw': (dict(problem='SCITS2016', coupling='slip-weakening_friction', mesh='cartesian', n=N_STD,
+                      lq=LQ_STD, rq=RQ_STD,
+                      aqrs=[(-3.0, 2.0, 16.0), (0.0, 2.0, 16.0)], bqrs=[(0.0, 8.0, 24.0), (3.0, 8.0, 24.0)]),
+                 [('rhs', ['0.2']), ('traj', ['20'])]),
+    'tpv36_sw': (dict(problem='TPV36', coupling='slip-weakening_friction', mesh='curvilinear', n=N_STD,
+                      lq=LQ_STD, rq=RQ_STD, profile='analytical_tpv36',
+                      aqrs=[(-40.0, 0.0, -5.0), (0.0, 0.0, -5.0)], bqrs=[(0.0, 8.0, 5.0), (40.0, 8.0, 5.0)]),
+                 [('rhs', ['0.4']), ('traj', ['20'])]),
+    'tpv37_sw': (dict(problem='TPV37', coupling='slip-weakening_friction', mesh='curvilinear', n=N_STD,
+                      lq=LQ_STD, rq=RQ_STD, profile='analytical_tpv36',
+                      aqrs=[(-40.0, 0.0, -5.0), (0.0, 0.0, -5.0)], bqrs=[(0.0, 8.0, 5.0), (40.0, 8.0, 5.0)]),
+                 [('rhs', ['0.4'])]),
+    # wasatch100's prestress puts no shear traction on a vertical fault: the
+    # grid spans the shallow depths (y + 2.149 from 0.05 to 6.1 km) where the
+    # pattern stresses exceed the strength, over its fd, Dc and cohesion ramps
+    'wasatch_sw': (dict(problem='wasatch100', coupling='slip-weakening_friction', mesh='cartesian', n=N_STD,
+                        lq=LQ_STD, rq=RQ_STD,
+                        aqrs=[(-4.0, -2.1, 30.0), (0.0, -2.1, 30.0)], bqrs=[(0.0, 4.0, 46.0), (4.0, 4.0, 46.0)]),
+                   [('rhs', ['0.3']), ('traj', ['20'])]),
+    # TPV30's plasticity: the TPV29/30 background stress in the volume
+    'tpv30_plastic': (dict(problem='TPV30', coupling='slip-weakening_friction', mesh='cartesian', n=N_STD,
+                           lq=LQ_STD, rq=RQ_STD, response='plastic', plastic_model='scec',
+                           aqrs=[(-6.0, 2.0, -9.0), (0.0, 2.0, -9.0)], bqrs=[(0.0, 24.0, 3.0), (6.0, 24.0, 3.0)]),
+                      [('traj', ['20', '-1.0', '13.0', '-3.0', '4.0', '2.0'])]),
+    # init_material's layered media on a locked pair with a velocity pulse
+    'loh1_locked': (dict(problem='LOH1', coupling='locked', mesh='cartesian', n=N_STD, lq=LQ_STD, rq=RQ_STD,
+                         keep_mat=True, aqrs=[(-3.0, 0.0, -1.5), (0.0, 0.0, -1.5)],
+                         bqrs=[(0.0, 3.0, 1.5), (3.0, 3.0, 1.5)]),
+                    [('traj', ['12', '1.0', '1.5', '0.0', '0.5', '1.0'])]),
+    'oklahoma_locked': (dict(problem='OKLAHOMA', coupling='locked', mesh='cartesian', n=N_STD, lq=LQ_STD,
+                             rq=RQ_STD, keep_mat=True, aqrs=[(-3.0, 0.0, -1.5), (0.0, 0.0, -1.5)],
+                             bqrs=[(0.0, 3.0, 1.5), (9.0, 3.0, 1.5)]),
+                        [('traj', ['12', '2.0', '1.5', '0.0', '0.6', '1.0'])]),
+    'grid_tpv28': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 16, 28),
+                        lq=LQ_STD, rq=RQ_STD, profile='analytical_tpv28',
+                        aqrs=[(-6.0, 0.0, -13.5), (0.0, 0.0, -13.5)], bqrs=[(0.0, 15.0, 13.5), (6.0, 15.0, 13.5)]),
+                   [('rhs', ['0.1'])]),
+    'grid_tpv10': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 21, 13),
+                        lq=LQ_STD, rq=RQ_STD, profile='analytical_tpv10',
+                        aqrs=[(-12.0, 0.0, -6.0), (0.0, 0.0, -6.0)], bqrs=[(0.0, 20.0, 6.0), (12.0, 20.0, 6.0)]),
+                   [('rhs', ['0.1'])]),
+    'grid_tpv36': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 25, 13),
+                        lq=LQ_STD, rq=RQ_STD, profile='analytical_tpv36',
+                        aqrs=[(-40.0, 0.0, -6.0), (0.0, 0.0, -6.0)], bqrs=[(0.0, 12.0, 6.0), (40.0, 12.0, 6.0)]),
+                   [('rhs', ['0.1'])]),
+    'grid_fractal': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 14, 15),
+                          lq=LQ_STD, rq=RQ_STD, profile='read_from_memomry_fractal', profile_path='fault.dat',
+                          fractal=dict(seed=11, amp=0.04)),
+                     [('rhs', ['0.1'])]),
 }
+
+
+def write_fractal(work, kw):
+    """the fault-profile text file of a grid case: my*mz rows 'x y z', y
+    fastest; smooth random x offsets"""
+    nq, nr, ns = kw['n']
+    rng = np.random.default_rng(kw['fractal']['seed'])
+    w = rng.standard_normal((ns, nr))
+    k0 = np.fft.fftfreq(ns)[:, None]
+    k1 = np.fft.fftfreq(nr)[None, :]
+    x = np.real(np.fft.ifft2(np.fft.fft2(w) * np.exp(-(k0 ** 2 + k1 ** 2) * 60.0)))
+    x *= kw['fractal']['amp'] / max(np.abs(x).max(), 1e-300)
+    rows = [f'{x[k, j]:.17e} {float(j):.1f} {float(k):.1f}' for k in range(ns) for j in range(nr)]
+    with open(os.path.join(work, kw['profile_path']), 'w') as f:
+        f.write('\n'.join(rows) + '\n')
+    return x.T.copy()  # [nr][ns]
 
 
 def read_block(prefix, b, name, shape):
@@ -211,6 +312,7 @@ def run_case(name, kw, runs, work):
     inp = os.path.join(work, f'{name}.in')
     with open(inp, 'w') as f:
         f.write(namelist(name, **kw))
+    fractal = write_fractal(work, kw) if kw.get('fractal') else None
     for d in ('data', 'seismogram/block1/data', 'seismogram/block2/data'):
         os.makedirs(os.path.join(work, d), exist_ok=True)
     nq, nr, ns = kw['n']
@@ -225,8 +327,10 @@ def run_case(name, kw, runs, work):
         vol = lambda b, nm, nf: read_block(prefix, b, nm, (nq, nr, ns, nf)).transpose(3, 2, 1, 0)
         face = lambda b, nm, nf: read_block(prefix, b, nm, (nr, ns, nf)).transpose(2, 1, 0)
         for b in (1, 2):
-            if os.path.exists(f'{prefix}_b{b}_Mat.bin'):
+            if os.path.exists(f'{prefix}_b{b}_Qsinv.bin') or (os.path.exists(f'{prefix}_b{b}_Mat.bin') and
+                                                              kw.get('keep_mat')):
                 out[f'b{b}_Mat'] = vol(b, 'Mat', 3)
+            if os.path.exists(f'{prefix}_b{b}_Qsinv.bin'):
                 out[f'b{b}_Qsinv'] = vol(b, 'Qsinv', 1)
             for tag in ('Deta', 'eta'):
                 if os.path.exists(f'{prefix}_b{b}_{tag}4.bin'):
@@ -272,6 +376,18 @@ def run_case(name, kw, runs, work):
                         a = np.fromfile(f'{prefix}_b{b}_nml{side}.bin', '<f8')
                         nml.append(a)
                     out[f'b{b}_nml'] = np.concatenate(nml)
+    if name.startswith('grid_'):
+        for b in (1, 2):
+            out[f'b{b}_X'] = vol(b, 'X', 3)
+        out = {k: v for k, v in out.items() if k.endswith('_X') or k == 'n'}
+        if fractal is not None:
+            out['profile_x'] = fractal
+    for k in list(out):
+        if k.endswith('_Mat') and not kw.get('response') == 'anelastic':
+            a = out[k]
+            flat = a.reshape(a.shape[0], -1)
+            if (flat == flat[:, :1]).all():
+                out[k] = flat[:, 0].copy()
     if kw.get('response') == 'anelastic':
         # the physics under test is pointwise: keep the rates with all terms
         # (DFfull), the memory variables and the state; drop what the other

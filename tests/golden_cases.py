@@ -101,3 +101,35 @@ def pulse(prob, X, args):
 def rel_l2(a, b):
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+GRID_CASES = ['grid_tpv28', 'grid_tpv10', 'grid_tpv36', 'grid_fractal']
+
+
+def grid_problem(case, tmpdir):
+    """the Problem of a fault-profile grid case; the profile-file case gets its
+    file (the stored x column, rows 'x y z' with y fastest) written beside a
+    copy of its input file in tmpdir"""
+    src = os.path.join(GOLDEN, 'inputs', f'{case}.in')
+    z = load(case)
+    if 'profile_x' not in z:
+        return P.from_namelist(src)
+    x = z['profile_x']
+    nr, ns = x.shape
+    with open(os.path.join(tmpdir, 'fault.dat'), 'w') as f:
+        for k in range(ns):
+            for j in range(nr):
+                f.write(f'{x[j, k]:.17e} {float(j):.1f} {float(k):.1f}\n')
+    dst = os.path.join(tmpdir, f'{case}.in')
+    with open(src) as a, open(dst, 'w') as b:
+        b.write(a.read())
+    return P.from_namelist(dst)
+
+
+# the slip-weakening problems the reference keys by name (per-node fault
+# tables) and init_material's layered media
+KEY_RHS_CASES = ['tpv10_sw', 'tpv29_sw', 'tpv31_sw', 'tpv32_sw', 'tpv33_sw', 'tpv34_sw', 'scits_sw', 'tpv36_sw',
+                 'tpv37_sw', 'wasatch_sw']
+KEY_TRAJ_CASES = ['tpv10_sw', 'tpv29_sw', 'tpv31_sw', 'tpv32_sw', 'tpv33_sw', 'tpv34_sw', 'scits_sw', 'tpv36_sw',
+                  'wasatch_sw', 'tpv30_plastic', 'loh1_locked', 'oklahoma_locked']
+MEDIUM_CASES = ['tpv31_sw', 'tpv32_sw', 'tpv33_sw', 'loh1_locked', 'oklahoma_locked']

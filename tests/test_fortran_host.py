@@ -203,3 +203,29 @@ def test_fortran_run_matches_python_host(prec):
     assert max_f > 0
     assert S.max() == pytest.approx(max_f, rel=1e-12)
     assert S.sum() == pytest.approx(sum_f, rel=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not _have(MAIN), reason='Fortran host not built (no flang)')
+@pytest.mark.parametrize('case', G.GRID_CASES)
+def test_fortran_main_fault_profile_grids(case, tmp_path):
+    """the Fortran host's grids of the TPV28, TPV10, TPV36 and file fault
+    profiles (wql_grid_block) against the reference build's"""
+    _say(f'[gpu] fortran grid {case}')
+    z = G.load(case)
+    prob = G.grid_problem(case, str(tmp_path))
+    inp = os.path.join(str(tmp_path), f'{case}.in')
+    if not os.path.exists(inp):
+        inp = os.path.join(G.GOLDEN, 'inputs', f'{case}.in')
+    dump = os.path.join(str(tmp_path), 'grid.bin')
+    r = subprocess.run([MAIN, inp, 'grid', dump], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    a = np.fromfile(dump, '<f8')
+    o = 0
+    for ib, b in enumerate(prob.blocks):
+        nq, nr, ns = b.n
+        X = a[o:o + 3 * nq * nr * ns].reshape((nq, nr, ns, 3), order='F').transpose(3, 2, 1, 0)
+        o += 3 * nq * nr * ns
+        ref = z[f'b{ib + 1}_X']
+        assert np.abs(X - ref).max() <= 1e-13 * np.abs(ref).max(), ib
+    assert o == a.size

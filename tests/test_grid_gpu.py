@@ -1,8 +1,10 @@
 """Block grids generated on the device (wql_grid_block) against the host
 formulation (wql/geometry.py, which reproduces the reference build's grid of
 curv_sw to 1e-14, tests/test_oracle_golden.py): the uniform grid, the
-'analytical_test_problem' bump profile of both golden blocks, and planar
-faces with a fault-surface and a free-surface offset field."""
+'analytical_test_problem' bump profile of both golden blocks, planar faces
+with a fault-surface and a free-surface offset field; and against the
+reference build's grids of the TPV28, TPV10 and TPV36 fault profiles and of a
+fault surface read from a file."""
 import sys
 
 import numpy as np
@@ -43,3 +45,15 @@ def test_offset_fields():
         X = device_grid(b)
         ref = b.coords()
         assert np.abs(X - ref).max() <= 1e-14 * np.abs(ref).max(), side
+
+
+@pytest.mark.parametrize('case', G.GRID_CASES)
+def test_fault_profile_grids(case, tmp_path):
+    from wql.solver import device_grid
+    _say(f'[gpu] grid {case}')
+    z = G.load(case)
+    prob = G.grid_problem(case, str(tmp_path))
+    for ib, b in enumerate(prob.blocks):
+        X = device_grid(b)
+        ref = z[f'b{ib + 1}_X']
+        assert np.abs(X - ref).max() <= 1e-13 * np.abs(ref).max(), ib

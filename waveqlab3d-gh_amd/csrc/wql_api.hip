@@ -801,6 +801,7 @@ void reset_typed(wql_ctx *c)
 
 int valid_problem(const wql_problem *prob);
 int sbp_index(const wql_problem *prob);
+void fault_tables(wql_ctx *c);
 
 int ctx_reset(wql_ctx *c, const wql_problem *prob)
 {
@@ -808,6 +809,8 @@ int ctx_reset(wql_ctx *c, const wql_problem *prob)
     if (prob->precision != c->prec || prob->plastic != c->prob.plastic || prob->s0_kind != c->prob.s0_kind)
         return fail(WQL_E_ARG, "a context is reused only for the same precision and plasticity model");
     if (sbp_index(prob) != c->sbp) return fail(WQL_E_ARG, "a context is reused only for the same spatial operator");
+    if (prob->fault.key != c->prob.fault.key)
+        return fail(WQL_E_ARG, "a context is reused only for the same keyed fault problem");
     if (prob->anelastic != c->prob.anelastic ||
         (prob->anelastic && (prob->an_c != c->prob.an_c || prob->an_weight_exp != c->prob.an_weight_exp ||
                              prob->an_fref != c->prob.an_fref)))
@@ -823,6 +826,7 @@ int ctx_reset(wql_ctx *c, const wql_problem *prob)
     c->vol_launches = 0;
     if (c->prec == WQL_FP64) reset_typed<double>(c);
     else reset_typed<float>(c);
+    fault_tables(c);
     return check_launch();
 }
 
@@ -981,7 +985,7 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
                                                                           d.frame[side], d.fgm[side], d.fmat[side],
                                                                           fault_side ? d.fx : nullptr, d_err);
         }
-        if (c->prob.plastic && c->prob.s0_kind == 1) {
+        if (c->prob.plastic && c->prob.s0_kind >= 1) {
             if (int rc = dalloc(c, (void **)&d.Y, NP * sizeof(double))) return rc;
             k_pad<double><<<(unsigned)((NP + 255) / 256), 256, 0, c->stream>>>(1, d.nq, d.nr, d.ns, d.px, X + ND, d.Y);
         }
@@ -1047,11 +1051,28 @@ int create_typed(wql_ctx *c, const wql_block_desc blocks[2])
             double v = (k == 4) ? 1.0e9 : 0.0;  // trup = 1e9 (iface.f90:41)
             k_fill<double><<<(unsigned)((nf[k] * FP + 255) / 256), 256, 0, c->stream>>>(nf[k] * FP, *arrs[k], v);
         }
+        if (c->fault.key)
+            if (int rc = dalloc(c, (void **)&I.ftab, 8 * FP * sizeof(double))) return rc;
     }
+    fault_tables(c);
     if (int rc = dalloc(c, (void **)&c->d_flag, sizeof(int))) return rc;
     HIPCHK(hipMemset(c->d_flag, 0, sizeof(int)));
     HIPCHK(hipStreamSynchronize(c->stream));
     return check_launch();
+}
+
+// the per-node fault fields of a keyed problem (k_fault_tables), from each
+// block's fault-face coordinates, frame and material
+void fault_tables(wql_ctx *c)
+{
+    if (!c->fault.key) return;
+    for (int ib = 0; ib < 2; ib++) {
+        const DevBlock &d = c->b[ib];
+        const int side = ib == 0 ? 1 : 0;
+        const long FP = (long)d.nr * d.ns;
+        k_fault_tables<<<(unsigned)((FP + 127) / 128), 128, 0, c->stream>>>(c->fault.key, FP, d.fx, d.frame[side],
+                                                                          d.fmat[side], c->I[ib].ftab);
+    }
 }
 
 long face_nodes(const DevBlock &d, int side)
@@ -1213,10 +1234,12 @@ int valid_problem(const wql_problem *prob)
         f.nuc < 0 || f.nuc > 3 || f.rs_law < 0 || f.rs_law > 2 || f.rs_init < 0 || f.rs_init > 1 ||
         f.sw_law < 0 || f.sw_law > 1 || f.rs_solver < 0 || f.rs_solver > 1)
         return fail(WQL_E_ARG, "bad fault description");
-    if (f.law == WQL_SLIP_WEAKENING && !(f.Dc > 0)) return fail(WQL_E_ARG, "slip weakening needs Dc > 0");
+    if (f.key < 0 || f.key > WQL_KEY_WASATCH100) return fail(WQL_E_ARG, "bad fault key %d", f.key);
+    if (f.key && f.law != WQL_SLIP_WEAKENING) return fail(WQL_E_ARG, "a keyed fault problem is slip weakening");
+    if (f.law == WQL_SLIP_WEAKENING && !f.key && !(f.Dc > 0)) return fail(WQL_E_ARG, "slip weakening needs Dc > 0");
     if (f.law == WQL_RATE_STATE && !(f.rs_b > 0 && f.rs_L > 0 && f.rs_V0 > 0 && f.rs_a > 0))
         return fail(WQL_E_ARG, "rate and state needs a, b, L, V0 > 0");
-    if (prob->plastic < 0 || prob->plastic > 2 || prob->s0_kind < 0 || prob->s0_kind > 1)
+    if (prob->plastic < 0 || prob->plastic > 2 || prob->s0_kind < 0 || prob->s0_kind > 2)
         return fail(WQL_E_ARG, "bad plasticity description");
     return 0;
 }
@@ -1818,6 +1841,19 @@ int wql_get_fault_rates(wql_ctx *c, int ib, double *DS, double *DW)
     return 0;
 }
 
+// the slip-weakening problems with per-node fault fields (wql_fault_params.key)
+int fault_key(const std::string &pk)
+{
+    static const struct { const char *name; int key; } keys[] = {
+        {"TPV10", WQL_KEY_TPV10}, {"TPV29", WQL_KEY_TPV29}, {"TPV30", WQL_KEY_TPV29}, {"TPV31", WQL_KEY_TPV31},
+        {"TPV32", WQL_KEY_TPV32}, {"TPV33", WQL_KEY_TPV33}, {"TPV34", WQL_KEY_TPV34},
+        {"SCITS2016", WQL_KEY_SCITS2016}, {"TPV36", WQL_KEY_TPV36}, {"TPV37", WQL_KEY_TPV37},
+        {"wasatch100", WQL_KEY_WASATCH100}};
+    for (const auto &k : keys)
+        if (pk == k.name) return k.key;
+    return WQL_KEY_NONE;
+}
+
 int wql_problem_keys(const char *problem, const char *coupling, const char *response, const char *plastic_model,
                      const double mu_beta_eta[3], wql_problem *out)
 {
@@ -1867,6 +1903,10 @@ int wql_problem_keys(const char *problem, const char *coupling, const char *resp
         f.sw_law = 1;
         f.coh[0] = 0.4; f.coh[1] = 0.72; f.coh[2] = 5.0;
         f.frc[0] = 10.0; f.frc[1] = -5.0; f.frc[2] = 4.0; f.frc[3] = 3.464; f.frc[4] = 0.5;
+    } else if (ck == "slip-weakening_friction" && fault_key(pk) != WQL_KEY_NONE) {
+        // the keyed problems whose fault fields k_fault_tables evaluates per node
+        f.law = WQL_SLIP_WEAKENING;
+        f.key = fault_key(pk);
     } else if (ck == "rate-and-state_friction" && (pk == "TPV101" || pk == "TPV102")) {
         // 386-400, 433-436, 2324-2385, 2756-2819, initial_stress_condition.f90:114-120
         f.law = WQL_RATE_STATE;
@@ -1904,6 +1944,8 @@ int wql_problem_keys(const char *problem, const char *coupling, const char *resp
         // tensor whatever the coupling (initial_stress_condition.f90:8-254)
         if (pk == "TPV26" || pk == "TPV27") {
             p.s0_kind = 1;
+        } else if (pk == "TPV29" || pk == "TPV30") {
+            p.s0_kind = 2;
         } else if (pk == "TPV28") {
             const double s0[6] = {-60.0, 0.0, 60.0, 0.0, 29.38, 0.0};
             memcpy(f.s0, s0, sizeof s0);
@@ -1934,11 +1976,34 @@ int wql_problem_keys(const char *problem, const char *coupling, const char *resp
     return 0;
 }
 
+int wql_problem_material(const char *problem, int nq, int nr, int ns, const double *X, double *M)
+{
+    if (!problem) return fail(WQL_E_ARG, "null argument");
+    static const struct { const char *name; int model; } media[] = {
+        {"LOH1", WQL_MEDIUM_LOH1}, {"LOH1_Harmonic", WQL_MEDIUM_LOH1_HARMONIC}, {"TPV31", WQL_MEDIUM_TPV31},
+        {"TPV32", WQL_MEDIUM_TPV32}, {"TPV33", WQL_MEDIUM_TPV33}, {"OKLAHOMA", WQL_MEDIUM_OKLAHOMA}};
+    int model = WQL_MEDIUM_NONE;
+    for (const auto &m : media)
+        if (strcmp(problem, m.name) == 0) model = m.model;
+    if (model == WQL_MEDIUM_NONE) return 0;
+    if (!X || !M || nq < 1 || nr < 1 || ns < 1) return fail(WQL_E_ARG, "bad argument");
+    const long N = (long)nq * nr * ns;
+    for (long p = 0; p < N; p++) {
+        double vp, vs, rho;
+        if (!layered_medium(model, X[p], X[N + p], vp, vs, rho))
+            return fail(WQL_E_ARG, "node %ld: coordinate outside every layer of the %s medium", p, problem);
+        M[p] = rho * (vp * vp - 2.0 * (vs * vs));  // lambda (material.f90: rho (Vp**2 - 2 Vs**2))
+        M[N + p] = rho * (vs * vs);                // mu
+        M[2 * N + p] = rho;
+    }
+    return 1;
+}
+
 int wql_grid_block(const wql_grid_desc *gd, double *X)
 {
     if (!gd || !X) return fail(WQL_E_ARG, "null argument");
     if (gd->nq < 2 || gd->nr < 2 || gd->ns < 2) return fail(WQL_E_ARG, "a grid needs two nodes per direction");
-    if (gd->mesh < 0 || gd->mesh > 1 || gd->profile < 0 || gd->profile > 1 || gd->fault_side < 0 ||
+    if (gd->mesh < 0 || gd->mesh > 1 || gd->profile < 0 || gd->profile > 4 || gd->fault_side < 0 ||
         gd->fault_side > 1)
         return fail(WQL_E_ARG, "bad grid description");
     const long N = (long)gd->nq * gd->nr * gd->ns;

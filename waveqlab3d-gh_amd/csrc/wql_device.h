@@ -46,6 +46,7 @@ struct FaultParams {
     double coh[3];
     double frc[5];
     int rs_solver;           // 0 regula falsi (reference), 1 bracketed Newton
+    int key;                 // keyed problem whose fault fields are per-node tables (0: none)
 };
 
 // One moment-tensor point source, identical in meaning to wql_source.
@@ -131,6 +132,10 @@ struct Iface {
     double *Svel;            // [3][nr*ns]
     double *Uhat, *Vhat;     // [6][nr*ns]
     double *psr;             // [nr*ns] peak tangential slip rate over the run (stage-1 samples)
+    // keyed problems: [8][nr*ns] static prestress T0_n, T0_m, T0_l, strength
+    // fs, fd, Dc, cohesion C0 (barriers included) and the time of forced
+    // rupture of this block's fault face (null otherwise)
+    double *ftab;
 };
 
 __device__ __forceinline__ long pidx(int nq, int nr, int i, int j, int k)

@@ -1,8 +1,9 @@
 // wql_grid.hip -- block grids on the device (part of libwql; included by
 // wql_api.hip): the uniform grid of cartesian_grid_3d (grid.f90:213-240) and
 // the transfinite-interpolation grid of Curve_Grid_3D3 (grid.f90:471-1907)
-// for planar or bump-profiled q faces ('analytical_test_problem',
-// 775-831), with optional caller-supplied fault-face and free-surface offset
+// for planar or profiled q faces ('analytical_test_problem', 775-831;
+// 'analytical_tpv28', 'analytical_tpv10', 'analytical_tpv36', 625-773),
+// with optional caller-supplied fault-face and free-surface offset
 // fields (the surfaces the reference reads from files through the same
 // interpolation, 832-1430).
 //
@@ -37,7 +38,8 @@ struct GridArgs {
     int nq, nr, ns;
     double a[3], b[3];
     double lc, rc;
-    int profile;              // 0 planar q faces, 1 'analytical_test_problem'
+    int profile;              // 0 planar q faces, 1 'analytical_test_problem', 2 'analytical_tpv28',
+                              // 3 'analytical_tpv10', 4 'analytical_tpv36'
     const double *fault_x;    // [ns][nr] x offsets of one q face (or null)
     int fault_side;           // 0: q = 0 face, 1: q = nq-1 face
     const double *top_y;      // [ns][nq] y offsets of the r = 0 face (or null)
@@ -67,6 +69,36 @@ __global__ void k_grid_qfaces(GridArgs g)
     const double Z = (1.0 - q1) * zk + q1 * zk + (1.0 - r1) * az + r1 * bz - (1.0 - q1) * (1.0 - r1) * z0 -
                      q1 * (1.0 - r1) * z0 - r1 * (1.0 - q1) * bz - (r1 * q1) * bz;
     double XL, XR;
+    if (g.profile >= 2) {
+        // the formula profiles set the face nodes' y and z directly
+        // (grid.f90:636-639) and add rc f, lc f to the planar faces; the
+        // dipping ones evaluate f when either weight is positive
+        const double PI = 3.141592653589793;
+        const double y = yj, z = zk;
+        double f = 0.0;
+        if (g.profile == 2) {  // TPV28 hills (627-656)
+            const double r1 = sqrt((z + 10.5) * (z + 10.5) + (y - 7.5) * (y - 7.5));
+            const double r2 = sqrt((z - 10.5) * (z - 10.5) + (y - 7.5) * (y - 7.5));
+            if (r1 <= 3.0) f = -0.3 * (1.0 + cos(PI * r1 / 3.0));
+            if (r2 <= 3.0) f = -0.3 * (1.0 + cos(PI * r2 / 3.0));
+        } else if (g.rc > 0.0 || g.lc > 0.0) {
+            if (g.profile == 3) {  // TPV10, 60 degrees (658-713)
+                const double ang = 60.0 / 180.0 * PI;
+                if (y <= 15.0) f = (1.0 / tan(ang)) * (y - 0.5 * (ay + by));
+                else f = (1.0 / tan(ang)) * (15.0 - 0.5 * (ay + by)) + 0.5 * atan(4.0 * (y - 15.0)) * exp(-5.0 * (y - 15.0));
+            } else {  // TPV36, 15 degrees (715-773)
+                const double ang = 15.0 / 180.0 * PI;
+                if (y <= 8.0) f = 1.0 / tan(ang) * y - 15.0;
+                else f = 1.0 / tan(ang) * 8.0 - 15.0 + 0.25 * atan(4.0 * (y - 8.0)) * exp(-0.0 * (y - 8.0));
+            }
+        }
+        const long FP = (long)my * mz;
+        g.qf[t] = y;
+        g.qf[FP + t] = z;
+        g.qf[2 * FP + t] = g.a[0] + g.rc * f;
+        g.qf[3 * FP + t] = g.b[0] + g.lc * f;
+        return;
+    }
     if (g.profile == 1) {
         const double PI = 3.141592653589793;
         const double cy = 0.5 * (ay + by), cz = 0.5 * (az + bz);

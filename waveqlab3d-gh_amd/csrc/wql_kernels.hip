@@ -15,6 +15,7 @@
 //                    update (1782-2201)
 //   k_plastic        last stage only: viscoplastic return map (plastic.f90)
 #include "wql_device.h"
+#include "wql_models.h"
 
 #include <math.h>
 
@@ -1949,9 +1950,31 @@ __device__ void tpv26_stress(double y, double s0[6])
     s0[5] = 0.0;
 }
 
-__device__ void prestress(const FaultParams &f, const double x[3], double t, const double l[3], const double m[3],
-                          const double n[3], double &Tn, double &Tm, double &Tl)
+// TPV29/30 background stress at depth y (initial_stress_condition.f90:87-112):
+// the TPV26 form with the taper between 17 and 22 km and its own ratios
+__device__ void tpv29_stress(double y, double s0[6])
 {
+    double Omega = 0.0;
+    if (y <= 17.0) Omega = 1.0;
+    else if (y >= 17.0 && y <= 22.0) Omega = (22.0 - y) / 5.0;
+    else if (y >= 22.0) Omega = 0.0;
+    const double b11 = 1.025837, b33 = 0.974162, b13 = -0.158649;
+    const double Pf = 9.8 * y, s22 = -(2.67 * 9.8 * y);
+    s0[0] = Omega * (b33 * (s22 + Pf) - Pf) + (1.0 - Omega) * s22 + Pf;
+    s0[1] = s22 + Pf;
+    s0[2] = Omega * (b11 * (s22 + Pf) - Pf) + (1.0 - Omega) * s22 + Pf;
+    s0[3] = 0.0;
+    s0[4] = Omega * (b13 * (s22 + Pf));
+    s0[5] = 0.0;
+}
+
+__device__ void prestress(const FaultParams &f, const double x[3], double t, const double l[3], const double m[3],
+                          const double n[3], double &Tn, double &Tm, double &Tl, const double *kt = nullptr)
+{
+    if (f.key) {  // a keyed problem: the node's static prestress, tabled at create
+        Tn = kt[0]; Tm = kt[1]; Tl = kt[2];
+        return;
+    }
     if (f.stress_mode == 0) {
         Tn = f.T0[0]; Tm = f.T0[1]; Tl = f.T0[2];
         for (int k = 0; k < f.npatch; k++)
@@ -2003,8 +2026,30 @@ __device__ __forceinline__ double rs_a_at(const FaultParams &f, const double x[3
 }
 
 // shear strength of the slip-weakening laws (Tau_strength, 1268-1797)
-__device__ double tau_strength(const FaultParams &f, double sigma_n, double D, const double x[3], double t)
+__device__ double tau_strength(const FaultParams &f, double sigma_n, double D, const double x[3], double t,
+                               const double *kt = nullptr)
 {
+    if (f.key) {
+        // a keyed problem: fs, fd, Dc and C0 (barriers included) and the
+        // forced-rupture time of the node, tabled at create.  TPV10, TPV31-34
+        // and SCITS2016 weaken linearly in slip; TPV29/30, TPV36/37 and
+        // wasatch100 also by forced rupture over t0 = 0.5 s.  sigma_n >= 0
+        // here, so the reference's max(sigma_n, 0) is sigma_n.
+        const double fs = kt[3], fd = kt[4], Dc = kt[5], C0 = kt[6], tf = kt[7];
+        double fr;
+        if (f.key == WQL_KEY_TPV29 || f.key == WQL_KEY_TPV36 || f.key == WQL_KEY_TPV37 || f.key == WQL_KEY_WASATCH100) {
+            const double t0 = 0.5;
+            const double f1 = fmin(D / Dc, 1.0);
+            double f2;
+            if (t < tf) f2 = 0.0;
+            else if (t < tf + t0) f2 = (t - tf) / t0;
+            else f2 = 1.0;
+            fr = fs - (fs - fd) * fmax(f1, f2);
+        } else {
+            fr = fs - (fs - fd) * fmin(D, Dc) / Dc;
+        }
+        return C0 + fr * sigma_n;
+    }
     if (f.sw_law == 1) {  // TPV26/27 (1376-1458)
         const double yc = f.frc[0], zc = f.frc[1], r_crit = f.frc[2], Vs = f.frc[3], t0 = f.frc[4];
         const double dy = x[1] - yc, dz = x[2] - zc;
@@ -2031,6 +2076,209 @@ __device__ double tau_strength(const FaultParams &f, double sigma_n, double D, c
     const double fs = f.fs + f.bar_big * (1.0 - boxcar(x[1], f.bar_wy) * boxcar(x[2], f.bar_wz));
     const double fr = fs - (fs - f.fd) * fmin(D, f.Dc) / f.Dc;
     return sigma_n * fr;
+}
+
+// ---------------------------------------------------------------------------
+// keyed slip-weakening problems (wql_fault_params.key): the static fault
+// fields of one block's fault face, once at create, from the node's
+// coordinates x (x[1] depth, x[2] along strike), its frame (n, m, l) and the
+// block's material there.  Every expression is the reference's, in its
+// operation order (FP64, no contraction).
+
+// Frictional_Cohesion (Interface_Condition.f90:2491-2580) at depth d
+__device__ __forceinline__ double key_cohesion(int key, double d)
+{
+    switch (key) {
+    case WQL_KEY_TPV29: case WQL_KEY_WASATCH100:
+        return d <= 4.0 ? 0.4 + 0.2 * (4.0 - d) : 0.4;
+    case WQL_KEY_TPV31: case WQL_KEY_TPV32: case WQL_KEY_TPV34: case WQL_KEY_SCITS2016:
+        return d <= 2.4 ? 0.425 * (2.4 - d) : 0.0;
+    case WQL_KEY_TPV36:
+        return d <= 8.0 ? 0.5 * (8.0 - d) : 0.0;
+    case WQL_KEY_TPV37:
+        return d <= 8.0 ? 1.875 * (8.0 - d) : 0.0;
+    default:
+        return 0.0;
+    }
+}
+
+// Time_of_Forced_Rupture (2583-2660): r_crit 4 km; thr and big are the
+// per-problem cut-off offset and "never" time
+__device__ __forceinline__ double key_forced_rupture(double r, double Vs, double thr, double big)
+{
+    const double r_crit = 4.0;
+    if (r < r_crit - thr) {
+        const double q = r / r_crit;
+        return r / (0.7 * Vs) + 0.081 * r_crit / (0.7 * Vs) * (1.0 / (1.0 - q * q) - 1.0);
+    }
+    return big;
+}
+
+// tractions of a stress tensor on the face frame (prestress 1802-2435)
+__device__ __forceinline__ void key_tractions(const double s[6], const double n[3], const double m[3],
+                                              const double l[3], double &Tn, double &Tm, double &Tl)
+{
+    const double Tx = n[0] * s[0] + n[1] * s[3] + n[2] * s[4];
+    const double Ty = n[0] * s[3] + n[1] * s[1] + n[2] * s[5];
+    const double Tz = n[0] * s[4] + n[1] * s[5] + n[2] * s[2];
+    Tn = n[0] * Tx + n[1] * Ty + n[2] * Tz;
+    Tm = m[0] * Tx + m[1] * Ty + m[2] * Tz;
+    Tl = l[0] * Tx + l[1] * Ty + l[2] * Tz;
+}
+
+// cosine-tapered nucleation disk of prestress (TPV31/32/34, TPV33,
+// SCITS2016): a for r <= r1, ha (1 + cos(pi (r - r1) / w)) for r1 <= r <= r2,
+// both times s (the shear modulus over u0; 1 for TPV33), as
+// (a s) / u0 -- the reference's order -- when s_num is given
+__device__ __forceinline__ double key_nucleation(double r, double r1, double w, double r2, double a, double ha,
+                                                 double mu, double u0)
+{
+    const double pi = 3.141592653589793;
+    if (r <= r1) return u0 > 0.0 ? a * mu / u0 : a;
+    if (r >= r1 && r <= r2) {
+        const double v = ha * (1.0 + cos(pi * (r - r1) / w));
+        return u0 > 0.0 ? v * mu / u0 : v;
+    }
+    return 0.0;
+}
+
+__global__ void k_fault_tables(int key, long FP, const double *__restrict__ fx, const double *__restrict__ frame,
+                               const double *__restrict__ fmat, double *__restrict__ tab)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= FP) return;
+    const double pi = 3.141592653589793;
+    double x[3], n[3], m[3], l[3];
+    for (int c = 0; c < 3; c++) {
+        x[c] = fx[c * FP + t];
+        n[c] = frame[c * FP + t]; m[c] = frame[(3 + c) * FP + t]; l[c] = frame[(6 + c) * FP + t];
+    }
+    double Tn = 0.0, Tm = 0.0, Tl = 0.0, fs = 0.0, fd = 0.0, Dc = 1.0, C0 = 0.0, tf = 1.0e9;
+    double s0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double u0 = 32.03812032;
+    switch (key) {
+    case WQL_KEY_TPV10: {  // 1591-1610, 1882-1912: 60-degree normal fault
+        const double B = boxcar(x[1] * (2.0 / sqrt(3.0)), 15.0) * boxcar(x[2] - 0.0, 15.0);
+        fs = 0.760 + (1.0e4 - 0.760) * (1.0 - B);
+        fd = 0.448; Dc = 0.50;
+        C0 = 0.2 + 1.0e3 * (1.0 - B);
+        Tn = -(7.378 * x[1]) * (2.0 / sqrt(3.0));
+        Tm = -(0.55 * Tn);
+        Tl = 0.0;
+        Tm = Tm + (-((0.760 + 0.0057) * Tn) + 0.2 - Tm) * boxcar(x[1] * (2.0 / sqrt(3.0)) - 12.0, 1.5) *
+                      boxcar(x[2] - 0.0, 1.5);
+        break;
+    }
+    case WQL_KEY_TPV29: {  // 1477-1518, 2081-2101; initial_stress_condition.f90:87-112
+        fs = 0.18; fd = 0.12; Dc = 0.3;
+        const double Vs = 3.464;
+        const double dy = x[1] - 10.0, dz = x[2] + 5.0;
+        const double r = sqrt(dy * dy + dz * dz);
+        C0 = key_cohesion(key, x[1]) + 1.0e10 * (1.0 - boxcar(x[1], 20.0) * boxcar(x[2], 20.0));
+        tf = key_forced_rupture(r, Vs, 1.0e-9, 1.0e9);
+        tpv29_stress(x[1], s0);
+        key_tractions(s0, n, m, l, Tn, Tm, Tl);
+        break;
+    }
+    case WQL_KEY_TPV31: case WQL_KEY_TPV32: case WQL_KEY_TPV34: case WQL_KEY_SCITS2016: {
+        // 1520-1590 and 2102-2135, 2169-2240; the shear modulus of the fault
+        // node: the TPV31 / TPV32 depth profile (Interface_RHS 133-254), the
+        // block's own for TPV34 and SCITS2016 (255-280)
+        double mu;
+        if (key == WQL_KEY_TPV31 || key == WQL_KEY_TPV32) {
+            double vp = 0.0, vs = 0.0, rho = 0.0;
+            layered_medium(key == WQL_KEY_TPV31 ? WQL_MEDIUM_TPV31 : WQL_MEDIUM_TPV32, x[0], x[1], vp, vs, rho);
+            mu = rho * (vs * vs);
+        } else {
+            mu = fmat[FP + t];
+        }
+        fd = 0.45; Dc = 0.18;
+        if (key == WQL_KEY_TPV34) {
+            fs = 0.58 + 1.0e10 * (1.0 - boxcar(x[0], 15.0) * boxcar(x[1], 15.0));
+            C0 = key_cohesion(key, x[0]);
+        } else if (key == WQL_KEY_SCITS2016) {
+            fs = 0.58 + 1.0e10 * (1.0 - boxcar(x[1] - 5.0, 2.0) * boxcar(x[2] - 20.0, 3.0));
+            fd = 0.35; Dc = 0.09;
+            C0 = key_cohesion(key, x[1]);
+        } else {
+            fs = 0.58 + 1.0e10 * (1.0 - boxcar(x[1], 15.0) * boxcar(x[2], 15.0));
+            C0 = key_cohesion(key, x[1]);
+        }
+        s0[0] = -(60.0 * mu) / u0;
+        s0[2] = -(60.0 * mu) / u0;
+        s0[4] = 30.0 * mu / u0;
+        key_tractions(s0, n, m, l, Tn, Tm, Tl);
+        if (key == WQL_KEY_SCITS2016) {
+            const double dy = x[1] - 4.5, dz = x[2] - 20.0;
+            const double r = sqrt(dy * dy + dz * dz);
+            Tl = Tl + key_nucleation(r, 1.4, 0.6, 2.0, 14.95, 7.475, mu, u0);
+        } else {
+            const double dy = x[1] - 7.5, dz = x[2];
+            const double r = sqrt(dy * dy + dz * dz);
+            Tl = Tl + key_nucleation(r, 1.4, 0.6, 2.0, 4.95, 2.475, mu, u0);
+        }
+        break;
+    }
+    case WQL_KEY_TPV33: {  // 1538-1553, 2136-2168; initial_stress_condition.f90:133-158
+        fs = 0.55 + 1.0e10 * (1.0 - boxcar(x[0], 10.0) * boxcar(x[1] + 4.0, 8.0));
+        fd = 0.45; Dc = 0.18; C0 = 0.0;
+        double Rx, Ry;
+        if (x[2] < -9.8) Rx = (-x[2] - 9.8) / 10.0;
+        else if (x[2] > 1.1) Rx = (x[2] - 1.1) / 10.0;
+        else Rx = 0.0;
+        if (x[1] < 2.3) Ry = (-x[1] + 2.3) / 10.0;
+        else if (x[1] > 8.0) Ry = (x[1] - 8.0) / 10.0;
+        else Ry = 0.0;
+        const double Rt = fmin(1.0, sqrt(Rx * Rx + Ry * Ry));
+        s0[0] = -60.0;
+        s0[4] = 30.0 * (1.0 - Rt);
+        key_tractions(s0, n, m, l, Tn, Tm, Tl);
+        const double dy = x[1] - 6.0, dz = x[2] + 6.0;
+        const double r = sqrt(dy * dy + dz * dz);
+        Tl = Tl + key_nucleation(r, 0.55, 0.25, 0.8, 3.15, 1.575, 1.0, 0.0);
+        break;
+    }
+    case WQL_KEY_TPV36: case WQL_KEY_TPV37: {  // 1612-1690, 1914-1920: 15-degree thrust
+        fs = 0.575; fd = 0.450; Dc = 0.180;
+        const double Vs = 3.464;
+        const double sd = sin(15.0 / 180.0 * pi);
+        const double d = x[1] / sd;
+        const double r = sqrt((d - 18.0) * (d - 18.0) + x[2] * x[2]);
+        C0 = key_cohesion(key, d) + 1.0e10 * (1.0 - boxcar(d, 28.0) * boxcar(x[2], 15.0));
+        tf = key_forced_rupture(r, Vs, 10.0e-10, 10.0e10);
+        Tn = -(4.24 * x[1]) / sd;
+        Tm = -(2.12 * x[1]) / sd;
+        Tl = 0.0;
+        break;
+    }
+    case WQL_KEY_WASATCH100: {  // 1692-1742, 1922-1990; initial_stress_condition.f90:222-244
+        const double d = x[1] + 2.149;
+        fs = 0.6;
+        if (d <= 4.0) fd = 0.44 + (0.6 - 0.44) * 0.25 * (4.0 - d);
+        else if (d > 4.0 && d <= 17.0) fd = 0.44;
+        else if (d > 17.0 && d <= 18.0) fd = 0.44 + (0.6 - 0.44) * (d - 17.0);
+        else if (d > 18.0) fd = 0.6;
+        if (d <= 4.0) Dc = 0.3 + 0.2 * (4.0 - d);
+        else if (d >= 4.0) Dc = 0.3;
+        const double Vs = 3.464;
+        const double a = 1.3054 * (d - 14.0), b = x[2] - 38.0;
+        const double r = sqrt(a * a + b * b);
+        const double B = boxcar(d, 18.0) * boxcar(x[2] - 38.0, 18.6);
+        fs = fs + 1.0e10 * (1.0 - B);
+        C0 = key_cohesion(key, d) + 1.0e10 * (1.0 - B);
+        tf = key_forced_rupture(r, Vs, 1.0e-9, 1.0e9);
+        const double Pf = 9.8 * (x[1] + 2.149), s22 = -(2.7 * 9.8) * (x[1] + 2.149);
+        s0[0] = s22 + Pf - 0.4 * s22;
+        s0[1] = s22 + Pf;
+        s0[2] = (s0[0] + s0[1]) * 0.5;
+        key_tractions(s0, n, m, l, Tn, Tm, Tl);
+        break;
+    }
+    default:
+        break;
+    }
+    const double v[8] = {Tn, Tm, Tl, fs, fd, Dc, C0, tf};
+    for (int c = 0; c < 8; c++) tab[c * FP + t] = v[c];
 }
 
 __device__ __forceinline__ double rs_f(double V, double Phi, double eta, double sn, double psi, double V0, double a)
@@ -2106,7 +2354,7 @@ __device__ double newton_rs(double V, double Phi, double eta, double sn, double 
 __device__ int hat_vars(const FaultParams &f, const double u[6], const double v[6], const double m1[3],
                         const double m2[3], const double x[3], const double nn[3], const double mm[3],
                         const double ll[3], double slip, double psi, double t, double uo[6], double vo[6],
-                        double &Vel, double T0[3])
+                        double &Vel, double T0[3], const double *kt)
 {
     T0[0] = T0[1] = T0[2] = 0.0;
     double c1p = sqrt((2.0 * m1[1] + m1[0]) / m1[2]), c1s = sqrt(m1[1] / m1[2]);
@@ -2133,7 +2381,7 @@ __device__ int hat_vars(const FaultParams &f, const double u[6], const double v[
     double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
     double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
     double T0n, T0m, T0l;
-    prestress(f, x, t, ll, mm, nn, T0n, T0m, T0l);
+    prestress(f, x, t, ll, mm, nn, T0n, T0m, T0l, kt);
     T0[0] = T0n; T0[1] = T0m; T0[2] = T0l;
     double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
     double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
@@ -2144,7 +2392,7 @@ __device__ int hat_vars(const FaultParams &f, const double u[6], const double v[
     double vv_m, vv_l, Tu_m, Tu_l, Tv_m, Tv_l;
     if (f.law == 1) {
         double tau_lock = sqrt((T0l + phi_l) * (T0l + phi_l) + (T0m + phi_m) * (T0m + phi_m));
-        double tau_str = tau_strength(f, sigma_n, slip, x, t);
+        double tau_str = tau_strength(f, sigma_n, slip, x, t, kt);
         if (tau_lock > tau_str) {
             double phi_1 = T0m + phi_m, phi_2 = T0l + phi_l;
             double Phi = sqrt(phi_1 * phi_1 + phi_2 * phi_2);
@@ -2306,7 +2554,10 @@ __device__ int fault_node(const Pair<real> &P, const Iface &I, const FaultParams
     }
     double slip = S0[3];
     double uh[6], vh[6], Vel, T0[3];
-    int rc = hat_vars(fp, uL, vL, mat1, mat2, x, nn, mm, ll, slip, psi, t, uh, vh, Vel, T0);
+    double kt[8];
+    if (fp.key)
+        for (int c = 0; c < 8; c++) kt[c] = I.ftab[c * FP + fpi];
+    int rc = hat_vars(fp, uL, vL, mat1, mat2, x, nn, mm, ll, slip, psi, t, uh, vh, Vel, T0, kt);
     if (rc) return rc;
     // fault state rates (Interface_Condition.f90:431-436, 470-476) and RK update
     double slip_rate = sqrt((vh[1] - uh[1]) * (vh[1] - uh[1]) + (vh[2] - uh[2]) * (vh[2] - uh[2]));
@@ -2601,6 +2852,8 @@ __global__ void k_plastic(Pair<real> P, PlasticArgs a)
     double s0[6];
     if (a.s0_kind == 1) {
         tpv26_stress(B.Y[p], s0);
+    } else if (a.s0_kind == 2) {
+        tpv29_stress(B.Y[p], s0);
     } else {
         for (int c = 0; c < 6; c++) s0[c] = a.s0[c];
     }

@@ -26,6 +26,7 @@ module wql_domain
   type :: block_type
      integer :: nq = 0, nr = 0, ns = 0
      real(kind = wp), allocatable :: X(:,:,:,:)   ! X(q, r, s, 1:3)
+     real(kind = wp), allocatable :: M(:,:,:,:)   ! M(q, r, s, 1:3) of a keyed medium (material.f90:179-690)
   end type block_type
 
   !> one block's stations (seismogram_type, datatypes.f90): the located
@@ -87,6 +88,7 @@ contains
     type(wql_grid_desc) :: g
     integer :: ib, k
     real(kind = wp) :: dt(2)
+    real(kind = wp), allocatable, target :: fx(:,:)
 
     call read_input(ifname, D%p)
     D%name = D%p%name; D%problem = D%p%problem; D%coupling = D%p%coupling; D%response = D%p%response
@@ -112,12 +114,39 @@ contains
          else
             error stop 'init_domain: type_of_mesh must be cartesian or curvilinear'
          end if
+         ! fault profiles of Curve_Grid_3D3 (grid.f90:625-925); other names
+         ! keep the q faces planar, as the reference does
          g%profile = 0
-         if (trim(b%profile_type) == 'analytical_test_problem') g%profile = 1
-         if (g%mesh == 1 .and. g%profile == 0 .and. trim(b%profile_type) /= 'none' .and. &
-             trim(b%profile_type) /= 'flat') error stop 'init_domain: fault profile not supported by this host'
+         g%fault_x = c_null_ptr
+         select case (trim(b%profile_type))
+         case ('analytical_test_problem')
+            g%profile = 1
+         case ('analytical_tpv28')
+            g%profile = 2
+         case ('analytical_tpv10')
+            g%profile = 3
+         case ('analytical_tpv36')
+            g%profile = 4
+         case ('read_from_memomry_fractal')
+            ! the x column of the profile file times the face weight, as an
+            ! offset field of the weighted q face (read_2darray, grid.f90:2457)
+            if (g%mesh == 1 .and. (b%lc /= 0.0_wp .or. b%rc /= 0.0_wp)) then
+               if (b%lc /= 0.0_wp .and. b%rc /= 0.0_wp) &
+                    error stop 'init_domain: a fault profile file on both q faces of a block is not supported'
+               call read_fault_profile(trim(b%profile_path), g%nr, g%ns, fx)
+               if (b%rc /= 0.0_wp) then
+                  fx = b%rc * fx
+                  g%fault_side = 0
+               else
+                  fx = b%lc * fx
+                  g%fault_side = 1
+               end if
+               g%fault_x = c_loc(fx)
+            end if
+         end select
          if (trim(b%topography_type) /= 'none') error stop 'init_domain: topography from files is not read by this host'
          call wql_check(wql_grid_block(g, D%B(ib)%X), 'wql_grid_block')
+         if (allocated(fx)) deallocate(fx)
          D%desc(ib)%nq = g%nq; D%desc(ib)%nr = g%nr; D%desc(ib)%ns = g%ns
          ! boundary codes Lx Rx Ly Ry Lz Rz and PML sides (block.f90:27-269)
          D%desc(ib)%bc = [b%lqrs(1), b%rqrs(1), b%lqrs(2), b%rqrs(2), b%lqrs(3), b%rqrs(3)]
@@ -129,6 +158,15 @@ contains
          D%desc(ib)%cartesian = merge(1, 0, g%mesh == 0)
          D%desc(ib)%X = c_loc(D%B(ib)%X)
          D%desc(ib)%M = c_null_ptr
+         ! the medium init_material keys by problem name, if any
+         allocate(D%B(ib)%M(g%nq, g%nr, g%ns, 3))
+         k = wql_problem_material(c_string(D%problem), g%nq, g%nr, g%ns, D%B(ib)%X, D%B(ib)%M)
+         if (k < 0) call wql_check(k, 'wql_problem_material')
+         if (k == 1) then
+            D%desc(ib)%M = c_loc(D%B(ib)%M)
+         else
+            deallocate(D%B(ib)%M)
+         end if
          D%desc(ib)%rho_s_p = b%rho_s_p
        end associate
     end do
@@ -379,5 +417,26 @@ contains
     write(D%fault_units(6)) real(Uh(:, :, 1:3), 4)
     write(D%fault_units(7)) real(tr, 4)
   end subroutine write_fault_stage1
+
+  !> x offsets fx(nr, ns) of a fault profile file: nr*ns rows 'x y z', y
+  !> fastest (read_2darray / reshape_2darray, grid.f90:2457-2505)
+  subroutine read_fault_profile(path, nr, ns, fx)
+    character(*), intent(in) :: path
+    integer(c_int), intent(in) :: nr, ns
+    real(kind = wp), allocatable, intent(out) :: fx(:,:)
+    integer :: u, stat, j, k
+    real(kind = wp) :: row(3)
+    allocate(fx(nr, ns))
+    open(newunit = u, file = path, status = 'old', action = 'read', iostat = stat)
+    if (stat /= 0) error stop 'init_domain: cannot open the fault profile file'
+    do k = 1, ns
+       do j = 1, nr
+          read(u, *, iostat = stat) row
+          if (stat /= 0) error stop 'init_domain: short fault profile file'
+          fx(j, k) = row(1)
+       end do
+    end do
+    close(u)
+  end subroutine read_fault_profile
 
 end module wql_domain

@@ -43,6 +43,7 @@ module wql_gpu
      integer(c_int) :: sw_law = 0           ! 0 linear slip weakening, 1 TPV26/27
      real(c_double) :: coh(3) = 0, frc(5) = [0d0, 0d0, 1d0, 1d0, 1d0]
      integer(c_int) :: rs_solver = 0        ! 0 regula falsi, 1 bracketed Newton
+     integer(c_int) :: key = 0              ! WQL_KEY_*: per-node fault fields of a keyed problem
   end type wql_fault_params
 
   type, bind(C) :: wql_problem
@@ -63,7 +64,8 @@ module wql_gpu
      integer(c_int) :: nq = 0, nr = 0, ns = 0
      real(c_double) :: a(3) = 0, b(3) = 0
      integer(c_int) :: mesh = 0              ! 0 uniform, 1 transfinite interpolation
-     integer(c_int) :: profile = 0           ! 0 planar q faces, 1 'analytical_test_problem'
+     integer(c_int) :: profile = 0           ! 0 planar q faces, 1 'analytical_test_problem',
+                                             ! 2 'analytical_tpv28', 3 'analytical_tpv10', 4 'analytical_tpv36'
      real(c_double) :: lc = 0, rc = 0
      type(c_ptr) :: fault_x = c_null_ptr     ! optional fault_x(nr, ns)
      integer(c_int) :: fault_side = 0
@@ -107,6 +109,13 @@ module wql_gpu
      type(c_ptr) function wql_last_error() bind(C, name='wql_last_error')
        import :: c_ptr
      end function wql_last_error
+     integer(c_int) function wql_problem_material(problem, nq, nr, ns, X, M) bind(C, name='wql_problem_material')
+       import :: c_int, c_char, c_double
+       character(kind=c_char), intent(in) :: problem(*)
+       integer(c_int), value :: nq, nr, ns
+       real(c_double), intent(in) :: X(*)
+       real(c_double), intent(out) :: M(*)
+     end function wql_problem_material
      integer(c_int) function wql_abi_version() bind(C, name='wql_abi_version')
        import :: c_int
      end function wql_abi_version

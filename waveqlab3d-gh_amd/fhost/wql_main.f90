@@ -18,12 +18,13 @@ program wql_main
   character(len=1024) :: ifname, arg, dump
   integer :: n, nsteps, narg, stat, ib, u, k
   integer(c_int) :: prec
+  logical :: grid_only = .false.
   integer(8) :: c0, c1, rate
   type(problem_params) :: p
   real(kind = wp), allocatable :: F(:,:,:,:), S(:,:,:), W(:,:,:), tr(:,:,:), Sv(:,:,:)
 
   narg = command_argument_count()
-  if (narg < 1) stop 'usage: wql_main input.in [nsteps] [fp32] [dump.bin]'
+  if (narg < 1) stop 'usage: wql_main input.in [nsteps] [fp32] [dump.bin] | check | grid dump.bin'
   call get_command_argument(1, ifname, status = stat)
   if (stat /= 0) stop ':: Problem reading input filename from commandline.'
   nsteps = 0
@@ -44,6 +45,8 @@ program wql_main
                 trim(p%btp(ib)%profile_type)
         end do
         stop
+     else if (trim(arg) == 'grid') then
+        grid_only = .true.
      else if (trim(arg) == 'fp32') then
         prec = WQL_FP32
      else if (index(arg, '.bin') > 0) then
@@ -55,6 +58,17 @@ program wql_main
 
   call init_RK(RK)
   call init_domain(D, trim(ifname), prec)
+  if (grid_only) then
+     ! the node coordinates X(q, r, s, 1:3) of both blocks, then stop
+     if (len_trim(dump) == 0) stop 'usage: wql_main input.in grid dump.bin'
+     open(newunit = u, file = trim(dump), access = 'stream', form = 'unformatted', status = 'replace')
+     do ib = 1, 2
+        write(u) D%B(ib)%X
+     end do
+     close(u)
+     call close_domain(D)
+     stop
+  end if
   if (nsteps > 0) D%nt = nsteps
   call system_clock(c0, rate)
   do n = 1, D%nt

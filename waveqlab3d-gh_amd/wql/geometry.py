@@ -6,7 +6,11 @@ The coordinates are the input of the metric computation done on the device.
 * ``cartesian`` restates cartesian_grid_3d (grid.f90:213-240).
 * ``curvilinear`` restates the transfinite-interpolation generator
   Curve_Grid_3D3 (grid.f90:471-1907) for the cases the hot path needs: the
-  'analytical_test_problem' fault profile (grid.f90:775-831), plus
+  'analytical_test_problem' fault profile (grid.f90:775-831), the
+  'analytical_tpv28' hills, the 'analytical_tpv10' 60-degree and
+  'analytical_tpv36' 15-degree dipping faults (grid.f90:625-773), the
+  'read_from_memomry_fractal' fault surface read from a text file
+  (grid.f90:832-925, read_2darray 2457-2481), plus
   caller-supplied fault-surface and free-surface perturbation fields (the
   seeded topography / rough-fault inputs of the benchmark workloads, which the
   reference reads from files through the same TFI; grid.f90:1349-1430).
@@ -78,13 +82,54 @@ def _tfi3(n, Xleft, Xright, Xtop, Xbottom, Xfront, Xback):
     return U + V + W - UV - UW - VW + UVW1 + UVW2
 
 
+# fault-surface profiles of Curve_Grid_3D3 given by a formula: x offset f(y)
+# or f(y, z) of the q faces, weighted by rc (left face) and lc (right face)
+ANALYTICAL_PROFILES = ('analytical_tpv28', 'analytical_tpv10', 'analytical_tpv36')
+
+
+def profile_offset(profile, Y, Z, ay, by, use):
+    """f of an analytical fault profile at face nodes (Y, Z); `use` False
+    gives 0 (the dipping profiles evaluate f only on a face whose weight is
+    positive)"""
+    f = np.zeros_like(Y)
+    if profile == 'analytical_tpv28':  # grid.f90:627-656: two cosine hills
+        r1 = np.sqrt((Z + 10.5) ** 2 + (Y - 7.5) ** 2)
+        r2 = np.sqrt((Z - 10.5) ** 2 + (Y - 7.5) ** 2)
+        f = np.where(r1 <= 3.0, -0.3 * (1.0 + np.cos(PI * r1 / 3.0)), f)
+        f = np.where(r2 <= 3.0, -0.3 * (1.0 + np.cos(PI * r2 / 3.0)), f)
+    elif profile == 'analytical_tpv10' and use:  # 658-713: 60 degrees, rounded below 15 km
+        ang = 60.0 / 180.0 * PI
+        deep = (1.0 / np.tan(ang)) * (15.0 - 0.5 * (ay + by)) + \
+            0.5 * np.arctan(4.0 * (Y - 15.0)) * np.exp(-5.0 * (Y - 15.0))
+        f = np.where(Y <= 15.0, (1.0 / np.tan(ang)) * (Y - 0.5 * (ay + by)), deep)
+    elif profile == 'analytical_tpv36' and use:  # 715-773: 15 degrees to 8 km depth
+        ang = 15.0 / 180.0 * PI
+        deep = 1.0 / np.tan(ang) * 8.0 - 15.0 + 0.25 * np.arctan(4.0 * (Y - 8.0)) * np.exp(-0.0 * (Y - 8.0))
+        f = np.where(Y <= 8.0, 1.0 / np.tan(ang) * Y - 15.0, deep)
+    return f
+
+
+def read_fault_profile(path, my, mz):
+    """x offsets [my][mz] of a 'read_from_memomry_fractal' fault surface: a
+    text file of my*mz rows 'x y z', y fastest (read_2darray and
+    reshape_2darray, grid.f90:2457-2505); only x is used (grid.f90:832-925)"""
+    d = np.loadtxt(path, dtype=np.float64, ndmin=2)
+    if d.shape[0] < my * mz or d.shape[1] < 3:
+        raise ValueError(f'fault profile {path}: {d.shape[0]} rows of {d.shape[1]} values, '
+                         f'{my * mz} rows of 3 needed')
+    return d[:my * mz, 0].reshape(mz, my).T.copy()
+
+
 def curvilinear(n, a, b, lc=0.0, rc=0.0, profile='analytical_test_problem',
-                fault_left=None, fault_right=None, top=None):
+                fault_left=None, fault_right=None, top=None, profile_path=None):
     """Curvilinear block with n=(nq,nr,ns) nodes over [a, b].
 
     profile 'analytical_test_problem' adds the Gaussian-plus-sinusoid bump of
     grid.f90:812-827 to the q-faces, weighted by rc (left face) and lc (right
-    face).  profile 'flat' keeps the q-faces planar; then ``fault_left`` /
+    face); 'analytical_tpv28', 'analytical_tpv10' and 'analytical_tpv36' add
+    their formulas the same way, 'read_from_memomry_fractal' the x column of
+    the file `profile_path`.  Any other profile keeps the q-faces planar (as
+    the reference does for names it does not know); then ``fault_left`` /
     ``fault_right`` ([nr][ns] arrays of x offsets) perturb the q-faces and
     ``top`` ([nq][ns] array of y offsets) perturbs the r = 1 face.
     """
@@ -106,6 +151,24 @@ def curvilinear(n, a, b, lc=0.0, rc=0.0, profile='analytical_test_problem',
         Xright = (bx + 0.1 * lc * np.exp(-(1.0 * (Yright - 0.5 * (ay + by)) ** 2
                                            + 1.0 * (Zright - 0.5 * (az + bz)) ** 2) / 0.025)
                   + 0.1 * lc * np.sin(2.0 / (by - ay) * PI * Yright) * np.sin(2.0 / (bz - az) * PI * Zright))
+    elif profile in ANALYTICAL_PROFILES:
+        # these branches set the face nodes' y and z directly (grid.f90:636-639)
+        Yleft = np.broadcast_to(jy[:, None], (my, mz)).copy()
+        Zleft = np.broadcast_to(kz[None, :], (my, mz)).copy()
+        Yright, Zright = Yleft.copy(), Zleft.copy()
+        # one f for both faces: the dipping profiles evaluate it when either
+        # weight is positive (from Yleft or Yright, which are equal)
+        f = profile_offset(profile, Yleft, Zleft, ay, by, rc > 0.0 or lc > 0.0)
+        Xleft = ax + rc * f
+        Xright = bx + lc * f
+    elif profile == 'read_from_memomry_fractal':
+        X1 = read_fault_profile(profile_path, my, mz)
+        Xleft = np.full((my, mz), float(ax))
+        Xright = np.full((my, mz), float(bx))
+        if rc != 0.0:
+            Xleft = Xleft + rc * X1
+        if lc != 0.0:
+            Xright = Xright + lc * X1
     else:
         Xleft = np.full((my, mz), float(ax))
         Xright = np.full((my, mz), float(bx))

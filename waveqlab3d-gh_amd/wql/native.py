@@ -24,7 +24,7 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_server_submit', 'wql_server_poll', 'wql_server_result', 'wql_server_status', 'wql_server_set_policy', 'wql_set_tile', 'wql_get_tile', 'wql_set_tile_order', 'wql_get_tile_order', 'wql_tile_variants', 'wql_use_graph', 'wql_profile',
            'wql_volume_kernel_ms', 'wql_get_fault_faces', 'wql_get_plastic',
            'wql_set_anelastic', 'wql_get_anelastic',
-           'wql_get_fields_noalloc', 'wql_problem_keys', 'wql_grid_block',
+           'wql_get_fields_noalloc', 'wql_problem_keys', 'wql_grid_block', 'wql_problem_material',
            'wql_stage', 'wql_server_pool_stats', 'wql_server_set_shape', 'wql_find_stations',
            'wql_set_stations', 'wql_get_stations', 'wql_locate_sources', 'wql_set_sources']
 
@@ -44,7 +44,12 @@ class FaultParams(C.Structure):
                 ('rs_Vin', C.c_double),
                 ('rs_law', C.c_int), ('rs_fw', C.c_double), ('rs_Vw', C.c_double), ('rs_dVw', C.c_double),
                 ('rs_init', C.c_int), ('rs_W0', C.c_double), ('sw_law', C.c_int), ('coh', C.c_double * 3),
-                ('frc', C.c_double * 5), ('rs_solver', C.c_int)]
+                ('frc', C.c_double * 5), ('rs_solver', C.c_int), ('key', C.c_int)]
+
+
+# wql_fault_params.key: slip-weakening problems with per-node fault fields
+FAULT_KEYS = {'TPV10': 1, 'TPV29': 2, 'TPV30': 2, 'TPV31': 3, 'TPV32': 4, 'TPV33': 5, 'TPV34': 6,
+              'SCITS2016': 7, 'TPV36': 8, 'TPV37': 9, 'wasatch100': 10}
 
 
 class GridDesc(C.Structure):
@@ -103,6 +108,7 @@ def lib():
             'wql_set_anelastic': ([vp, C.c_int, dp], C.c_int),
             'wql_get_anelastic': ([vp, C.c_int, C.c_int, dp], C.c_int),
             'wql_get_fields_noalloc': ([vp, C.c_int, dp], C.c_int),
+            'wql_problem_material': ([C.c_char_p, C.c_int, C.c_int, C.c_int, dp, dp], C.c_int),
             'wql_problem_keys': ([C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, dp, C.POINTER(ProblemDesc)],
                                  C.c_int),
             'wql_grid_block': ([C.POINTER(GridDesc), dp], C.c_int),

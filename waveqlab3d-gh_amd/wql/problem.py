@@ -7,6 +7,7 @@ plus the fault law parameters that the reference keys by problem name
 input file; ``fault_params`` turns a problem key into explicit parameters.
 """
 import math
+import os
 import re
 from dataclasses import dataclass, field
 from typing import List, Optional, Tuple
@@ -31,6 +32,7 @@ class Block:
     rc: float = 0.0
     mesh: str = 'cartesian'                                    # 'cartesian' | 'curvilinear'
     profile: str = 'analytical_test_problem'
+    profile_path: Optional[str] = None                         # 'read_from_memomry_fractal' profile file
     fault_offset: Optional[np.ndarray] = None                  # [nr][ns] x-offsets of the fault face
     top_offset: Optional[np.ndarray] = None                    # [nq][ns] y-offsets of the r=1 face
     material: Optional[np.ndarray] = None                      # [3][ns][nr][nq] lambda, mu, rho (heterogeneous)
@@ -45,7 +47,8 @@ class Block:
             else:
                 fl = self.fault_offset
         return geometry.curvilinear(self.n, self.a, self.b, self.lc, self.rc,
-                                    self.profile, fault_left=fl, fault_right=fr, top=self.top_offset)
+                                    self.profile, fault_left=fl, fault_right=fr, top=self.top_offset,
+                                    profile_path=self.profile_path)
 
     def lame(self):
         """M = (lambda, mu, rho) on every node (material.f90:181-188)."""
@@ -97,12 +100,22 @@ class Fault:
     coh: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     frc: Tuple[float, ...] = (0.0, 0.0, 1.0, 1.0, 1.0)
     rs_solver: int = 0         # 0 regula falsi (the reference's), 1 bracketed Newton
+    # a slip-weakening problem the reference keys by name (native.FAULT_KEYS:
+    # TPV10, TPV29/30, TPV31-34, SCITS2016, TPV36/37, wasatch100) whose
+    # prestress, strength, cohesion and forced-rupture time the library
+    # evaluates at every fault node; 0: the fields above describe the fault
+    key: int = 0
 
 
 def fault_params(problem: str, coupling: str) -> Fault:
     """Explicit parameters for the reference's problem keys."""
+    from .native import FAULT_KEYS
     if coupling == 'locked':
         return Fault(law=LOCKED)
+    if coupling == 'slip-weakening_friction' and problem in FAULT_KEYS:
+        # prestress 1802-2435, Tau_strength 1268-1797, Frictional_Cohesion
+        # 2491-2580, Time_of_Forced_Rupture 2583-2660, evaluated per node
+        return Fault(law=SLIP_WEAKENING, key=FAULT_KEYS[problem])
     if coupling == 'slip-weakening_friction':
         if problem == 'TPV5':   # Interface_Condition.f90:1283-1299, 1822-1830
             return Fault(law=SLIP_WEAKENING, stress_mode=0, T0=(-120.0, 0.0, 70.0),
@@ -161,7 +174,7 @@ class Problem:
     pl_mu: float = 0.0
     pl_beta: float = 0.0
     pl_eta: float = 0.0
-    s0_kind: int = 0           # plastic background stress: 0 = fault.s0 everywhere, 1 = TPV26 depth profile
+    s0_kind: int = 0           # plastic background stress: 0 = fault.s0 everywhere, 1 = TPV26/27, 2 = TPV29/30 depth profile
     name: str = 'default'
     w_stride: int = 1          # fault output every w_stride steps (time_step.f90:137)
     w_fault: bool = False      # write the fault output files (fault_output.f90)
@@ -297,11 +310,63 @@ def read_namelist(text, group):
     return out
 
 
+def problem_material(problem, block):
+    """lambda, mu, rho [3][ns][nr][nq] of init_material's medium for this
+    problem name at the block's nodes (material.f90:179-690: 'LOH1',
+    'LOH1_Harmonic', 'TPV31', 'TPV32', 'TPV33', 'OKLAHOMA'), or None for the
+    homogeneous rho_s_p medium of every other problem"""
+    from . import native
+    nq, nr, ns = block.n
+    X = np.ascontiguousarray(block.coords(), dtype=np.float64)
+    M = np.empty((3, ns, nr, nq))
+    dp = native.dp
+    rc = native.lib().wql_problem_material(problem.encode(), nq, nr, ns, X.ctypes.data_as(dp), M.ctypes.data_as(dp))
+    if rc < 0:
+        native.check(rc)
+    return M if rc == 1 else None
+
+
+def _data_path(p, base):
+    """a file named in an input file: as given when it exists (the reference
+    opens it from the run directory), else relative to the input file's
+    directory; None for the reference's placeholders"""
+    if not p or p.strip().lower() in ('no-path', 'no_path', 'none'):
+        return None
+    p = p.strip()
+    if os.path.isabs(p) or os.path.exists(p):
+        return p
+    return os.path.join(base, p)
+
+
+def block_list_file(text, base):
+    """the file named by '&block_list = 'file'' in an input file, relative to
+    the input file's directory unless absolute (locate_block_list_file,
+    input_utils.f90:9-94), or None"""
+    for line in text.splitlines():
+        line = line.split('!', 1)[0].strip()
+        m = re.match(r'&block_list\s*=\s*([\'"])(.*?)\1', line, re.I)
+        if m:
+            return m.group(2) if os.path.isabs(m.group(2)) else os.path.join(base, m.group(2))
+        if re.match(r'&block_list\b', line, re.I):
+            return None
+    return None
+
+
 def from_namelist(path):
-    """Build a Problem from a reference input file (&problem_list, &block_list)."""
+    """Build a Problem from a reference input file (&problem_list, &block_list;
+    the block list may sit in its own file, domain.f90:165-211)."""
     text = open(path).read()
+    base = os.path.dirname(os.path.abspath(path))
     pl = read_namelist(text, 'problem_list')
-    bl = read_namelist(text, 'block_list')
+    blf = block_list_file(text, base)
+    if blf is not None:
+        try:
+            btext = open(blf).read()
+        except OSError:
+            raise ValueError(f'error opening block list file {blf}') from None
+        bl = read_namelist(btext, 'block_list')
+    else:
+        bl = read_namelist(text, 'block_list')
     problem = pl.get('problem', 'TPV5')
     coupling = pl.get('coupling', 'locked')
     mesh = pl.get('type_of_mesh', 'cartesian')
@@ -317,7 +382,10 @@ def from_namelist(path):
             bc=(lq[0], rq[0], lq[1], rq[1], lq[2], rq[2]),
             pml=(pml_l[0], pml_r[0], pml_l[1], pml_r[1], pml_l[2], pml_r[2]),
             npml=int(g('npml', 0) or 0), lc=float(g('lc', 0.0) or 0.0), rc=float(g('rc', 0.0) or 0.0),
-            mesh=mesh, profile=g('profile_type', 'analytical_test_problem')))
+            mesh=mesh, profile=g('profile_type', 'analytical_test_problem'),
+            profile_path=_data_path(g('profile_path', None), base)))
+    for b in blocks:
+        b.material = problem_material(problem, b)
     response = pl.get('response', 'elastic')
     model = pl.get('plastic_model', 'default')
     plastic = 0
@@ -339,7 +407,7 @@ def from_namelist(path):
         prob.pl_mu, prob.pl_beta, prob.pl_eta = mbe[0]
     if plastic:
         f = fault_params(problem, 'slip-weakening_friction') if problem == 'TPV28' else prob.fault
-        prob.s0_kind = 1 if problem in ('TPV26', 'TPV27') else 0
+        prob.s0_kind = 1 if problem in ('TPV26', 'TPV27') else 2 if problem in ('TPV29', 'TPV30') else 0
         if problem == 'TPV30':
             prob.pl_c, prob.pl_nu, prob.pl_Tv = 1.18, 0.1680, 0.05
         if prob.fault.stress_mode == 0 and problem == 'TPV28':

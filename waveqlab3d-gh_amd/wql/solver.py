@@ -8,7 +8,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import native
+from . import geometry, native
 from .native import check, dp
 
 
@@ -32,6 +32,7 @@ def fault_struct(f):
         setattr(s, k, getattr(f, k))
     s.coh[:] = f.coh
     s.frc[:] = f.frc
+    s.key = f.key
     return s
 
 
@@ -314,6 +315,10 @@ class Simulation:
         return native.lib().wql_volume_kernel_ms(self.ctx)
 
 
+# wql_grid_desc.profile of the reference's profile_type names (others: planar)
+PROFILE_CODES = {'analytical_test_problem': 1, 'analytical_tpv28': 2, 'analytical_tpv10': 3, 'analytical_tpv36': 4}
+
+
 def device_grid(block):
     """node coordinates X[3][ns][nr][nq] of a Block generated on the device
     by wql_grid_block (the same grid as block.coords())"""
@@ -322,14 +327,24 @@ def device_grid(block):
     g.a[:] = block.a
     g.b[:] = block.b
     g.mesh = 0 if block.mesh == 'cartesian' else 1
-    g.profile = 1 if block.profile == 'analytical_test_problem' else 0
+    g.profile = PROFILE_CODES.get(block.profile, 0)
     g.lc, g.rc = block.lc, block.rc
     keep = []
+    off, side = None, 0
     if block.fault_offset is not None:
-        fx = np.ascontiguousarray(np.asarray(block.fault_offset, dtype=np.float64).T)  # [ns][nr]
+        off, side = block.fault_offset, 1 if block.bc[1] == 0 else 0
+    if block.mesh != 'cartesian' and block.profile == 'read_from_memomry_fractal':
+        # the file's x column times the face weight, as an offset field
+        if block.lc != 0.0 and block.rc != 0.0:
+            raise ValueError('a fault profile file on both q faces of one block is not supported')
+        if block.lc != 0.0 or block.rc != 0.0:
+            X1 = geometry.read_fault_profile(block.profile_path, block.n[1], block.n[2])
+            off, side = (block.rc * X1, 0) if block.rc != 0.0 else (block.lc * X1, 1)
+    if off is not None:
+        fx = np.ascontiguousarray(np.asarray(off, dtype=np.float64).T)  # [ns][nr]
         keep.append(fx)
         g.fault_x = _p(fx)
-        g.fault_side = 1 if block.bc[1] == 0 else 0
+        g.fault_side = side
     if block.top_offset is not None:
         ty = np.ascontiguousarray(np.asarray(block.top_offset, dtype=np.float64).T)  # [ns][nq]
         keep.append(ty)
