@@ -44,7 +44,7 @@ def kernel_name(prob, precision, variant):
         # the anelastic response runs on 32 x 8 tiles, fused or held
         fam, tx, ty = ('hold', 32, 8) if fam == 'hold' else ('fused', 32, 8)
     curv = 'true' if any(b.mesh != 'cartesian' for b in prob.blocks) else 'false'
-    het = 'true' if any(b.material is not None for b in prob.blocks) else 'false'
+    het = 'true' if any(b.material is not None or b.medium is not None for b in prob.blocks) else 'false'
     real = 'float' if precision == 'fp32' else 'double'
     if fam == 'split':
         return f'k_volume_split<{real}, {tx}, {ty}, {curv}, {het}, 0>'  # interior (not s-closure) instantiation
@@ -99,7 +99,7 @@ def bytes_per_node(prob, precision):
         vals += 98 - 24 / 5
     if any(b.mesh != 'cartesian' for b in prob.blocks):
         vals += 10
-    if any(b.material is not None for b in prob.blocks):
+    if any(b.material is not None or b.medium is not None for b in prob.blocks):
         vals += 3
     return vals * (4 if precision == 'fp32' else 8)
 

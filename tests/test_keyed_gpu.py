@@ -82,7 +82,7 @@ def test_keyed_medium_on_the_device(case):
     prob = G.problem(case)
     for ib, b in enumerate(prob.blocks):
         ref = z[f'b{ib + 1}_Mat']
-        M = b.material
+        M = b.medium_material()
         if ref.ndim == 1:
             ref = np.broadcast_to(ref[:, None, None, None], M.shape)
         assert np.array_equal(M, ref), ib

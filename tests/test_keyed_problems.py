@@ -39,16 +39,18 @@ def test_medium_vs_reference(case):
     z = G.load(case)
     prob = G.problem(case)
     for ib, b in enumerate(prob.blocks):
-        assert b.material is not None
+        assert b.material is None and b.medium is not None
+        M = b.medium_material()
         ref = z[f'b{ib + 1}_Mat']
         if ref.ndim == 1:
-            ref = np.broadcast_to(ref[:, None, None, None], b.material.shape)
-        assert np.array_equal(b.material, ref), ib
+            ref = np.broadcast_to(ref[:, None, None, None], M.shape)
+        assert np.array_equal(M, ref), ib
+        assert np.array_equal(b.lame(), M)
 
 
 def test_no_medium_for_other_problems():
     prob = G.problem('cart_sw')
-    assert all(b.material is None for b in prob.blocks)
+    assert all(b.material is None and b.medium is None for b in prob.blocks)
     assert problem_material('TPV5', prob.blocks[0]) is None
 
 
@@ -56,7 +58,7 @@ def test_medium_layers_exercised():
     """the graded layers of each medium are reached by the golden grids"""
     for case, n_min in (('tpv31_sw', 5), ('tpv32_sw', 8), ('oklahoma_locked', 4), ('tpv33_sw', 2)):
         prob = G.problem(case)
-        mus = np.unique(np.concatenate([b.material[1].ravel() for b in prob.blocks]))
+        mus = np.unique(np.concatenate([b.medium_material()[1].ravel() for b in prob.blocks]))
         assert mus.size >= n_min, case
 
 

@@ -27,7 +27,7 @@ ZCHUNKS = (4, 6, 8, 12, 16, 24)
 
 def key(prob, precision):
     b = prob.blocks[0]
-    kind = ('curv' if b.mesh != 'cartesian' else 'cart') + ('_het' if b.material is not None else '') + \
+    kind = ('curv' if b.mesh != 'cartesian' else 'cart') + ('_het' if b.material is not None or b.medium is not None else '') + \
            ('_pml' if any(b.pml) else '')
     return f'{"x".join(map(str, b.n))}:{precision}:{kind}'
 

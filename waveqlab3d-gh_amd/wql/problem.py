@@ -36,6 +36,7 @@ class Block:
     fault_offset: Optional[np.ndarray] = None                  # [nr][ns] x-offsets of the fault face
     top_offset: Optional[np.ndarray] = None                    # [nq][ns] y-offsets of the r=1 face
     material: Optional[np.ndarray] = None                      # [3][ns][nr][nq] lambda, mu, rho (heterogeneous)
+    medium: Optional[str] = None                               # init_material's keyed medium (problem name)
 
     def coords(self):
         if self.mesh == 'cartesian':
@@ -50,10 +51,19 @@ class Block:
                                     self.profile, fault_left=fl, fault_right=fr, top=self.top_offset,
                                     profile_path=self.profile_path)
 
+    def medium_material(self, X=None):
+        """lambda, mu, rho of the keyed medium at the nodes X (default: this
+        block's grid), or None without one"""
+        if self.medium is None:
+            return None
+        return problem_material(self.medium, self, X)
+
     def lame(self):
         """M = (lambda, mu, rho) on every node (material.f90:181-188)."""
         if self.material is not None:
             return np.ascontiguousarray(self.material, dtype=np.float64)
+        if self.medium is not None:
+            return self.medium_material()
         rho, cs, cp = self.rho_s_p
         nq, nr, ns = self.n
         M = np.empty((3, ns, nr, nq))
@@ -310,14 +320,18 @@ def read_namelist(text, group):
     return out
 
 
-def problem_material(problem, block):
+MEDIA = ('LOH1', 'LOH1_Harmonic', 'TPV31', 'TPV32', 'TPV33', 'OKLAHOMA')
+
+
+def problem_material(problem, block, X=None):
     """lambda, mu, rho [3][ns][nr][nq] of init_material's medium for this
-    problem name at the block's nodes (material.f90:179-690: 'LOH1',
-    'LOH1_Harmonic', 'TPV31', 'TPV32', 'TPV33', 'OKLAHOMA'), or None for the
-    homogeneous rho_s_p medium of every other problem"""
+    problem name at the block's nodes X (default: block.coords(); material.
+    f90:179-690: 'LOH1', 'LOH1_Harmonic', 'TPV31', 'TPV32', 'TPV33',
+    'OKLAHOMA'), or None for the homogeneous rho_s_p medium of every other
+    problem"""
     from . import native
     nq, nr, ns = block.n
-    X = np.ascontiguousarray(block.coords(), dtype=np.float64)
+    X = np.ascontiguousarray(block.coords() if X is None else X, dtype=np.float64)
     M = np.empty((3, ns, nr, nq))
     dp = native.dp
     rc = native.lib().wql_problem_material(problem.encode(), nq, nr, ns, X.ctypes.data_as(dp), M.ctypes.data_as(dp))
@@ -385,7 +399,9 @@ def from_namelist(path):
             mesh=mesh, profile=g('profile_type', 'analytical_test_problem'),
             profile_path=_data_path(g('profile_path', None), base)))
     for b in blocks:
-        b.material = problem_material(problem, b)
+        # init_material's keyed medium, evaluated on the block's grid when a
+        # simulation is set up (Block.medium_material)
+        b.medium = problem if problem in MEDIA else None
     response = pl.get('response', 'elastic')
     model = pl.get('plastic_model', 'default')
     plastic = 0

@@ -66,8 +66,9 @@ def block_descs(prob, coords=None):
         d.npml = b.npml
         d.cartesian = 1 if b.mesh == 'cartesian' else 0
         d.X = _p(X)
-        if b.material is not None:
-            M = np.ascontiguousarray(b.material, dtype=np.float64)
+        Mb = b.material if b.material is not None else b.medium_material(X)
+        if Mb is not None:
+            M = np.ascontiguousarray(Mb, dtype=np.float64)
             keep.append(M)
             d.M = _p(M)
         d.rho_s_p[:] = b.rho_s_p
