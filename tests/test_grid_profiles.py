@@ -59,3 +59,19 @@ def test_block_list_file(tmp_path):
     (tmp_path / 'bad.in').write_text(src[:start] + "&block_list = 'missing.txt' /\n" + src[end + 1:])
     with pytest.raises(ValueError):
         from_namelist(str(tmp_path / 'bad.in'))
+
+
+@pytest.mark.parametrize('edit', [("nblocks = 2", "nblocks = 3"),
+                                  ("w_stride = 1,", "w_stride = 1, use_topography = T,"),
+                                  ("w_stride = 1,", "w_stride = 1, mollify_source = T,"),
+                                  ("w_stride = 1,", "w_stride = 1, material_source = 'file',"),
+                                  ("use_mms = F", "use_mms = T")])
+def test_unsupported_inputs_fail(edit, tmp_path):
+    """input features outside the path raise instead of running as something
+    else"""
+    from wql.problem import from_namelist
+    src = open(os.path.join(G.GOLDEN, 'inputs', 'cart_sw.in')).read()
+    assert edit[0] in src
+    (tmp_path / 'x.in').write_text(src.replace(edit[0], edit[1], 1))
+    with pytest.raises(NotImplementedError):
+        from_namelist(str(tmp_path / 'x.in'))

@@ -100,8 +100,11 @@ uint64_t fnv(uint64_t h, const void *p, size_t bytes)
 uint64_t scenario_key(const wql_scenario *sc)
 {
     uint64_t h = 1469598103934665603ull;
-    const int flags[5] = {sc->prob.precision, sc->prob.plastic, sc->prob.s0_kind, sc->prob.fd_type,
-                          sc->prob.fd_type ? sc->prob.fd_order : 0};
+    // what a pooled context keeps from its create: precision, plasticity,
+    // operator, the keyed fault's per-node tables (recomputed by the reset,
+    // but allocated only for a keyed problem)
+    const int flags[6] = {sc->prob.precision, sc->prob.plastic, sc->prob.s0_kind, sc->prob.fd_type,
+                          sc->prob.fd_type ? sc->prob.fd_order : 0, sc->prob.fault.key};
     h = fnv(h, flags, sizeof flags);
     const double an[4] = {(double)sc->prob.anelastic, sc->prob.an_c, sc->prob.an_weight_exp, sc->prob.an_fref};
     if (sc->prob.anelastic) h = fnv(h, an, sizeof an);

@@ -94,6 +94,8 @@ contains
     D%name = D%p%name; D%problem = D%p%problem; D%coupling = D%p%coupling; D%response = D%p%response
     D%t_final = D%p%t_final; D%CFL = D%p%CFL; D%w_stride = D%p%w_stride; D%w_fault = D%p%w_fault
     if (D%p%nblocks /= 2) error stop 'init_domain: the path runs two blocks joined by one interface'
+    if (D%p%mollify_source) error stop 'init_domain: the Gaussian-mollified moment-tensor source is not built'
+    if (trim(D%p%material_source) == 'file') error stop "init_domain: material_source = 'file' is not read by this host"
     D%t = 0.0_wp
     do ib = 1, 2
        dt(ib) = block_time_step(D%p%btp(ib), D%CFL)
@@ -144,7 +146,7 @@ contains
                g%fault_x = c_loc(fx)
             end if
          end select
-         if (trim(b%topography_type) /= 'none') error stop 'init_domain: topography from files is not read by this host'
+         if (D%p%use_topography) error stop "init_domain: the reference's topography branch is not restated by this host"
          call wql_check(wql_grid_block(g, D%B(ib)%X), 'wql_grid_block')
          if (allocated(fx)) deallocate(fx)
          D%desc(ib)%nq = g%nq; D%desc(ib)%nr = g%nr; D%desc(ib)%ns = g%ns

@@ -372,6 +372,19 @@ def from_namelist(path):
     text = open(path).read()
     base = os.path.dirname(os.path.abspath(path))
     pl = read_namelist(text, 'problem_list')
+    # inputs outside this library's path fail here rather than run as
+    # something else
+    if int(pl.get('nblocks', 2)) != 2:
+        raise NotImplementedError('the path runs two blocks joined by one interface')
+    if pl.get('use_topography', False):
+        raise NotImplementedError("use_topography: the reference's topography branch of Curve_Grid_3D3 "
+                                  '(grid.f90:1349-1900) is not restated; pass the grid as coordinates')
+    if pl.get('mollify_source', False):
+        raise NotImplementedError('mollify_source: the Gaussian-mollified moment-tensor source is not built')
+    if str(pl.get('material_source', 'hardcode')) == 'file':
+        raise NotImplementedError("material_source = 'file': set Block.material from the files instead")
+    if read_namelist(text, 'mms_list').get('use_mms', False):
+        raise NotImplementedError('use_mms: the manufactured-solution forcing is not built')
     blf = block_list_file(text, base)
     if blf is not None:
         try:
