@@ -568,8 +568,12 @@ __device__ void volume_node_op(const Blk<real> &B, int i, int j, int k, real A, 
     node_tail<real>(B, i, j, k, p, U, fin, Jc, A, bdt);
 }
 
+// three waves per SIMD in FP32 (a few spilled registers cost less than the
+// third wave gains: the per-node loads are L2-latency bound), two in FP64
+// (three spill 260 bytes a lane and run slower)
 template <typename real>
-__global__ void __launch_bounds__(256) k_volume_op(Pair<real> P, BoxList L, real A0, real A1, real bdt)
+__global__ void __launch_bounds__(256, sizeof(real) == 4 ? 3 : 2)
+    k_volume_op(Pair<real> P, BoxList L, real A0, real A1, real bdt)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.total) return;
@@ -1693,8 +1697,10 @@ struct OptRows {
     int nl, nr, lo, hi;
 };
 
+// FP32 at three waves per SIMD (168 VGPRs and about 200 spilled bytes a
+// lane): 18% shorter DRP-6 steps than at two; FP64 one wave
 template <typename real, int TX, int TY, int H>
-__global__ void __launch_bounds__(TX * TY, (sizeof(real) == 4 ? 2 : 1))
+__global__ void __launch_bounds__(TX * TY, (sizeof(real) == 4 ? 3 : 1))
     k_volume_opt(Pair<real> P, TileList TL, real A0, real A1, real bdt)
 {
     constexpr int HS = H < 5 ? H : 5;  // interior rows reach at most 5 planes
