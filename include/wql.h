@@ -176,6 +176,15 @@ typedef struct {
 /* X[3][ns][nr][nq] (the reference's X(q,r,s,1:3)) */
 int wql_grid_block(const wql_grid_desc *grid, double *X);
 
+/* Gen3_Curve_Mesh (grid.f90:2290-2416) on the device from given faces:
+ * X[3][ns][nr][nq] from faces[3][...], per component x, y, z the q = 0 and
+ * q = nq-1 faces [ns][nr], the r = 0 and r = nr-1 faces [ns][nq], the s = 0
+ * and s = ns-1 faces [nr][nq].  The faces of the topography branch of
+ * Curve_Grid_3D3 (use_topography, grid.f90:1349-1860) are built from
+ * surface data (the topography and fault-profile files); the volume
+ * interpolation, the O(n^3) part, runs here. */
+int wql_grid_faces(int nq, int nr, int ns, const double *faces, double *X);
+
 /* init_material's media keyed by problem name (material.f90:179-690): for
  * 'LOH1', 'LOH1_Harmonic', 'TPV31', 'TPV32', 'TPV33' and 'OKLAHOMA' fills
  * M[3][ns][nr][nq] (lambda, mu, rho, as wql_block_desc.M) from the node

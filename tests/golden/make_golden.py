@@ -34,7 +34,7 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
              npml=0, lc=(1.0, 0.0), rc=(0.0, 1.0), profile='analytical_test_problem',
              rho_s_p=(2.67, 3.464, 6.0), t_final=1.0, aqrs=None, bqrs=None, mu_beta_eta=(0.0, 0.0, 0.0),
              moment=None, fd_type=None, order=None, anelastic=None, profile_path='No-path', fractal=None,
-             keep_mat=False):
+             keep_mat=False, topography=None, faultsize=None):
     T = lambda b: 'T' if b else 'F'
     s = ["&problem_list",
          f" name = 'data/{name}',", f" problem = '{problem}',", f" response = '{response}',",
@@ -42,7 +42,10 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
     if fd_type is not None:
         s += [f" fd_type = '{fd_type}',", f" order = {order},"]
     s += [
-         " CFL = 0.5d0,", f" coupling = '{coupling}',", f" type_of_mesh = '{mesh}',",
+         " CFL = 0.5d0,", f" coupling = '{coupling}',", f" type_of_mesh = '{mesh}',"]
+    if topography is not None:
+        s += [" use_topography = T,", f" topo = {topography['topo']}d0,"]
+    s += [
          " w_stride = 1,", " w_fault = F /", "", "&block_list"]
     for b in range(2):
         a = (aqrs or BASE_BLOCKS['aqrs'])[b]
@@ -57,11 +60,13 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
               f"btp({i})%rqrs = {rq[b][0]}, {rq[b][1]}, {rq[b][2]},",
               f"btp({i})%lc = {lc[b]}d0,", f"btp({i})%rc = {rc[b]}d0,",
               f"btp({i})%profile_type = '{profile}',", f"btp({i})%profile_path = '{profile_path}',",
-              f"btp({i})%topography_type = 'none',", f"btp({i})%topography_path = 'No-path',",
+              f"btp({i})%topography_type = '{topography['type'] if topography else 'none'}',",
+              f"btp({i})%topography_path = '{f'topo{i}.dat' if topography and topography['type'] == 'read_topo_from_memory' else 'No-path'}',",
               "btp(%d)%%material_path = 'none', 'none', 'none'," % i,
               f"btp({i})%pml_lqrs = {', '.join(T(x) for x in pml_l[b])},",
               f"btp({i})%pml_rqrs = {', '.join(T(x) for x in pml_r[b])},",
-              f"btp({i})%npml = {npml},", f"btp({i})%faultsize = {n[1]}, {n[2]}" + (',' if b == 0 else ' /')]
+              f"btp({i})%npml = {npml},",
+              f"btp({i})%faultsize = {(faultsize or n[1:])[0]}, {(faultsize or n[1:])[1]}" + (',' if b == 0 else ' /')]
     if moment is None:
         s += ["", "&moment_list", " use_moment_tensor = F /"]
     else:
@@ -280,6 +285,26 @@ CASES = {
                         lq=LQ_STD, rq=RQ_STD, profile='analytical_tpv36',
                         aqrs=[(-40.0, 0.0, -6.0), (0.0, 0.0, -6.0)], bqrs=[(0.0, 12.0, 6.0), (40.0, 12.0, 6.0)]),
                    [('rhs', ['0.1'])]),
+    # the topography branch of Curve_Grid_3D3 (use_topography, grid.f90:
+    # 1349-1860): a topography file (x, y, z per top-face node, scaled by
+    # topo), the analytical hill at (x, z) = (10, 10), and a topography file
+    # with a fault-profile file larger than the face (faultsize)
+    'grid_topo_read': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 12, 15),
+                            lq=LQ_STD, rq=RQ_STD, profile='no_path',
+                            topography=dict(type='read_topo_from_memory', topo=0.5, seed=13, amp=0.3),
+                            aqrs=[(-4.0, 0.0, -3.0), (0.0, 0.0, -3.0)], bqrs=[(0.0, 6.0, 3.0), (4.0, 6.0, 3.0)]),
+                       [('rhs', ['0.1'])]),
+    'grid_topo_analytic': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 12, 15),
+                                lq=LQ_STD, rq=RQ_STD, profile='no_path',
+                                topography=dict(type='analytical_topography', topo=1.0),
+                                aqrs=[(4.0, 0.0, 5.0), (10.0, 0.0, 5.0)], bqrs=[(10.0, 6.0, 15.0), (16.0, 6.0, 15.0)]),
+                           [('rhs', ['0.1'])]),
+    'grid_topo_fractal': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 12, 15),
+                               lq=LQ_STD, rq=RQ_STD, profile='read_from_memomry_fractal', profile_path='fault.dat',
+                               fractal=dict(seed=17, amp=0.05), faultsize=(14, 18),
+                               topography=dict(type='read_topo_from_memory', topo=1.0, seed=19, amp=0.2),
+                               aqrs=[(-4.0, 0.0, -3.0), (0.0, 0.0, -3.0)], bqrs=[(0.0, 6.0, 3.0), (4.0, 6.0, 3.0)]),
+                          [('rhs', ['0.1'])]),
     'grid_fractal': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 14, 15),
                           lq=LQ_STD, rq=RQ_STD, profile='read_from_memomry_fractal', profile_path='fault.dat',
                           fractal=dict(seed=11, amp=0.04)),
@@ -287,10 +312,50 @@ CASES = {
 }
 
 
+def write_topography(work, kw):
+    """the topography files of a grid case (read_topo_from_memory): per block
+    nq*ns rows 'x y z' with z fastest, x and z on the block's uniform lines,
+    y one smooth random surface over both blocks (equal on the shared face)"""
+    nq, nr, ns = kw['n']
+    aq, bq = kw['aqrs'], kw['bqrs']
+    rng = np.random.default_rng(kw['topography']['seed'])
+    xs = np.concatenate([aq[0][0] + np.arange(nq) * (bq[0][0] - aq[0][0]) / (nq - 1),
+                         aq[1][0] + np.arange(1, nq) * (bq[1][0] - aq[1][0]) / (nq - 1)])
+    w = rng.standard_normal((xs.size, ns))
+    k0 = np.fft.fftfreq(xs.size)[:, None]
+    k1 = np.fft.fftfreq(ns)[None, :]
+    y = np.real(np.fft.ifft2(np.fft.fft2(w) * np.exp(-(k0 ** 2 + k1 ** 2) * 80.0)))
+    y *= kw['topography']['amp'] / max(np.abs(y).max(), 1e-300)
+    out = []
+    for b in range(2):
+        x = aq[b][0] + np.arange(nq) * (bq[b][0] - aq[b][0]) / (nq - 1)
+        z = aq[b][2] + np.arange(ns) * (bq[b][2] - aq[b][2]) / (ns - 1)
+        yb = y[b * (nq - 1): b * (nq - 1) + nq]
+        rows = [f'{x[i]:.17e} {yb[i, k]:.17e} {z[k]:.17e}' for i in range(nq) for k in range(ns)]
+        with open(os.path.join(work, f'topo{b + 1}.dat'), 'w') as f:
+            f.write('\n'.join(rows) + '\n')
+        out.append(np.stack([np.broadcast_to(x[:, None], (nq, ns)), yb, np.broadcast_to(z[None, :], (nq, ns))]))
+    return out
+
+
 def write_fractal(work, kw):
     """the fault-profile text file of a grid case: my*mz rows 'x y z', y
-    fastest; smooth random x offsets"""
+    fastest; smooth random x offsets.  In the topography branch (which takes
+    the face's y and z from the file too, over faultsize rows) y and z are the
+    face's uniform lines, continued past the face for the extra rows"""
     nq, nr, ns = kw['n']
+    if kw.get('topography'):
+        ny, nz = kw['faultsize']
+        aq, bq = kw['aqrs'][0], kw['bqrs'][0]
+        rng = np.random.default_rng(kw['fractal']['seed'])
+        x = kw['fractal']['amp'] * np.sin(np.arange(ny)[:, None] * 0.7 + rng.uniform()) * \
+            np.cos(np.arange(nz)[None, :] * 0.5)
+        y = aq[1] + np.arange(ny) * (bq[1] - aq[1]) / (nr - 1)
+        z = aq[2] + np.arange(nz) * (bq[2] - aq[2]) / (ns - 1)
+        rows = [f'{x[j, k]:.17e} {y[j]:.17e} {z[k]:.17e}' for k in range(nz) for j in range(ny)]
+        with open(os.path.join(work, kw['profile_path']), 'w') as f:
+            f.write('\n'.join(rows) + '\n')
+        return np.stack([x, np.broadcast_to(y[:, None], (ny, nz)), np.broadcast_to(z[None, :], (ny, nz))])
     rng = np.random.default_rng(kw['fractal']['seed'])
     w = rng.standard_normal((ns, nr))
     k0 = np.fft.fftfreq(ns)[:, None]
@@ -313,6 +378,8 @@ def run_case(name, kw, runs, work):
     with open(inp, 'w') as f:
         f.write(namelist(name, **kw))
     fractal = write_fractal(work, kw) if kw.get('fractal') else None
+    topo = write_topography(work, kw) if kw.get('topography', {}) and kw['topography']['type'] == \
+        'read_topo_from_memory' else None
     for d in ('data', 'seismogram/block1/data', 'seismogram/block2/data'):
         os.makedirs(os.path.join(work, d), exist_ok=True)
     nq, nr, ns = kw['n']
@@ -382,6 +449,8 @@ def run_case(name, kw, runs, work):
         out = {k: v for k, v in out.items() if k.endswith('_X') or k == 'n'}
         if fractal is not None:
             out['profile_x'] = fractal
+        if topo is not None:
+            out['topo1'], out['topo2'] = topo
     for k in list(out):
         if k.endswith('_Mat') and not kw.get('response') == 'anelastic':
             a = out[k]

@@ -103,7 +103,8 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
-GRID_CASES = ['grid_tpv28', 'grid_tpv10', 'grid_tpv36', 'grid_fractal']
+GRID_CASES = ['grid_tpv28', 'grid_tpv10', 'grid_tpv36', 'grid_fractal', 'grid_topo_read', 'grid_topo_analytic',
+              'grid_topo_fractal']
 
 
 def grid_problem(case, tmpdir):
@@ -112,14 +113,27 @@ def grid_problem(case, tmpdir):
     copy of its input file in tmpdir"""
     src = os.path.join(GOLDEN, 'inputs', f'{case}.in')
     z = load(case)
-    if 'profile_x' not in z:
+    if 'profile_x' not in z and 'topo1' not in z:
         return P.from_namelist(src)
-    x = z['profile_x']
-    nr, ns = x.shape
-    with open(os.path.join(tmpdir, 'fault.dat'), 'w') as f:
-        for k in range(ns):
-            for j in range(nr):
-                f.write(f'{x[j, k]:.17e} {float(j):.1f} {float(k):.1f}\n')
+    if 'profile_x' in z:
+        x = z['profile_x']
+        with open(os.path.join(tmpdir, 'fault.dat'), 'w') as f:
+            if x.ndim == 3:  # x, y, z of the topography branch's profile file
+                for k in range(x.shape[2]):
+                    for j in range(x.shape[1]):
+                        f.write(f'{x[0, j, k]:.17e} {x[1, j, k]:.17e} {x[2, j, k]:.17e}\n')
+            else:
+                nr, ns = x.shape
+                for k in range(ns):
+                    for j in range(nr):
+                        f.write(f'{x[j, k]:.17e} {float(j):.1f} {float(k):.1f}\n')
+    for b in (1, 2):
+        if f'topo{b}' in z:  # read_topo_from_memory files: rows x y z, z fastest
+            t = z[f'topo{b}']
+            with open(os.path.join(tmpdir, f'topo{b}.dat'), 'w') as f:
+                for i in range(t.shape[1]):
+                    for k in range(t.shape[2]):
+                        f.write(f'{t[0, i, k]:.17e} {t[1, i, k]:.17e} {t[2, i, k]:.17e}\n')
     dst = os.path.join(tmpdir, f'{case}.in')
     with open(src) as a, open(dst, 'w') as b:
         b.write(a.read())

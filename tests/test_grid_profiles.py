@@ -1,8 +1,10 @@
 """The fault profiles of the curvilinear grid generator on the host
 (wql/geometry.py) against the reference build's grids: the TPV28 hills, the
 TPV10 60-degree and TPV36 15-degree dipping faults, and a fault surface read
-from a text file (grid.f90:625-925, read_2darray 2457-2481); the block-list
-file of an input file (domain.f90:165-211, input_utils.f90:9-94)."""
+from a text file (grid.f90:625-925, read_2darray 2457-2481); the topography
+branch (use_topography: a topography file, the analytical hill, a topography
+file with a fault-profile file, grid.f90:1349-1860); the block-list file of an
+input file (domain.f90:165-211, input_utils.f90:9-94)."""
 import os
 
 import numpy as np
@@ -17,12 +19,17 @@ def test_profile_grid_vs_reference(case, tmp_path):
     prob = G.grid_problem(case, str(tmp_path))
     for ib, b in enumerate(prob.blocks):
         assert b.mesh == 'curvilinear' and b.profile == str(prob.blocks[0].profile)
+        assert (b.topography is not None) == case.startswith('grid_topo')
         X = b.coords()
         ref = z[f'b{ib + 1}_X']
         assert np.abs(X - ref).max() <= 1e-14 * np.abs(ref).max(), ib
-        # the fault face is not planar
-        face = ref[0][:, :, -1] if ib == 0 else ref[0][:, :, 0]
-        assert np.ptp(face) > 1e-3
+        # the fault face, or the free surface of the topography branch, is
+        # not planar
+        if b.topography:
+            assert np.ptp(ref[1][:, 0, :]) > 1e-3
+        if b.profile != 'no_path':
+            face = ref[0][:, :, -1] if ib == 0 else ref[0][:, :, 0]
+            assert np.ptp(face) > 1e-3
 
 
 def test_profiles_vanish_with_zero_weights(tmp_path):
@@ -62,7 +69,6 @@ def test_block_list_file(tmp_path):
 
 
 @pytest.mark.parametrize('edit', [("nblocks = 2", "nblocks = 3"),
-                                  ("w_stride = 1,", "w_stride = 1, use_topography = T,"),
                                   ("w_stride = 1,", "w_stride = 1, mollify_source = T,"),
                                   ("w_stride = 1,", "w_stride = 1, material_source = 'file',"),
                                   ("use_mms = F", "use_mms = T")])

@@ -1982,6 +1982,31 @@ int wql_problem_keys(const char *problem, const char *coupling, const char *resp
     return 0;
 }
 
+int wql_grid_faces(int nq, int nr, int ns, const double *faces, double *X)
+{
+    if (!faces || !X) return fail(WQL_E_ARG, "null argument");
+    if (nq < 2 || nr < 2 || ns < 2) return fail(WQL_E_ARG, "a grid needs two nodes per direction");
+    const long N = (long)nq * nr * ns;
+    const long FQ = (long)nr * ns, FS = (long)nq * nr, FR = (long)nq * ns;
+    const long NF = 3 * 2 * (FQ + FR + FS);
+    double *d = nullptr;
+    if (hipMalloc(&d, (size_t)(3 * N + NF) * sizeof(double)) != hipSuccess) return fail(WQL_E_NOMEM, "hipMalloc(grid)");
+    GridArgs g;
+    memset(&g, 0, sizeof g);
+    g.nq = nq; g.nr = nr; g.ns = ns;
+    g.X = d;
+    g.faces = d + 3 * N;
+    hipError_t e = hipMemcpy(d + 3 * N, faces, (size_t)NF * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_grid_volume<<<(unsigned)((3 * N + 255) / 256), 256>>>(g);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(X, d, (size_t)(3 * N) * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(WQL_E_HIP, "grid faces: %s", hipGetErrorString(e));
+    return 0;
+}
+
 int wql_problem_material(const char *problem, int nq, int nr, int ns, const double *X, double *M)
 {
     if (!problem) return fail(WQL_E_ARG, "null argument");

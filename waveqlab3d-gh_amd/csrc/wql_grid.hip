@@ -43,6 +43,10 @@ struct GridArgs {
     const double *fault_x;    // [ns][nr] x offsets of one q face (or null)
     int fault_side;           // 0: q = 0 face, 1: q = nq-1 face
     const double *top_y;      // [ns][nq] y offsets of the r = 0 face (or null)
+    // all six faces given (wql_grid_faces): per component x, y, z the q = 0,
+    // q = nq-1 faces [ns][nr], the r = 0, r = nr-1 faces [ns][nq], the s = 0,
+    // s = ns-1 faces [nr][nq] (or null: the faces of the kernels above)
+    const double *faces;
     // work arrays (device)
     double *qf;               // [4][nr ns]: Yleft (= Yright), Zleft (= Zright), Xleft, Xright; node j + nr k
     double *sf;               // [2][2][nq nr]: X, Y of the front (k = 0) and back (k = ns-1) faces; node i + nq j
@@ -193,33 +197,41 @@ __global__ void k_grid_volume(GridArgs g)
     const int i = (int)(p % mx), j = (int)((p / mx) % my), k = (int)(p / ((long)mx * my));
     const long FQ = (long)my * mz, FS = (long)mx * my, FR = (long)mx * mz;
     const double ay = g.a[1], by = g.b[1], az = g.a[2], bz = g.b[2];
+    // the given faces (wql_grid_faces) of component c
+    const double *F = g.faces ? g.faces + (long)c * 2 * (FQ + FR + FS) : nullptr;
     // face arrays of component c
     auto L = [&](int jj, int kk) -> double {  // q = 0 face
         const long o = jj + (long)my * kk;
+        if (F) return F[o];
         return c == 0 ? g.qf[2 * FQ + o] : c == 1 ? g.qf[o] : g.qf[FQ + o];
     };
     auto R = [&](int jj, int kk) -> double {  // q = nq-1 face
         const long o = jj + (long)my * kk;
+        if (F) return F[FQ + o];
         return c == 0 ? g.qf[3 * FQ + o] : c == 1 ? g.qf[o] : g.qf[FQ + o];
     };
     auto Tp = [&](int ii, int kk) -> double {  // r = 0 face
         const long o = ii + (long)mx * kk;
+        if (F) return F[2 * FQ + o];
         if (c == 0) return g.rf[o];
         if (c == 2) return g.rf[FR + o];
         return g.top_y ? ay + g.top_y[o] : ay;
     };
     auto Bt = [&](int ii, int kk) -> double {  // r = nr-1 face
         const long o = ii + (long)mx * kk;
+        if (F) return F[2 * FQ + FR + o];
         if (c == 0) return g.rf[2 * FR + o];
         if (c == 2) return g.rf[3 * FR + o];
         return by;
     };
     auto Fr = [&](int ii, int jj) -> double {  // s = 0 face
         const long o = ii + (long)mx * jj;
+        if (F) return F[2 * FQ + 2 * FR + o];
         return c == 0 ? g.sf[o] : c == 1 ? g.sf[FS + o] : az;
     };
     auto Bk = [&](int ii, int jj) -> double {  // s = ns-1 face
         const long o = ii + (long)mx * jj;
+        if (F) return F[2 * FQ + 2 * FR + FS + o];
         return c == 0 ? g.sf[2 * FS + o] : c == 1 ? g.sf[3 * FS + o] : bz;
     };
     const double q = (double)i * (1.0 / (double)(mx - 1));

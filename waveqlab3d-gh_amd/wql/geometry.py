@@ -211,3 +211,98 @@ def curvilinear(n, a, b, lc=0.0, rc=0.0, profile='analytical_test_problem',
     X[1] = _tfi3(n, Yleft, Yright, faces['Ytop'], faces['Ybottom'], faces['Yfront'], faces['Yback'])
     X[2] = _tfi3(n, Zleft, Zright, faces['Ztop'], faces['Zbottom'], faces['Zfront'], faces['Zback'])
     return X
+
+
+def read_topography(path, mx, mz):
+    """XT, YT, ZT [mx][mz] of a 'read_topo_from_memory' file: mx*mz rows
+    'x y z', z fastest (read_2darray and reshape_2darray to (mz, mx),
+    grid.f90:1394-1420)"""
+    d = np.loadtxt(path, dtype=np.float64, ndmin=2)
+    if d.shape[0] < mx * mz or d.shape[1] < 3:
+        raise ValueError(f'topography {path}: {d.shape[0]} rows of {d.shape[1]} values, {mx * mz} rows of 3 needed')
+    return [d[:mx * mz, c].reshape(mx, mz) for c in range(3)]
+
+
+def topography_faces(n, a, b, lc=0.0, rc=0.0, profile='no_path', profile_path=None, faultsize=None,
+                     topography='read_topo_from_memory', topography_path=None, topo=1.0):
+    """The six faces of Curve_Grid_3D3's topography branch (use_topography,
+    grid.f90:1349-1860): the top face r = 0 from the topography, the q faces
+    interpolated between it and y = b_y (with the x column, y and z of a
+    'read_from_memomry_fractal' file of faultsize rows on a weighted face),
+    the s faces and the bottom face from their edges.  Returns
+    {'X': (left, right, top, bottom, front, back), 'Y': ..., 'Z': ...} in
+    _tfi3's layouts."""
+    mx, my, mz = n
+    ax, ay, az = a
+    bx, by, bz = b
+    Ybottom = np.full((mx, mz), float(by))
+    if topography == 'analytical_topography':  # 1364-1392 (both radii from (10, 10))
+        hx = (bx - ax) / float(mx - 1)
+        hz = (bz - az) / float(mz - 1)
+        Xtop = np.broadcast_to((ax + np.arange(mx, dtype=np.float64) * hx)[:, None], (mx, mz)).copy()
+        Ztop = np.broadcast_to((az + np.arange(mz, dtype=np.float64) * hz)[None, :], (mx, mz)).copy()
+        r1 = np.sqrt((Xtop - 10.0) ** 2 + (Ztop - 10.0) ** 2)
+        f = np.where(r1 <= 3.0, -0.3 * (1.0 + np.cos(PI * r1 / 3.0)), 0.0)
+        Ytop = ay - topo * 1.5 * f
+    elif topography == 'read_topo_from_memory':  # 1394-1420
+        XT, YT, ZT = read_topography(topography_path, mx, mz)
+        Xtop, Ytop, Ztop = XT, ay + topo * YT, ZT
+    else:
+        raise ValueError(f"specify topography profile (got {topography!r})")
+    hy, hz = 1.0 / float(my - 1), 1.0 / float(mz - 1)
+    r = np.arange(my, dtype=np.float64) * hy
+    sz = np.arange(mz, dtype=np.float64) * hz
+    zc = az + (bz - az) * sz
+
+    def qface(i):  # 1422-1480: the q face at column i of the top face
+        YL, YR = Ytop[i, :], np.full(mz, float(by))
+        YB = Ytop[i, 0] + (by - Ytop[i, 0]) * r
+        YT_ = Ytop[i, mz - 1] + (by - Ytop[i, mz - 1]) * r
+        return _surf(my, mz, YL, YR, YB, YT_), _surf(my, mz, zc, zc, np.full(my, float(az)), np.full(my, float(bz)))
+    Yleft, Zleft = qface(0)
+    Yright, Zright = qface(mx - 1)
+    Xleft = np.full((my, mz), float(ax))
+    Xright = np.full((my, mz), float(bx))
+    if profile == 'read_from_memomry_fractal':  # 1561-1610
+        ny, nz = faultsize if faultsize else (my, mz)
+        d = np.loadtxt(profile_path, dtype=np.float64, ndmin=2)
+        if d.shape[0] < ny * nz or d.shape[1] < 3:
+            raise ValueError(f'fault profile {profile_path}: {ny * nz} rows of 3 needed')
+        X1, Y1, Z1 = [d[:ny * nz, c].reshape(nz, ny).T[:my, :mz] for c in range(3)]
+        if rc != 0.0:
+            Xleft, Yleft, Zleft = ax + X1, Y1.copy(), Z1.copy()
+        if lc != 0.0:
+            Xright, Yright, Zright = bx + X1, Y1.copy(), Z1.copy()
+    elif profile != 'no_path':
+        raise ValueError(f"specify fault profile (the topography branch takes 'no_path' or "
+                         f"'read_from_memomry_fractal', got {profile!r})")
+    hx = 1.0 / float(mx - 1)
+    q = np.arange(mx, dtype=np.float64) * hx
+
+    def sface(k):  # 1700-1745: the s face at plane k
+        XT_ = Xleft[my - 1, k] + (Xright[my - 1, k] - Xleft[my - 1, k]) * q
+        X = _surf(mx, my, Xleft[:, k], Xright[:, k], Xtop[:, k], XT_)
+        Y = _surf(mx, my, Yleft[:, k], Yright[:, k], Ytop[:, k], Ybottom[:, k])
+        return X, Y
+    Xfront, Yfront = sface(0)
+    Xback, Yback = sface(mz - 1)
+    Zfront = np.full((mx, my), float(az))
+    Zback = np.full((mx, my), float(bz))
+    # bottom face (1747-1762)
+    Xbottom = _surf(mx, mz, Xleft[my - 1, :], Xright[my - 1, :], Xfront[:, my - 1], Xback[:, my - 1])
+    Zbottom = _surf(mx, mz, Zleft[my - 1, :], Zright[my - 1, :], Zfront[:, my - 1], Zback[:, my - 1])
+    return {'X': (Xleft, Xright, Xtop, Xbottom, Xfront, Xback),
+            'Y': (Yleft, Yright, Ytop, Ybottom, Yfront, Yback),
+            'Z': (Zleft, Zright, Ztop, Zbottom, Zfront, Zback)}
+
+
+def curvilinear_topography(n, a, b, **kw):
+    """X[3][ns][nr][nq] of Curve_Grid_3D3's topography branch (faces of
+    topography_faces, volume Gen3_Curve_Mesh)"""
+    faces = topography_faces(n, a, b, **kw)
+    mx, my, mz = n
+    X = np.empty((3, mz, my, mx))
+    for c, key in enumerate('XYZ'):
+        X[c] = _tfi3(n, *faces[key])
+    return X
+

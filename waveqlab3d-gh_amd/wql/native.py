@@ -25,6 +25,7 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_volume_kernel_ms', 'wql_get_fault_faces', 'wql_get_plastic',
            'wql_set_anelastic', 'wql_get_anelastic',
            'wql_get_fields_noalloc', 'wql_problem_keys', 'wql_grid_block', 'wql_problem_material',
+           'wql_grid_faces',
            'wql_stage', 'wql_server_pool_stats', 'wql_server_set_shape', 'wql_find_stations',
            'wql_set_stations', 'wql_get_stations', 'wql_locate_sources', 'wql_set_sources']
 
@@ -109,6 +110,7 @@ def lib():
             'wql_get_anelastic': ([vp, C.c_int, C.c_int, dp], C.c_int),
             'wql_get_fields_noalloc': ([vp, C.c_int, dp], C.c_int),
             'wql_problem_material': ([C.c_char_p, C.c_int, C.c_int, C.c_int, dp, dp], C.c_int),
+            'wql_grid_faces': ([C.c_int, C.c_int, C.c_int, dp, dp], C.c_int),
             'wql_problem_keys': ([C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, dp, C.POINTER(ProblemDesc)],
                                  C.c_int),
             'wql_grid_block': ([C.POINTER(GridDesc), dp], C.c_int),

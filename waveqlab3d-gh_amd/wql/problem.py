@@ -33,6 +33,13 @@ class Block:
     mesh: str = 'cartesian'                                    # 'cartesian' | 'curvilinear'
     profile: str = 'analytical_test_problem'
     profile_path: Optional[str] = None                         # 'read_from_memomry_fractal' profile file
+    # use_topography: the topography branch of Curve_Grid_3D3 with this
+    # topography_type ('analytical_topography' or 'read_topo_from_memory'),
+    # its file, the &problem_list topo scale and btp%faultsize
+    topography: Optional[str] = None
+    topography_path: Optional[str] = None
+    topo: float = 1.0
+    faultsize: Optional[Tuple[int, int]] = None
     fault_offset: Optional[np.ndarray] = None                  # [nr][ns] x-offsets of the fault face
     top_offset: Optional[np.ndarray] = None                    # [nq][ns] y-offsets of the r=1 face
     material: Optional[np.ndarray] = None                      # [3][ns][nr][nq] lambda, mu, rho (heterogeneous)
@@ -41,6 +48,11 @@ class Block:
     def coords(self):
         if self.mesh == 'cartesian':
             return geometry.cartesian(self.n, self.a, self.b)
+        if self.topography:
+            return geometry.curvilinear_topography(
+                self.n, self.a, self.b, lc=self.lc, rc=self.rc, profile=self.profile,
+                profile_path=self.profile_path, faultsize=self.faultsize, topography=self.topography,
+                topography_path=self.topography_path, topo=self.topo)
         fl = fr = None
         if self.fault_offset is not None:
             if self.bc[1] == 0:
@@ -376,9 +388,6 @@ def from_namelist(path):
     # something else
     if int(pl.get('nblocks', 2)) != 2:
         raise NotImplementedError('the path runs two blocks joined by one interface')
-    if pl.get('use_topography', False):
-        raise NotImplementedError("use_topography: the reference's topography branch of Curve_Grid_3D3 "
-                                  '(grid.f90:1349-1900) is not restated; pass the grid as coordinates')
     if pl.get('mollify_source', False):
         raise NotImplementedError('mollify_source: the Gaussian-mollified moment-tensor source is not built')
     if str(pl.get('material_source', 'hardcode')) == 'file':
@@ -411,6 +420,16 @@ def from_namelist(path):
             npml=int(g('npml', 0) or 0), lc=float(g('lc', 0.0) or 0.0), rc=float(g('rc', 0.0) or 0.0),
             mesh=mesh, profile=g('profile_type', 'analytical_test_problem'),
             profile_path=_data_path(g('profile_path', None), base)))
+        if pl.get('use_topography', False) and mesh == 'curvilinear':
+            # the topography branch of Curve_Grid_3D3 (domain.f90:280-288:
+            # faultsize sizes a profile file only with a topography file)
+            b = blocks[-1]
+            b.topography = g('topography_type', 'none')
+            b.topography_path = _data_path(g('topography_path', None), base)
+            b.topo = float(pl.get('topo', 1.0))
+            fs = g('faultsize', None)
+            if fs and b.profile == 'read_from_memomry_fractal' and b.topography == 'read_topo_from_memory':
+                b.faultsize = (int(fs[0]), int(fs[1]))
     for b in blocks:
         # init_material's keyed medium, evaluated on the block's grid when a
         # simulation is set up (Block.medium_material)

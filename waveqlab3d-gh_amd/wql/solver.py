@@ -323,6 +323,18 @@ PROFILE_CODES = {'analytical_test_problem': 1, 'analytical_tpv28': 2, 'analytica
 def device_grid(block):
     """node coordinates X[3][ns][nr][nq] of a Block generated on the device
     by wql_grid_block (the same grid as block.coords())"""
+    nq, nr, ns = block.n
+    if block.mesh != 'cartesian' and block.topography:
+        # the topography branch: its faces from the surface data on the host,
+        # the volume interpolation on the device
+        f = geometry.topography_faces(block.n, block.a, block.b, lc=block.lc, rc=block.rc, profile=block.profile,
+                                      profile_path=block.profile_path, faultsize=block.faultsize,
+                                      topography=block.topography, topography_path=block.topography_path,
+                                      topo=block.topo)
+        faces = np.ascontiguousarray(np.concatenate([a.T.ravel() for c in 'XYZ' for a in f[c]]))
+        X = np.empty((3, ns, nr, nq))
+        check(native.lib().wql_grid_faces(nq, nr, ns, _p(faces), _p(X)))
+        return X
     g = native.GridDesc()
     g.nq, g.nr, g.ns = block.n
     g.a[:] = block.a
