@@ -297,6 +297,16 @@ int wql_locate_sources(const double *X, int nq, int nr, int ns, int n, const dou
  * 0, 25, 50 or 100) is the first source's, as the reference uses
  * alpha(1) for all of a block's sources */
 int wql_set_sources(wql_ctx *ctx, int block, int n, const wql_source *src, int order, int alpha);
+/* the Gaussian-mollified form of the same sources (mollify_source = T,
+ * set_moment_tensor_smooth, moment_tensor.f90:1400-1475; chosen in set_rates,
+ * domain.f90:926-953): every source of the block's list -- all of the
+ * unified list in both blocks -- adds -A(t) M dx dy dz at every node, dx a
+ * normal density in x about the source with standard deviation twice the
+ * node's spacing along q (likewise y along r, z along s).  Only type,
+ * duration, t_init, m and x of src are read; X [3][ns][nr][nq] is the block's
+ * node coordinates, copied to the device.  Replaces the block's point
+ * sources (and wql_set_sources replaces these). */
+int wql_set_sources_smooth(wql_ctx *ctx, int block, int n, const wql_source *src, const double *X);
 
 /* station seismograms (seismogram.f90:12-527).  wql_find_stations is
  * Find_Coordinates (seismogram.f90:531-650) on the host, no device call:

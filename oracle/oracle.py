@@ -198,6 +198,9 @@ class OracleDomain:
             setattr(self.d, k, getattr(prob, k))
         self.d.t = 0.0
         self.d.dt = prob.dt()
+        if getattr(prob, 'mollify_source', False):
+            raise NotImplementedError('the oracle restates the point-source form only (the mollified sources '
+                                      'are pinned against reference-build goldens)')
         if getattr(prob, 'moment', None):
             from wql.problem import block_sources
             for ib in range(2):

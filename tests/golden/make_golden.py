@@ -34,11 +34,13 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
              npml=0, lc=(1.0, 0.0), rc=(0.0, 1.0), profile='analytical_test_problem',
              rho_s_p=(2.67, 3.464, 6.0), t_final=1.0, aqrs=None, bqrs=None, mu_beta_eta=(0.0, 0.0, 0.0),
              moment=None, fd_type=None, order=None, anelastic=None, profile_path='No-path', fractal=None,
-             keep_mat=False, topography=None, faultsize=None):
+             keep_mat=False, topography=None, faultsize=None, mollify=False):
     T = lambda b: 'T' if b else 'F'
     s = ["&problem_list",
          f" name = 'data/{name}',", f" problem = '{problem}',", f" response = '{response}',",
          f" plastic_model = '{plastic_model}',", " nblocks = 2,", f" t_final = {t_final}d0,"]
+    if mollify:
+        s += [" mollify_source = T,"]
     if fd_type is not None:
         s += [f" fd_type = '{fd_type}',", f" order = {order},"]
     s += [
@@ -150,6 +152,21 @@ CASES = {
                                  V=['cosine 0.1 0.0 0.3 0.3 -0.6 0.2 0.5 0.1 1.1 7.1 -0.3 25',
                                     'sine_moment_time_history 0.06 0.0 0.1 -0.4 0.2 0.0 0.2 0.3 1.6 7.9 0.5 25'])),
                 [('rhs', ['0.03']), ('traj', ['12'])]),
+    # the Gaussian-mollified sources (mollify_source = T): the unified list on
+    # the curvilinear grid (both blocks spread every source) and the per-block
+    # lists on the Cartesian one
+    'mt_smooth': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=N_STD, lq=LQ_STD, rq=RQ_STD,
+                       mollify=True,
+                       moment=dict(order=2, preference=1,
+                                   union=['gaussian 20.0 0.05 1.0 0.5 -0.3 0.2 0.1 -0.4 -1.73 7.31 0.22 0',
+                                          'Brune 30.0 0.0 -0.5 0.8 0.4 0.0 0.3 0.1 1.41 7.62 -0.37 0'])),
+                  [('rhs', ['0.04']), ('traj', ['12'])]),
+    'mt_smooth_lists': (dict(problem='TPV5', coupling='locked', mesh='cartesian', n=N_STD, lq=LQ_STD, rq=RQ_STD,
+                             mollify=True,
+                             moment=dict(order=2, preference=1,
+                                         U=["'Cos-Bell' 0.0 0.08 0.2 0.2 0.2 0.0 0.0 0.0 -2.2 8.1 -0.9 0"],
+                                         V=['cosine 0.1 0.0 0.3 0.3 -0.6 0.2 0.5 0.1 1.1 7.1 -0.3 0'])),
+                        [('rhs', ['0.03']), ('traj', ['12'])]),
     'tpv26_plastic': (dict(problem='TPV26', coupling='slip-weakening_friction', mesh='cartesian',
                            n=N_STD, lq=LQ_STD, rq=RQ_STD, response='plastic', plastic_model='scec',
                            aqrs=[(-6.0, 2.0, -9.0), (0.0, 2.0, -9.0)], bqrs=[(0.0, 24.0, 3.0), (6.0, 24.0, 3.0)]),

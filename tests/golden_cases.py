@@ -146,4 +146,6 @@ KEY_RHS_CASES = ['tpv10_sw', 'tpv29_sw', 'tpv31_sw', 'tpv32_sw', 'tpv33_sw', 'tp
                  'tpv37_sw', 'wasatch_sw']
 KEY_TRAJ_CASES = ['tpv10_sw', 'tpv29_sw', 'tpv31_sw', 'tpv32_sw', 'tpv33_sw', 'tpv34_sw', 'scits_sw', 'tpv36_sw',
                   'wasatch_sw', 'tpv30_plastic', 'loh1_locked', 'oklahoma_locked']
+# the Gaussian-mollified moment-tensor sources (mollify_source = T)
+SMOOTH_CASES = ['mt_smooth', 'mt_smooth_lists']
 MEDIUM_CASES = ['tpv31_sw', 'tpv32_sw', 'tpv33_sw', 'loh1_locked', 'oklahoma_locked']

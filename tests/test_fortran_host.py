@@ -112,7 +112,7 @@ def _run_main(inp, steps, cwd, extra=()):
 @pytest.mark.gpu
 @pytest.mark.skipif(not _have(MAIN), reason='Fortran host not built (no flang)')
 @pytest.mark.parametrize('case', ['cart_sw', 'curv_sw', 'curv_rs_long', 'plastic_sw', 'rw_plastic', 'tpv26_sw',
-                                  'mt_cart', 'mt_curv', 'up4_curv', 'drp66_cart', 'an_curv'])
+                                  'mt_cart', 'mt_curv', 'mt_smooth', 'up4_curv', 'drp66_cart', 'an_curv'])
 def test_fortran_main_matches_reference(case, tmp_path):
     _say(f'[gpu] fortran main {case}')
     z = G.load(case)

@@ -69,7 +69,6 @@ def test_block_list_file(tmp_path):
 
 
 @pytest.mark.parametrize('edit', [("nblocks = 2", "nblocks = 3"),
-                                  ("w_stride = 1,", "w_stride = 1, mollify_source = T,"),
                                   ("w_stride = 1,", "w_stride = 1, material_source = 'file',"),
                                   ("use_mms = F", "use_mms = T")])
 def test_unsupported_inputs_fail(edit, tmp_path):
@@ -81,3 +80,14 @@ def test_unsupported_inputs_fail(edit, tmp_path):
     (tmp_path / 'x.in').write_text(src.replace(edit[0], edit[1], 1))
     with pytest.raises(NotImplementedError):
         from_namelist(str(tmp_path / 'x.in'))
+
+
+def test_mollified_sources_parse(tmp_path):
+    """mollify_source = T reaches the Problem (the sources then spread as
+    Gaussians); the oracle, which restates the point form only, refuses it"""
+    import oracle as O
+    prob = G.problem('mt_smooth')
+    assert prob.mollify_source and prob.moment
+    assert not G.problem('mt_cart').mollify_source
+    with pytest.raises(NotImplementedError):
+        O.OracleDomain(prob)

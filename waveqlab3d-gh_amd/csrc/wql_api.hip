@@ -133,6 +133,9 @@ struct DevBlock {
     int nsrc = 0, src_cap = 0, src_order = 2, src_alpha = 0;
     int src_lo[3] = {0, 0, 0}, src_hi[3] = {0, 0, 0};
     SourceDev *src = nullptr;
+    // the mollified form (wql_set_sources_smooth): node coordinates [3][ND]
+    int src_smooth = 0;
+    double *Xs = nullptr;
 };
 
 }  // namespace
@@ -486,6 +489,8 @@ void launch_sources(wql_ctx *c, Pair<real> &P, double t, const double *tdev, dou
         a.tdev = tdev;
         a.tinc = tinc;
         a.bdt = bdt;
+        a.smooth = d.src_smooth;
+        a.X = d.Xs;
         k_sources<real><<<(unsigned)((nbox + 255) / 256), 256, 0, c->stream>>>(P.b[ib], a);
     }
 }
@@ -1731,12 +1736,53 @@ int wql_set_sources(wql_ctx *c, int ib, int n, const wql_source *src, int order,
     }
     drop_graphs(c);
     d.nsrc = n;
+    d.src_smooth = 0;
     d.src_order = order;
     d.src_alpha = alpha;
     for (int k = 0; k < 3; k++) { d.src_lo[k] = lo[k]; d.src_hi[k] = hi[k]; }
     if (n > 0) {
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipMemcpy(d.src, h.data(), n * sizeof(SourceDev), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int wql_set_sources_smooth(wql_ctx *c, int ib, int n, const wql_source *src, const double *X)
+{
+    if (!c || ib < 0 || ib > 1 || n < 0 || (n > 0 && (!src || !X))) return fail(WQL_E_ARG, "bad argument");
+    DevBlock &d = c->b[ib];
+    std::vector<SourceDev> h(n);
+    for (int s = 0; s < n; s++) {
+        if (src[s].type < WQL_STF_GAUSSIAN || src[s].type > WQL_STF_SINE_MOMENT)
+            return fail(WQL_E_ARG, "source %d: unknown source time function %d", s, src[s].type);
+        SourceDev &o = h[s];
+        o.type = src[s].type;
+        o.duration = src[s].duration;
+        o.t_init = src[s].t_init;
+        for (int k = 0; k < 6; k++) o.m[k] = src[s].m[k];
+        for (int k = 0; k < 3; k++) {
+            o.x[k] = src[s].x[k];
+            o.node[k] = -1;
+            o.node_x[k] = -1.0;
+        }
+    }
+    if (n > d.src_cap) {
+        if (int rc = dalloc(c, (void **)&d.src, n * sizeof(SourceDev))) return rc;
+        d.src_cap = n;
+    }
+    if (n > 0 && !d.Xs)
+        if (int rc = dalloc(c, (void **)&d.Xs, 3 * (size_t)d.ND * sizeof(double))) return rc;
+    drop_graphs(c);
+    d.nsrc = n;
+    d.src_smooth = 1;
+    d.src_order = 2;
+    d.src_alpha = 0;
+    d.src_lo[0] = d.src_lo[1] = d.src_lo[2] = 0;
+    d.src_hi[0] = d.nq; d.src_hi[1] = d.nr; d.src_hi[2] = d.ns;
+    if (n > 0) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(d.src, h.data(), n * sizeof(SourceDev), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.Xs, X, 3 * (size_t)d.ND * sizeof(double), hipMemcpyHostToDevice));
     }
     return 0;
 }

@@ -3129,6 +3129,10 @@ struct SourceArgs {
     const double *tdev;    // non-null: stage time = *tdev + tinc
     double tinc;
     double bdt;
+    // the Gaussian-mollified form (set_moment_tensor_smooth): every source
+    // spreads over the whole block; X [3][ns][nr][nq] dense node coordinates
+    int smooth;
+    const double *X;
 };
 
 // DF -= A M delta at the nodes of the box, sources in list order, and the
@@ -3152,7 +3156,33 @@ __global__ void k_sources(Blk<real> B, SourceArgs a)
     for (int c = 0; c < 6; c++) df[c] = (double)B.DF[(3 + c) * B.NP + p];
     const double df0[6] = {df[0], df[1], df[2], df[3], df[4], df[5]};
     bool hit = false;
-    for (int s = 0; s < a.n; s++) {
+    if (a.smooth) {
+        // set_moment_tensor_smooth (moment_tensor.f90:1400-1475): a normal
+        // density in each coordinate with standard deviation twice the local
+        // spacing (backward differences of X along q, Y along r, Z along s;
+        // forward at the first node), with the reference's pi = 3.14159265359
+        const long N = (long)B.nq * B.nr * B.ns, rs = B.nq, ss = (long)B.nq * B.nr;
+        const long d = i + rs * j + ss * k;
+        const double *X = a.X, *Y = a.X + N, *Z = a.X + 2 * N;
+        const double hx = i == 0 ? X[d + 1] - X[d] : X[d] - X[d - 1];
+        const double hy = j == 0 ? Y[d + rs] - Y[d] : Y[d] - Y[d - rs];
+        const double hz = k == 0 ? Z[d + ss] - Z[d] : Z[d] - Z[d - ss];
+        const double sx = 2.0 * hx, sy = 2.0 * hy, sz = 2.0 * hz;
+        const double r2pi = sqrt(2.0 * 3.14159265359);
+        for (int s = 0; s < a.n; s++) {
+            const SourceDev &S = a.src[s];
+            const double ux = (X[d] - S.x[0]) / sx, uy = (Y[d] - S.x[1]) / sy, uz = (Z[d] - S.x[2]) / sz;
+            const double dx = exp(-(0.5 * (ux * ux))) / (sx * r2pi);
+            const double dy = exp(-(0.5 * (uy * uy))) / (sy * r2pi);
+            const double dz = exp(-(0.5 * (uz * uz))) / (sz * r2pi);
+            const double A = source_time_fn(S.type, ts, S.t_init, S.duration);
+            const double delta = dx * dy * dz;
+            if (delta == 0.0) continue;
+            hit = true;
+            for (int c = 0; c < 6; c++) df[c] = df[c] - A * S.m[c] * delta;
+        }
+    }
+    for (int s = 0; s < (a.smooth ? 0 : a.n); s++) {
         const SourceDev &S = a.src[s];
         const double ax = (S.x[0] - S.node_x[0]) / hx;
         const double ay = (S.x[1] - S.node_x[1]) / hy;

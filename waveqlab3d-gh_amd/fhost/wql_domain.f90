@@ -94,7 +94,6 @@ contains
     D%name = D%p%name; D%problem = D%p%problem; D%coupling = D%p%coupling; D%response = D%p%response
     D%t_final = D%p%t_final; D%CFL = D%p%CFL; D%w_stride = D%p%w_stride; D%w_fault = D%p%w_fault
     if (D%p%nblocks /= 2) error stop 'init_domain: the path runs two blocks joined by one interface'
-    if (D%p%mollify_source) error stop 'init_domain: the Gaussian-mollified moment-tensor source is not built'
     if (trim(D%p%material_source) == 'file') error stop "init_domain: material_source = 'file' is not read by this host"
     D%t = 0.0_wp
     do ib = 1, 2
@@ -296,6 +295,14 @@ contains
           if (src(s)%type < 0) error stop 'invalid source type'
           src(s)%duration = dur(s); src(s)%t_init = t0(s); src(s)%m = mt(:, s); src(s)%x = xyz(:, s)
        end do
+       if (D%p%mollify_source) then
+          ! set_moment_tensor_smooth: every source of the block's list spreads
+          ! over the block's nodes (domain.f90:926-953)
+          call wql_check(wql_set_sources_smooth(D%ctx, int(ib - 1, c_int), int(n, c_int), src, D%B(ib)%X), &
+                         'wql_set_sources_smooth')
+          deallocate(src)
+          cycle
+       end if
        call wql_check(wql_locate_sources(D%B(ib)%X, int(D%B(ib)%nq, c_int), int(D%B(ib)%nr, c_int), &
                                          int(D%B(ib)%ns, c_int), int(n, c_int), xyz, src), 'wql_locate_sources')
        if (union .and. ib /= mp%tensor_block_preference) then

@@ -213,6 +213,13 @@ module wql_gpu
        type(wql_source), intent(in) :: src(*)
        integer(c_int), value :: order, alpha
      end function wql_set_sources
+     integer(c_int) function wql_set_sources_smooth(ctx, block, n, src, X) bind(C, name='wql_set_sources_smooth')
+       import :: c_int, c_ptr, c_double, wql_source
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: block, n
+       type(wql_source), intent(in) :: src(*)
+       real(c_double), intent(in) :: X(*)
+     end function wql_set_sources_smooth
      integer(c_int) function wql_set_stations(ctx, block, n, ijk) bind(C, name='wql_set_stations')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

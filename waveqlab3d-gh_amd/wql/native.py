@@ -25,7 +25,7 @@ EXPORTS = ['wql_create', 'wql_destroy', 'wql_last_error', 'wql_abi_version', 'wq
            'wql_volume_kernel_ms', 'wql_get_fault_faces', 'wql_get_plastic',
            'wql_set_anelastic', 'wql_get_anelastic',
            'wql_get_fields_noalloc', 'wql_problem_keys', 'wql_grid_block', 'wql_problem_material',
-           'wql_grid_faces',
+           'wql_grid_faces', 'wql_set_sources_smooth',
            'wql_stage', 'wql_server_pool_stats', 'wql_server_set_shape', 'wql_find_stations',
            'wql_set_stations', 'wql_get_stations', 'wql_locate_sources', 'wql_set_sources']
 
@@ -133,6 +133,7 @@ def lib():
             'wql_set_stations': ([vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
             'wql_locate_sources': ([dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.POINTER(Source)], C.c_int),
             'wql_set_sources': ([vp, C.c_int, C.c_int, C.POINTER(Source), C.c_int, C.c_int], C.c_int),
+            'wql_set_sources_smooth': ([vp, C.c_int, C.c_int, C.POINTER(Source), dp], C.c_int),
             'wql_get_stations': ([vp, C.c_int, dp], C.c_int),
             'wql_server_set_shape': ([vp, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
                                      C.c_int),

@@ -205,6 +205,9 @@ class Problem:
     # {'U': [...], 'V': [...]} of source dicts (type, duration, t_init, m, x,
     # alpha); see block_sources
     moment: Optional[dict] = None
+    # mollify_source: the sources spread as Gaussians over every node
+    # (set_moment_tensor_smooth) instead of the discrete delta
+    mollify_source: bool = False
     # spatial operator (&problem_list fd_type, order; domain.f90:106, 135-136):
     # 'traditional' (sixth-order central), 'upwind' (order 2-9) or
     # 'upwind_drp' (order 3-7, 66, 679); the order is unused for traditional
@@ -388,8 +391,6 @@ def from_namelist(path):
     # something else
     if int(pl.get('nblocks', 2)) != 2:
         raise NotImplementedError('the path runs two blocks joined by one interface')
-    if pl.get('mollify_source', False):
-        raise NotImplementedError('mollify_source: the Gaussian-mollified moment-tensor source is not built')
     if str(pl.get('material_source', 'hardcode')) == 'file':
         raise NotImplementedError("material_source = 'file': set Block.material from the files instead")
     if read_namelist(text, 'mms_list').get('use_mms', False):
@@ -443,6 +444,7 @@ def from_namelist(path):
                    cfl=float(pl.get('cfl', 0.5)), t_final=float(pl.get('t_final', 0.0)),
                    plastic=plastic, name=pl.get('name', 'default'), w_stride=int(pl.get('w_stride', 1)),
                    w_fault=bool(pl.get('w_fault', False)), moment=read_moment(text),
+                   mollify_source=bool(pl.get('mollify_source', False)),
                    fd_type=pl.get('fd_type', 'traditional'), order=int(pl.get('order', 5)))
     if response == 'anelastic':
         al = read_namelist(text, 'anelastic_list')

@@ -134,6 +134,11 @@ class Simulation:
             arr[k].m[:] = list(s_['m'])
             arr[k].x[:] = list(s_['x'])
         X = np.ascontiguousarray(X, dtype=np.float64)
+        if getattr(self.prob, 'mollify_source', False):
+            # every source of the block's list, held or not (the Gaussian
+            # reaches every node of the block)
+            check(native.lib().wql_set_sources_smooth(self.ctx, ib, n, arr, _p(X)))
+            return []
         nq, nr, ns = self.prob.blocks[ib].n
         xyz = np.ascontiguousarray([s_['x'] for s_ in src] or [[0.0, 0.0, 0.0]], dtype=np.float64)
         check(native.lib().wql_locate_sources(_p(X), nq, nr, ns, n, _p(xyz), arr))
