@@ -34,13 +34,15 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
              npml=0, lc=(1.0, 0.0), rc=(0.0, 1.0), profile='analytical_test_problem',
              rho_s_p=(2.67, 3.464, 6.0), t_final=1.0, aqrs=None, bqrs=None, mu_beta_eta=(0.0, 0.0, 0.0),
              moment=None, fd_type=None, order=None, anelastic=None, profile_path='No-path', fractal=None,
-             keep_mat=False, topography=None, faultsize=None, mollify=False):
+             keep_mat=False, topography=None, faultsize=None, mollify=False, material_files=None):
     T = lambda b: 'T' if b else 'F'
     s = ["&problem_list",
          f" name = 'data/{name}',", f" problem = '{problem}',", f" response = '{response}',",
          f" plastic_model = '{plastic_model}',", " nblocks = 2,", f" t_final = {t_final}d0,"]
     if mollify:
         s += [" mollify_source = T,"]
+    if material_files is not None:
+        s += [" material_source = 'file',"]
     if fd_type is not None:
         s += [f" fd_type = '{fd_type}',", f" order = {order},"]
     s += [
@@ -64,7 +66,8 @@ def namelist(name, problem, coupling, mesh, n, lq, rq, *, response='elastic',
               f"btp({i})%profile_type = '{profile}',", f"btp({i})%profile_path = '{profile_path}',",
               f"btp({i})%topography_type = '{topography['type'] if topography else 'none'}',",
               f"btp({i})%topography_path = '{f'topo{i}.dat' if topography and topography['type'] == 'read_topo_from_memory' else 'No-path'}',",
-              "btp(%d)%%material_path = 'none', 'none', 'none'," % i,
+              ("btp(%d)%%material_path = 'mat%d_lam.bin', 'mat%d_mu.bin', 'mat%d_rho.bin'," % (i, i, i, i)
+               if material_files is not None else "btp(%d)%%material_path = 'none', 'none', 'none'," % i),
               f"btp({i})%pml_lqrs = {', '.join(T(x) for x in pml_l[b])},",
               f"btp({i})%pml_rqrs = {', '.join(T(x) for x in pml_r[b])},",
               f"btp({i})%npml = {npml},",
@@ -167,6 +170,10 @@ CASES = {
                                          U=["'Cos-Bell' 0.0 0.08 0.2 0.2 0.2 0.0 0.0 0.0 -2.2 8.1 -0.9 0"],
                                          V=['cosine 0.1 0.0 0.3 0.3 -0.6 0.2 0.5 0.1 1.1 7.1 -0.3 0'])),
                         [('rhs', ['0.03']), ('traj', ['12'])]),
+    # a block medium read from files (material_source = 'file')
+    'matfile_sw': (dict(problem='TPV5', coupling='slip-weakening_friction', mesh='curvilinear', n=N_STD,
+                        lq=LQ_STD, rq=RQ_STD, material_files=dict(seed=23), keep_mat=True),
+                   [('rhs', ['0.3']), ('traj', ['12'])]),
     'tpv26_plastic': (dict(problem='TPV26', coupling='slip-weakening_friction', mesh='cartesian',
                            n=N_STD, lq=LQ_STD, rq=RQ_STD, response='plastic', plastic_model='scec',
                            aqrs=[(-6.0, 2.0, -9.0), (0.0, 2.0, -9.0)], bqrs=[(0.0, 24.0, 3.0), (6.0, 24.0, 3.0)]),
@@ -385,6 +392,31 @@ def write_fractal(work, kw):
     return x.T.copy()  # [nr][ns]
 
 
+def write_material(work, kw):
+    """the three material files of each block (init_material_from_file):
+    raw FP64 lambda, mu, rho with q fastest, a smooth random medium around
+    rho = 2.67, vs = 3.464, vp = 6 (continuous across the shared face)"""
+    nq, nr, ns = kw['n']
+    rng = np.random.default_rng(kw['material_files']['seed'])
+    out = []
+    ph = rng.uniform(0, 6.0, 3)
+    for b in range(2):
+        i = np.arange(nq)[None, None, :] + b * (nq - 1)
+        j = np.arange(nr)[None, :, None]
+        k = np.arange(ns)[:, None, None]
+        w = np.sin(0.3 * i + ph[0]) * np.cos(0.4 * j + ph[1]) * np.sin(0.25 * k + ph[2])
+        rho = 2.67 * (1.0 + 0.05 * w)
+        vs = 3.464 * (1.0 + 0.08 * w)
+        vp = 6.0 * (1.0 + 0.06 * w)
+        mu = rho * vs * vs
+        lam = rho * vp * vp - 2.0 * mu
+        M = np.stack([lam, mu, rho])  # [3][ns][nr][nq]
+        for c, tag in enumerate(('lam', 'mu', 'rho')):
+            M[c].astype('<f8').tofile(os.path.join(work, f'mat{b + 1}_{tag}.bin'))
+        out.append(M)
+    return out
+
+
 def read_block(prefix, b, name, shape):
     return np.fromfile(f'{prefix}_b{b}_{name}.bin', '<f8').reshape(shape, order='F')
 
@@ -395,6 +427,8 @@ def run_case(name, kw, runs, work):
     with open(inp, 'w') as f:
         f.write(namelist(name, **kw))
     fractal = write_fractal(work, kw) if kw.get('fractal') else None
+    if kw.get('material_files') is not None:
+        write_material(work, kw)  # the fixture keeps the medium as b*_Mat
     topo = write_topography(work, kw) if kw.get('topography', {}) and kw['topography']['type'] == \
         'read_topo_from_memory' else None
     for d in ('data', 'seismogram/block1/data', 'seismogram/block2/data'):

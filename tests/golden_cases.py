@@ -149,3 +149,21 @@ KEY_TRAJ_CASES = ['tpv10_sw', 'tpv29_sw', 'tpv31_sw', 'tpv32_sw', 'tpv33_sw', 't
 # the Gaussian-mollified moment-tensor sources (mollify_source = T)
 SMOOTH_CASES = ['mt_smooth', 'mt_smooth_lists']
 MEDIUM_CASES = ['tpv31_sw', 'tpv32_sw', 'tpv33_sw', 'loh1_locked', 'oklahoma_locked']
+
+
+# a block medium read from files (material_source = 'file')
+MATFILE_CASES = ['matfile_sw']
+
+
+def matfile_problem(case, tmpdir):
+    """(Problem, input path) of a material-file case: the three files of each
+    block (raw FP64 lambda, mu, rho, q fastest) written from the fixture's
+    medium beside a copy of its input file in tmpdir"""
+    z = load(case)
+    for b in (1, 2):
+        for c, tag in enumerate(('lam', 'mu', 'rho')):
+            np.ascontiguousarray(z[f'b{b}_Mat'][c]).astype('<f8').tofile(os.path.join(tmpdir, f'mat{b}_{tag}.bin'))
+    dst = os.path.join(tmpdir, f'{case}.in')
+    with open(os.path.join(GOLDEN, 'inputs', f'{case}.in')) as a, open(dst, 'w') as f:
+        f.write(a.read())
+    return P.from_namelist(dst), dst

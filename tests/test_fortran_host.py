@@ -135,6 +135,23 @@ def test_fortran_main_matches_reference(case, tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not _have(MAIN), reason='Fortran host not built (no flang)')
+def test_fortran_main_material_files(tmp_path):
+    """material_source = 'file': the host reads each block's three files
+    from the run directory (init_material_from_file)"""
+    _say('[gpu] fortran main material files')
+    z = G.load('matfile_sw')
+    prob, inp = G.matfile_problem('matfile_sw', str(tmp_path))
+    steps = int(z['traj_args'][0])
+    dump, out = _run_main(inp, steps, str(tmp_path))
+    got = _read_dump(dump, prob)
+    for ib in (1, 2):
+        assert G.rel_l2(got[f'b{ib}_F'], z[f'b{ib}_F']) <= 1e-12, f'F block {ib}'
+        ref = z[f'b{ib}_S']
+        assert np.abs(got[f'b{ib}_S'] - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not _have(MAIN), reason='Fortran host not built (no flang)')
 def test_fortran_main_block_list_file(tmp_path):
     """the same run with the block list moved to a second file named by
     `&block_list = 'blocks/curv_sw.nml'` (input_utils.f90:45-94)"""

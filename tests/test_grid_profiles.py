@@ -69,7 +69,6 @@ def test_block_list_file(tmp_path):
 
 
 @pytest.mark.parametrize('edit', [("nblocks = 2", "nblocks = 3"),
-                                  ("w_stride = 1,", "w_stride = 1, material_source = 'file',"),
                                   ("use_mms = F", "use_mms = T")])
 def test_unsupported_inputs_fail(edit, tmp_path):
     """input features outside the path raise instead of running as something
@@ -91,3 +90,17 @@ def test_mollified_sources_parse(tmp_path):
     assert not G.problem('mt_cart').mollify_source
     with pytest.raises(NotImplementedError):
         O.OracleDomain(prob)
+
+
+def test_material_files_read(tmp_path):
+    """material_source = 'file' fills each block's medium from its three files
+    (init_material_from_file); a short file is an error"""
+    z = G.load('matfile_sw')
+    prob, inp = G.matfile_problem('matfile_sw', str(tmp_path))
+    for ib, b in enumerate(prob.blocks):
+        assert b.medium is None
+        assert np.array_equal(b.lame(), z[f'b{ib + 1}_Mat'])
+    (tmp_path / 'mat2_rho.bin').write_bytes((tmp_path / 'mat2_rho.bin').read_bytes()[:-8])
+    from wql.problem import from_namelist
+    with pytest.raises(ValueError):
+        from_namelist(inp)

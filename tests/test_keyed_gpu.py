@@ -89,3 +89,32 @@ def test_keyed_medium_on_the_device(case):
     with Simulation(prob) as sim:
         sim.step(1)
         sim.sync()
+
+
+@pytest.mark.parametrize('case', G.MATFILE_CASES)
+def test_material_file_case(case, tmp_path):
+    """a medium read from files (material_source = 'file'): rates of the
+    pattern state with every term, and the FP64 trajectory, against the
+    reference build"""
+    from wql.solver import Simulation
+    _say(f'[gpu] material file {case}')
+    z = G.load(case)
+    prob, _ = G.matfile_problem(case, str(tmp_path))
+    with Simulation(prob) as sim:
+        H.load_pattern_state(sim, prob)
+        sim.eval_rates(float(z['rhs_t']), what=1, stage=2)
+        for ib in range(2):
+            assert G.rel_l2(sim.rates(ib), z[f'b{ib + 1}_DFfull']) <= TOL, f'DF block {ib + 1}'
+    args = z['traj_args']
+    with Simulation(prob) as sim:
+        F0 = H.pulse_init(prob, args)
+        for ib in range(2):
+            sim.set_fields(ib, F0[ib])
+        sim.init_state()
+        sim.step(int(args[0]))
+        sim.sync()
+        for ib in range(2):
+            assert G.rel_l2(sim.fields(ib), z[f'b{ib + 1}_F']) <= TOL, f'F block {ib + 1}'
+            f = sim.fault(ib)
+            ref = z[f'b{ib + 1}_S']
+            assert np.abs(f['S'] - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())

@@ -86,7 +86,7 @@ contains
     character(*), intent(in) :: ifname
     integer(c_int), intent(in), optional :: precision
     type(wql_grid_desc) :: g
-    integer :: ib, k
+    integer :: ib, k, u, stat
     real(kind = wp) :: dt(2)
     real(kind = wp), allocatable, target :: fx(:,:)
 
@@ -94,7 +94,6 @@ contains
     D%name = D%p%name; D%problem = D%p%problem; D%coupling = D%p%coupling; D%response = D%p%response
     D%t_final = D%p%t_final; D%CFL = D%p%CFL; D%w_stride = D%p%w_stride; D%w_fault = D%p%w_fault
     if (D%p%nblocks /= 2) error stop 'init_domain: the path runs two blocks joined by one interface'
-    if (trim(D%p%material_source) == 'file') error stop "init_domain: material_source = 'file' is not read by this host"
     D%t = 0.0_wp
     do ib = 1, 2
        dt(ib) = block_time_step(D%p%btp(ib), D%CFL)
@@ -159,9 +158,23 @@ contains
          D%desc(ib)%cartesian = merge(1, 0, g%mesh == 0)
          D%desc(ib)%X = c_loc(D%B(ib)%X)
          D%desc(ib)%M = c_null_ptr
-         ! the medium init_material keys by problem name, if any
+         ! the medium init_material keys by problem name, if any, or
+         ! init_material_from_file (material.f90:706-747): lambda, mu, rho
+         ! from the block's three material_path files, raw FP64 with q fastest
          allocate(D%B(ib)%M(g%nq, g%nr, g%ns, 3))
-         k = wql_problem_material(c_string(D%problem), g%nq, g%nr, g%ns, D%B(ib)%X, D%B(ib)%M)
+         if (trim(D%p%material_source) == 'file') then
+            do k = 1, 3
+               open(newunit=u, file=trim(b%material_path(k)), access='stream', form='unformatted', &
+                    status='old', action='read', iostat=stat)
+               if (stat /= 0) error stop 'init_domain: cannot open a material_path file'
+               read(u, iostat=stat) D%B(ib)%M(:, :, :, k)
+               if (stat /= 0) error stop 'init_domain: a material_path file is shorter than its block'
+               close(u)
+            end do
+            k = 1
+         else
+            k = wql_problem_material(c_string(D%problem), g%nq, g%nr, g%ns, D%B(ib)%X, D%B(ib)%M)
+         end if
          if (k < 0) call wql_check(k, 'wql_problem_material')
          if (k == 1) then
             D%desc(ib)%M = c_loc(D%B(ib)%M)

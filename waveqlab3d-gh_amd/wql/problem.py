@@ -367,6 +367,25 @@ def _data_path(p, base):
     return os.path.join(base, p)
 
 
+def read_material_files(paths, n):
+    """lambda, mu, rho [3][ns][nr][nq] from the three files of
+    init_material_from_file (material.f90:706-747, read_file_distributed over
+    the block's global array): raw native FP64, q fastest"""
+    if len(paths) != 3 or any(p is None for p in paths):
+        raise ValueError("material_source = 'file' needs three material_path files (lambda, mu, rho)")
+    nq, nr, ns = n
+    M = np.empty((3, ns, nr, nq))
+    for k, p in enumerate(paths):
+        try:
+            a = np.fromfile(p, dtype=np.float64, count=nq * nr * ns)
+        except OSError:
+            raise ValueError(f'error opening material file {p}') from None
+        if a.size != nq * nr * ns:
+            raise ValueError(f'material file {p}: {a.size} values, the block has {nq * nr * ns}')
+        M[k] = a.reshape(ns, nr, nq)
+    return M
+
+
 def block_list_file(text, base):
     """the file named by '&block_list = 'file'' in an input file, relative to
     the input file's directory unless absolute (locate_block_list_file,
@@ -391,8 +410,6 @@ def from_namelist(path):
     # something else
     if int(pl.get('nblocks', 2)) != 2:
         raise NotImplementedError('the path runs two blocks joined by one interface')
-    if str(pl.get('material_source', 'hardcode')) == 'file':
-        raise NotImplementedError("material_source = 'file': set Block.material from the files instead")
     if read_namelist(text, 'mms_list').get('use_mms', False):
         raise NotImplementedError('use_mms: the manufactured-solution forcing is not built')
     blf = block_list_file(text, base)
@@ -431,7 +448,13 @@ def from_namelist(path):
             fs = g('faultsize', None)
             if fs and b.profile == 'read_from_memomry_fractal' and b.topography == 'read_topo_from_memory':
                 b.faultsize = (int(fs[0]), int(fs[1]))
-    for b in blocks:
+    for i, b in enumerate(blocks, 1):
+        if str(pl.get('material_source', 'hardcode')) == 'file':
+            # init_material_from_file (material.f90:706-747): lambda, mu, rho
+            # from the block's three material_path files
+            paths = bl.get(f'btp({i})%material_path', None) or []
+            b.material = read_material_files([_data_path(p, base) for p in paths], b.n)
+            continue
         # init_material's keyed medium, evaluated on the block's grid when a
         # simulation is set up (Block.medium_material)
         b.medium = problem if problem in MEDIA else None
