@@ -313,17 +313,17 @@ CASES = {
     # 1349-1860): a topography file (x, y, z per top-face node, scaled by
     # topo), the analytical hill at (x, z) = (10, 10), and a topography file
     # with a fault-profile file larger than the face (faultsize)
-    'grid_topo_read': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 12, 15),
+    'grid_topo_read': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 13, 15),
                             lq=LQ_STD, rq=RQ_STD, profile='no_path',
                             topography=dict(type='read_topo_from_memory', topo=0.5, seed=13, amp=0.3),
                             aqrs=[(-4.0, 0.0, -3.0), (0.0, 0.0, -3.0)], bqrs=[(0.0, 6.0, 3.0), (4.0, 6.0, 3.0)]),
                        [('rhs', ['0.1'])]),
-    'grid_topo_analytic': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 12, 15),
+    'grid_topo_analytic': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 13, 15),
                                 lq=LQ_STD, rq=RQ_STD, profile='no_path',
                                 topography=dict(type='analytical_topography', topo=1.0),
                                 aqrs=[(4.0, 0.0, 5.0), (10.0, 0.0, 5.0)], bqrs=[(10.0, 6.0, 15.0), (16.0, 6.0, 15.0)]),
                            [('rhs', ['0.1'])]),
-    'grid_topo_fractal': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 12, 15),
+    'grid_topo_fractal': (dict(problem='TPV5', coupling='locked', mesh='curvilinear', n=(13, 13, 15),
                                lq=LQ_STD, rq=RQ_STD, profile='read_from_memomry_fractal', profile_path='fault.dat',
                                fractal=dict(seed=17, amp=0.05), faultsize=(14, 18),
                                topography=dict(type='read_topo_from_memory', topo=1.0, seed=19, amp=0.2),

@@ -224,9 +224,7 @@ def test_fortran_run_matches_python_host(prec):
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not _have(MAIN), reason='Fortran host not built (no flang)')
-# the Fortran host refuses use_topography (its deck reader stops with an
-# error); the topography branch runs through the Python host
-@pytest.mark.parametrize('case', [c for c in G.GRID_CASES if not c.startswith('grid_topo')])
+@pytest.mark.parametrize('case', G.GRID_CASES)
 def test_fortran_main_fault_profile_grids(case, tmp_path):
     """the Fortran host's grids of the TPV28, TPV10, TPV36 and file fault
     profiles (wql_grid_block) against the reference build's"""

@@ -144,8 +144,14 @@ contains
                g%fault_x = c_loc(fx)
             end if
          end select
-         if (D%p%use_topography) error stop "init_domain: the reference's topography branch is not restated by this host"
-         call wql_check(wql_grid_block(g, D%B(ib)%X), 'wql_grid_block')
+         if (D%p%use_topography .and. g%mesh == 1) then
+            ! the topography branch of Curve_Grid_3D3: faces here, volume on
+            ! the device
+            if (allocated(fx)) deallocate(fx)
+            call topography_grid(b, D%p%topo, D%B(ib)%X)
+         else
+            call wql_check(wql_grid_block(g, D%B(ib)%X), 'wql_grid_block')
+         end if
          if (allocated(fx)) deallocate(fx)
          D%desc(ib)%nq = g%nq; D%desc(ib)%nr = g%nr; D%desc(ib)%ns = g%ns
          ! boundary codes Lx Rx Ly Ry Lz Rz and PML sides (block.f90:27-269)
@@ -442,6 +448,136 @@ contains
 
   !> x offsets fx(nr, ns) of a fault profile file: nr*ns rows 'x y z', y
   !> fastest (read_2darray / reshape_2darray, grid.f90:2457-2505)
+  !> Gen_Surf_Mesh (grid.f90:2230-2288): S(i, j) of the 2-D transfinite
+  !> interpolation of the edges XL, XR (along j) and XB, XT (along i)
+  pure function surf(mx, my, XL, XR, XB, XT) result(S)
+    integer, intent(in) :: mx, my
+    real(kind = wp), intent(in) :: XL(my), XR(my), XB(mx), XT(mx)
+    real(kind = wp) :: S(mx, my), q, r
+    integer :: i, j
+    do j = 1, my
+       r = real(j - 1, wp) * (1.0_wp / real(my - 1, wp))
+       do i = 1, mx
+          q = real(i - 1, wp) * (1.0_wp / real(mx - 1, wp))
+          S(i, j) = (((((((1.0_wp - q) * XL(j) + q * XR(j)) + (1.0_wp - r) * XB(i)) + r * XT(i)) &
+                    - ((1.0_wp - q) * (1.0_wp - r)) * XL(1)) - (q * (1.0_wp - r)) * XR(1)) &
+                    - (r * (1.0_wp - q)) * XT(1)) - (r * q) * XT(mx)
+       end do
+    end do
+  end function surf
+
+  !> the topography branch of Curve_Grid_3D3 (use_topography, grid.f90:
+  !> 1349-1860): the top face from 'analytical_topography' or a
+  !> 'read_topo_from_memory' file (rows x y z, z fastest), the q faces between
+  !> it and y = b_y (or x, y, z of a 'read_from_memomry_fractal' file of
+  !> faultsize rows on a weighted face), the s faces and the bottom face from
+  !> their edges; wql_grid_faces interpolates the volume on the device
+  subroutine topography_grid(b, topo, X)
+    type(block_temp_parameters), intent(in) :: b
+    real(kind = wp), intent(in) :: topo
+    real(kind = wp), intent(out) :: X(:,:,:,:)
+    real(kind = wp), parameter :: pi = 3.141592653589793_wp
+    real(kind = wp), allocatable :: Xt(:,:), Yt(:,:), Zt(:,:), Xl(:,:), Yl(:,:), Zl(:,:), Xr(:,:), Yr(:,:), Zr(:,:)
+    real(kind = wp), allocatable :: Xf(:,:), Yf(:,:), Xk(:,:), Yk(:,:), Xb(:,:), Zb(:,:), d(:,:), F(:), zc(:), rr(:), qq(:)
+    real(kind = wp) :: row(3), r1, hill, hx, hz
+    integer :: mx, my, mz, i, k, u, stat, ny, nz, o
+    mx = b%nqrs(1); my = b%nqrs(2); mz = b%nqrs(3)
+    allocate(Xt(mx, mz), Yt(mx, mz), Zt(mx, mz))
+    select case (trim(b%topography_type))
+    case ('analytical_topography')
+       hx = (b%bqrs(1) - b%aqrs(1)) / real(mx - 1, wp)
+       hz = (b%bqrs(3) - b%aqrs(3)) / real(mz - 1, wp)
+       do k = 1, mz
+          do i = 1, mx
+             Xt(i, k) = b%aqrs(1) + real(i - 1, wp) * hx
+             Zt(i, k) = b%aqrs(3) + real(k - 1, wp) * hz
+             r1 = sqrt((Xt(i, k) - 10.0_wp) * (Xt(i, k) - 10.0_wp) + (Zt(i, k) - 10.0_wp) * (Zt(i, k) - 10.0_wp))
+             hill = 0.0_wp
+             if (r1 <= 3.0_wp) hill = -0.3_wp * (1.0_wp + cos(pi * r1 / 3.0_wp))
+             Yt(i, k) = b%aqrs(2) - (topo * 1.5_wp) * hill
+          end do
+       end do
+    case ('read_topo_from_memory')
+       open(newunit = u, file = trim(b%topography_path), status = 'old', action = 'read', iostat = stat)
+       if (stat /= 0) error stop 'init_domain: cannot open the topography file'
+       do i = 1, mx
+          do k = 1, mz
+             read(u, *, iostat = stat) row
+             if (stat /= 0) error stop 'init_domain: short topography file'
+             Xt(i, k) = row(1); Yt(i, k) = b%aqrs(2) + topo * row(2); Zt(i, k) = row(3)
+          end do
+       end do
+       close(u)
+    case default
+       error stop 'init_domain: specify topography profile'
+    end select
+    allocate(zc(mz), rr(my), qq(mx))
+    do k = 1, mz
+       zc(k) = b%aqrs(3) + (b%bqrs(3) - b%aqrs(3)) * (real(k - 1, wp) * (1.0_wp / real(mz - 1, wp)))
+    end do
+    do i = 1, my
+       rr(i) = real(i - 1, wp) * (1.0_wp / real(my - 1, wp))
+    end do
+    do i = 1, mx
+       qq(i) = real(i - 1, wp) * (1.0_wp / real(mx - 1, wp))
+    end do
+    allocate(Xl(my, mz), Yl(my, mz), Zl(my, mz), Xr(my, mz), Yr(my, mz), Zr(my, mz))
+    Yl = surf(my, mz, Yt(1, :), spread(b%bqrs(2), 1, mz), Yt(1, 1) + (b%bqrs(2) - Yt(1, 1)) * rr, &
+              Yt(1, mz) + (b%bqrs(2) - Yt(1, mz)) * rr)
+    Yr = surf(my, mz, Yt(mx, :), spread(b%bqrs(2), 1, mz), Yt(mx, 1) + (b%bqrs(2) - Yt(mx, 1)) * rr, &
+              Yt(mx, mz) + (b%bqrs(2) - Yt(mx, mz)) * rr)
+    Zl = surf(my, mz, zc, zc, spread(b%aqrs(3), 1, my), spread(b%bqrs(3), 1, my))
+    Zr = Zl
+    Xl = b%aqrs(1); Xr = b%bqrs(1)
+    select case (trim(b%profile_type))
+    case ('read_from_memomry_fractal')
+       ny = my; nz = mz
+       if (all(b%faultsize > 0)) then
+          ny = b%faultsize(1); nz = b%faultsize(2)
+       end if
+       if (ny < my .or. nz < mz) error stop 'init_domain: faultsize smaller than the fault face'
+       allocate(d(3, ny * nz))
+       open(newunit = u, file = trim(b%profile_path), status = 'old', action = 'read', iostat = stat)
+       if (stat /= 0) error stop 'init_domain: cannot open the fault profile file'
+       do i = 1, ny * nz
+          read(u, *, iostat = stat) d(:, i)
+          if (stat /= 0) error stop 'init_domain: short fault profile file'
+       end do
+       close(u)
+       do k = 1, mz
+          do i = 1, my
+             o = i + ny * (k - 1)
+             if (b%rc /= 0.0_wp) then
+                Xl(i, k) = b%aqrs(1) + d(1, o); Yl(i, k) = d(2, o); Zl(i, k) = d(3, o)
+             end if
+             if (b%lc /= 0.0_wp) then
+                Xr(i, k) = b%bqrs(1) + d(1, o); Yr(i, k) = d(2, o); Zr(i, k) = d(3, o)
+             end if
+          end do
+       end do
+    case ('no_path')
+    case default
+       error stop "init_domain: the topography branch takes the fault profile 'no_path' or 'read_from_memomry_fractal'"
+    end select
+    allocate(Xf(mx, my), Yf(mx, my), Xk(mx, my), Yk(mx, my))
+    Xf = surf(mx, my, Xl(:, 1), Xr(:, 1), Xt(:, 1), Xl(my, 1) + (Xr(my, 1) - Xl(my, 1)) * qq)
+    Yf = surf(mx, my, Yl(:, 1), Yr(:, 1), Yt(:, 1), spread(b%bqrs(2), 1, mx))
+    Xk = surf(mx, my, Xl(:, mz), Xr(:, mz), Xt(:, mz), Xl(my, mz) + (Xr(my, mz) - Xl(my, mz)) * qq)
+    Yk = surf(mx, my, Yl(:, mz), Yr(:, mz), Yt(:, mz), spread(b%bqrs(2), 1, mx))
+    allocate(Xb(mx, mz), Zb(mx, mz))
+    Xb = surf(mx, mz, Xl(my, :), Xr(my, :), Xf(:, my), Xk(:, my))
+    Zb = surf(mx, mz, Zl(my, :), Zr(my, :), spread(b%aqrs(3), 1, mx), spread(b%bqrs(3), 1, mx))
+    ! wql_grid_faces' layout: per component the q = 0, q = nq-1, r = 0,
+    ! r = nr-1, s = 0, s = ns-1 faces, each in Fortran order
+    F = [reshape(Xl, [my * mz]), reshape(Xr, [my * mz]), reshape(Xt, [mx * mz]), reshape(Xb, [mx * mz]), &
+         reshape(Xf, [mx * my]), reshape(Xk, [mx * my]), &
+         reshape(Yl, [my * mz]), reshape(Yr, [my * mz]), reshape(Yt, [mx * mz]), &
+         spread(b%bqrs(2), 1, mx * mz), reshape(Yf, [mx * my]), reshape(Yk, [mx * my]), &
+         reshape(Zl, [my * mz]), reshape(Zr, [my * mz]), reshape(Zt, [mx * mz]), reshape(Zb, [mx * mz]), &
+         spread(b%aqrs(3), 1, mx * my), spread(b%bqrs(3), 1, mx * my)]
+    call wql_check(wql_grid_faces(int(mx, c_int), int(my, c_int), int(mz, c_int), F, X), 'wql_grid_faces')
+  end subroutine topography_grid
+
   subroutine read_fault_profile(path, nr, ns, fx)
     character(*), intent(in) :: path
     integer(c_int), intent(in) :: nr, ns

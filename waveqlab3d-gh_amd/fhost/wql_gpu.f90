@@ -96,6 +96,12 @@ module wql_gpu
        type(wql_grid_desc), intent(in) :: grid
        real(c_double), intent(out) :: X(*)
      end function wql_grid_block
+     integer(c_int) function wql_grid_faces(nq, nr, ns, faces, X) bind(C, name='wql_grid_faces')
+       import :: c_int, c_double
+       integer(c_int), value :: nq, nr, ns
+       real(c_double), intent(in) :: faces(*)
+       real(c_double), intent(out) :: X(*)
+     end function wql_grid_faces
      integer(c_int) function wql_create(prob, blocks, ctx) bind(C, name='wql_create')
        import :: c_int, c_ptr, wql_problem, wql_block_desc
        type(wql_problem), intent(in) :: prob
